@@ -1,0 +1,182 @@
+"""Benchmark: particle-steps/s of the explicit correspondence-PD increment with the
+Johnson-Cook VUMAT on one MI355X, on the 500k-particle JC tensile bar (200×50×50 lattice,
+Δx = 0.2 mm, δ = 3.015Δx, FP64, critical-stretch damage s_c = 0.02, end-face velocity
+1 mm/ms, dt = 0.5·Δx/c_p), with the dominant kernel's HBM roofline and the reference's CPU
+path timed on this host beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line. A step is one fused increment (F + damage bond pass, material point
+update, force bond pass + velocity-Verlet) over all 500k particles, inputs resident in HBM.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vumat-pd_amd"))
+
+# Johnson-Cook A=350 MPa, B=275 MPa, n=0.36, C=0.022, m=1, ε̇₀ = 1/s = 1e-3/ms, T_room 293 K,
+# T_melt 1793 K, E = 200 GPa, ν = 0.3, ρ = 7850 kg/m³, β = 0.9, c_p = 500, workToHeat = 1.
+JC_PROPS = [200000.0, 0.3, 350.0, 275.0, 0.022, 0.36, 1.0, 1.0e-3, 293.0, 1793.0, 7850.0, 0.9,
+            500.0, 1.0]
+RHO = 7.85e-3  # N·ms²/mm⁴
+HBM_PEAK = 8.0e12
+
+
+def cfl_dt(h, fraction=0.5, props=JC_PROPS, rho=RHO):
+    E, nu = props[0], props[1]
+    cp = math.sqrt(E * (1 - nu) / (rho * (1 + nu) * (1 - 2 * nu)))
+    return fraction * h / cp
+
+
+WORKLOAD = dict(name="jc_tensile_bar_500k", nx=200, ny=50, nz=50, spacing=0.2,
+                horizon=3.015 * 0.2, velocity=1.0, damage=(0.02, 0.0))
+
+
+def bytes_per_particle(kernel, bbar, damage):
+    """Algorithmic HBM bytes per particle and launch: every FP64 field the kernel must read
+    or write once, int32 CSR indices once, bit-packed bond flags (one read, one write)."""
+    flags = bbar / 8.0 if damage else 0.0
+    if kernel == "defgrad_bond_pass":
+        # X 24 + u 24 + V 8 + K⁻¹ 72 + rowptr 4 + col 4b̄ + F 72 out (+ flags r/w)
+        return 24 + 24 + 8 + 72 + 4 + 4 * bbar + 72 + 2 * flags
+    if kernel == "material_jc":
+        # F 72 + (F_n, V, R) 216 r + 216 w + σ̄ 48+48 + statev(6) 48+48 + T 8 w
+        # + energies 16+16 + K⁻¹ 72 + A 72 out
+        return 72 + 432 + 96 + 96 + 8 + 32 + 72 + 72
+    if kernel == "force_bond_pass":
+        # X 24 + V 8 + A 72 + rowptr 4 + col 4b̄ + u, v, a 72 r + 72 w + code 1 (+ flags r)
+        return 24 + 8 + 72 + 4 + 4 * bbar + 144 + 1 + flags
+    if kernel == "predict":
+        return 144 + 1
+    return 0.0
+
+
+def step_bytes_per_particle(bbar, damage):
+    """The whole-step model: 816 persistent state + 136 (X, V both passes, K⁻¹) + 144 (A)
+    + 24 (u re-read) + 8 (rowptr) + 8b̄ (col, both passes) + b̄/4 (flags r/w)."""
+    return 816 + 136 + 144 + 24 + 8 + 8 * bbar + (bbar / 4.0 if damage else 0.0)
+
+
+def cpu_baseline(sample_steps, w):
+    """The reference's own OpenMP increment (oracle/_ref) on this host, or the restated
+    oracle when the reference build is absent, on the same bar for `sample_steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    kw = dict(nx=w["nx"], ny=w["ny"], nz=w["nz"], spacing=w["spacing"], horizon=w["horizon"],
+              props=JC_PROPS, dt=cfl_dt(w["spacing"]), density=RHO, amplitude=1.0,
+              rise_time=1.0, damage=w["damage"])
+    if O.ref_available():
+        kind = "reference"
+        # The reference runner only offers the half-cosine ramp; the velocity-driven band
+        # changes which constraint values are written, not the cost of a step.
+        bar = O.RefBar(**kw)
+    else:
+        kind = "port"
+        O.build()
+        bar = O.OracleBar(**dict(kw, velocity=w["velocity"]))
+    n = w["nx"] * w["ny"] * w["nz"]
+    t0 = time.perf_counter()
+    bar.step(sample_steps)
+    dt = time.perf_counter() - t0
+    threads = O.lib().orc_num_threads()
+    return dict(value=n * sample_steps / dt, unit="particle-steps/s", cores=threads, kind=kind,
+                sample="steps 1-%d of the %d-particle bar (setup untimed), %d OpenMP threads"
+                % (sample_steps, n, threads), seconds=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this path is not sharded across GPUs "
+              "(see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import vpd
+
+    w = WORKLOAD
+    dt = cfl_dt(w["spacing"])
+    print("[bench] building %s on the GPU" % w["name"], file=sys.stderr, flush=True)
+    bar = vpd.Bar(w["nx"], w["ny"], w["nz"], w["spacing"], w["horizon"], JC_PROPS, dt, RHO,
+                  velocity=w["velocity"], damage=w["damage"])
+    n, nnz = bar.n, bar.ctx.nnz
+    bbar = nnz / n
+    if args.warmup:
+        bar.step(args.warmup)
+        bar.ctx.synchronize()
+        bar.reset()
+    bar.ctx.set_timing(True)
+    bar.ctx.synchronize()
+    print("[bench] timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    bar.step(args.steps)
+    bar.ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = bar.ctx.collect_timing()
+    u = bar.get("u")
+    eqps = bar.get("statev")[:, 1]
+    broken = bar.get("damage").sum()
+    if not np.isfinite(u).all():
+        raise SystemExit("non-finite displacement after the timed run")
+
+    value = n * args.steps / elapsed
+    dom = max((k for k in kt if k != "neighbor_build"), key=lambda k: kt[k][0])
+    avg_s = kt[dom][0] / kt[dom][1] / 1e3
+    per_launch = bytes_per_particle(dom, bbar, w["damage"] is not None) * n
+    achieved = per_launch / avg_s
+    step_bps = step_bytes_per_particle(bbar, w["damage"] is not None)
+    out = {
+        "metric": "particle-steps/s (500k JC bar, δ=3.015Δx) GPU plain; %HBM peak",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (deterministic lattice, JC material of the benchmark config)",
+        "config": {"workload": w["name"], "particles": n, "bonds": nnz, "grid": [w["nx"], w["ny"], w["nz"]],
+                   "spacing_mm": w["spacing"], "horizon_over_spacing": 3.015, "dt_ms": dt,
+                   "end_velocity_mm_per_ms": w["velocity"], "damage": "critical_stretch s_c=0.02",
+                   "return_map": "bisection (parity)", "steps_from": 1},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9 * 1e9,
+                     "unit": "B/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_s * 1e3},
+        "step_model": {"bytes_per_particle_step": step_bps,
+                       "hbm_fraction": step_bps * value / HBM_PEAK},
+        "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
+        "result": {"max_eqps": float(eqps.max()), "broken_bonds": int(broken)},
+    }
+    out["roofline"]["achieved"] /= 1e9
+    out["roofline"]["peak"] = HBM_PEAK / 1e9
+    out["roofline"]["unit"] = "GB/s"
+    if not args.no_cpu_baseline:
+        del bar
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        cb = cpu_baseline(args.cpu_steps, w)
+        cb["gpu_over_cpu"] = value / cb["value"]
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

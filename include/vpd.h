@@ -1,0 +1,217 @@
+/*
+ * vpd.h — C ABI of the MI355X (gfx950) explicit correspondence-PD increment with the
+ * Johnson-Cook VUMAT. Plain C: opaque handles, plain pointers and sizes, status codes.
+ *
+ * One vpd_context owns one particle block on one GPU and one HIP stream; calls on one
+ * context are not reentrant (as the reference's objects are not thread-safe for
+ * concurrent callers, docs/architecture.md:28). Every call returns VPD_OK (0) or an error
+ * status; vpd_last_error() gives the message, worded as the reference's exceptions are,
+ * so a C++ wrapper can rethrow std::runtime_error(message) unchanged.
+ *
+ * Host arrays crossing this boundary use the reference's in-memory layouts:
+ *   points     double[n][3]   (std::vector<Vec3>,    include/PdTypes.hpp:9)
+ *   tensors    double[n][9]   row-major 3x3 (std::vector<Tensor3>, include/PdTypes.hpp:10)
+ *   neighbours CSR: int32 rowptr[n+1], int32 col[nnz], rows ascending
+ *              (std::vector<std::vector<int>>, include/PdTypes.hpp:18)
+ *   VUMAT      column-major blocks [k + nblock*c] (src/material/VumatMaterialAdapter.cpp:77-80)
+ * On the device every particle field is SoA (one plane per component).
+ */
+#ifndef VPD_H
+#define VPD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPD_ABI_VERSION 1
+
+enum vpd_status {
+  VPD_OK = 0,
+  VPD_ERR_ARGUMENT = 1,  /* invalid argument / size mismatch (std::runtime_error in the reference) */
+  VPD_ERR_NUMERIC = 2,   /* singular tensor, bad determinant, non-finite coordinate */
+  VPD_ERR_STATE = 3,     /* call out of order (e.g. step before initialisation) */
+  VPD_ERR_DEVICE = 4,    /* HIP runtime failure or no device */
+  VPD_ERR_UNSUPPORTED = 5
+};
+
+/* Return-map solver of the Johnson-Cook stage. */
+enum vpd_return_map {
+  VPD_RETURN_MAP_BISECTION = 0 /* exact bracketed bisection of JohnsonCook.f90:114-136 */
+};
+
+enum vpd_damage_mode {
+  VPD_DAMAGE_NONE = 0,
+  VPD_DAMAGE_CRITICAL_STRETCH = 1 /* src/damage/PdSimpleDamage.cpp:63-84 */
+};
+
+/* Constraint code per particle and axis (BoundaryConditions.cpp:81-115): bits 2d..2d+1
+ * of codes[i] for axis d: 0 free, 1 held at zero (u = v = a = 0), 2 follows the ramp. */
+enum vpd_constraint { VPD_FREE = 0, VPD_ZERO = 1, VPD_RAMP = 2 };
+
+/* Field identifiers for vpd_upload / vpd_download (host layouts above). */
+enum vpd_field {
+  VPD_FIELD_REFERENCE_POSITION = 0, /* n×3 (read-only after vpd_set_particles) */
+  VPD_FIELD_VOLUME = 1,             /* n */
+  VPD_FIELD_DISPLACEMENT = 2,       /* n×3 ParticleBlock::displacement */
+  VPD_FIELD_VELOCITY = 3,           /* n×3 ExplicitDynamicState::velocity */
+  VPD_FIELD_ACCELERATION = 4,       /* n×3 ExplicitDynamicState::acceleration */
+  VPD_FIELD_DEFGRAD_N = 5,          /* n×9 MaterialState::deformationGradientN */
+  VPD_FIELD_LEFT_STRETCH_N = 6,     /* n×9 MaterialState::leftStretchN */
+  VPD_FIELD_ROTATION_N = 7,         /* n×9 MaterialState::rotationN */
+  VPD_FIELD_STRESS_N = 8,           /* n×9 MaterialState::rawStressN (corotational, symmetric) */
+  VPD_FIELD_STATEV = 9,             /* n×nstatev MaterialState::vumatStateN */
+  VPD_FIELD_TEMPERATURE = 10,       /* n */
+  VPD_FIELD_INTERNAL_ENERGY = 11,   /* n */
+  VPD_FIELD_INELASTIC_ENERGY = 12,  /* n */
+  VPD_FIELD_STRESS = 13,            /* n×9 MaterialUpdateResult::rawStress (spatial; download) */
+  VPD_FIELD_STRAIN = 14,            /* n×9 MaterialUpdateResult::strain (Green-Lagrange; download) */
+  VPD_FIELD_FORCE_DENSITY = 15,     /* n×3 computeForceDensity output (download) */
+  VPD_FIELD_BOND_DAMAGE = 16,       /* nnz doubles 0/1 PdDamageState::bondDamageN */
+  VPD_FIELD_DAMAGE_INDEX = 17,      /* n PdDamageResult::damageIndex (download) */
+  VPD_FIELD_SHAPE_INVERSE = 18,     /* n×9 CorrespondenceState::shapeTensorInverse */
+  VPD_FIELD_DEFGRAD = 19,           /* n×9 computeDeformationGradients output */
+  VPD_FIELD_RETURN_MAP_ITERATIONS = 20 /* n int32, iterations of the last return map */
+};
+
+typedef struct vpd_config {
+  /* PdCorrespondenceSkeleton(horizon), include/PdCorrespondenceSkeleton.hpp:12 */
+  double horizon;
+  /* ExplicitTimeIntegrator(density), include/ExplicitTimeIntegrator.hpp:17 (N·ms²/mm⁴) */
+  double density;
+  /* VumatPdCorrespondenceConfig, include/VumatPdCorrespondenceMaterial.hpp:13-19, with the
+   * Johnson-Cook props of materials/examples/JohnsonCook.f90:9 (14 values). */
+  int nprops;
+  const double *props;
+  int nstatev;                 /* >= 6 */
+  const double *initial_state; /* nstatev values, or NULL for zeros */
+  double reference_temperature;
+  double char_length;
+  double material_density; /* VumatMaterialConfig::density (kg/m³, passed to the VUMAT) */
+  /* SimpleDamageConfig, include/SimpleDamageModel.hpp:14-23 (critical_stretch only) */
+  int damage_mode;
+  double damage_initiation;
+  double damage_failure;
+  int return_map; /* enum vpd_return_map */
+} vpd_config;
+
+typedef struct vpd_context vpd_context;
+
+/* Per-kernel timing collected with HIP events on the context stream. */
+typedef struct vpd_kernel_time {
+  const char *name;
+  double total_ms;
+  int64_t launches;
+} vpd_kernel_time;
+
+typedef struct vpd_step_args {
+  double dt;
+  double time;       /* step·dt: constraints are applied at this time */
+  double step_time;  /* VUMAT stepTime = totalTime = (step−1)·dt (BoundaryValueCaseRunner.cpp:218-220) */
+  double ramp[3];    /* loaded-band (u, v, a) at `time` (RegionalConstraintSystem::ramp) */
+  int fuse_next;     /* 1: also predict the next step and constrain it with ramp_next */
+  double ramp_next[3];
+  int want_fields;   /* 1: keep spatial stress, strain and force density for download */
+} vpd_step_args;
+
+/* ---- lifetime ------------------------------------------------------------------- */
+int vpd_abi_version(void);
+int vpd_device_count(int *count);
+/* Replaces the constructors of PdCorrespondenceSkeleton (src/pd/PdCorrespondenceSkeleton.cpp:144-147),
+ * VumatPdCorrespondenceMaterial (src/material/VumatPdCorrespondenceMaterial.cpp:33-46),
+ * PdSimpleDamage (src/damage/PdSimpleDamage.cpp:39-40) and ExplicitTimeIntegrator
+ * (src/pd/ExplicitTimeIntegrator.cpp:9-12), with their argument checks. */
+int vpd_create(const vpd_config *config, vpd_context **out);
+void vpd_destroy(vpd_context *ctx);
+const char *vpd_last_error(const vpd_context *ctx); /* ctx may be NULL: last vpd_create error */
+
+/* ---- the particle block (include/PdTypes.hpp:14-19) ----------------------------- */
+int vpd_set_particles(vpd_context *ctx, const double *reference_position, const double *volume,
+                      int64_t n);
+/* buildNeighborList (src/pd/PdCorrespondenceSkeleton.cpp:95-129) on the GPU. */
+int vpd_build_neighbors(vpd_context *ctx, int64_t *nnz);
+/* ParticleBlock::neighbors supplied by the caller (validated: 0 <= j < n). */
+int vpd_set_neighbors(vpd_context *ctx, const int32_t *rowptr, const int32_t *col, int64_t nnz);
+int vpd_get_neighbors(vpd_context *ctx, int32_t *rowptr, int32_t *col);
+int vpd_set_constraints(vpd_context *ctx, const uint8_t *codes); /* n packed codes, or NULL */
+
+/* ---- initialisation -------------------------------------------------------------- */
+/* PdCorrespondenceSkeleton::initialize (src/pd/PdCorrespondenceSkeleton.cpp:149-170). */
+int vpd_initialize_shape(vpd_context *ctx);
+/* VumatPdCorrespondenceMaterial::initialize (src/material/VumatPdCorrespondenceMaterial.cpp:48-70)
+ * with F₀ = computeDeformationGradients at the current displacement. */
+int vpd_initialize_material(vpd_context *ctx);
+/* ExplicitTimeIntegrator::initialize (src/pd/ExplicitTimeIntegrator.cpp:14-21) and
+ * PdSimpleDamage::initialize (src/damage/PdSimpleDamage.cpp:42-49). */
+int vpd_initialize_dynamics(vpd_context *ctx);
+
+/* ---- state transfer ------------------------------------------------------------- */
+int vpd_upload(vpd_context *ctx, int field, const void *host);
+int vpd_download(vpd_context *ctx, int field, void *host);
+
+/* ---- the increment, piece by piece (for per-function parity) --------------------- */
+/* ExplicitTimeIntegrator::predictDisplacement (src/pd/ExplicitTimeIntegrator.cpp:23-34)
+ * followed by RegionalConstraintSystem::apply (src/host/BoundaryConditions.cpp:81-115). */
+int vpd_predict(vpd_context *ctx, double dt, const double ramp[3]);
+int vpd_apply_constraints(vpd_context *ctx, const double ramp[3]);
+/* PdCorrespondenceSkeleton::computeDeformationGradients (src/pd/PdCorrespondenceSkeleton.cpp:172-197);
+ * with damage on, also PdSimpleDamage::update's bond loop (src/damage/PdSimpleDamage.cpp:63-84)
+ * on the same positions. */
+int vpd_compute_deformation_gradient(vpd_context *ctx);
+/* VumatPdCorrespondenceMaterial::update (src/material/VumatPdCorrespondenceMaterial.cpp:72-176)
+ * on the last computed F, plus cauchyToFirstPiola·K⁻¹ (src/pd/PdCorrespondenceSkeleton.cpp:209-210). */
+int vpd_update_material(vpd_context *ctx, double dt, double step_time, double total_time);
+/* computeForceDensity's gather (src/pd/PdCorrespondenceSkeleton.cpp:213-227) and
+ * ExplicitTimeIntegrator::correctVelocity (src/pd/ExplicitTimeIntegrator.cpp:36-52), then the
+ * constraints at `ramp`. */
+int vpd_compute_force_and_correct(vpd_context *ctx, double dt, const double ramp[3]);
+
+/* ---- the fused increment ---------------------------------------------------------- */
+/* One step of the loop body in src/host/BoundaryValueCaseRunner.cpp:207-229 (without its
+ * diagnostics): [predict + constrain unless the previous step fused it] → F (+ damage) →
+ * material → force + correct + constrain [+ next predict + constrain]. Asynchronous on the
+ * context stream; numeric errors surface at the next vpd_synchronize / download. */
+int vpd_step(vpd_context *ctx, const vpd_step_args *args);
+int vpd_synchronize(vpd_context *ctx);
+
+/* ---- per-kernel timing ------------------------------------------------------------ */
+int vpd_set_timing(vpd_context *ctx, int enable);
+/* Synchronises, sums the recorded launches per kernel and clears them. Writes up to
+ * `capacity` entries and returns their count in *count. */
+int vpd_collect_timing(vpd_context *ctx, vpd_kernel_time *out, int capacity, int *count);
+
+/* ---- the VUMAT on device arrays ---------------------------------------------------- */
+/* vumat_adapter_call (materials/bridge/vumat_c_wrapper_jinfo.f90:1-44; typedef in
+ * include/VumatMaterialAdapter.hpp:35-39) running the Johnson-Cook routine
+ * (materials/examples/JohnsonCook.f90:12-172) on the GPU: the same 33 arguments, scalars
+ * (nblock … lanneal, stepTime, totalTime, dt, cmname) on the host, every array a device
+ * pointer in column-major [k + nblock*c] layout. Synchronous. Dimension errors that make
+ * the Fortran routine STOP return VPD_ERR_ARGUMENT instead. */
+int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstatev, int *nfieldv,
+                                  int *nprops, int *lanneal, double *stepTime, double *totalTime,
+                                  double *dt, char *cmname, double *coordMp, double *charLength,
+                                  double *props, double *density, double *strainInc,
+                                  double *relSpinInc, double *tempOld, double *stretchOld,
+                                  double *defgradOld, double *fieldOld, double *stressOld,
+                                  double *stateOld, double *enerInternOld, double *enerInelasOld,
+                                  double *tempNew, double *stretchNew, double *defgradNew,
+                                  double *fieldNew, double *stressNew, double *stateNew,
+                                  double *enerInternNew, double *enerInelasNew);
+
+/* ---- host helpers of the case runner (no GPU work) --------------------------------- */
+/* makeParticlePatch (src/host/Geometry.cpp:10-34): x[n][3], volume[n], n = nx·ny·nz. */
+int vpd_make_patch(int nx, int ny, int nz, double spacing, double *x, double *volume);
+/* RegionalConstraintSystem (src/host/BoundaryConditions.cpp:54-68, 81-115) reduced to per-
+ * particle codes. component 0: support left / load right along x; 1: bottom / top along y. */
+int vpd_regional_constraint_codes(const double *x, int64_t n, int component, int region_layers,
+                                  double spacing, int plane_strain, uint8_t *codes,
+                                  int64_t *anchor, int64_t *rotation_pin);
+/* RegionalConstraintSystem::ramp (src/host/BoundaryConditions.cpp:70-79). */
+void vpd_smooth_ramp(double amplitude, double rise_time, double time, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VPD_H */

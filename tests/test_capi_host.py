@@ -1,0 +1,73 @@
+"""CPU-side checks of the product library: libvpd.so loads, exports every entry point that
+include/vpd.h declares, and its host helpers (lattice, boundary bands, constraint codes,
+ramp) agree with the oracle. No GPU work is issued here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "vpd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vpd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_header(orc):
+    import vpd
+    lib = vpd.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.vpd_abi_version() == 1
+
+
+@pytest.mark.parametrize("dims", [(30, 8, 8, 1.0), (11, 5, 3, 0.2), (2, 2, 2, 1.0)])
+def test_patch_matches_oracle(orc, dims):
+    import vpd
+    x, v = vpd.make_patch(*dims)
+    xo, vo = orc.make_patch(*dims)
+    assert np.array_equal(x, xo) and np.array_equal(v, vo)
+
+
+@pytest.mark.parametrize("component", [0, 1])
+@pytest.mark.parametrize("plane", [False, True])
+@pytest.mark.parametrize("layers", [1, 3])
+def test_constraint_codes_match_oracle(orc, component, plane, layers):
+    import vpd
+    x, _ = vpd.make_patch(13, 9, 4, 0.5)
+    codes, anchor, pin = vpd.regional_constraint_codes(x, component, layers, 0.5, plane)
+    c3, a, p = orc.constraint_codes(x, component, layers, 0.5, plane)
+    assert np.array_equal(vpd.unpack_codes(codes), c3)
+    assert (anchor, pin) == (a, p)
+
+
+def test_ramp_matches_oracle(orc):
+    import vpd
+    for t in (0.0, 1e-5, 0.03, 0.0599, 0.06, 0.2):
+        assert np.array_equal(vpd.smooth_ramp(2.0, 0.06, t), orc.ramp(2.0, 0.06, t))
+
+
+def test_create_without_device_fails_loudly():
+    import vpd
+    if vpd.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(vpd.VpdError, match="No HIP device"):
+        vpd.Context(1.0, 1.0, [1.0] * 14)
+
+
+def test_config_rejections_precede_device():
+    import vpd
+    with pytest.raises(vpd.VpdError, match="horizon > 0"):
+        vpd.Context(0.0, 1.0, [1.0] * 14)
+    with pytest.raises(vpd.VpdError, match="density > 0"):
+        vpd.Context(1.0, 0.0, [1.0] * 14)
+    with pytest.raises(vpd.VpdError, match="14 props and 6 states"):
+        vpd.Context(1.0, 1.0, [1.0] * 13)
+    with pytest.raises(vpd.VpdError, match="initiation > 0"):
+        vpd.Context(1.0, 1.0, [1.0] * 14, damage=(0.0, 0.0))

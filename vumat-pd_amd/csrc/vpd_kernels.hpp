@@ -1,0 +1,447 @@
+// gfx950 kernels of the explicit correspondence-PD increment. One lane owns one particle
+// (or one material point); particle fields are SoA planes of FP64 ([component][N]), so a
+// wavefront of 64 consecutive particles reads each plane with unit stride, and the
+// neighbour gathers of a lattice-ordered block (row p+1 ≈ row p shifted by one index)
+// stay close to unit stride as well. Bond sums run over each CSR row in order, one lane
+// per row, which keeps the reference's summation order and needs no atomics.
+#pragma once
+
+#include "vpd_math.hpp"
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vpd {
+
+// Planes of an SoA field with stride N.
+struct Planes {
+  double *p;
+  int64_t n;
+  __device__ __forceinline__ double &operator()(int c, int64_t i) const { return p[c * n + i]; }
+};
+
+__device__ __forceinline__ void raise_error(int *err, int bit) { atomicOr(err, bit); }
+
+// Workgroups are handed out to the 8 XCDs round-robin by workgroup id. Remapping the id so
+// that each XCD walks one contiguous eighth of the particle range keeps every XCD's
+// neighbour working set (a few lattice planes) inside its own L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
+  const int64_t b = blockIdx.x;
+  const int64_t per = (nblocks + 7) / 8;
+  const int64_t xcd = b % 8, slot = b / 8;
+  const int64_t mapped = xcd * per + slot;
+  return mapped < nblocks ? mapped : -1;
+}
+
+// Per-component constraint codes: 2 bits per axis, 0 free, 1 held at zero, 2 follows the
+// ramp (BoundaryConditions.cpp:81-115 collapsed to what its sequence of writes leaves).
+struct Ramp {
+  double u, v, a;
+};
+
+__device__ __forceinline__ void constrain(uint8_t code, const Ramp &r, double u[3], double v[3],
+                                          double a[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int c = (code >> (2 * d)) & 3;
+    if (c == 1) {
+      u[d] = 0.0;
+      v[d] = 0.0;
+      a[d] = 0.0;
+    } else if (c == 2) {
+      u[d] = r.u;
+      v[d] = r.v;
+      a[d] = r.a;
+    }
+  }
+}
+
+// ExplicitTimeIntegrator.cpp:23-34 then the constraints at t (BoundaryConditions.cpp:81).
+__global__ void __launch_bounds__(256) k_predict(int64_t n, Planes u, Planes v, Planes a,
+                                                 const uint8_t *__restrict__ codes, double dt,
+                                                 Ramp r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  double uu[3], vv[3], aa[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    vv[d] = v(d, i);
+    aa[d] = a(d, i);
+    uu[d] = u(d, i) + (dt * vv[d] + 0.5 * dt * dt * aa[d]);
+  }
+  constrain(codes[i], r, uu, vv, aa);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    u(d, i) = uu[d];
+    v(d, i) = vv[d];
+    a(d, i) = aa[d];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_constrain(int64_t n, Planes u, Planes v, Planes a,
+                                                   const uint8_t *__restrict__ codes, Ramp r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const uint8_t c = codes[i];
+  if (c == 0)
+    return;
+  double uu[3], vv[3], aa[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    uu[d] = u(d, i);
+    vv[d] = v(d, i);
+    aa[d] = a(d, i);
+  }
+  constrain(c, r, uu, vv, aa);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    u(d, i) = uu[d];
+    v(d, i) = vv[d];
+    a(d, i) = aa[d];
+  }
+}
+
+// PdCorrespondenceSkeleton.cpp:149-170: K_i = Σ_j V_j ξ⊗ξ, stored as K_i⁻¹.
+__global__ void __launch_bounds__(256) k_shape(int64_t n, Planes x, const double *__restrict__ vol,
+                                               const int *__restrict__ rowptr,
+                                               const int *__restrict__ col, Planes kinv,
+                                               int *err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
+  T3 k;
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    k.m[c] = 0.0;
+  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+    const int j = col[p];
+    if (j < 0 || j >= n) {
+      raise_error(err, ERR_NEIGHBOR_RANGE);
+      continue;
+    }
+    const double xi[3] = {x(0, j) - x0, x(1, j) - x1, x(2, j) - x2};
+    const double vj = vol[j];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        k.m[3 * a + b] += vj * xi[a] * xi[b];
+  }
+  T3 ki;
+  if (!t3_inverse_skeleton(k, ki)) {
+    raise_error(err, ERR_SINGULAR_SHAPE);
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      ki.m[c] = 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    kinv(c, i) = ki.m[c];
+}
+
+// Bond pass 1. PdCorrespondenceSkeleton.cpp:172-197: F_i = (Σ_j V_j (y_j − y_i)⊗ξ) K_i⁻¹,
+// and, with DAMAGE, PdSimpleDamage.cpp:63-84 on the same bond and the same predicted
+// positions: a bond breaks (irreversibly) when its stretch exceeds `initiation` or, with a
+// compression threshold armed, falls below −failure. Damage does not enter F (:181-196).
+template <bool DAMAGE>
+__global__ void __launch_bounds__(256)
+    k_defgrad(int64_t n, int64_t nblocks, Planes x, Planes u, const double *__restrict__ vol,
+              const int *__restrict__ rowptr, const int *__restrict__ col, Planes kinv,
+              Planes fout, uint32_t *broken, double initiation, double failure,
+              int useCompression) {
+  const int64_t blk = xcd_block(nblocks);
+  if (blk < 0)
+    return;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
+  const double yi0 = x0 + u(0, i), yi1 = x1 + u(1, i), yi2 = x2 + u(2, i);
+  double num[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    num[c] = 0.0;
+  const int pb = rowptr[i], pe = rowptr[i + 1];
+  for (int p = pb; p < pe; ++p) {
+    const int j = col[p];
+    const double xj0 = x(0, j), xj1 = x(1, j), xj2 = x(2, j);
+    const double xi[3] = {xj0 - x0, xj1 - x1, xj2 - x2};
+    const double yij[3] = {(xj0 + u(0, j)) - yi0, (xj1 + u(1, j)) - yi1, (xj2 + u(2, j)) - yi2};
+    const double vj = vol[j];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+        num[3 * a + b] += vj * yij[a] * xi[b];
+    if (DAMAGE) {
+      const uint32_t bit = 1u << (p & 31);
+      if (!(broken[p >> 5] & bit)) {
+        const double initial = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+        const double stretch =
+            (sqrt(yij[0] * yij[0] + yij[1] * yij[1] + yij[2] * yij[2]) - initial) / initial;
+        if (stretch > initiation || (useCompression && stretch < -failure))
+          atomicOr(&broken[p >> 5], bit);
+      }
+    }
+  }
+  T3 nm, ki;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    nm.m[c] = num[c];
+    ki.m[c] = kinv(c, i);
+  }
+  const T3 f = t3_mul(nm, ki);
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    fout(c, i) = f.m[c];
+}
+
+// Material point update. VumatPdCorrespondenceMaterial.cpp:110-173 for every point at
+// once (nblock = N; the 256-row padding of the CPU host leaves results unchanged because
+// the routine is point-wise): Flanagan-Taylor kinematics, strainInc = D̄·dt and stressOld
+// packed as Voigt (VumatMaterialAdapter.cpp:290-330), the Johnson-Cook return map,
+// σ = R σ̄ Rᵀ, history commit; then A_i = (J σ F⁻ᵀ) K_i⁻¹ for the force pass
+// (PdCorrespondenceSkeleton.cpp:209-210). sigOut/strainOut (spatial σ, Green-Lagrange E)
+// are written only when non-null (diagnostics and snapshots).
+struct MaterialArgs {
+  int64_t n;
+  int64_t nblocks;
+  Planes fnew, fN, vN, rN, sig, statev, kinv, amap;
+  double *temp, *eint, *einel;
+  Planes sigOut, strainOut;
+  int *iterOut;
+  JcParams jc;
+  double dt;
+  int *err;
+};
+
+__global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
+  const int64_t blk = xcd_block(g.nblocks);
+  if (blk < 0)
+    return;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= g.n)
+    return;
+  T3 f1, f0, v0, r0;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    f1.m[c] = g.fnew(c, i);
+    f0.m[c] = g.fN(c, i);
+    v0.m[c] = g.vN(c, i);
+    r0.m[c] = g.rN(c, i);
+  }
+  Corot k;
+  if (!corotational(f0, f1, v0, r0, g.dt, k)) {
+    raise_error(g.err, ERR_SINGULAR_COROT);
+    return;
+  }
+  double inc[6], sOld[6], sNew[6], st[6];
+  to_voigt(k.Dbar, inc);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    inc[c] *= g.dt;
+    sOld[c] = g.sig(c, i);
+    st[c] = g.statev(c, i);
+  }
+  double eint = g.eint[i], einel = g.einel[i];
+  const int it = jc_point(g.jc, g.dt, inc, sOld, st, eint, einel, sNew);
+  const T3 sFull = from_voigt(sNew);
+  const T3 sigma = t3_mul(k.R1, t3_mul(sFull, t3_transpose(k.R1)));
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    g.fN(c, i) = f1.m[c];
+    g.vN(c, i) = k.V1.m[c];
+    g.rN(c, i) = k.R1.m[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    g.sig(c, i) = sNew[c];
+    g.statev(c, i) = st[c];
+  }
+  g.temp[i] = st[2];
+  g.eint[i] = eint;
+  g.einel[i] = einel;
+  if (g.iterOut)
+    g.iterOut[i] = it;
+  if (g.sigOut.p) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      g.sigOut(c, i) = sigma.m[c];
+  }
+  if (g.strainOut.p) { // CorotationalKinematics.cpp:141-149
+    T3 e = t3_mul(t3_transpose(f1), f1);
+    e.m[0] -= 1.0;
+    e.m[4] -= 1.0;
+    e.m[8] -= 1.0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      g.strainOut(c, i) = e.m[c] * 0.5;
+  }
+  T3 P;
+  if (!first_piola(sigma, f1, P)) {
+    raise_error(g.err, ERR_PIOLA_DET);
+    return;
+  }
+  T3 ki;
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    ki.m[c] = g.kinv(c, i);
+  const T3 A = t3_mul(P, ki);
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    g.amap(c, i) = A.m[c];
+}
+
+// Bond pass 2. PdCorrespondenceSkeleton.cpp:213-227: f_i = Σ_k (1 − d_ik)(A_i ξ − A_j(−ξ)) V_j
+// in CSR order (a broken bond contributes exactly zero, so it is skipped), then the
+// velocity-Verlet corrector (ExplicitTimeIntegrator.cpp:36-52) and the constraints at t;
+// with FUSE_NEXT also the next step's predictor and constraints at t + dt
+// (ExplicitTimeIntegrator.cpp:23-34), which only touch the lane's own particle.
+struct ForceArgs {
+  int64_t n;
+  int64_t nblocks;
+  Planes x, u, v, a, amap, forceOut;
+  const double *vol;
+  const int *rowptr, *col;
+  const uint32_t *broken;
+  const uint8_t *codes;
+  double dt, density;
+  Ramp rampNow, rampNext;
+};
+
+template <bool DAMAGE, bool FUSE_NEXT>
+__global__ void __launch_bounds__(256) k_force(ForceArgs g) {
+  const int64_t blk = xcd_block(g.nblocks);
+  if (blk < 0)
+    return;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= g.n)
+    return;
+  const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
+  T3 ai;
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    ai.m[c] = g.amap(c, i);
+  double acc[3] = {0.0, 0.0, 0.0};
+  const int pb = g.rowptr[i], pe = g.rowptr[i + 1];
+  for (int p = pb; p < pe; ++p) {
+    if (DAMAGE && (g.broken[p >> 5] & (1u << (p & 31))))
+      continue;
+    const int j = g.col[p];
+    V3 xi;
+    xi.v[0] = g.x(0, j) - x0;
+    xi.v[1] = g.x(1, j) - x1;
+    xi.v[2] = g.x(2, j) - x2;
+    const V3 ti = t3_mulv(ai, xi);
+    V3 nxi;
+    nxi.v[0] = -xi.v[0];
+    nxi.v[1] = -xi.v[1];
+    nxi.v[2] = -xi.v[2];
+    T3 aj;
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      aj.m[c] = g.amap(c, j);
+    const V3 tj = t3_mulv(aj, nxi);
+    const double vj = g.vol[j];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      acc[d] += (ti.v[d] - tj.v[d]) * vj;
+  }
+  if (g.forceOut.p) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      g.forceOut(d, i) = acc[d];
+  }
+  double uu[3], vv[3], aa[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double next = acc[d] / g.density;
+    vv[d] = g.v(d, i) + 0.5 * g.dt * (g.a(d, i) + next);
+    aa[d] = next;
+    uu[d] = g.u(d, i);
+  }
+  const uint8_t code = g.codes[i];
+  constrain(code, g.rampNow, uu, vv, aa);
+  if (FUSE_NEXT) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      uu[d] = uu[d] + (g.dt * vv[d] + 0.5 * g.dt * g.dt * aa[d]);
+    constrain(code, g.rampNext, uu, vv, aa);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    g.u(d, i) = uu[d];
+    g.v(d, i) = vv[d];
+    g.a(d, i) = aa[d];
+  }
+}
+
+// PdSimpleDamage.cpp:23-36: D_i = Σ d V_j / Σ V_j in CSR order.
+__global__ void __launch_bounds__(256) k_damage_index(int64_t n, const double *__restrict__ vol,
+                                                      const int *__restrict__ rowptr,
+                                                      const int *__restrict__ col,
+                                                      const uint32_t *__restrict__ broken,
+                                                      double *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  double damaged = 0.0, total = 0.0;
+  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+    const double vj = vol[col[p]];
+    const double d = (broken[p >> 5] >> (p & 31)) & 1u ? 1.0 : 0.0;
+    damaged += d * vj;
+    total += vj;
+  }
+  out[i] = total > 0.0 ? damaged / total : 0.0;
+}
+
+// The Johnson-Cook VUMAT on a caller's column-major device block ([k + nblock*c]),
+// JohnsonCook.f90:74-156, one lane per point. Unread VUMAT outputs are copied through as
+// the routine does (stretch, defgrad, field).
+struct VumatArgs {
+  int64_t nblock;
+  int nstatev, nfieldv;
+  JcParams jc;
+  double dt;
+  const double *strainInc, *tempOld, *stretchOld, *defgradOld, *fieldOld, *stressOld, *stateOld,
+      *enerInternOld, *enerInelasOld;
+  double *tempNew, *stretchNew, *defgradNew, *fieldNew, *stressNew, *stateNew, *enerInternNew,
+      *enerInelasNew;
+};
+
+__global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.nblock)
+    return;
+  const int64_t nb = g.nblock;
+  double inc[6], sOld[6], sNew[6], st[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    inc[c] = g.strainInc[k + nb * c];
+    sOld[c] = g.stressOld[k + nb * c];
+    st[c] = g.stateOld[k + nb * c];
+    g.stretchNew[k + nb * c] = g.stretchOld[k + nb * c];
+  }
+  for (int c = 6; c < g.nstatev; ++c)
+    g.stateNew[k + nb * c] = g.stateOld[k + nb * c];
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    g.defgradNew[k + nb * c] = g.defgradOld[k + nb * c];
+  for (int c = 0; c < g.nfieldv; ++c)
+    g.fieldNew[k + nb * c] = g.fieldOld[k + nb * c];
+  double eint = g.enerInternOld[k], einel = g.enerInelasOld[k];
+  jc_point(g.jc, g.dt, inc, sOld, st, eint, einel, sNew);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    g.stressNew[k + nb * c] = sNew[c];
+    g.stateNew[k + nb * c] = st[c];
+  }
+  g.tempNew[k] = st[2];
+  g.enerInternNew[k] = eint;
+  g.enerInelasNew[k] = einel;
+}
+
+} // namespace vpd

@@ -1,0 +1,356 @@
+// Device-side FP64 tensor algebra and the point-wise constitutive pieces of the PD
+// increment: Flanagan-Taylor corotational update, Voigt packing, Johnson-Cook radial
+// return, first Piola stress. Every routine keeps the operand order of the reference
+// expression it stands for (cited per function), and the library is compiled with
+// -ffp-contract=off, so +, -, *, / and sqrt round exactly as the reference's CPU build
+// does; sin, pow and log come from the device math library (within an ulp of libm).
+//
+// Tensors are 3x3 row-major in registers: t[3*r + c].
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace vpd {
+
+struct T3 {
+  double m[9];
+};
+struct V3 {
+  double v[3];
+};
+
+__device__ __forceinline__ T3 t3_ident() {
+  T3 t;
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    t.m[c] = (c % 4 == 0) ? 1.0 : 0.0;
+  return t;
+}
+
+// PdCorrespondenceSkeleton.cpp:47-54 / CorotationalKinematics.cpp:27-34: o(i,j) starts
+// at zero and accumulates a(i,k) b(k,j) for k = 0, 1, 2.
+__device__ __forceinline__ T3 t3_mul(const T3 &a, const T3 &b) {
+  T3 o;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        s += a.m[3 * i + k] * b.m[3 * k + j];
+      o.m[3 * i + j] = s;
+    }
+  return o;
+}
+
+// PdCorrespondenceSkeleton.cpp:55-59
+__device__ __forceinline__ V3 t3_mulv(const T3 &a, const V3 &x) {
+  V3 o;
+  o.v[0] = a.m[0] * x.v[0] + a.m[1] * x.v[1] + a.m[2] * x.v[2];
+  o.v[1] = a.m[3] * x.v[0] + a.m[4] * x.v[1] + a.m[5] * x.v[2];
+  o.v[2] = a.m[6] * x.v[0] + a.m[7] * x.v[1] + a.m[8] * x.v[2];
+  return o;
+}
+
+__device__ __forceinline__ T3 t3_transpose(const T3 &a) {
+  T3 o;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      o.m[3 * i + j] = a.m[3 * j + i];
+  return o;
+}
+
+// PdCorrespondenceSkeleton.cpp:63-66
+__device__ __forceinline__ double t3_det(const T3 &t) {
+  const double *a = t.m;
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+         a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+// PdCorrespondenceSkeleton.cpp:67-76: cofactors (off-diagonal ones negated) over det.
+__device__ __forceinline__ bool t3_inverse_skeleton(const T3 &t, T3 &o) {
+  const double *a = t.m;
+  const double d = t3_det(t);
+  if (fabs(d) < 1.0e-14)
+    return false;
+  o.m[0] = (a[4] * a[8] - a[5] * a[7]) / d;
+  o.m[1] = -(a[1] * a[8] - a[2] * a[7]) / d;
+  o.m[2] = (a[1] * a[5] - a[2] * a[4]) / d;
+  o.m[3] = -(a[3] * a[8] - a[5] * a[6]) / d;
+  o.m[4] = (a[0] * a[8] - a[2] * a[6]) / d;
+  o.m[5] = -(a[0] * a[5] - a[2] * a[3]) / d;
+  o.m[6] = (a[3] * a[7] - a[4] * a[6]) / d;
+  o.m[7] = -(a[0] * a[7] - a[1] * a[6]) / d;
+  o.m[8] = (a[0] * a[4] - a[1] * a[3]) / d;
+  return true;
+}
+
+// CorotationalKinematics.cpp:41-50: the same cofactors written with swapped operands.
+__device__ __forceinline__ bool t3_inverse_corot(const T3 &t, T3 &o) {
+  const double *a = t.m;
+  const double d = t3_det(t);
+  if (fabs(d) < 1.0e-14)
+    return false;
+  o.m[0] = (a[4] * a[8] - a[5] * a[7]) / d;
+  o.m[1] = (a[2] * a[7] - a[1] * a[8]) / d;
+  o.m[2] = (a[1] * a[5] - a[2] * a[4]) / d;
+  o.m[3] = (a[5] * a[6] - a[3] * a[8]) / d;
+  o.m[4] = (a[0] * a[8] - a[2] * a[6]) / d;
+  o.m[5] = (a[2] * a[3] - a[0] * a[5]) / d;
+  o.m[6] = (a[3] * a[7] - a[4] * a[6]) / d;
+  o.m[7] = (a[1] * a[6] - a[0] * a[7]) / d;
+  o.m[8] = (a[0] * a[4] - a[1] * a[3]) / d;
+  return true;
+}
+
+// Error bits a kernel raises through the context's device flag.
+enum : int {
+  ERR_SINGULAR_SHAPE = 1,
+  ERR_SINGULAR_COROT = 2,
+  ERR_PIOLA_DET = 4,
+  ERR_NEIGHBOR_RANGE = 8,
+  ERR_NONFINITE_COORD = 16,
+  ERR_NEIGHBOR_OVERFLOW = 32,
+  ERR_VUMAT_DIMS = 64,
+};
+
+struct Corot {
+  T3 R1, V1, Dbar;
+};
+
+// CorotationalKinematics.cpp:61-135 (Flanagan-Taylor). The relative-spin increment
+// (:125-134) feeds only VUMAT arguments the Johnson-Cook routine never reads, so it is
+// not formed here.
+__device__ __forceinline__ bool corotational(const T3 &fN, const T3 &fNP1, const T3 &vNt,
+                                             const T3 &rN, double dt, Corot &out) {
+  T3 fDot;
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    fDot.m[i] = (fNP1.m[i] - fN.m[i]) / dt;
+  T3 fNi;
+  if (!t3_inverse_corot(fN, fNi))
+    return false;
+  const T3 L = t3_mul(fDot, fNi);
+  T3 d, w;
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      d.m[3 * r + c] = 0.5 * (L.m[3 * r + c] + L.m[3 * c + r]);
+      w.m[3 * r + c] = 0.5 * (L.m[3 * r + c] - L.m[3 * c + r]);
+    }
+  const double *vN = vNt.m;
+  const double zX =
+      -vN[2] * d.m[3] - vN[5] * d.m[4] - vN[8] * d.m[5] + vN[1] * d.m[6] + vN[4] * d.m[7] +
+      vN[7] * d.m[8];
+  const double zY = vN[2] * d.m[0] + vN[5] * d.m[1] + vN[8] * d.m[2] - vN[0] * d.m[6] -
+                    vN[3] * d.m[7] - vN[6] * d.m[8];
+  const double zZ =
+      -vN[1] * d.m[0] - vN[4] * d.m[1] - vN[7] * d.m[2] + vN[0] * d.m[3] + vN[3] * d.m[4] +
+      vN[6] * d.m[5];
+  const double wX = 0.5 * (w.m[7] - w.m[5]);
+  const double wY = 0.5 * (w.m[2] - w.m[6]);
+  const double wZ = 0.5 * (w.m[3] - w.m[1]);
+  const double trV = vN[0] + vN[4] + vN[8];
+  T3 m = t3_ident();
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    m.m[i] = m.m[i] * trV - vN[i];
+  T3 c;
+  if (!t3_inverse_corot(m, c))
+    return false;
+  const double oX = wX + c.m[0] * zX + c.m[1] * zY + c.m[2] * zZ;
+  const double oY = wY + c.m[3] * zX + c.m[4] * zY + c.m[5] * zZ;
+  const double oZ = wZ + c.m[6] * zX + c.m[7] * zY + c.m[8] * zZ;
+  T3 om;
+  om.m[0] = 0.0;
+  om.m[1] = -oZ;
+  om.m[2] = oY;
+  om.m[3] = oZ;
+  om.m[4] = 0.0;
+  om.m[5] = -oX;
+  om.m[6] = -oY;
+  om.m[7] = oX;
+  om.m[8] = 0.0;
+  const double oSq = oX * oX + oY * oY + oZ * oZ;
+  T3 q = t3_ident();
+  if (oSq > 1.0e-30) {
+    const double mag = sqrt(oSq);
+    const double s1 = sin(dt * mag) / mag;
+    const double hs = sin(0.5 * dt * mag);
+    const double s2 = 2.0 * hs * hs / oSq;
+    const T3 o2 = t3_mul(om, om);
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      q.m[i] += s1 * om.m[i] + s2 * o2.m[i];
+  }
+  out.R1 = t3_mul(q, rN);
+  const T3 ra = t3_mul(L, vNt);
+  const T3 rb = t3_mul(vNt, om);
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    out.V1.m[i] = vN[i] + dt * (ra.m[i] - rb.m[i]);
+  out.Dbar = t3_mul(t3_transpose(out.R1), t3_mul(d, out.R1));
+  return true;
+}
+
+// TensorVoigtMap.cpp:9-16: [11, 22, 33, 23, 13, 12], shears symmetrised.
+__device__ __forceinline__ void to_voigt(const T3 &f, double v[6]) {
+  v[0] = f.m[0];
+  v[1] = f.m[4];
+  v[2] = f.m[8];
+  v[3] = 0.5 * (f.m[5] + f.m[7]);
+  v[4] = 0.5 * (f.m[2] + f.m[6]);
+  v[5] = 0.5 * (f.m[1] + f.m[3]);
+}
+// TensorVoigtMap.cpp:18-20
+__device__ __forceinline__ T3 from_voigt(const double v[6]) {
+  T3 t;
+  t.m[0] = v[0];
+  t.m[1] = v[5];
+  t.m[2] = v[4];
+  t.m[3] = v[5];
+  t.m[4] = v[1];
+  t.m[5] = v[3];
+  t.m[6] = v[4];
+  t.m[7] = v[3];
+  t.m[8] = v[2];
+  return t;
+}
+
+// PdCorrespondenceSkeleton.cpp:83-91: P = J σ F⁻ᵀ.
+__device__ __forceinline__ bool first_piola(const T3 &s, const T3 &f, T3 &p) {
+  const double j = t3_det(f);
+  if (fabs(j) < 1.0e-14)
+    return false;
+  T3 fi;
+  if (!t3_inverse_skeleton(f, fi))
+    return false;
+  p = t3_mul(s, t3_transpose(fi));
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    p.m[i] *= j;
+  return true;
+}
+
+// Johnson-Cook parameters (props 1..14 of materials/examples/JohnsonCook.f90:9) and the
+// derived constants of :70-72.
+struct JcParams {
+  double young, poisson, a0, b0, rateC, hardN, thermalM, rate0, tref, tmelt, rho, beta, cp,
+      workToHeat;
+  double mu, lambda, tolerance;
+};
+
+// JohnsonCook.f90:160-170
+__device__ __forceinline__ double jc_yield(const JcParams &p, double ep, double rate,
+                                           double temp) {
+  double theta = (temp - p.tref) / (p.tmelt - p.tref);
+  theta = fmin(1.0, fmax(0.0, theta));
+  double er = rate;
+  if (er == 0.0)
+    er = p.rate0;
+  return (p.a0 + p.b0 * pow(ep, p.hardN)) * (1.0 - pow(theta, p.thermalM)) *
+         (1.0 + p.rateC * log(er / p.rate0));
+}
+
+// One material point of JohnsonCook.f90:74-156 in the exact bracketed-bisection form.
+// in: stressOld[6], strainInc[6], state[6] (in/out), energies (in/out). Writes
+// stressNew[6]; returns the iteration count (0 for the init branch).
+__device__ __forceinline__ int jc_point(const JcParams &p, double dt, const double inc[6],
+                                        const double stressOld[6], double state[6],
+                                        double &eint, double &einel, double stressNew[6]) {
+  double trial[6];
+  const double traceInc = inc[0] + inc[1] + inc[2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    trial[i] = stressOld[i] + p.lambda * traceInc + 2.0 * p.mu * inc[i];
+#pragma unroll
+  for (int i = 3; i < 6; ++i)
+    trial[i] = stressOld[i] + 2.0 * p.mu * inc[i];
+
+  if (state[0] == 0.0) { // JohnsonCook.f90:92-102
+    state[0] = 1.0;
+    state[1] = 0.0;
+    state[2] = p.tref;
+    state[3] = 0.0;
+    state[4] = 0.0;
+    state[5] = 1.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      stressNew[i] = trial[i];
+    return 0;
+  }
+  const double epOld = state[1], temperature = state[2], oldYield = state[3];
+  const double pressure = (trial[0] + trial[1] + trial[2]) / 3.0;
+  double dev[6];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    dev[i] = trial[i] - pressure;
+#pragma unroll
+  for (int i = 3; i < 6; ++i)
+    dev[i] = trial[i];
+  const double qTrial =
+      sqrt(1.5 * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
+                  2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5])));
+
+  double epInc = 0.0, epLower = 0.0, epUpper = qTrial / (2.0 * p.mu);
+  double yield = oldYield;
+  int iteration = 0;
+  while (iteration < 50) { // JohnsonCook.f90:119-136
+    iteration = iteration + 1;
+    const double epRate = epInc / dt;
+    yield = jc_yield(p, epOld + epInc, epRate, temperature);
+    yield = fmax(yield, oldYield);
+    const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
+    const double residual = qNew - yield;
+    if (fabs(residual) < p.tolerance)
+      break;
+    if (epInc == 0.0 && residual < 0.0)
+      break;
+    if (residual >= 0.0)
+      epLower = fmax(epLower, epInc);
+    else
+      epUpper = fmin(epUpper, epInc);
+    epInc = 0.5 * (epLower + epUpper);
+    if (iteration == 1)
+      epInc = fmin(fmax(state[4], epLower), epUpper);
+  }
+
+  double upd[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+    upd[i] = trial[i];
+  if (qTrial > 0.0) { // JohnsonCook.f90:138-142
+    const double scale = fmax(0.0, 1.0 - 3.0 * p.mu * epInc / qTrial);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      upd[i] = pressure + scale * dev[i];
+#pragma unroll
+    for (int i = 3; i < 6; ++i)
+      upd[i] = scale * dev[i];
+  }
+  const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
+  double work = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+    work = work + (stressOld[i] + upd[i]) * inc[i];
+  const double workInc = 0.5 * work;
+  const double plasticWork = 0.5 * epInc * qNew;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+    stressNew[i] = upd[i];
+  eint = eint + workInc / p.rho;
+  einel = einel + plasticWork / p.rho;
+  state[0] = 1.0;
+  state[1] = epOld + epInc;
+  state[2] = temperature + p.workToHeat * p.beta * plasticWork / (p.rho * p.cp);
+  state[3] = yield;
+  state[4] = epInc;
+  state[5] = state[5] + static_cast<double>(iteration);
+  return iteration;
+}
+
+} // namespace vpd

@@ -1,0 +1,384 @@
+"""Python front end of libvpd.so (include/vpd.h) — the MI355X explicit correspondence-PD
+increment with the Johnson-Cook VUMAT.
+
+The library is loaded from this directory and is required: there is no CPU fallback, and
+every entry point raises VpdError when the HIP library is missing or a call fails.
+Arrays use the reference's host layouts: points (n, 3), tensors (n, 9) row-major,
+neighbours CSR int32 with ascending rows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvpd.so")
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+FIELD = dict(x=0, volume=1, u=2, v=3, a=4, fN=5, vN=6, rN=7, stressN=8, statev=9, temp=10,
+             eint=11, einel=12, sigma=13, strain=14, force=15, damage=16, damage_index=17,
+             kinv=18, F=19, iterations=20)
+DAMAGE_NONE, DAMAGE_CRITICAL_STRETCH = 0, 1
+RETURN_MAP_BISECTION = 0
+
+
+class VpdError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [("horizon", C.c_double), ("density", C.c_double), ("nprops", C.c_int),
+                ("props", C.POINTER(C.c_double)), ("nstatev", C.c_int),
+                ("initial_state", C.POINTER(C.c_double)), ("reference_temperature", C.c_double),
+                ("char_length", C.c_double), ("material_density", C.c_double),
+                ("damage_mode", C.c_int), ("damage_initiation", C.c_double),
+                ("damage_failure", C.c_double), ("return_map", C.c_int)]
+
+
+class StepArgs(C.Structure):
+    _fields_ = [("dt", C.c_double), ("time", C.c_double), ("step_time", C.c_double),
+                ("ramp", C.c_double * 3), ("fuse_next", C.c_int), ("ramp_next", C.c_double * 3),
+                ("want_fields", C.c_int)]
+
+
+class KernelTime(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("total_ms", C.c_double), ("launches", C.c_int64)]
+
+
+_lib = None
+
+
+def build() -> None:
+    """Compile libvpd.so for gfx950 (hipcc cross-compiles without a GPU)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VpdError("libvpd.so is not built: run `make -C vumat-pd_amd` "
+                           "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.vpd_last_error.restype = C.c_char_p
+        L.vpd_last_error.argtypes = [P]
+        L.vpd_create.argtypes = [C.POINTER(Config), C.POINTER(P)]
+        L.vpd_destroy.argtypes = [P]
+        L.vpd_destroy.restype = None
+        L.vpd_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.vpd_set_particles.argtypes = [P, _D, _D, C.c_int64]
+        L.vpd_build_neighbors.argtypes = [P, C.POINTER(C.c_int64)]
+        L.vpd_set_neighbors.argtypes = [P, _I32, _I32, C.c_int64]
+        L.vpd_get_neighbors.argtypes = [P, _I32, _I32]
+        L.vpd_set_constraints.argtypes = [P, C.c_void_p]
+        for f in ("vpd_initialize_shape", "vpd_initialize_material", "vpd_initialize_dynamics",
+                  "vpd_compute_deformation_gradient", "vpd_synchronize"):
+            getattr(L, f).argtypes = [P]
+        L.vpd_upload.argtypes = [P, C.c_int, C.c_void_p]
+        L.vpd_download.argtypes = [P, C.c_int, C.c_void_p]
+        L.vpd_predict.argtypes = [P, C.c_double, _D]
+        L.vpd_apply_constraints.argtypes = [P, _D]
+        L.vpd_update_material.argtypes = [P, C.c_double, C.c_double, C.c_double]
+        L.vpd_compute_force_and_correct.argtypes = [P, C.c_double, _D]
+        L.vpd_step.argtypes = [P, C.POINTER(StepArgs)]
+        L.vpd_set_timing.argtypes = [P, C.c_int]
+        L.vpd_collect_timing.argtypes = [P, C.POINTER(KernelTime), C.c_int,
+                                         C.POINTER(C.c_int)]
+        L.vpd_make_patch.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _D, _D]
+        L.vpd_regional_constraint_codes.argtypes = [_D, C.c_int64, C.c_int, C.c_int, C.c_double,
+                                                    C.c_int, _U8, C.POINTER(C.c_int64),
+                                                    C.POINTER(C.c_int64)]
+        L.vpd_smooth_ramp.argtypes = [C.c_double, C.c_double, C.c_double, _D]
+        L.vpd_smooth_ramp.restype = None
+        L.vpd_vumat_adapter_call_device.argtypes = ([C.POINTER(C.c_int)] * 7 +
+                                                    [C.POINTER(C.c_double)] * 3 + [C.c_char_p] +
+                                                    [C.c_void_p] * 22)
+        _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    lib().vpd_device_count(C.byref(n))
+    return n.value
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+# ------------------------------------------------------------------ host helpers
+def make_patch(nx, ny, nz, spacing):
+    n = nx * ny * nz
+    x, vol = np.zeros((n, 3)), np.zeros(n)
+    if lib().vpd_make_patch(nx, ny, nz, spacing, x, vol):
+        raise VpdError("Invalid patch dimensions.")
+    return x, vol
+
+
+def regional_constraint_codes(x, component, region_layers, spacing, plane_strain=False):
+    n = x.shape[0]
+    codes = np.zeros(n, np.uint8)
+    anchor, pin = C.c_int64(), C.c_int64()
+    if lib().vpd_regional_constraint_codes(_f64(x), n, component, region_layers, spacing,
+                                           int(plane_strain), codes, C.byref(anchor),
+                                           C.byref(pin)):
+        raise VpdError("Boundary-region selection returned no particles.")
+    return codes, anchor.value, pin.value
+
+
+def unpack_codes(codes):
+    """Packed per-particle codes -> (n, 3) array of 0 free / 1 zero / 2 ramp."""
+    c = np.asarray(codes, np.uint8)
+    return np.stack([(c >> (2 * d)) & 3 for d in range(3)], 1)
+
+
+def pack_codes(codes3):
+    c = np.asarray(codes3, np.uint8)
+    return (c[:, 0] | (c[:, 1] << 2) | (c[:, 2] << 4)).astype(np.uint8)
+
+
+def smooth_ramp(amplitude, rise_time, t):
+    out = np.zeros(3)
+    lib().vpd_smooth_ramp(amplitude, rise_time, t, out)
+    return out
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One particle block on the GPU (PdCorrespondenceSkeleton + VumatPdCorrespondenceMaterial
+    + PdSimpleDamage + ExplicitTimeIntegrator state behind one HIP stream)."""
+
+    def __init__(self, horizon, density, props, nstatev=6, initial_state=None,
+                 reference_temperature=293.0, char_length=1.0, material_density=7850.0,
+                 damage=None, return_map=RETURN_MAP_BISECTION):
+        self._props = _f64(props)
+        self._init = None if initial_state is None else _f64(initial_state)
+        cfg = Config()
+        cfg.horizon = horizon
+        cfg.density = density
+        cfg.nprops = len(self._props)
+        cfg.props = self._props.ctypes.data_as(C.POINTER(C.c_double))
+        cfg.nstatev = nstatev
+        cfg.initial_state = (None if self._init is None else
+                             self._init.ctypes.data_as(C.POINTER(C.c_double)))
+        cfg.reference_temperature = reference_temperature
+        cfg.char_length = char_length
+        cfg.material_density = material_density
+        if damage is not None:
+            cfg.damage_mode = DAMAGE_CRITICAL_STRETCH
+            cfg.damage_initiation, cfg.damage_failure = damage
+        cfg.return_map = return_map
+        self.h = C.c_void_p()
+        self.n = 0
+        self.nnz = 0
+        self.nstatev = nstatev
+        if lib().vpd_create(C.byref(cfg), C.byref(self.h)):
+            raise VpdError(lib().vpd_last_error(None).decode())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().vpd_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise VpdError(lib().vpd_last_error(self.h).decode())
+
+    def set_particles(self, x, volume):
+        x = _f64(x)
+        self.n = x.shape[0]
+        self._check(lib().vpd_set_particles(self.h, x, _f64(volume), self.n))
+
+    def build_neighbors(self):
+        nnz = C.c_int64()
+        self._check(lib().vpd_build_neighbors(self.h, C.byref(nnz)))
+        self.nnz = nnz.value
+        return self.nnz
+
+    def set_neighbors(self, rowptr, col):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        if col.size == 0:
+            col = np.zeros(1, np.int32)
+        self.nnz = int(rowptr[-1])
+        self._check(lib().vpd_set_neighbors(self.h, rowptr, col, self.nnz))
+
+    def get_neighbors(self):
+        rowptr = np.zeros(self.n + 1, np.int32)
+        col = np.zeros(max(self.nnz, 1), np.int32)
+        self._check(lib().vpd_get_neighbors(self.h, rowptr, col))
+        return rowptr, col[:self.nnz]
+
+    def set_constraints(self, codes):
+        codes = np.ascontiguousarray(codes, np.uint8)
+        self._check(lib().vpd_set_constraints(self.h, codes.ctypes.data_as(C.c_void_p)))
+
+    def initialize_shape(self):
+        self._check(lib().vpd_initialize_shape(self.h))
+
+    def initialize_material(self):
+        self._check(lib().vpd_initialize_material(self.h))
+
+    def initialize_dynamics(self):
+        self._check(lib().vpd_initialize_dynamics(self.h))
+
+    def _shape(self, name):
+        if name == "damage":
+            return (self.nnz,), np.float64
+        if name == "iterations":
+            return (self.n,), np.int32
+        w = {"x": 3, "u": 3, "v": 3, "a": 3, "force": 3, "volume": 1, "temp": 1, "eint": 1,
+             "einel": 1, "damage_index": 1, "statev": self.nstatev}.get(name, 9)
+        return ((self.n,) if w == 1 else (self.n, w)), np.float64
+
+    def upload(self, name, arr):
+        shape, dt = self._shape(name)
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(shape)
+        self._check(lib().vpd_upload(self.h, FIELD[name], a.ctypes.data_as(C.c_void_p)))
+
+    def download(self, name):
+        shape, dt = self._shape(name)
+        out = np.zeros(shape, dt)
+        self._check(lib().vpd_download(self.h, FIELD[name], out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def predict(self, dt, ramp):
+        self._check(lib().vpd_predict(self.h, dt, _f64(ramp)))
+
+    def apply_constraints(self, ramp):
+        self._check(lib().vpd_apply_constraints(self.h, _f64(ramp)))
+
+    def compute_deformation_gradient(self):
+        self._check(lib().vpd_compute_deformation_gradient(self.h))
+
+    def update_material(self, dt, step_time=0.0, total_time=0.0):
+        self._check(lib().vpd_update_material(self.h, dt, step_time, total_time))
+
+    def force_and_correct(self, dt, ramp):
+        self._check(lib().vpd_compute_force_and_correct(self.h, dt, _f64(ramp)))
+
+    def step(self, dt, time, step_time, ramp, ramp_next=None, want_fields=False):
+        a = StepArgs()
+        a.dt, a.time, a.step_time = dt, time, step_time
+        a.ramp[:] = list(ramp)
+        if ramp_next is not None:
+            a.fuse_next = 1
+            a.ramp_next[:] = list(ramp_next)
+        a.want_fields = int(want_fields)
+        self._check(lib().vpd_step(self.h, C.byref(a)))
+
+    def synchronize(self):
+        self._check(lib().vpd_synchronize(self.h))
+
+    def set_timing(self, on=True):
+        self._check(lib().vpd_set_timing(self.h, int(on)))
+
+    def collect_timing(self):
+        buf = (KernelTime * 16)()
+        cnt = C.c_int()
+        self._check(lib().vpd_collect_timing(self.h, buf, 16, C.byref(cnt)))
+        return {buf[i].name.decode(): (buf[i].total_ms, buf[i].launches) for i in range(cnt.value)}
+
+
+def vumat_device(props, dt, strain_inc, stress_old, state_old, temp_old, eint_old, einel_old,
+                 nfieldv=1):
+    """vpd_vumat_adapter_call_device on torch device tensors, column-major blocks:
+    strain_inc/stress_old (6, nb), state_old (nstatev, nb), the rest (nb,). Returns a dict
+    of device tensors (temp, stretch, defgrad, field, stress, state, eint, einel)."""
+    import torch
+
+    dev = strain_inc.device
+    nb = strain_inc.shape[1]
+    nst = state_old.shape[0]
+    z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=dev)
+    p = torch.as_tensor(np.asarray(props, np.float64), device=dev)
+    ins = dict(coord=z(3, nb), charlen=z(nb), density=z(nb), spin=z(3, nb), stretch=z(6, nb),
+               defgrad=z(9, nb), field=z(max(1, nfieldv), nb))
+    out = dict(temp=z(nb), stretch=z(6, nb), defgrad=z(9, nb), field=z(max(1, nfieldv), nb),
+               stress=z(6, nb), state=z(nst, nb), eint=z(nb), einel=z(nb))
+    ints = [C.c_int(v) for v in (nb, 3, 3, nst, nfieldv, len(props), 0)]
+    dbl = [C.c_double(v) for v in (0.0, 0.0, dt)]
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    srcs = [strain_inc, stress_old, state_old, temp_old, eint_old, einel_old]
+    srcs = [s.contiguous() for s in srcs]
+    rc = lib().vpd_vumat_adapter_call_device(
+        *[C.byref(i) for i in ints], *[C.byref(d) for d in dbl], b"JOHNSONCOOK".ljust(80),
+        ptr(ins["coord"]), ptr(ins["charlen"]), ptr(p), ptr(ins["density"]), ptr(srcs[0]),
+        ptr(ins["spin"]), ptr(srcs[3]), ptr(ins["stretch"]), ptr(ins["defgrad"]),
+        ptr(ins["field"]), ptr(srcs[1]), ptr(srcs[2]), ptr(srcs[4]), ptr(srcs[5]),
+        ptr(out["temp"]), ptr(out["stretch"]), ptr(out["defgrad"]), ptr(out["field"]),
+        ptr(out["stress"]), ptr(out["state"]), ptr(out["eint"]), ptr(out["einel"]))
+    if rc:
+        raise VpdError(lib().vpd_last_error(None).decode())
+    return out
+
+
+# ------------------------------------------------------------------ a bar case
+class Bar:
+    """The reference's regional-constraint bar (BoundaryValueCaseRunner.cpp:142-229) on the GPU:
+    lattice, boundary bands, GPU neighbour CSR, K⁻¹, material and dynamics initialisation,
+    then fused steps. `velocity` replaces the half-cosine ramp by a constant end velocity
+    (u = v·t, v, a = 0) for the velocity-driven bars."""
+
+    def __init__(self, nx, ny, nz, spacing, horizon, props, dt, density, amplitude=0.0,
+                 rise_time=1.0, component=0, region_layers=3, plane_strain=False, nstatev=6,
+                 tref=293.0, initial_state=None, damage=None, velocity=None,
+                 material_density=7850.0, return_map=RETURN_MAP_BISECTION, rowptr=None,
+                 col=None):
+        self.dt, self.amplitude, self.rise_time, self.velocity = dt, amplitude, rise_time, velocity
+        self.ctx = Context(horizon, density, props, nstatev, initial_state, tref, spacing,
+                           material_density, damage, return_map)
+        self.x, self.vol = make_patch(nx, ny, nz, spacing)
+        self.n = self.x.shape[0]
+        self.ctx.set_particles(self.x, self.vol)
+        if rowptr is None:
+            self.ctx.build_neighbors()
+        else:
+            self.ctx.set_neighbors(rowptr, col)
+        self.codes, self.anchor, self.pin = regional_constraint_codes(
+            self.x, component, region_layers, spacing, plane_strain)
+        self.ctx.set_constraints(self.codes)
+        self.ctx.initialize_shape()
+        self.ctx.initialize_material()
+        self.ctx.initialize_dynamics()
+        self.ctx.apply_constraints(self.ramp(0.0))
+        self.step_index = 0
+
+    def reset(self):
+        """Back to the state before step 1 (zero displacement, fresh history, intact bonds)."""
+        self.ctx.upload("u", np.zeros((self.n, 3)))
+        self.ctx.initialize_material()
+        self.ctx.initialize_dynamics()
+        self.ctx.apply_constraints(self.ramp(0.0))
+        self.step_index = 0
+
+    def ramp(self, t):
+        if self.velocity is not None:
+            return np.array([self.velocity * t, self.velocity, 0.0])
+        return smooth_ramp(self.amplitude, self.rise_time, t)
+
+    def step(self, k=1, fuse=True, want_fields=False):
+        """k increments; the last one is left unfused so the state reads as the reference's
+        end-of-step state."""
+        for s in range(k):
+            self.step_index += 1
+            t = self.step_index * self.dt
+            last = s == k - 1
+            nxt = None if (last or not fuse) else self.ramp(t + self.dt)
+            self.ctx.step(self.dt, t, (self.step_index - 1) * self.dt, self.ramp(t), nxt,
+                          want_fields and last)
+
+    def get(self, name):
+        return self.ctx.download(name)
