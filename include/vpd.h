@@ -167,6 +167,30 @@ int vpd_update_material(vpd_context *ctx, double dt, double step_time, double to
  * constraints at `ramp`. */
 int vpd_compute_force_and_correct(vpd_context *ctx, double dt, const double ramp[3]);
 
+/* PdCorrespondenceSkeleton::computeForceDensity (src/pd/PdCorrespondenceSkeleton.cpp:199-230)
+ * for a caller-supplied spatial Cauchy stress (n×9), with the context's last F (upload
+ * VPD_FIELD_DEFGRAD or compute it), K⁻¹ and bond damage. Writes force density (n×3). */
+int vpd_compute_force_density(vpd_context *ctx, const double *cauchy, double *force);
+
+/* ExplicitTimeIntegrator::correctVelocity (src/pd/ExplicitTimeIntegrator.cpp:36-52) for a
+ * caller-supplied force density (n×3). */
+int vpd_correct_velocity(vpd_context *ctx, double dt, const double *force);
+/* PdSimpleDamage::update, critical_stretch bond loop (src/damage/PdSimpleDamage.cpp:63-84), on
+ * the context's current displacement; no-op for VPD_DAMAGE_NONE. */
+int vpd_update_damage(vpd_context *ctx);
+
+/* ---- per-point kinematics on host arrays (computed on the GPU) --------------------- */
+/* updateCorotationalKinematics (src/pd/CorotationalKinematics.cpp:61-135) for n points:
+ * velocity gradient, unrotated rate of deformation, relative spin increment, R_{n+1},
+ * V_{n+1}; all n×9. */
+int vpd_corotational_kinematics(const double *fN, const double *fNP1, const double *vN,
+                                const double *rN, double dt, int64_t n, double *L, double *dbar,
+                                double *rel_spin, double *r1, double *v1);
+/* rotateCorotationalStressToSpatial (:137-139) and greenLagrangeStrain (:141-149) for n
+ * points; either pair of pointers may be NULL. */
+int vpd_rotate_and_strain(const double *stress, const double *rotation, const double *defgrad,
+                          int64_t n, double *spatial_stress, double *strain);
+
 /* ---- the fused increment ---------------------------------------------------------- */
 /* One step of the loop body in src/host/BoundaryValueCaseRunner.cpp:207-229 (without its
  * diagnostics): [predict + constrain unless the previous step fused it] → F (+ damage) →
@@ -198,6 +222,13 @@ int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstate
                                   double *tempNew, double *stretchNew, double *defgradNew,
                                   double *fieldNew, double *stressNew, double *stateNew,
                                   double *enerInternNew, double *enerInelasNew);
+
+/* ---- device memory for callers without a HIP runtime of their own ------------------ */
+/* Plumbing for the device VUMAT entry (no reference counterpart): allocate, free and copy
+ * device buffers through this library's HIP runtime. to_device 1: host → device. */
+int vpd_device_malloc(uint64_t bytes, void **out);
+int vpd_device_free(void *ptr);
+int vpd_memcpy(void *dst, const void *src, uint64_t bytes, int to_device);
 
 /* ---- host helpers of the case runner (no GPU work) --------------------------------- */
 /* makeParticlePatch (src/host/Geometry.cpp:10-34): x[n][3], volume[n], n = nx·ny·nz. */

@@ -66,7 +66,7 @@ def test_config_rejections_precede_device():
     with pytest.raises(vpd.VpdError, match="horizon > 0"):
         vpd.Context(0.0, 1.0, [1.0] * 14)
     with pytest.raises(vpd.VpdError, match="density > 0"):
-        vpd.Context(1.0, 0.0, [1.0] * 14)
+        vpd.Context(1.0, -1.0, [1.0] * 14)
     with pytest.raises(vpd.VpdError, match="14 props and 6 states"):
         vpd.Context(1.0, 1.0, [1.0] * 13)
     with pytest.raises(vpd.VpdError, match="initiation > 0"):

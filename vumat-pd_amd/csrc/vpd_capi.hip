@@ -65,8 +65,13 @@ struct vpd_context {
   double *fnew = nullptr, *amap = nullptr;
   double *sigOut = nullptr, *strainOut = nullptr, *force = nullptr, *dmgIndex = nullptr;
   int *iters = nullptr;
-  uint32_t *broken = nullptr;
+  uint32_t *broken = nullptr; // one bit per sliced bond slot
   int64_t brokenWords = 0;
+  int *colS = nullptr;         // sliced bond layout (see Sell in vpd_kernels.hpp)
+  int64_t *sliceBase = nullptr;
+  int64_t sellTotal = 0;
+  std::vector<int> hostRowptr;
+  std::vector<int64_t> hostBase;
   int *derr = nullptr;
   bool haveNeighbors = false, haveShape = false, haveMaterial = false, haveDynamics = false;
   bool haveFnew = false, haveFields = false, haveForce = false, predicted = false;
@@ -161,13 +166,18 @@ JcParams make_jc(const std::vector<double> &p) {
 }
 
 // Maps the device error bits to the reference's exception messages.
-int check_device(vpd_context *c) {
-  int bits = 0;
-  HIPTRY(c, hipMemcpyAsync(&bits, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+// Reports (and clears) the error bits in `mask`; other bits stay pending for a later check.
+// The material update leaves the first-Piola bit pending: the reference raises it only when
+// the force state is assembled (PdCorrespondenceSkeleton.cpp:83-91, called from :209).
+int check_device(vpd_context *c, int mask = ~0) {
+  int all = 0;
+  HIPTRY(c, hipMemcpyAsync(&all, c->derr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPTRY(c, hipStreamSynchronize(c->stream));
+  const int bits = all & mask;
   if (!bits)
     return VPD_OK;
-  HIPTRY(c, hipMemsetAsync(c->derr, 0, sizeof(int), c->stream));
+  const int keep = all & ~mask;
+  HIPTRY(c, hipMemcpyAsync(c->derr, &keep, sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPTRY(c, hipStreamSynchronize(c->stream));
   if (bits & ERR_NEIGHBOR_RANGE)
     return fail(c, VPD_ERR_ARGUMENT, "PD neighbor index is out of range.");
@@ -330,6 +340,47 @@ int ensure_outputs(vpd_context *c) {
   return VPD_OK;
 }
 
+// Sliced copy of the bond lists: slice s holds particles 64s..64s+63, padded to its
+// longest row; bond flags are indexed by sliced slot.
+int build_sell(vpd_context *c) {
+  const int64_t n = c->n;
+  c->hostRowptr.resize(n + 1);
+  HIPTRY(c, hipMemcpyAsync(c->hostRowptr.data(), c->rowptr, sizeof(int) * (n + 1),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  const int64_t ns = ceil_div(n, 64);
+  c->hostBase.assign(ns + 1, 0);
+  for (int64_t s = 0; s < ns; ++s) {
+    int m = 0;
+    for (int64_t i = 64 * s; i < std::min<int64_t>(n, 64 * s + 64); ++i)
+      m = std::max(m, c->hostRowptr[i + 1] - c->hostRowptr[i]);
+    c->hostBase[s + 1] = c->hostBase[s] + 64 * (int64_t)m;
+  }
+  c->sellTotal = c->hostBase[ns];
+  if (dalloc(c, c->sliceBase, ns + 1) || dalloc(c, c->colS, c->sellTotal))
+    return VPD_ERR_DEVICE;
+  HIPTRY(c, hipMemcpyAsync(c->sliceBase, c->hostBase.data(), sizeof(int64_t) * (ns + 1),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPTRY(c, hipMemsetAsync(c->colS, 0, sizeof(int) * std::max<int64_t>(1, c->sellTotal),
+                           c->stream));
+  k_sell_fill<<<grid_of(n), 256, 0, c->stream>>>(n, c->rowptr, c->col, c->sliceBase, c->colS);
+  HIPTRY(c, hipGetLastError());
+  c->brokenWords = ceil_div(c->sellTotal, 32);
+  if (dalloc(c, c->broken, c->brokenWords))
+    return VPD_ERR_DEVICE;
+  HIPTRY(c, hipMemsetAsync(c->broken, 0, sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords),
+                           c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  return VPD_OK;
+}
+
+// Sliced slot of CSR entry p of row i.
+int64_t sell_slot(const vpd_context *c, int64_t i, int64_t p) {
+  return c->hostBase[i >> 6] + 64 * (p - c->hostRowptr[i]) + (i & 63);
+}
+
+Sell sell_of(const vpd_context *c) { return Sell{c->sliceBase, c->colS, c->rowptr}; }
+
 Ramp ramp_of(const double r[3]) { return Ramp{r[0], r[1], r[2]}; }
 
 int launch_predict(vpd_context *c, double dt, const double ramp[3]) {
@@ -347,12 +398,12 @@ int launch_defgrad(vpd_context *c) {
   return timed(c, K_DEFGRAD, [&] {
     if (dmg)
       k_defgrad<true><<<grid8_of(c->n), 256, 0, c->stream>>>(
-          c->n, nblocks, planes(c->x, c->n), planes(c->u, c->n), c->vol, c->rowptr, c->col,
+          c->n, nblocks, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c),
           planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
           useCompression);
     else
       k_defgrad<false><<<grid8_of(c->n), 256, 0, c->stream>>>(
-          c->n, nblocks, planes(c->x, c->n), planes(c->u, c->n), c->vol, c->rowptr, c->col,
+          c->n, nblocks, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c),
           planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
           useCompression);
   });
@@ -394,8 +445,7 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.amap = planes(c->amap, c->n);
   g.forceOut = planes(fields ? c->force : nullptr, c->n);
   g.vol = c->vol;
-  g.rowptr = c->rowptr;
-  g.col = c->col;
+  g.bonds = sell_of(c);
   g.broken = c->broken;
   g.codes = c->codes;
   g.dt = dt;
@@ -461,13 +511,14 @@ int vpd_create(const vpd_config *cfg, vpd_context **out) {
   *out = nullptr;
   if (!(cfg->horizon > 0.0))
     return fail(nullptr, VPD_ERR_ARGUMENT, "PD correspondence skeleton requires horizon > 0.");
-  if (!(cfg->density > 0.0))
+  if (cfg->density < 0.0 || std::isnan(cfg->density))
     return fail(nullptr, VPD_ERR_ARGUMENT, "Explicit dynamics requires density > 0.");
-  if (!(cfg->char_length > 0.0))
+  const bool material = cfg->nprops > 0 || cfg->props;
+  if (material && !(cfg->char_length > 0.0))
     return fail(nullptr, VPD_ERR_ARGUMENT, "VUMAT-PD material bridge requires charLength > 0.");
-  if (cfg->nprops <= 0 || !cfg->props)
+  if (material && (cfg->nprops <= 0 || !cfg->props))
     return fail(nullptr, VPD_ERR_ARGUMENT, "VUMAT props cannot be empty.");
-  if (cfg->nprops < 14 || cfg->nstatev < 6)
+  if (material && (cfg->nprops < 14 || cfg->nstatev < 6))
     return fail(nullptr, VPD_ERR_ARGUMENT, "JohnsonCook VUMAT requires 14 props and 6 states");
   if (cfg->damage_mode != VPD_DAMAGE_NONE && cfg->damage_mode != VPD_DAMAGE_CRITICAL_STRETCH)
     return fail(nullptr, VPD_ERR_UNSUPPORTED, "Unsupported damage mode.");
@@ -484,16 +535,18 @@ int vpd_create(const vpd_config *cfg, vpd_context **out) {
   c->charLength = cfg->char_length;
   c->tref = cfg->reference_temperature;
   c->materialDensity = cfg->material_density;
-  c->props.assign(cfg->props, cfg->props + cfg->nprops);
-  c->nstatev = cfg->nstatev;
-  c->initialState.assign(cfg->nstatev, 0.0);
-  if (cfg->initial_state)
+  if (material)
+    c->props.assign(cfg->props, cfg->props + cfg->nprops);
+  c->nstatev = material ? cfg->nstatev : 0;
+  c->initialState.assign(c->nstatev, 0.0);
+  if (material && cfg->initial_state)
     c->initialState.assign(cfg->initial_state, cfg->initial_state + cfg->nstatev);
   c->damageMode = cfg->damage_mode;
   c->initiation = cfg->damage_initiation;
   c->failure = cfg->damage_failure;
   c->returnMap = cfg->return_map;
-  c->jc = make_jc(c->props);
+  if (material)
+    c->jc = make_jc(c->props);
   if (hipGetDevice(&c->device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(reinterpret_cast<void **>(&c->derr), sizeof(int)) != hipSuccess ||
@@ -541,6 +594,8 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->dmgIndex);
   dfree(c->iters);
   dfree(c->broken);
+  dfree(c->colS);
+  dfree(c->sliceBase);
   dfree(c->derr);
   if (c->stream)
     (void)hipStreamDestroy(c->stream);
@@ -601,12 +656,8 @@ int vpd_set_neighbors(vpd_context *c, const int32_t *rowptr, const int32_t *col,
                            c->stream));
   if (nnz)
     HIPTRY(c, hipMemcpyAsync(c->col, col, sizeof(int) * nnz, hipMemcpyHostToDevice, c->stream));
-  c->brokenWords = ceil_div(nnz, 32);
-  if (dalloc(c, c->broken, c->brokenWords))
-    return VPD_ERR_DEVICE;
-  HIPTRY(c, hipMemsetAsync(c->broken, 0, sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords),
-                           c->stream));
-  HIPTRY(c, hipStreamSynchronize(c->stream));
+  if (int st = build_sell(c))
+    return st;
   c->haveNeighbors = true;
   c->haveShape = c->haveMaterial = false;
   return VPD_OK;
@@ -712,12 +763,8 @@ int vpd_build_neighbors(vpd_context *c, int64_t *nnzOut) {
   HIPTRY(c, hipStreamSynchronize(c->stream));
   freeAll();
   c->nnz = nnz;
-  c->brokenWords = ceil_div(nnz, 32);
-  if (dalloc(c, c->broken, c->brokenWords))
-    return VPD_ERR_DEVICE;
-  HIPTRY(c, hipMemsetAsync(c->broken, 0, sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords),
-                           c->stream));
-  HIPTRY(c, hipStreamSynchronize(c->stream));
+  if (int st = build_sell(c))
+    return st;
   c->haveNeighbors = true;
   c->haveShape = c->haveMaterial = false;
   if (nnzOut)
@@ -776,6 +823,8 @@ int vpd_initialize_shape(vpd_context *c) {
 int vpd_initialize_material(vpd_context *c) {
   if (int s = require_block(c))
     return s;
+  if (c->props.empty())
+    return fail(c, VPD_ERR_STATE, "VUMAT props cannot be empty.");
   if (!c->haveShape)
     return fail(c, VPD_ERR_STATE, "PD correspondence state is not initialized.");
   // F₀ at the current displacement into fN (VumatPdCorrespondenceMaterial.cpp:54).
@@ -806,6 +855,8 @@ int vpd_initialize_material(vpd_context *c) {
 int vpd_initialize_dynamics(vpd_context *c) {
   if (int s = require_block(c))
     return s;
+  if (!(c->density > 0.0))
+    return fail(c, VPD_ERR_STATE, "Explicit dynamics requires density > 0.");
   HIPTRY(c, hipMemsetAsync(c->v, 0, sizeof(double) * 3 * c->n, c->stream));
   HIPTRY(c, hipMemsetAsync(c->a, 0, sizeof(double) * 3 * c->n, c->stream));
   if (c->broken)
@@ -827,11 +878,14 @@ int vpd_upload(vpd_context *c, int field, const void *host) {
       return fail(c, VPD_ERR_STATE, "Neighbour lists have not been built.");
     const double *d = static_cast<const double *>(host);
     std::vector<uint32_t> bits(std::max<int64_t>(1, c->brokenWords), 0u);
-    for (int64_t p = 0; p < c->nnz; ++p)
-      if (std::min(std::max(d[p], 0.0), 1.0) >= 1.0)
-        bits[p >> 5] |= 1u << (p & 31);
-      else if (d[p] > 0.0)
-        return fail(c, VPD_ERR_UNSUPPORTED, "Partial bond damage is not representable.");
+    for (int64_t i = 0; i < c->n; ++i)
+      for (int64_t p = c->hostRowptr[i]; p < c->hostRowptr[i + 1]; ++p) {
+        const int64_t q = sell_slot(c, i, p);
+        if (std::min(std::max(d[p], 0.0), 1.0) >= 1.0)
+          bits[q >> 5] |= 1u << (q & 31);
+        else if (d[p] > 0.0)
+          return fail(c, VPD_ERR_UNSUPPORTED, "Partial bond damage is not representable.");
+      }
     HIPTRY(c, hipMemcpyAsync(c->broken, bits.data(), sizeof(uint32_t) * bits.size(),
                              hipMemcpyHostToDevice, c->stream));
     HIPTRY(c, hipStreamSynchronize(c->stream));
@@ -859,6 +913,15 @@ int vpd_upload(vpd_context *c, int field, const void *host) {
     return fail(c, VPD_ERR_ARGUMENT, "Unknown field.");
   if (field == VPD_FIELD_DEFGRAD)
     c->haveFnew = true;
+  if (field == VPD_FIELD_DEFGRAD_N && !c->props.empty())
+    c->haveMaterial = true; // the caller supplies the material history
+  if ((field == VPD_FIELD_VELOCITY || field == VPD_FIELD_ACCELERATION) && c->density > 0.0)
+    c->haveDynamics = true;
+  if (field == VPD_FIELD_SHAPE_INVERSE) {
+    if (!c->haveNeighbors)
+      return fail(c, VPD_ERR_STATE, "Neighbour lists have not been built.");
+    c->haveShape = true;
+  }
   if (field == VPD_FIELD_DISPLACEMENT || field == VPD_FIELD_VELOCITY ||
       field == VPD_FIELD_ACCELERATION)
     c->predicted = false;
@@ -878,8 +941,11 @@ int vpd_download(vpd_context *c, int field, void *host) {
                              hipMemcpyDeviceToHost, c->stream));
     HIPTRY(c, hipStreamSynchronize(c->stream));
     double *d = static_cast<double *>(host);
-    for (int64_t p = 0; p < c->nnz; ++p)
-      d[p] = (bits[p >> 5] >> (p & 31)) & 1u ? 1.0 : 0.0;
+    for (int64_t i = 0; i < c->n; ++i)
+      for (int64_t p = c->hostRowptr[i]; p < c->hostRowptr[i + 1]; ++p) {
+        const int64_t q = sell_slot(c, i, p);
+        d[p] = (bits[q >> 5] >> (q & 31)) & 1u ? 1.0 : 0.0;
+      }
     return VPD_OK;
   }
   if (field == VPD_FIELD_STRESS_N) {
@@ -907,8 +973,8 @@ int vpd_download(vpd_context *c, int field, void *host) {
       return fail(c, VPD_ERR_STATE, "Neighbour lists have not been built.");
     if (!c->dmgIndex && dalloc(c, c->dmgIndex, c->n))
       return VPD_ERR_DEVICE;
-    k_damage_index<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, c->vol, c->rowptr, c->col,
-                                                         c->broken, c->dmgIndex);
+    k_damage_index<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, c->vol, sell_of(c), c->broken,
+                                                         c->dmgIndex);
     HIPTRY(c, hipGetLastError());
   }
   if ((field == VPD_FIELD_STRESS || field == VPD_FIELD_STRAIN) && !c->haveFields)
@@ -922,8 +988,10 @@ int vpd_download(vpd_context *c, int field, void *host) {
 }
 
 int vpd_predict(vpd_context *c, double dt, const double ramp[3]) {
-  if (int s = require_ready(c))
+  if (int s = require_block(c))
     return s;
+  if (!c->haveDynamics)
+    return fail(c, VPD_ERR_STATE, "Invalid explicit dynamics predictor input.");
   if (!(dt > 0.0) || !ramp)
     return fail(c, VPD_ERR_ARGUMENT, "Invalid explicit dynamics predictor input.");
   c->predicted = false;
@@ -970,7 +1038,7 @@ int vpd_update_material(vpd_context *c, double dt, double stepTime, double total
   if (int s = launch_material(c, dt, true))
     return s;
   c->haveFields = true;
-  return check_device(c);
+  return check_device(c, ~ERR_PIOLA_DET);
 }
 
 int vpd_compute_force_and_correct(vpd_context *c, double dt, const double ramp[3]) {
@@ -1008,6 +1076,181 @@ int vpd_step(vpd_context *c, const vpd_step_args *g) {
   c->haveFnew = true;
   c->haveFields = g->want_fields != 0;
   c->haveForce = g->want_fields != 0;
+  return VPD_OK;
+}
+
+int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *force) {
+  if (int s = require_block(c))
+    return s;
+  if (!cauchy || !force)
+    return fail(c, VPD_ERR_ARGUMENT, "PD force-state inputs do not match particle count.");
+  if (!c->haveShape)
+    return fail(c, VPD_ERR_STATE, "PD correspondence state is not initialized.");
+  if (!c->haveFnew)
+    return fail(c, VPD_ERR_STATE, "No deformation gradient for the force state.");
+  if (int s = ensure_outputs(c))
+    return s;
+  if (int s = put_planes(c, c->sigOut, cauchy, 9))
+    return s;
+  k_piola<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->sigOut, c->n),
+                                                planes(c->fnew, c->n), planes(c->kinv, c->n),
+                                                planes(c->amap, c->n), c->derr);
+  HIPTRY(c, hipGetLastError());
+  if (int s = check_device(c))
+    return s;
+  ForceArgs g{};
+  g.n = c->n;
+  g.nblocks = ceil_div(c->n, 256);
+  g.x = planes(c->x, c->n);
+  g.amap = planes(c->amap, c->n);
+  g.forceOut = planes(c->force, c->n);
+  g.vol = c->vol;
+  g.bonds = sell_of(c);
+  g.broken = c->broken;
+  const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
+  if (int s = timed(c, K_FORCE, [&] {
+        if (dmg)
+          k_force<true, false, false><<<grid8_of(c->n), 256, 0, c->stream>>>(g);
+        else
+          k_force<false, false, false><<<grid8_of(c->n), 256, 0, c->stream>>>(g);
+      }))
+    return s;
+  c->haveForce = true;
+  return get_planes(c, c->force, force, 3);
+}
+
+int vpd_corotational_kinematics(const double *fN, const double *fNP1, const double *vN,
+                                const double *rN, double dt, int64_t n, double *L, double *dbar,
+                                double *relSpin, double *r1, double *v1) {
+  if (!(dt > 0.0))
+    return fail(nullptr, VPD_ERR_ARGUMENT, "Corotational update requires dt > 0.");
+  if (n <= 0)
+    return VPD_OK;
+  if (!fN || !fNP1 || !vN || !rN || !L || !dbar || !relSpin || !r1 || !v1)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "Null corotational argument.");
+  vpd_context tmp;
+  tmp.n = n;
+  vpd_context *c = &tmp;
+  double *buf = nullptr;
+  int *err = nullptr;
+  auto release = [&] {
+    dfree(buf);
+    dfree(err);
+    if (tmp.stream)
+      (void)hipStreamDestroy(tmp.stream);
+    tmp.stream = nullptr;
+  };
+  if (hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(nullptr, VPD_ERR_DEVICE, "HIP stream creation failed.");
+  if (dalloc(c, buf, 9 * 9 * n) || dalloc(c, err, 1)) {
+    release();
+    return fail(nullptr, VPD_ERR_DEVICE, tmp.err);
+  }
+  double *p[9];
+  for (int k = 0; k < 9; ++k)
+    p[k] = buf + (int64_t)k * 9 * n;
+  const double *in[4] = {fN, fNP1, vN, rN};
+  double *out[5] = {L, dbar, relSpin, r1, v1};
+  int st = VPD_OK;
+  for (int k = 0; k < 4 && !st; ++k)
+    st = put_planes(c, p[k], in[k], 9);
+  if (!st && hipMemsetAsync(err, 0, sizeof(int), tmp.stream) != hipSuccess)
+    st = VPD_ERR_DEVICE;
+  if (!st) {
+    k_corotational_full<<<grid_of(n), 256, 0, tmp.stream>>>(
+        n, planes(p[0], n), planes(p[1], n), planes(p[2], n), planes(p[3], n), dt,
+        planes(p[4], n), planes(p[5], n), planes(p[6], n), planes(p[7], n), planes(p[8], n), err);
+    int bits = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&bits, err, sizeof(int), hipMemcpyDeviceToHost, tmp.stream) != hipSuccess ||
+        hipStreamSynchronize(tmp.stream) != hipSuccess)
+      st = VPD_ERR_DEVICE;
+    else if (bits) {
+      release();
+      return fail(nullptr, VPD_ERR_NUMERIC, "Corotational update encountered a singular tensor.");
+    }
+  }
+  for (int k = 0; k < 5 && !st; ++k)
+    st = get_planes(c, p[4 + k], out[k], 9);
+  release();
+  return st ? fail(nullptr, st, tmp.err) : VPD_OK;
+}
+
+int vpd_rotate_and_strain(const double *stress, const double *rotation, const double *defgrad,
+                          int64_t n, double *spatialStress, double *strain) {
+  if (n <= 0)
+    return VPD_OK;
+  vpd_context tmp;
+  tmp.n = n;
+  vpd_context *c = &tmp;
+  double *buf = nullptr;
+  auto release = [&] {
+    dfree(buf);
+    if (tmp.stream)
+      (void)hipStreamDestroy(tmp.stream);
+    tmp.stream = nullptr;
+  };
+  if (hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(nullptr, VPD_ERR_DEVICE, "HIP stream creation failed.");
+  if (dalloc(c, buf, 5 * 9 * n)) {
+    release();
+    return fail(nullptr, VPD_ERR_DEVICE, tmp.err);
+  }
+  double *p[5];
+  for (int k = 0; k < 5; ++k)
+    p[k] = buf + (int64_t)k * 9 * n;
+  const bool rot = stress && rotation && spatialStress, gl = defgrad && strain;
+  int st = VPD_OK;
+  if (rot)
+    st = put_planes(c, p[0], stress, 9) || put_planes(c, p[1], rotation, 9);
+  if (!st && gl)
+    st = put_planes(c, p[2], defgrad, 9);
+  if (!st) {
+    k_rotate_gl<<<grid_of(n), 256, 0, tmp.stream>>>(
+        n, planes(p[0], n), planes(p[1], n), planes(rot ? p[3] : nullptr, n), planes(p[2], n),
+        planes(gl ? p[4] : nullptr, n));
+    if (hipGetLastError() != hipSuccess)
+      st = VPD_ERR_DEVICE;
+  }
+  if (!st && rot)
+    st = get_planes(c, p[3], spatialStress, 9);
+  if (!st && gl)
+    st = get_planes(c, p[4], strain, 9);
+  release();
+  return st ? fail(nullptr, st, tmp.err) : VPD_OK;
+}
+
+int vpd_correct_velocity(vpd_context *c, double dt, const double *force) {
+  if (int s = require_block(c))
+    return s;
+  if (!(dt > 0.0) || !force)
+    return fail(c, VPD_ERR_ARGUMENT, "Invalid explicit dynamics corrector input.");
+  if (!c->haveDynamics)
+    return fail(c, VPD_ERR_STATE, "Invalid explicit dynamics corrector input.");
+  if (int s = ensure_outputs(c))
+    return s;
+  if (int s = put_planes(c, c->force, force, 3))
+    return s;
+  k_correct<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->v, c->n), planes(c->a, c->n),
+                                                  planes(c->force, c->n), dt, c->density);
+  HIPTRY(c, hipGetLastError());
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  return VPD_OK;
+}
+
+int vpd_update_damage(vpd_context *c) {
+  if (int s = require_block(c))
+    return s;
+  if (!c->haveNeighbors)
+    return fail(c, VPD_ERR_STATE, "Neighbour lists have not been built.");
+  if (c->damageMode != VPD_DAMAGE_CRITICAL_STRETCH)
+    return VPD_OK;
+  const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
+  k_bond_damage<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->x, c->n),
+                                                      planes(c->u, c->n), sell_of(c), c->broken,
+                                                      c->initiation, c->failure, useCompression);
+  HIPTRY(c, hipGetLastError());
+  HIPTRY(c, hipStreamSynchronize(c->stream));
   return VPD_OK;
 }
 
@@ -1051,6 +1294,31 @@ int vpd_collect_timing(vpd_context *c, vpd_kernel_time *out, int capacity, int *
   if (count)
     *count = k;
   return VPD_OK;
+}
+
+int vpd_device_malloc(uint64_t bytes, void **out) {
+  if (!out)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "Null output pointer.");
+  *out = nullptr;
+  hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+  return e == hipSuccess ? VPD_OK
+                         : fail(nullptr, VPD_ERR_DEVICE,
+                                std::string("HIP error: ") + hipGetErrorString(e));
+}
+
+int vpd_device_free(void *ptr) {
+  hipError_t e = hipFree(ptr);
+  return e == hipSuccess ? VPD_OK
+                         : fail(nullptr, VPD_ERR_DEVICE,
+                                std::string("HIP error: ") + hipGetErrorString(e));
+}
+
+int vpd_memcpy(void *dst, const void *src, uint64_t bytes, int to_device) {
+  hipError_t e = hipMemcpy(dst, src, bytes, to_device ? hipMemcpyHostToDevice
+                                                      : hipMemcpyDeviceToHost);
+  return e == hipSuccess ? VPD_OK
+                         : fail(nullptr, VPD_ERR_DEVICE,
+                                std::string("HIP error: ") + hipGetErrorString(e));
 }
 
 int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstatev, int *nfieldv,

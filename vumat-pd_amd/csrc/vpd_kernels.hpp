@@ -22,6 +22,33 @@ struct Planes {
 
 __device__ __forceinline__ void raise_error(int *err, int bit) { atomicOr(err, bit); }
 
+// Bond lists for the per-step passes in a 64-wide sliced layout: the k-th bond of particle
+// i = 64 s + L sits at base[s] + 64 k + L, so at every k the lanes of a wavefront read 64
+// consecutive column indices, and bond flags for lanes 0-31 / 32-63 share one word each.
+// Within a row the bonds keep their CSR (ascending) order.
+struct Sell {
+  const int64_t *base;
+  const int *col;
+  const int *rowptr;
+};
+__device__ __forceinline__ int64_t sell_pos(const Sell &s, int64_t i, int k) {
+  return s.base[i >> 6] + 64 * (int64_t)k + (i & 63);
+}
+
+// Fills the sliced layout from CSR (one lane per particle).
+__global__ void __launch_bounds__(256) k_sell_fill(int64_t n, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ col,
+                                                   const int64_t *__restrict__ base,
+                                                   int *__restrict__ colS) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const int b = rowptr[i], e = rowptr[i + 1];
+  const int64_t o = base[i >> 6] + (i & 63);
+  for (int k = 0; k < e - b; ++k)
+    colS[o + 64 * (int64_t)k] = col[b + k];
+}
+
 // Workgroups are handed out to the 8 XCDs round-robin by workgroup id. Remapping the id so
 // that each XCD walks one contiguous eighth of the particle range keeps every XCD's
 // neighbour working set (a few lattice planes) inside its own L2.
@@ -149,9 +176,8 @@ __global__ void __launch_bounds__(256) k_shape(int64_t n, Planes x, const double
 template <bool DAMAGE>
 __global__ void __launch_bounds__(256)
     k_defgrad(int64_t n, int64_t nblocks, Planes x, Planes u, const double *__restrict__ vol,
-              const int *__restrict__ rowptr, const int *__restrict__ col, Planes kinv,
-              Planes fout, uint32_t *broken, double initiation, double failure,
-              int useCompression) {
+              Sell bonds, Planes kinv, Planes fout, uint32_t *broken, double initiation,
+              double failure, int useCompression) {
   const int64_t blk = xcd_block(nblocks);
   if (blk < 0)
     return;
@@ -164,9 +190,11 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int c = 0; c < 9; ++c)
     num[c] = 0.0;
-  const int pb = rowptr[i], pe = rowptr[i + 1];
-  for (int p = pb; p < pe; ++p) {
-    const int j = col[p];
+  const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
+  const int64_t p0 = sell_pos(bonds, i, 0);
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = p0 + 64 * (int64_t)k;
+    const int j = bonds.col[p];
     const double xj0 = x(0, j), xj1 = x(1, j), xj2 = x(2, j);
     const double xi[3] = {xj0 - x0, xj1 - x1, xj2 - x2};
     const double yij[3] = {(xj0 + u(0, j)) - yi0, (xj1 + u(1, j)) - yi1, (xj2 + u(2, j)) - yi2};
@@ -305,14 +333,14 @@ struct ForceArgs {
   int64_t nblocks;
   Planes x, u, v, a, amap, forceOut;
   const double *vol;
-  const int *rowptr, *col;
+  Sell bonds;
   const uint32_t *broken;
   const uint8_t *codes;
   double dt, density;
   Ramp rampNow, rampNext;
 };
 
-template <bool DAMAGE, bool FUSE_NEXT>
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true>
 __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
   const int64_t blk = xcd_block(g.nblocks);
   if (blk < 0)
@@ -326,11 +354,13 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
   for (int c = 0; c < 9; ++c)
     ai.m[c] = g.amap(c, i);
   double acc[3] = {0.0, 0.0, 0.0};
-  const int pb = g.rowptr[i], pe = g.rowptr[i + 1];
-  for (int p = pb; p < pe; ++p) {
+  const int len = g.bonds.rowptr[i + 1] - g.bonds.rowptr[i];
+  const int64_t p0 = sell_pos(g.bonds, i, 0);
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = p0 + 64 * (int64_t)k;
     if (DAMAGE && (g.broken[p >> 5] & (1u << (p & 31))))
       continue;
-    const int j = g.col[p];
+    const int j = g.bonds.col[p];
     V3 xi;
     xi.v[0] = g.x(0, j) - x0;
     xi.v[1] = g.x(1, j) - x1;
@@ -355,6 +385,8 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
     for (int d = 0; d < 3; ++d)
       g.forceOut(d, i) = acc[d];
   }
+  if (!INTEGRATE)
+    return;
   double uu[3], vv[3], aa[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -379,18 +411,158 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
   }
 }
 
+
+// cauchyToFirstPiola · K⁻¹ for a caller-supplied spatial stress (PdCorrespondenceSkeleton.cpp:
+// 83-91, 209-210): A_i = (J σ F⁻ᵀ) K_i⁻¹.
+__global__ void __launch_bounds__(256) k_piola(int64_t n, Planes sigma, Planes f, Planes kinv,
+                                               Planes amap, int *err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  T3 s, ff, ki, P;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    s.m[c] = sigma(c, i);
+    ff.m[c] = f(c, i);
+    ki.m[c] = kinv(c, i);
+  }
+  if (!first_piola(s, ff, P)) {
+    raise_error(err, ERR_PIOLA_DET);
+    return;
+  }
+  const T3 A = t3_mul(P, ki);
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    amap(c, i) = A.m[c];
+}
+
+// The full Flanagan-Taylor update of CorotationalKinematics.cpp:61-135 with every output,
+// including the velocity gradient and the relative spin increment (:125-134), for callers
+// of the per-point API.
+__global__ void __launch_bounds__(256) k_corotational_full(int64_t n, Planes fN, Planes fNP1,
+                                                           Planes vN, Planes rN, double dt,
+                                                           Planes L, Planes dbar, Planes spin,
+                                                           Planes r1, Planes v1, int *err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  T3 f0, f1, v0, r0;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    f0.m[c] = fN(c, i);
+    f1.m[c] = fNP1(c, i);
+    v0.m[c] = vN(c, i);
+    r0.m[c] = rN(c, i);
+  }
+  Corot k;
+  T3 lv, w, om;
+  if (!corotational_full(f0, f1, v0, r0, dt, k, lv, w, om)) {
+    raise_error(err, ERR_SINGULAR_COROT);
+    return;
+  }
+  T3 rs;
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    rs.m[c] = w.m[c] - om.m[c];
+  T3 sp = t3_mul(t3_transpose(k.R1), t3_mul(rs, k.R1));
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    L(c, i) = lv.m[c];
+    dbar(c, i) = k.Dbar.m[c];
+    spin(c, i) = sp.m[c] * dt;
+    r1(c, i) = k.R1.m[c];
+    v1(c, i) = k.V1.m[c];
+  }
+}
+
+// σ = R σ̄ Rᵀ (CorotationalKinematics.cpp:137-139) and E = ½(FᵀF − I) (:141-149).
+__global__ void __launch_bounds__(256) k_rotate_gl(int64_t n, Planes s, Planes r, Planes sOut,
+                                                   Planes f, Planes eOut) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  if (sOut.p) {
+    T3 a, b;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      a.m[c] = s(c, i);
+      b.m[c] = r(c, i);
+    }
+    const T3 o = t3_mul(b, t3_mul(a, t3_transpose(b)));
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      sOut(c, i) = o.m[c];
+  }
+  if (eOut.p) {
+    T3 ff;
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      ff.m[c] = f(c, i);
+    T3 e = t3_mul(t3_transpose(ff), ff);
+    e.m[0] -= 1.0;
+    e.m[4] -= 1.0;
+    e.m[8] -= 1.0;
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      eOut(c, i) = e.m[c] * 0.5;
+  }
+}
+
+
+// ExplicitTimeIntegrator.cpp:36-52 for a caller-supplied force density.
+__global__ void __launch_bounds__(256) k_correct(int64_t n, Planes v, Planes a, Planes force,
+                                                 double dt, double density) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double next = force(d, i) / density;
+    v(d, i) = v(d, i) + 0.5 * dt * (a(d, i) + next);
+    a(d, i) = next;
+  }
+}
+
+// The bond loop of PdSimpleDamage.cpp:63-84 on its own (critical stretch).
+__global__ void __launch_bounds__(256) k_bond_damage(int64_t n, Planes x, Planes u, Sell bonds,
+                                                     uint32_t *broken, double initiation,
+                                                     double failure, int useCompression) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
+  const double yi0 = x0 + u(0, i), yi1 = x1 + u(1, i), yi2 = x2 + u(2, i);
+  const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = sell_pos(bonds, i, k);
+    const uint32_t bit = 1u << (p & 31);
+    if (broken[p >> 5] & bit)
+      continue;
+    const int j = bonds.col[p];
+    const double xj0 = x(0, j), xj1 = x(1, j), xj2 = x(2, j);
+    const double xi[3] = {xj0 - x0, xj1 - x1, xj2 - x2};
+    const double yij[3] = {(xj0 + u(0, j)) - yi0, (xj1 + u(1, j)) - yi1, (xj2 + u(2, j)) - yi2};
+    const double initial = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+    const double stretch =
+        (sqrt(yij[0] * yij[0] + yij[1] * yij[1] + yij[2] * yij[2]) - initial) / initial;
+    if (stretch > initiation || (useCompression && stretch < -failure))
+      atomicOr(&broken[p >> 5], bit);
+  }
+}
+
 // PdSimpleDamage.cpp:23-36: D_i = Σ d V_j / Σ V_j in CSR order.
 __global__ void __launch_bounds__(256) k_damage_index(int64_t n, const double *__restrict__ vol,
-                                                      const int *__restrict__ rowptr,
-                                                      const int *__restrict__ col,
+                                                      Sell bonds,
                                                       const uint32_t *__restrict__ broken,
                                                       double *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
   double damaged = 0.0, total = 0.0;
-  for (int p = rowptr[i]; p < rowptr[i + 1]; ++p) {
-    const double vj = vol[col[p]];
+  const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = sell_pos(bonds, i, k);
+    const double vj = vol[bonds.col[p]];
     const double d = (broken[p >> 5] >> (p & 31)) & 1u ? 1.0 : 0.0;
     damaged += d * vj;
     total += vj;

@@ -121,11 +121,12 @@ struct Corot {
   T3 R1, V1, Dbar;
 };
 
-// CorotationalKinematics.cpp:61-135 (Flanagan-Taylor). The relative-spin increment
-// (:125-134) feeds only VUMAT arguments the Johnson-Cook routine never reads, so it is
-// not formed here.
-__device__ __forceinline__ bool corotational(const T3 &fN, const T3 &fNP1, const T3 &vNt,
-                                             const T3 &rN, double dt, Corot &out) {
+// CorotationalKinematics.cpp:61-135 (Flanagan-Taylor). Also hands back L, W and Ω so the
+// per-point API can form the relative-spin increment (:125-134); the fused material kernel
+// drops them, since they feed only VUMAT arguments the Johnson-Cook routine never reads.
+__device__ __forceinline__ bool corotational_full(const T3 &fN, const T3 &fNP1, const T3 &vNt,
+                                                  const T3 &rN, double dt, Corot &out, T3 &L,
+                                                  T3 &w, T3 &om) {
   T3 fDot;
 #pragma unroll
   for (int i = 0; i < 9; ++i)
@@ -133,8 +134,8 @@ __device__ __forceinline__ bool corotational(const T3 &fN, const T3 &fNP1, const
   T3 fNi;
   if (!t3_inverse_corot(fN, fNi))
     return false;
-  const T3 L = t3_mul(fDot, fNi);
-  T3 d, w;
+  L = t3_mul(fDot, fNi);
+  T3 d;
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -165,7 +166,6 @@ __device__ __forceinline__ bool corotational(const T3 &fN, const T3 &fNP1, const
   const double oX = wX + c.m[0] * zX + c.m[1] * zY + c.m[2] * zZ;
   const double oY = wY + c.m[3] * zX + c.m[4] * zY + c.m[5] * zZ;
   const double oZ = wZ + c.m[6] * zX + c.m[7] * zY + c.m[8] * zZ;
-  T3 om;
   om.m[0] = 0.0;
   om.m[1] = -oZ;
   om.m[2] = oY;
@@ -195,6 +195,12 @@ __device__ __forceinline__ bool corotational(const T3 &fN, const T3 &fNP1, const
     out.V1.m[i] = vN[i] + dt * (ra.m[i] - rb.m[i]);
   out.Dbar = t3_mul(t3_transpose(out.R1), t3_mul(d, out.R1));
   return true;
+}
+
+__device__ __forceinline__ bool corotational(const T3 &fN, const T3 &fNP1, const T3 &vNt,
+                                             const T3 &rN, double dt, Corot &out) {
+  T3 L, w, om;
+  return corotational_full(fN, fNP1, vNt, rN, dt, out, L, w, om);
 }
 
 // TensorVoigtMap.cpp:9-16: [11, 22, 33, 23, 13, 12], shears symmetrised.

@@ -97,6 +97,9 @@ def lib():
                                                     C.POINTER(C.c_int64)]
         L.vpd_smooth_ramp.argtypes = [C.c_double, C.c_double, C.c_double, _D]
         L.vpd_smooth_ramp.restype = None
+        L.vpd_device_malloc.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
+        L.vpd_device_free.argtypes = [C.c_void_p]
+        L.vpd_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
         L.vpd_vumat_adapter_call_device.argtypes = ([C.POINTER(C.c_int)] * 7 +
                                                     [C.POINTER(C.c_double)] * 3 + [C.c_char_p] +
                                                     [C.c_void_p] * 22)
@@ -292,37 +295,59 @@ class Context:
         return {buf[i].name.decode(): (buf[i].total_ms, buf[i].launches) for i in range(cnt.value)}
 
 
+class DeviceBuffer:
+    """A device allocation made through libvpd's own HIP runtime."""
+
+    def __init__(self, host):
+        self.host = np.ascontiguousarray(host, dtype=np.float64)
+        self.ptr = C.c_void_p()
+        if lib().vpd_device_malloc(self.host.nbytes, C.byref(self.ptr)):
+            raise VpdError(lib().vpd_last_error(None).decode())
+        if lib().vpd_memcpy(self.ptr, self.host.ctypes.data_as(C.c_void_p), self.host.nbytes, 1):
+            raise VpdError(lib().vpd_last_error(None).decode())
+
+    def read(self):
+        out = np.empty_like(self.host)
+        if lib().vpd_memcpy(out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes, 0):
+            raise VpdError(lib().vpd_last_error(None).decode())
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            lib().vpd_device_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+
 def vumat_device(props, dt, strain_inc, stress_old, state_old, temp_old, eint_old, einel_old,
                  nfieldv=1):
-    """vpd_vumat_adapter_call_device on torch device tensors, column-major blocks:
-    strain_inc/stress_old (6, nb), state_old (nstatev, nb), the rest (nb,). Returns a dict
-    of device tensors (temp, stretch, defgrad, field, stress, state, eint, einel)."""
-    import torch
-
-    dev = strain_inc.device
+    """vpd_vumat_adapter_call_device on device copies of column-major numpy blocks:
+    strain_inc/stress_old (6, nb), state_old (nstatev, nb), the rest (nb,). Returns the
+    outputs read back (temp, stretch, defgrad, field, stress, state, eint, einel)."""
     nb = strain_inc.shape[1]
     nst = state_old.shape[0]
-    z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=dev)
-    p = torch.as_tensor(np.asarray(props, np.float64), device=dev)
-    ins = dict(coord=z(3, nb), charlen=z(nb), density=z(nb), spin=z(3, nb), stretch=z(6, nb),
-               defgrad=z(9, nb), field=z(max(1, nfieldv), nb))
-    out = dict(temp=z(nb), stretch=z(6, nb), defgrad=z(9, nb), field=z(max(1, nfieldv), nb),
-               stress=z(6, nb), state=z(nst, nb), eint=z(nb), einel=z(nb))
+    nf = max(1, nfieldv)
+    D = DeviceBuffer
+    ins = dict(coord=D(np.zeros((3, nb))), charlen=D(np.ones(nb)), props=D(props),
+               density=D(np.zeros(nb)), strain=D(strain_inc), spin=D(np.zeros((3, nb))),
+               temp=D(temp_old), stretch=D(np.zeros((6, nb))), defgrad=D(np.zeros((9, nb))),
+               field=D(np.zeros((nf, nb))), stress=D(stress_old), state=D(state_old),
+               eint=D(eint_old), einel=D(einel_old))
+    out = dict(temp=D(np.zeros(nb)), stretch=D(np.zeros((6, nb))), defgrad=D(np.zeros((9, nb))),
+               field=D(np.zeros((nf, nb))), stress=D(np.zeros((6, nb))),
+               state=D(np.zeros((nst, nb))), eint=D(np.zeros(nb)), einel=D(np.zeros(nb)))
     ints = [C.c_int(v) for v in (nb, 3, 3, nst, nfieldv, len(props), 0)]
     dbl = [C.c_double(v) for v in (0.0, 0.0, dt)]
-    ptr = lambda t: C.c_void_p(t.data_ptr())
-    srcs = [strain_inc, stress_old, state_old, temp_old, eint_old, einel_old]
-    srcs = [s.contiguous() for s in srcs]
+    i = ins
     rc = lib().vpd_vumat_adapter_call_device(
-        *[C.byref(i) for i in ints], *[C.byref(d) for d in dbl], b"JOHNSONCOOK".ljust(80),
-        ptr(ins["coord"]), ptr(ins["charlen"]), ptr(p), ptr(ins["density"]), ptr(srcs[0]),
-        ptr(ins["spin"]), ptr(srcs[3]), ptr(ins["stretch"]), ptr(ins["defgrad"]),
-        ptr(ins["field"]), ptr(srcs[1]), ptr(srcs[2]), ptr(srcs[4]), ptr(srcs[5]),
-        ptr(out["temp"]), ptr(out["stretch"]), ptr(out["defgrad"]), ptr(out["field"]),
-        ptr(out["stress"]), ptr(out["state"]), ptr(out["eint"]), ptr(out["einel"]))
+        *[C.byref(v) for v in ints], *[C.byref(d) for d in dbl], b"JOHNSONCOOK".ljust(80),
+        i["coord"].ptr, i["charlen"].ptr, i["props"].ptr, i["density"].ptr, i["strain"].ptr,
+        i["spin"].ptr, i["temp"].ptr, i["stretch"].ptr, i["defgrad"].ptr, i["field"].ptr,
+        i["stress"].ptr, i["state"].ptr, i["eint"].ptr, i["einel"].ptr, out["temp"].ptr,
+        out["stretch"].ptr, out["defgrad"].ptr, out["field"].ptr, out["stress"].ptr,
+        out["state"].ptr, out["eint"].ptr, out["einel"].ptr)
     if rc:
         raise VpdError(lib().vpd_last_error(None).decode())
-    return out
+    return {k: v.read() for k, v in out.items()}
 
 
 # ------------------------------------------------------------------ a bar case
