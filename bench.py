@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this path is not sharded across GPUs "
@@ -115,6 +116,8 @@ def main():
                   velocity=w["velocity"], damage=w["damage"])
     n, nnz = bar.n, bar.ctx.nnz
     bbar = nnz / n
+    tuned = bar.ctx.get_tuning() if args.no_autotune else bar.ctx.autotune(3)
+    print("[bench] launch tuning %s" % tuned, file=sys.stderr, flush=True)
     if args.warmup:
         bar.step(args.warmup)
         bar.ctx.synchronize()
@@ -155,7 +158,7 @@ def main():
         "config": {"workload": w["name"], "particles": n, "bonds": nnz, "grid": [w["nx"], w["ny"], w["nz"]],
                    "spacing_mm": w["spacing"], "horizon_over_spacing": 3.015, "dt_ms": dt,
                    "end_velocity_mm_per_ms": w["velocity"], "damage": "critical_stretch s_c=0.02",
-                   "return_map": "bisection (parity)", "steps_from": 1},
+                   "return_map": "bisection (parity)", "steps_from": 1, "launch_tuning": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9 * 1e9,
                      "unit": "B/s", "frac": achieved / HBM_PEAK, "traffic": None,
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_s * 1e3},

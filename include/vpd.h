@@ -38,7 +38,10 @@ enum vpd_status {
 
 /* Return-map solver of the Johnson-Cook stage. */
 enum vpd_return_map {
-  VPD_RETURN_MAP_BISECTION = 0 /* exact bracketed bisection of JohnsonCook.f90:114-136 */
+  VPD_RETURN_MAP_BISECTION = 0,       /* exact bracketed bisection of JohnsonCook.f90:114-136 */
+  VPD_RETURN_MAP_BISECTION_TIGHT = 1, /* the same scheme to |r| <= rtol·q_trial */
+  VPD_RETURN_MAP_NEWTON = 2,          /* safeguarded Newton seeded with state 5 */
+  VPD_RETURN_MAP_LEARNED = 3          /* safeguarded Newton seeded by the warm-start network */
 };
 
 enum vpd_damage_mode {
@@ -72,7 +75,9 @@ enum vpd_field {
   VPD_FIELD_DAMAGE_INDEX = 17,      /* n PdDamageResult::damageIndex (download) */
   VPD_FIELD_SHAPE_INVERSE = 18,     /* n×9 CorrespondenceState::shapeTensorInverse */
   VPD_FIELD_DEFGRAD = 19,           /* n×9 computeDeformationGradients output */
-  VPD_FIELD_RETURN_MAP_ITERATIONS = 20 /* n int32, iterations of the last return map */
+  VPD_FIELD_RETURN_MAP_ITERATIONS = 20, /* n int32, iterations of the last return map */
+  VPD_FIELD_STRAIN_INCREMENT = 21,      /* n×6 Voigt strainInc = D̄·dt of the last update */
+  VPD_FIELD_RETURN_MAP_FALLBACKS = 22   /* n int32, Newton steps replaced by bisection */
 };
 
 typedef struct vpd_config {
@@ -199,6 +204,33 @@ int vpd_rotate_and_strain(const double *stress, const double *rotation, const do
 int vpd_step(vpd_context *ctx, const vpd_step_args *args);
 int vpd_synchronize(vpd_context *ctx);
 
+/* ---- return-map solver of the material stage -------------------------------------- */
+/* Selects the solver (enum vpd_return_map) and the fast modes' relative residual bound
+ * rtol (|r| <= rtol·q_trial; 1e-12 by default). Bisection is the parity mode; the others
+ * change state 6 (iteration count) and Δε̄ᵖ within the tolerance. */
+int vpd_set_return_map(vpd_context *ctx, int mode, double rtol);
+/* The network used by VPD_RETURN_MAP_LEARNED (owned by the caller; NULL detaches). */
+typedef struct vpd_mlp vpd_mlp;
+int vpd_attach_warm_start(vpd_context *ctx, vpd_mlp *mlp);
+
+/* ---- launch tuning ---------------------------------------------------------------- */
+/* Two knobs per bond pass (no reference counterpart):
+ *  - sweeps: the pass walks the particle range in this many consecutive sweeps, so the
+ *    neighbour data an XCD touches at once (its stretch of the sweep plus a horizon either
+ *    side) can fit its L2;
+ *  - LDS bytes reserved per 256-thread workgroup, capping workgroups per CU.
+ * vpd_autotune times both passes over a grid of the two on the context's own block and keeps
+ * the fastest; tuned[] receives the values indexed by enum vpd_tuning. */
+enum vpd_tuning {
+  VPD_TUNE_DEFGRAD_LDS = 0,
+  VPD_TUNE_FORCE_LDS = 1,
+  VPD_TUNE_DEFGRAD_ROUNDS = 2,
+  VPD_TUNE_FORCE_ROUNDS = 3
+};
+int vpd_set_tuning(vpd_context *ctx, int knob, int value);
+int vpd_get_tuning(vpd_context *ctx, int knob, int *value);
+int vpd_autotune(vpd_context *ctx, int reps, int *tuned);
+
 /* ---- per-kernel timing ------------------------------------------------------------ */
 int vpd_set_timing(vpd_context *ctx, int enable);
 /* Synchronises, sums the recorded launches per kernel and clears them. Writes up to
@@ -229,6 +261,44 @@ int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstate
 int vpd_device_malloc(uint64_t bytes, void **out);
 int vpd_device_free(void *ptr);
 int vpd_memcpy(void *dst, const void *src, uint64_t bytes, int to_device);
+
+/* The Johnson-Cook return map alone on device arrays (column-major [k + nblock*c]), for
+ * comparing solvers on identical inputs. No reference counterpart beyond the solver of
+ * materials/examples/JohnsonCook.f90:114-136. mode: 0 the reference bisection (parity),
+ * 1 the same bisection run to |r| <= rtol·q, 2 safeguarded Newton seeded with state 5,
+ * 3 safeguarded Newton seeded with `seed` (the learned warm start). iterations/fallbacks
+ * (per point, may be NULL) report the solver work; *elapsed_ms the kernel time. */
+int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *props, int nprops,
+                          double dt, double rtol, const double *strain_inc,
+                          const double *stress_old, const double *state_old,
+                          const double *ener_intern_old, const double *ener_inelas_old,
+                          const double *seed, double *stress_new, double *state_new,
+                          double *ener_intern_new, double *ener_inelas_new, int32_t *iterations,
+                          int32_t *fallbacks, double *elapsed_ms);
+
+/* ---- the return-map warm start (no reference counterpart) -------------------------- */
+/* An 8 → 64 → 64 → 64 → 1 tanh network mapping (6 Voigt strain increments, ε̄ᵖ, T) to a
+ * Δε̄ᵖ estimate that seeds the Newton return map (vpd_return_map_device mode 3). Trained on
+ * the GPU in FP32 (forward, backward, Adam); inference runs as bf16 MFMA with FP32
+ * accumulation. theta: 8961 FP32 values, W1 (64×8), b1, W2 (64×64), b2, W3 (64×64), b3,
+ * W4 (1×64), b4; norm: input mean[8], std[8], output mean, std. */
+int vpd_mlp_create(int max_batch, vpd_mlp **out);
+void vpd_mlp_destroy(vpd_mlp *mlp);
+const char *vpd_mlp_last_error(const vpd_mlp *mlp);
+int vpd_mlp_set_params(vpd_mlp *mlp, const float *theta, const float *norm);
+int vpd_mlp_get_params(vpd_mlp *mlp, float *theta, float *norm);
+/* MSE loss and its gradient over one batch of normalised inputs x[B][8], targets t[B]. */
+int vpd_mlp_loss_and_gradient(vpd_mlp *mlp, const float *x, const float *t, int batch,
+                              float *loss, float *grad);
+int vpd_mlp_set_dataset(vpd_mlp *mlp, const float *x, const float *t, int64_t n, uint64_t seed);
+/* `steps` Adam steps on minibatches of the dataset; *last_loss = loss of the last batch. */
+int vpd_mlp_train(vpd_mlp *mlp, int steps, int batch, float lr, float *last_loss);
+/* Raw network outputs (normalised units) for normalised inputs, through the MFMA kernel. */
+int vpd_mlp_predict(vpd_mlp *mlp, const float *x, int64_t n, float *y);
+/* Seeds Δε̄ᵖ for n points from device planes: inc[c*inc_stride + i] (c < 6) and the JC
+ * state planes state[c*state_stride + i] (ε̄ᵖ = c 1, T = c 2). */
+int vpd_mlp_seed_device(vpd_mlp *mlp, const double *inc, int64_t inc_stride, const double *state,
+                        int64_t state_stride, int64_t n, double *seed, double *elapsed_ms);
 
 /* ---- host helpers of the case runner (no GPU work) --------------------------------- */
 /* makeParticlePatch (src/host/Geometry.cpp:10-34): x[n][3], volume[n], n = nx·ny·nz. */

@@ -9,7 +9,7 @@
 // first), so that with -ffp-contract=off the results agree bit for bit with a
 // non-contracting build of the Fortran source (pow/log/sqrt come from libm in both).
 //
-// Parity pins: tests/test_oracle_jc.py checks it against the amdflang build of the
+// Parity pins: tests/test_oracle_material.py checks it against the amdflang build of the
 // Fortran source under oracle/_ref (bitwise), the elastic-trial closed form, the
 // init branch, the converged-residual bound |r| < E*1e-9, the epsilon=0/rate=0
 // special case Y = A(1-theta^m), and the heat-unit ratio of
@@ -36,15 +36,18 @@ double jcYield(double a, double b, double c, double n, double m, double refRate,
 
 extern "C" {
 
-// JohnsonCook.f90:12-156. Returns 0, or 1 for the dimension errors on which the Fortran
-// routine executes STOP (JohnsonCook.f90:52-53).
-int orc_vumat_jc(int nblock, int ndir, int nshr, int nstatev, int nfieldv, int nprops,
-                 const double *dtArray, const double *props, const double *strainInc,
-                 const double *tempOld, const double *stretchOld, const double *defgradOld,
-                 const double *fieldOld, const double *stressOld, const double *stateOld,
-                 const double *enerInternOld, const double *enerInelasOld, double *tempNew,
-                 double *stretchNew, double *defgradNew, double *fieldNew, double *stressNew,
-                 double *stateNew, double *enerInternNew, double *enerInelasNew) {
+// JohnsonCook.f90:12-156 with the bisection's stopping rule as a parameter: rtol <= 0 keeps
+// the reference's |r| < E·1e-9 and 50 iterations; rtol > 0 stops at |r| < rtol·q_trial or
+// maxIt iterations (the tightly converged judge of the fast return maps). Returns 0, or 1
+// for the dimension errors on which the Fortran routine executes STOP (JohnsonCook.f90:52-53).
+int orc_vumat_jc_tol(int nblock, int ndir, int nshr, int nstatev, int nfieldv, int nprops,
+                     const double *dtArray, const double *props, const double *strainInc,
+                     const double *tempOld, const double *stretchOld, const double *defgradOld,
+                     const double *fieldOld, const double *stressOld, const double *stateOld,
+                     const double *enerInternOld, const double *enerInelasOld, double *tempNew,
+                     double *stretchNew, double *defgradNew, double *fieldNew, double *stressNew,
+                     double *stateNew, double *enerInternNew, double *enerInelasNew, double rtol,
+                     int maxIt) {
   if (ndir != 3 || nshr != 3)
     return 1;
   if (nprops < 14 || nstatev < 6)
@@ -118,8 +121,10 @@ int orc_vumat_jc(int nblock, int ndir, int nshr, int nstatev, int nfieldv, int n
     // Bracketed bisection, JohnsonCook.f90:114-136.
     double epInc = 0.0, epLower = 0.0, epUpper = qTrial / (2.0 * mu);
     double yield = oldYield, qNew, residual;
+    const double tol = rtol > 0.0 ? rtol * qTrial : tolerance;
+    const int iterCap = rtol > 0.0 ? maxIt : 50;
     int iteration = 0;
-    while (iteration < 50) {
+    while (iteration < iterCap) {
       iteration = iteration + 1;
       const double epRate = epInc / dt;
       yield = jcYield(a0, b0, rateC, hardN, thermalM, rate0, tref, tmelt, epOld + epInc, epRate,
@@ -127,7 +132,7 @@ int orc_vumat_jc(int nblock, int ndir, int nshr, int nstatev, int nfieldv, int n
       yield = std::fmax(yield, oldYield);
       qNew = std::fmax(0.0, qTrial - 3.0 * mu * epInc);
       residual = qNew - yield;
-      if (std::fabs(residual) < tolerance)
+      if (std::fabs(residual) < tol)
         break;
       if (epInc == 0.0 && residual < 0.0)
         break;
@@ -167,6 +172,20 @@ int orc_vumat_jc(int nblock, int ndir, int nshr, int nstatev, int nfieldv, int n
     tempNew[k] = stateNew[ix(k, 2)];
   }
   return 0;
+}
+
+int orc_vumat_jc(int nblock, int ndir, int nshr, int nstatev, int nfieldv, int nprops,
+                 const double *dtArray, const double *props, const double *strainInc,
+                 const double *tempOld, const double *stretchOld, const double *defgradOld,
+                 const double *fieldOld, const double *stressOld, const double *stateOld,
+                 const double *enerInternOld, const double *enerInelasOld, double *tempNew,
+                 double *stretchNew, double *defgradNew, double *fieldNew, double *stressNew,
+                 double *stateNew, double *enerInternNew, double *enerInelasNew) {
+  return orc_vumat_jc_tol(nblock, ndir, nshr, nstatev, nfieldv, nprops, dtArray, props,
+                          strainInc, tempOld, stretchOld, defgradOld, fieldOld, stressOld,
+                          stateOld, enerInternOld, enerInelasOld, tempNew, stretchNew,
+                          defgradNew, fieldNew, stressNew, stateNew, enerInternNew,
+                          enerInelasNew, 0.0, 50);
 }
 
 #ifdef VPD_ORACLE_PLUGIN

@@ -92,6 +92,8 @@ def lib():
                                 C.c_int, _D, C.c_double, C.c_double] + [_D] * 12 + [_DN] +
                                [_D] * 3)
         L.orc_vumat_jc.argtypes = [C.c_int] * 6 + [_D] * 11 + [_D] * 8
+        L.orc_vumat_jc_tol.argtypes = [C.c_int] * 6 + [_D] * 11 + [_D] * 8 + [C.c_double,
+                                                                            C.c_int]
         _lib = L
     return _lib
 
@@ -169,18 +171,19 @@ def green_lagrange(F):
 
 
 def vumat_jc(props, dt, strain_inc, stress_old, state_old, temp_old, eint_old, einel_old,
-             nstatev=6):
-    """JohnsonCook.f90 on a column-major block: strain_inc/stress_old (6, nb), state (nstatev, nb)."""
+             nstatev=6, rtol=0.0, max_iter=50):
+    """JohnsonCook.f90 on a column-major block: strain_inc/stress_old (6, nb), state (nstatev, nb).
+    rtol > 0 runs the bisection to |r| < rtol·q_trial within max_iter iterations."""
     nb = strain_inc.shape[1]
     z6, z9, z1 = np.zeros((6, nb)), np.zeros((9, nb)), np.zeros((1, nb))
     out = dict(temp=np.zeros(nb), stretch=np.zeros((6, nb)), defgrad=np.zeros((9, nb)),
                field=np.zeros((1, nb)), stress=np.zeros((6, nb)), state=np.zeros((nstatev, nb)),
                eint=np.zeros(nb), einel=np.zeros(nb))
-    rc = lib().orc_vumat_jc(nb, 3, 3, nstatev, 1, len(props), f64([dt]), f64(props),
-                            f64(strain_inc), f64(temp_old), z6, z9, z1, f64(stress_old),
-                            f64(state_old), f64(eint_old), f64(einel_old), out["temp"],
-                            out["stretch"], out["defgrad"], out["field"], out["stress"],
-                            out["state"], out["eint"], out["einel"])
+    rc = lib().orc_vumat_jc_tol(nb, 3, 3, nstatev, 1, len(props), f64([dt]), f64(props),
+                                f64(strain_inc), f64(temp_old), z6, z9, z1, f64(stress_old),
+                                f64(state_old), f64(eint_old), f64(einel_old), out["temp"],
+                                out["stretch"], out["defgrad"], out["field"], out["stress"],
+                                out["state"], out["eint"], out["einel"], rtol, max_iter)
     if rc:
         raise RuntimeError("JohnsonCook VUMAT requires ndir=3,nshr=3, 14 props and 6 states")
     return out

@@ -1,0 +1,147 @@
+"""GPU checks of the return-map solvers and of the learned warm start.
+
+* Return-map solvers on identical inputs: the parity bisection equals the oracle; the tight
+  bisection, Newton and learned-seed Newton all land on the FP64 oracle solved to
+  |r| < 1e-14·q (stress within 1e-10 relative).
+* The warm-start network: FP32 GPU gradients against the FP64 oracle network, bf16 MFMA
+  inference against the FP64 forward pass, deterministic training that lowers the loss.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import JC_PROPS
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print("[gpu-case]", msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def vpd():
+    import vpd as V
+    if V.device_count() == 0:
+        pytest.fail("no HIP device visible to the GPU tests")
+    return V
+
+
+def plastic_block(n, seed):
+    """Points on the yield surface with a loading increment: state initialised, random
+    hardening history, trial stress beyond the current yield."""
+    rng = np.random.default_rng(seed)
+    inc = rng.standard_normal((6, n)) * 1e-3
+    inc[:3] += 1e-3
+    stress = rng.standard_normal((6, n)) * 50.0
+    state = np.zeros((6, n))
+    state[0] = 1.0
+    state[1] = rng.random(n) * 0.05
+    state[2] = 293.0 + rng.random(n) * 100
+    state[3] = 300.0 + rng.random(n) * 200.0
+    state[4] = rng.random(n) * 2e-4
+    state[5] = 1.0
+    return inc, stress, state
+
+
+def test_return_map_modes(vpd, orc):
+    banner("return-map solvers vs oracle")
+    n = 20000
+    dt = 8.5377e-5
+    inc, stress, state = plastic_block(n, 4)
+    z = np.zeros(n)
+    ref = orc.vumat_jc(JC_PROPS, dt, inc, stress, state, np.full(n, 293.0), z, z)
+    tight = orc.vumat_jc(JC_PROPS, dt, inc, stress, state, np.full(n, 293.0), z, z,
+                         rtol=1e-14, max_iter=400)
+    plastic = tight["state"][4] > 0
+    assert plastic.mean() > 0.5
+    scale = np.abs(tight["stress"]).max()
+    runs = {}
+    for mode, seed in ((0, None), (1, None), (2, None), (3, tight["state"][4].copy())):
+        r = vpd.return_map(mode, JC_PROPS, dt, inc, stress, state, z, z, seed=seed)
+        runs[mode] = r
+    # Parity mode is the reference algorithm.
+    assert np.abs(runs[0]["stress"] - ref["stress"]).max() <= 1e-12 * scale
+    assert (runs[0]["iterations"] == ref["state"][5] - state[5]).mean() >= 0.999
+    # Fast modes converge to the tight solution.
+    for mode in (1, 2, 3):
+        err = np.abs(runs[mode]["stress"] - tight["stress"]).max() / scale
+        assert err <= 1e-10, (mode, err)
+        dep = np.abs(runs[mode]["state"][4] - tight["state"][4]).max()
+        assert dep <= 1e-12, (mode, dep)
+    it = {m: runs[m]["iterations"][plastic].mean() for m in runs}
+    print("mean iterations on plastic points:", it)
+    assert it[2] < it[1]  # Newton beats bisection at equal accuracy
+    assert it[3] <= 3.0   # an exact seed needs only the acceptance test
+
+
+def test_return_map_elastic_points_unchanged(vpd, orc):
+    banner("return-map elastic points")
+    n = 5000
+    rng = np.random.default_rng(8)
+    inc = rng.standard_normal((6, n)) * 1e-6
+    stress = rng.standard_normal((6, n)) * 10.0
+    state = np.zeros((6, n))
+    state[0] = 1.0
+    state[2] = 293.0
+    state[3] = 350.0
+    z = np.zeros(n)
+    ref = orc.vumat_jc(JC_PROPS, 1e-4, inc, stress, state, np.full(n, 293.0), z, z)
+    for mode in (2, 3):
+        r = vpd.return_map(mode, JC_PROPS, 1e-4, inc, stress, state, z, z, seed=np.zeros(n))
+        assert np.array_equal(r["state"][4], ref["state"][4])  # all zero
+        assert np.abs(r["stress"] - ref["stress"]).max() <= 1e-12 * np.abs(ref["stress"]).max()
+
+
+def test_mlp_gradient_vs_fp64_oracle(vpd):
+    banner("warm-start network gradient")
+    import mlp
+    rng = np.random.default_rng(0)
+    theta = (mlp.init(5) + 0.02 * rng.standard_normal(mlp.n_params())).astype(np.float32)
+    x = rng.standard_normal((1000, 8)).astype(np.float32)
+    t = rng.standard_normal(1000).astype(np.float32)
+    net = vpd.WarmStart(1024)
+    net.set_params(theta)
+    loss, g = net.loss_and_gradient(x, t)
+    lref, gref = mlp.loss_and_grad(theta.astype(np.float64), x.astype(np.float64),
+                                   t.astype(np.float64))
+    assert abs(loss - lref) <= 1e-5 * lref
+    assert np.abs(g - gref).max() <= 1e-4 * np.abs(gref).max()
+    cos = float(g @ gref / (np.linalg.norm(g) * np.linalg.norm(gref)))
+    assert cos > 1 - 1e-8
+
+
+def test_mlp_mfma_inference_vs_fp64(vpd):
+    banner("warm-start bf16 MFMA inference")
+    import mlp
+    rng = np.random.default_rng(1)
+    theta = (mlp.init(7) + 0.05 * rng.standard_normal(mlp.n_params())).astype(np.float32)
+    net = vpd.WarmStart(256)
+    net.set_params(theta)
+    x = rng.standard_normal((1003, 8)).astype(np.float32)  # ragged: not a multiple of 16
+    y = net.predict(x)
+    yref, _ = mlp.forward(theta.astype(np.float64), x.astype(np.float64))
+    # bf16 operands (8-bit mantissa) through three layers: ~1e-2 of the output scale.
+    assert np.abs(y - yref).max() <= 3e-2 * max(1.0, np.abs(yref).max())
+    assert np.corrcoef(y, yref)[0, 1] > 0.999
+
+
+def test_mlp_training_deterministic_and_descends(vpd):
+    banner("warm-start training")
+    import mlp
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((20000, 8)).astype(np.float32)
+    t = (np.sin(x[:, 0]) + 0.5 * x[:, 1] * x[:, 2]).astype(np.float32)
+    losses = []
+    params = []
+    for _ in range(2):
+        net = vpd.WarmStart(1024)
+        net.set_params(mlp.init(3).astype(np.float32))
+        net.set_dataset(x, t, seed=11)
+        l0, _ = net.loss_and_gradient(x[:1024], t[:1024])
+        losses.append((l0, net.train(600, 1024, 3e-3)))
+        params.append(net.get_params()[0])
+    assert losses[0][1] < 0.25 * losses[0][0]
+    assert np.array_equal(params[0], params[1])  # a training run is deterministic

@@ -2,6 +2,7 @@
 // kernel launches on the context stream, state transfer and error mapping.
 #include "vpd.h"
 #include "vpd_kernels.hpp"
+#include "vpd_mlp.hpp"
 #include "vpd_neighbors.hpp"
 
 #include <hip/hip_runtime.h>
@@ -28,11 +29,13 @@ enum KernelId {
   K_FORCE,
   K_SHAPE,
   K_NEIGHBORS,
+  K_MLP_SEED,
+  K_RETURN_MAP,
   K_COUNT
 };
 const char *kKernelNames[K_COUNT] = {"predict",      "constrain", "defgrad_bond_pass",
                                      "material_jc", "force_bond_pass", "shape_tensor",
-                                     "neighbor_build"};
+                                     "neighbor_build", "mlp_seed_mfma", "return_map"};
 
 struct TimingRec {
   int kernel;
@@ -42,6 +45,19 @@ struct TimingRec {
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 } // namespace
+
+struct vpd_mlp {
+  hipStream_t stream = nullptr;
+  int maxB = 0;
+  float *theta = nullptr, *norm = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  float *xb = nullptr, *yb = nullptr, *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *y = nullptr;
+  float *dy = nullptr, *dz3 = nullptr, *dz2 = nullptr, *dz1 = nullptr, *part = nullptr;
+  float *sq = nullptr, *xs = nullptr, *ys = nullptr;
+  int *order = nullptr;
+  int64_t nData = 0, cursor = 0;
+  int adamStep = 0;
+  std::string err;
+};
 
 struct vpd_context {
   std::string err;
@@ -76,6 +92,18 @@ struct vpd_context {
   bool haveNeighbors = false, haveShape = false, haveMaterial = false, haveDynamics = false;
   bool haveFnew = false, haveFields = false, haveForce = false, predicted = false;
   bool timing = false;
+  // LDS bytes reserved per 256-thread workgroup of each bond pass: a cap on how many
+  // workgroups share a CU, and with it on the span of particles whose neighbour data an
+  // XCD's L2 must hold at once. Chosen by vpd_autotune.
+  int ldsDefgrad = 0, ldsForce = 0;
+  // Sweeps per bond pass (see Rounds in vpd_kernels.hpp). Chosen by vpd_autotune.
+  int roundsDefgrad = 1, roundsForce = 1;
+  // Return map of the material stage; the learned mode adds the network, the strain
+  // increments it reads and the seeds it writes.
+  double rtol = 1.0e-12;
+  vpd_mlp *mlp = nullptr;
+  double *inc = nullptr, *seed = nullptr;
+  int *fallbacks = nullptr;
   std::vector<TimingRec> recs;
   std::vector<hipEvent_t> eventPool;
 };
@@ -197,6 +225,16 @@ Planes planes(double *p, int64_t n) { return Planes{p, n}; }
 int grid_of(int64_t n) { return (int)ceil_div(n, 256); }
 // Grids of the XCD-remapped kernels are rounded up to whole rounds of 8 workgroups.
 int grid8_of(int64_t n) { return (int)(ceil_div(ceil_div(n, 256), 8) * 8); }
+
+Rounds rounds_of(int64_t n, int rounds) {
+  const int64_t blocks = ceil_div(n, 256);
+  Rounds r;
+  r.n = n;
+  r.rounds = (int)std::max<int64_t>(1, std::min<int64_t>(rounds, blocks));
+  r.blocksPerRound = ceil_div(blocks, r.rounds);
+  return r;
+}
+int grid_rounds(const Rounds &r) { return (int)(ceil_div(r.blocksPerRound, 8) * 8); }
 
 int components(const vpd_context *c, int field) {
   switch (field) {
@@ -337,6 +375,8 @@ int ensure_outputs(vpd_context *c) {
     return VPD_ERR_DEVICE;
   if (!c->iters && dalloc(c, c->iters, c->n))
     return VPD_ERR_DEVICE;
+  if (!c->fallbacks && dalloc(c, c->fallbacks, c->n))
+    return VPD_ERR_DEVICE;
   return VPD_OK;
 }
 
@@ -391,22 +431,25 @@ int launch_predict(vpd_context *c, double dt, const double ramp[3]) {
   });
 }
 
-int launch_defgrad(vpd_context *c) {
-  const int64_t nblocks = ceil_div(c->n, 256);
-  const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
+int launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, int *err = nullptr) {
+  (void)err;
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
-  return timed(c, K_DEFGRAD, [&] {
-    if (dmg)
-      k_defgrad<true><<<grid8_of(c->n), 256, 0, c->stream>>>(
-          c->n, nblocks, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c),
-          planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
-          useCompression);
-    else
-      k_defgrad<false><<<grid8_of(c->n), 256, 0, c->stream>>>(
-          c->n, nblocks, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c),
-          planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
-          useCompression);
-  });
+  const Rounds rd = rounds_of(c->n, rounds);
+  if (dmg)
+    k_defgrad<true><<<grid_rounds(rd), 256, lds, c->stream>>>(
+        rd, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c), planes(c->kinv, c->n),
+        planes(c->fnew, c->n), c->broken, c->initiation, c->failure, useCompression);
+  else
+    k_defgrad<false><<<grid_rounds(rd), 256, lds, c->stream>>>(
+        rd, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c), planes(c->kinv, c->n),
+        planes(c->fnew, c->n), c->broken, c->initiation, c->failure, useCompression);
+  return VPD_OK;
+}
+
+int launch_defgrad(vpd_context *c) {
+  const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
+  return timed(c, K_DEFGRAD,
+               [&] { launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad); });
 }
 
 int launch_material(vpd_context *c, double dt, bool fields) {
@@ -426,11 +469,40 @@ int launch_material(vpd_context *c, double dt, bool fields) {
   g.einel = c->einel;
   g.sigOut = planes(fields ? c->sigOut : nullptr, c->n);
   g.strainOut = planes(fields ? c->strainOut : nullptr, c->n);
+  g.incOut = planes(fields || c->returnMap == RM_LEARNED ? c->inc : nullptr, c->n);
   g.iterOut = fields ? c->iters : nullptr;
+  g.fallbackOut = fields ? c->fallbacks : nullptr;
+  g.seed = nullptr;
   g.jc = c->jc;
   g.dt = dt;
+  g.rtol = c->rtol;
   g.err = c->derr;
-  return timed(c, K_MATERIAL, [&] { k_material<<<grid8_of(c->n), 256, 0, c->stream>>>(g); });
+  const int grid = grid8_of(c->n);
+  switch (c->returnMap) {
+  case RM_BISECTION:
+    return timed(c, K_MATERIAL,
+                 [&] { k_material<RM_BISECTION><<<grid, 256, 0, c->stream>>>(g); });
+  case RM_BISECTION_TIGHT:
+    return timed(c, K_MATERIAL,
+                 [&] { k_material<RM_BISECTION_TIGHT><<<grid, 256, 0, c->stream>>>(g); });
+  case RM_NEWTON:
+    return timed(c, K_MATERIAL, [&] { k_material<RM_NEWTON><<<grid, 256, 0, c->stream>>>(g); });
+  default: {
+    if (!c->mlp)
+      return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
+    if (int s = timed(c, K_MATERIAL,
+                      [&] { k_material<RM_LEARNED, true><<<grid, 256, 0, c->stream>>>(g); }))
+      return s;
+    if (int s = timed(c, K_MLP_SEED, [&] {
+          k_mlp_seed<<<(int)ceil_div(c->n, 64), 256, 0, c->stream>>>(
+              c->n, c->inc, c->n, c->statev, c->n, c->mlp->theta, c->mlp->norm, c->seed);
+        }))
+      return s;
+    g.seed = c->seed;
+    return timed(c, K_RETURN_MAP,
+                 [&] { k_material_return<RM_LEARNED><<<grid, 256, 0, c->stream>>>(g); });
+  }
+  }
 }
 
 int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
@@ -453,16 +525,17 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.rampNow = ramp_of(ramp);
   g.rampNext = fuse ? ramp_of(rampNext) : Ramp{0, 0, 0};
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
-  const int grid = grid8_of(c->n);
+  g.rd = rounds_of(c->n, c->roundsForce);
+  const int grid = grid_rounds(g.rd);
   return timed(c, K_FORCE, [&] {
     if (dmg && fuse)
-      k_force<true, true><<<grid, 256, 0, c->stream>>>(g);
+      k_force<true, true><<<grid, 256, c->ldsForce, c->stream>>>(g);
     else if (dmg)
-      k_force<true, false><<<grid, 256, 0, c->stream>>>(g);
+      k_force<true, false><<<grid, 256, c->ldsForce, c->stream>>>(g);
     else if (fuse)
-      k_force<false, true><<<grid, 256, 0, c->stream>>>(g);
+      k_force<false, true><<<grid, 256, c->ldsForce, c->stream>>>(g);
     else
-      k_force<false, false><<<grid, 256, 0, c->stream>>>(g);
+      k_force<false, false><<<grid, 256, c->ldsForce, c->stream>>>(g);
   });
 }
 
@@ -524,7 +597,7 @@ int vpd_create(const vpd_config *cfg, vpd_context **out) {
     return fail(nullptr, VPD_ERR_UNSUPPORTED, "Unsupported damage mode.");
   if (cfg->damage_mode == VPD_DAMAGE_CRITICAL_STRETCH && !(cfg->damage_initiation > 0.0))
     return fail(nullptr, VPD_ERR_ARGUMENT, "critical_stretch requires initiation > 0.");
-  if (cfg->return_map != VPD_RETURN_MAP_BISECTION)
+  if (cfg->return_map < VPD_RETURN_MAP_BISECTION || cfg->return_map > VPD_RETURN_MAP_LEARNED)
     return fail(nullptr, VPD_ERR_UNSUPPORTED, "Unsupported return-map mode.");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -593,6 +666,9 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->force);
   dfree(c->dmgIndex);
   dfree(c->iters);
+  dfree(c->fallbacks);
+  dfree(c->inc);
+  dfree(c->seed);
   dfree(c->broken);
   dfree(c->colS);
   dfree(c->sliceBase);
@@ -618,7 +694,7 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   for (size_t k = 0; k < sizeof(sz) / sizeof(sz[0]); ++k)
     if (dalloc(c, *bufs[k], sz[k]))
       return VPD_ERR_DEVICE;
-  if (dalloc(c, c->codes, n))
+  if (dalloc(c, c->codes, n) || dalloc(c, c->inc, 6 * n) || dalloc(c, c->seed, n))
     return VPD_ERR_DEVICE;
   HIPTRY(c, hipMemsetAsync(c->codes, 0, n, c->stream));
   HIPTRY(c, hipMemsetAsync(c->u, 0, sizeof(double) * 3 * n, c->stream));
@@ -627,6 +703,7 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   dfree(c->force);
   dfree(c->dmgIndex);
   dfree(c->iters);
+  dfree(c->fallbacks);
   c->haveNeighbors = c->haveShape = c->haveMaterial = c->haveDynamics = false;
   c->haveFnew = c->haveFields = c->haveForce = c->predicted = false;
   if (int s = put_planes(c, c->x, x, 3))
@@ -960,6 +1037,19 @@ int vpd_download(vpd_context *c, int field, void *host) {
     }
     return VPD_OK;
   }
+  if (field == VPD_FIELD_STRAIN_INCREMENT) {
+    if (!c->haveFields)
+      return fail(c, VPD_ERR_STATE, "Field is not available: run a step with want_fields.");
+    return get_planes(c, c->inc, static_cast<double *>(host), 6);
+  }
+  if (field == VPD_FIELD_RETURN_MAP_FALLBACKS) {
+    if (!c->fallbacks || !c->haveFields)
+      return fail(c, VPD_ERR_STATE, "Field is not available: run a step with want_fields.");
+    HIPTRY(c, hipMemcpyAsync(host, c->fallbacks, sizeof(int) * c->n, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return VPD_OK;
+  }
   if (field == VPD_FIELD_RETURN_MAP_ITERATIONS) {
     if (!c->iters || !c->haveFields)
       return fail(c, VPD_ERR_STATE, "Field is not available: run a step with want_fields.");
@@ -1108,11 +1198,12 @@ int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *forc
   g.bonds = sell_of(c);
   g.broken = c->broken;
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
+  g.rd = rounds_of(c->n, c->roundsForce);
   if (int s = timed(c, K_FORCE, [&] {
         if (dmg)
-          k_force<true, false, false><<<grid8_of(c->n), 256, 0, c->stream>>>(g);
+          k_force<true, false, false><<<grid_rounds(g.rd), 256, 0, c->stream>>>(g);
         else
-          k_force<false, false, false><<<grid8_of(c->n), 256, 0, c->stream>>>(g);
+          k_force<false, false, false><<<grid_rounds(g.rd), 256, 0, c->stream>>>(g);
       }))
     return s;
   c->haveForce = true;
@@ -1254,6 +1345,136 @@ int vpd_update_damage(vpd_context *c) {
   return VPD_OK;
 }
 
+int vpd_set_tuning(vpd_context *c, int knob, int value) {
+  if (!c)
+    return fail(c, VPD_ERR_ARGUMENT, "Null vpd_context.");
+  if (knob == VPD_TUNE_DEFGRAD_LDS || knob == VPD_TUNE_FORCE_LDS) {
+    if (value < 0 || value > 65536)
+      return fail(c, VPD_ERR_ARGUMENT, "LDS reservation must lie in [0, 65536] bytes.");
+    (knob == VPD_TUNE_DEFGRAD_LDS ? c->ldsDefgrad : c->ldsForce) = value;
+  } else if (knob == VPD_TUNE_DEFGRAD_ROUNDS || knob == VPD_TUNE_FORCE_ROUNDS) {
+    if (value < 1 || value > 1024)
+      return fail(c, VPD_ERR_ARGUMENT, "Sweeps per bond pass must lie in [1, 1024].");
+    (knob == VPD_TUNE_DEFGRAD_ROUNDS ? c->roundsDefgrad : c->roundsForce) = value;
+  } else {
+    return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
+  }
+  return VPD_OK;
+}
+
+int vpd_get_tuning(vpd_context *c, int knob, int *value) {
+  if (!c || !value)
+    return fail(c, VPD_ERR_ARGUMENT, "Null argument.");
+  switch (knob) {
+  case VPD_TUNE_DEFGRAD_LDS:
+    *value = c->ldsDefgrad;
+    return VPD_OK;
+  case VPD_TUNE_FORCE_LDS:
+    *value = c->ldsForce;
+    return VPD_OK;
+  case VPD_TUNE_DEFGRAD_ROUNDS:
+    *value = c->roundsDefgrad;
+    return VPD_OK;
+  case VPD_TUNE_FORCE_ROUNDS:
+    *value = c->roundsForce;
+    return VPD_OK;
+  default:
+    return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
+  }
+}
+
+// Times both bond passes over a grid of (sweeps per pass) × (workgroups per CU) and keeps
+// the fastest for each. The timing runs use the passes' side-effect-free forms: F into its
+// scratch plane, and the force gather without the integrator.
+int vpd_autotune(vpd_context *c, int reps, int *tuned) {
+  if (int s = require_ready(c))
+    return s;
+  if (reps < 1)
+    reps = 3;
+  if (int s = ensure_outputs(c))
+    return s;
+  const int ldsChoices[] = {0, 36408}; // ≤ 8 and 4 workgroups per CU
+  const int roundChoices[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
+  hipEvent_t a, b;
+  HIPTRY(c, hipEventCreate(&a));
+  HIPTRY(c, hipEventCreate(&b));
+  ForceArgs g{};
+  g.n = c->n;
+  g.nblocks = ceil_div(c->n, 256);
+  g.x = planes(c->x, c->n);
+  g.amap = planes(c->amap, c->n);
+  g.forceOut = planes(c->force, c->n);
+  g.vol = c->vol;
+  g.bonds = sell_of(c);
+  g.broken = c->broken;
+  const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
+  float best[2] = {1e30f, 1e30f};
+  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1};
+  for (int lds : ldsChoices)
+    for (int rounds : roundChoices) {
+      if (rounds > 1 && rounds > ceil_div(c->n, 256))
+        continue;
+      for (int pass = 0; pass < 2; ++pass) {
+        g.rd = rounds_of(c->n, rounds);
+        float total = 0.f;
+        for (int r = 0; r <= reps; ++r) {
+          HIPTRY(c, hipEventRecord(a, c->stream));
+          if (pass == 0)
+            launch_defgrad_with(c, false, lds, rounds);
+          else if (dmg)
+            k_force<true, false, false><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+          else
+            k_force<false, false, false><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+          HIPTRY(c, hipGetLastError());
+          HIPTRY(c, hipEventRecord(b, c->stream));
+          HIPTRY(c, hipEventSynchronize(b));
+          float ms = 0.f;
+          HIPTRY(c, hipEventElapsedTime(&ms, a, b));
+          if (r > 0) // the first run warms caches and code
+            total += ms;
+        }
+        if (total < best[pass]) {
+          best[pass] = total;
+          pickLds[pass] = lds;
+          pickRounds[pass] = rounds;
+        }
+      }
+    }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  c->ldsDefgrad = pickLds[0];
+  c->roundsDefgrad = pickRounds[0];
+  c->ldsForce = pickLds[1];
+  c->roundsForce = pickRounds[1];
+  c->haveFnew = false; // F scratch now holds the tuning runs' values
+  if (tuned) {
+    tuned[VPD_TUNE_DEFGRAD_LDS] = c->ldsDefgrad;
+    tuned[VPD_TUNE_FORCE_LDS] = c->ldsForce;
+    tuned[VPD_TUNE_DEFGRAD_ROUNDS] = c->roundsDefgrad;
+    tuned[VPD_TUNE_FORCE_ROUNDS] = c->roundsForce;
+  }
+  return check_device(c);
+}
+
+int vpd_set_return_map(vpd_context *c, int mode, double rtol) {
+  if (!c)
+    return fail(c, VPD_ERR_ARGUMENT, "Null vpd_context.");
+  if (mode < RM_BISECTION || mode > RM_LEARNED)
+    return fail(c, VPD_ERR_ARGUMENT, "Unknown return-map mode.");
+  if (!(rtol > 0.0) || rtol > 1e-3)
+    return fail(c, VPD_ERR_ARGUMENT, "Return-map tolerance must lie in (0, 1e-3].");
+  c->returnMap = mode;
+  c->rtol = rtol;
+  return VPD_OK;
+}
+
+int vpd_attach_warm_start(vpd_context *c, vpd_mlp *mlp) {
+  if (!c)
+    return fail(c, VPD_ERR_ARGUMENT, "Null vpd_context.");
+  c->mlp = mlp;
+  return VPD_OK;
+}
+
 int vpd_synchronize(vpd_context *c) {
   if (!c)
     return fail(c, VPD_ERR_ARGUMENT, "Null vpd_context.");
@@ -1321,6 +1542,75 @@ int vpd_memcpy(void *dst, const void *src, uint64_t bytes, int to_device) {
                                 std::string("HIP error: ") + hipGetErrorString(e));
 }
 
+int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *props, int nprops,
+                          double dt, double rtol, const double *strainInc,
+                          const double *stressOld, const double *stateOld,
+                          const double *enerInternOld, const double *enerInelasOld,
+                          const double *seed, double *stressNew, double *stateNew,
+                          double *enerInternNew, double *enerInelasNew, int32_t *iterations,
+                          int32_t *fallbacks, double *elapsedMs) {
+  if (mode < RM_BISECTION || mode > RM_LEARNED)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "Unknown return-map mode.");
+  if (!props || nprops < 14 || nstatev < 6)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "JohnsonCook VUMAT requires 14 props and 6 states");
+  if (mode == RM_LEARNED && !seed)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "The learned return map needs a seed array.");
+  if (!(dt > 0.0))
+    return fail(nullptr, VPD_ERR_ARGUMENT, "VUMAT update requires dt > 0.");
+  if (nblock <= 0)
+    return VPD_OK;
+  VumatArgs g{};
+  g.nblock = nblock;
+  g.nstatev = nstatev;
+  g.nfieldv = 0;
+  g.jc = make_jc(std::vector<double>(props, props + nprops));
+  g.dt = dt;
+  g.rtol = rtol;
+  g.strainInc = strainInc;
+  g.stressOld = stressOld;
+  g.stateOld = stateOld;
+  g.enerInternOld = enerInternOld;
+  g.enerInelasOld = enerInelasOld;
+  g.seed = seed;
+  g.stressNew = stressNew;
+  g.stateNew = stateNew;
+  g.enerInternNew = enerInternNew;
+  g.enerInelasNew = enerInelasNew;
+  g.iterations = iterations;
+  g.fallbacks = fallbacks;
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+    return fail(nullptr, VPD_ERR_DEVICE, "HIP event creation failed.");
+  (void)hipEventRecord(a, nullptr);
+  switch (mode) {
+  case RM_BISECTION:
+    k_vumat_jc<RM_BISECTION><<<grid_of(nblock), 256>>>(g);
+    break;
+  case RM_BISECTION_TIGHT:
+    k_vumat_jc<RM_BISECTION_TIGHT><<<grid_of(nblock), 256>>>(g);
+    break;
+  case RM_NEWTON:
+    k_vumat_jc<RM_NEWTON><<<grid_of(nblock), 256>>>(g);
+    break;
+  default:
+    k_vumat_jc<RM_LEARNED><<<grid_of(nblock), 256>>>(g);
+  }
+  hipError_t e = hipGetLastError();
+  (void)hipEventRecord(b, nullptr);
+  if (e == hipSuccess)
+    e = hipEventSynchronize(b);
+  float ms = 0.f;
+  if (e == hipSuccess)
+    e = hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (e != hipSuccess)
+    return fail(nullptr, VPD_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e));
+  if (elapsedMs)
+    *elapsedMs = ms;
+  return VPD_OK;
+}
+
 int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstatev, int *nfieldv,
                                   int *nprops, int *lanneal, double *stepTime, double *totalTime,
                                   double *dt, char *cmname, double *coordMp, double *charLength,
@@ -1374,12 +1664,289 @@ int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstate
   g.stateNew = stateNew;
   g.enerInternNew = enerInternNew;
   g.enerInelasNew = enerInelasNew;
-  k_vumat_jc<<<grid_of(g.nblock), 256>>>(g);
+  g.rtol = 0.0;
+  g.seed = nullptr;
+  g.iterations = nullptr;
+  g.fallbacks = nullptr;
+  k_vumat_jc<RM_BISECTION><<<grid_of(g.nblock), 256>>>(g);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
     e = hipDeviceSynchronize();
   if (e != hipSuccess)
     return fail(nullptr, VPD_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e));
+  return VPD_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- warm-start network
+
+
+namespace {
+
+int mfail(vpd_mlp *m, int status, const std::string &msg) {
+  if (m)
+    m->err = msg;
+  g_error = msg;
+  return status;
+}
+
+#define MLPTRY(m, expr)                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return mfail((m), VPD_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_));   \
+  } while (0)
+
+template <class T> int malloc_n(vpd_mlp *m, T *&p, int64_t count) {
+  if (p)
+    (void)hipFree(p);
+  p = nullptr;
+  MLPTRY(m, hipMalloc(reinterpret_cast<void **>(&p), sizeof(T) * (size_t)std::max<int64_t>(1, count)));
+  return VPD_OK;
+}
+
+// Forward and backward over the batch in xb/yb; leaves the gradient in m->grad and the
+// per-sample squared errors in m->sq.
+void mlp_forward_backward(vpd_mlp *m, int B) {
+  hipStream_t s = m->stream;
+  float *t = m->theta, *g = m->grad;
+  const int H = MLP_H;
+  auto grid = [](int64_t n) { return (int)ceil_div(n, 256); };
+  k_mlp_dense<<<grid((int64_t)B * H), 256, 0, s>>>(B, MLP_IN, H, m->xb, t + MLP_OFF_W1,
+                                                    t + MLP_OFF_B1, m->h1, 1);
+  k_mlp_dense<<<grid((int64_t)B * H), 256, 0, s>>>(B, H, H, m->h1, t + MLP_OFF_W2, t + MLP_OFF_B2,
+                                                    m->h2, 1);
+  k_mlp_dense<<<grid((int64_t)B * H), 256, 0, s>>>(B, H, H, m->h2, t + MLP_OFF_W3, t + MLP_OFF_B3,
+                                                    m->h3, 1);
+  k_mlp_dense<<<grid(B), 256, 0, s>>>(B, H, 1, m->h3, t + MLP_OFF_W4, t + MLP_OFF_B4, m->y, 0);
+  k_mlp_loss_grad<<<grid(B), 256, 0, s>>>(B, m->y, m->yb, m->dy, m->sq);
+  const int chunks = (int)ceil_div(B, MLP_CHUNK);
+  auto wgrad = [&](int K, int N, const float *dZ, const float *X, int offW, int offB) {
+    dim3 gp((unsigned)ceil_div((int64_t)N * (K + 1), 256), (unsigned)chunks);
+    k_mlp_wgrad_part<<<gp, 256, 0, s>>>(B, K, N, dZ, X, m->part);
+    k_mlp_wgrad_reduce<<<grid((int64_t)N * (K + 1)), 256, 0, s>>>(chunks, K, N, m->part, g + offW,
+                                                                 g + offB);
+  };
+  wgrad(H, 1, m->dy, m->h3, MLP_OFF_W4, MLP_OFF_B4);
+  k_mlp_back_input<<<grid((int64_t)B * H), 256, 0, s>>>(B, H, 1, m->dy, t + MLP_OFF_W4, m->h3,
+                                                         m->dz3);
+  wgrad(H, H, m->dz3, m->h2, MLP_OFF_W3, MLP_OFF_B3);
+  k_mlp_back_input<<<grid((int64_t)B * H), 256, 0, s>>>(B, H, H, m->dz3, t + MLP_OFF_W3, m->h2,
+                                                         m->dz2);
+  wgrad(H, H, m->dz2, m->h1, MLP_OFF_W2, MLP_OFF_B2);
+  k_mlp_back_input<<<grid((int64_t)B * H), 256, 0, s>>>(B, H, H, m->dz2, t + MLP_OFF_W2, m->h1,
+                                                         m->dz1);
+  wgrad(MLP_IN, H, m->dz1, m->xb, MLP_OFF_W1, MLP_OFF_B1);
+}
+
+int mlp_batch_loss(vpd_mlp *m, int B, float *loss) {
+  std::vector<float> sq(B);
+  MLPTRY(m, hipMemcpyAsync(sq.data(), m->sq, sizeof(float) * B, hipMemcpyDeviceToHost, m->stream));
+  MLPTRY(m, hipStreamSynchronize(m->stream));
+  double acc = 0.0;
+  for (float v : sq)
+    acc += v;
+  if (loss)
+    *loss = (float)(acc / B);
+  return VPD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int vpd_mlp_create(int maxBatch, vpd_mlp **out) {
+  if (!out || maxBatch <= 0)
+    return mfail(nullptr, VPD_ERR_ARGUMENT, "vpd_mlp_create requires a batch size and output.");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return mfail(nullptr, VPD_ERR_DEVICE, "No HIP device available.");
+  auto *m = new vpd_mlp();
+  m->maxB = maxBatch;
+  const int64_t B = maxBatch, H = MLP_H;
+  const int64_t chunks = ceil_div(B, MLP_CHUNK);
+  if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+      malloc_n(m, m->theta, MLP_PARAMS) || malloc_n(m, m->norm, MLP_NORM) ||
+      malloc_n(m, m->m, MLP_PARAMS) || malloc_n(m, m->v, MLP_PARAMS) ||
+      malloc_n(m, m->grad, MLP_PARAMS) || malloc_n(m, m->xb, B * MLP_IN) ||
+      malloc_n(m, m->yb, B) || malloc_n(m, m->h1, B * H) || malloc_n(m, m->h2, B * H) ||
+      malloc_n(m, m->h3, B * H) || malloc_n(m, m->y, B) || malloc_n(m, m->dy, B) ||
+      malloc_n(m, m->dz3, B * H) || malloc_n(m, m->dz2, B * H) || malloc_n(m, m->dz1, B * H) ||
+      malloc_n(m, m->part, chunks * H * (H + 1)) || malloc_n(m, m->sq, B)) {
+    vpd_mlp_destroy(m);
+    return mfail(nullptr, VPD_ERR_DEVICE, "Allocating the warm-start network failed.");
+  }
+  (void)hipMemsetAsync(m->theta, 0, sizeof(float) * MLP_PARAMS, m->stream);
+  (void)hipMemsetAsync(m->m, 0, sizeof(float) * MLP_PARAMS, m->stream);
+  (void)hipMemsetAsync(m->v, 0, sizeof(float) * MLP_PARAMS, m->stream);
+  std::vector<float> norm(MLP_NORM, 0.f);
+  for (int i = MLP_IN; i < 2 * MLP_IN; ++i)
+    norm[i] = 1.f;
+  norm[2 * MLP_IN + 1] = 1.f;
+  (void)hipMemcpyAsync(m->norm, norm.data(), sizeof(float) * MLP_NORM, hipMemcpyHostToDevice,
+                       m->stream);
+  if (hipStreamSynchronize(m->stream) != hipSuccess) {
+    vpd_mlp_destroy(m);
+    return mfail(nullptr, VPD_ERR_DEVICE, "Initialising the warm-start network failed.");
+  }
+  *out = m;
+  return VPD_OK;
+}
+
+void vpd_mlp_destroy(vpd_mlp *m) {
+  if (!m)
+    return;
+  if (m->stream)
+    (void)hipStreamSynchronize(m->stream);
+  float **bufs[] = {&m->theta, &m->norm, &m->m,   &m->v,   &m->grad, &m->xb, &m->yb,
+                    &m->h1,    &m->h2,   &m->h3,  &m->y,   &m->dy,   &m->dz3, &m->dz2,
+                    &m->dz1,   &m->part, &m->sq,  &m->xs,  &m->ys};
+  for (float **b : bufs)
+    dfree(*b);
+  dfree(m->order);
+  if (m->stream)
+    (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+const char *vpd_mlp_last_error(const vpd_mlp *m) { return m ? m->err.c_str() : g_error.c_str(); }
+
+int vpd_mlp_set_params(vpd_mlp *m, const float *theta, const float *norm) {
+  if (!m)
+    return mfail(m, VPD_ERR_ARGUMENT, "Null network.");
+  if (theta)
+    MLPTRY(m, hipMemcpyAsync(m->theta, theta, sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice,
+                             m->stream));
+  if (norm) {
+    for (int i = MLP_IN; i < 2 * MLP_IN; ++i)
+      if (!(norm[i] > 0.f))
+        return mfail(m, VPD_ERR_ARGUMENT, "Input scales must be positive.");
+    MLPTRY(m, hipMemcpyAsync(m->norm, norm, sizeof(float) * MLP_NORM, hipMemcpyHostToDevice,
+                             m->stream));
+  }
+  MLPTRY(m, hipMemsetAsync(m->m, 0, sizeof(float) * MLP_PARAMS, m->stream));
+  MLPTRY(m, hipMemsetAsync(m->v, 0, sizeof(float) * MLP_PARAMS, m->stream));
+  MLPTRY(m, hipStreamSynchronize(m->stream));
+  m->adamStep = 0;
+  return VPD_OK;
+}
+
+int vpd_mlp_get_params(vpd_mlp *m, float *theta, float *norm) {
+  if (!m)
+    return mfail(m, VPD_ERR_ARGUMENT, "Null network.");
+  if (theta)
+    MLPTRY(m, hipMemcpyAsync(theta, m->theta, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost,
+                             m->stream));
+  if (norm)
+    MLPTRY(m, hipMemcpyAsync(norm, m->norm, sizeof(float) * MLP_NORM, hipMemcpyDeviceToHost,
+                             m->stream));
+  MLPTRY(m, hipStreamSynchronize(m->stream));
+  return VPD_OK;
+}
+
+int vpd_mlp_loss_and_gradient(vpd_mlp *m, const float *x, const float *t, int B, float *loss,
+                              float *grad) {
+  if (!m || !x || !t || B <= 0 || B > m->maxB)
+    return mfail(m, VPD_ERR_ARGUMENT, "Batch exceeds the network's capacity.");
+  MLPTRY(m, hipMemcpyAsync(m->xb, x, sizeof(float) * B * MLP_IN, hipMemcpyHostToDevice,
+                           m->stream));
+  MLPTRY(m, hipMemcpyAsync(m->yb, t, sizeof(float) * B, hipMemcpyHostToDevice, m->stream));
+  mlp_forward_backward(m, B);
+  MLPTRY(m, hipGetLastError());
+  if (grad)
+    MLPTRY(m, hipMemcpyAsync(grad, m->grad, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost,
+                             m->stream));
+  return mlp_batch_loss(m, B, loss);
+}
+
+int vpd_mlp_set_dataset(vpd_mlp *m, const float *x, const float *t, int64_t n, uint64_t seed) {
+  if (!m || !x || !t || n <= 0 || n > INT_MAX)
+    return mfail(m, VPD_ERR_ARGUMENT, "Invalid training set.");
+  if (malloc_n(m, m->xs, n * MLP_IN) || malloc_n(m, m->ys, n) || malloc_n(m, m->order, n))
+    return VPD_ERR_DEVICE;
+  // A fixed shuffle (64-bit LCG driven Fisher-Yates) makes training reproducible.
+  std::vector<int> order(n);
+  for (int64_t i = 0; i < n; ++i)
+    order[i] = (int)i;
+  uint64_t st = seed * 6364136223846793005ULL + 1442695040888963407ULL;
+  for (int64_t i = n - 1; i > 0; --i) {
+    st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+    const int64_t j = (int64_t)((st >> 33) % (uint64_t)(i + 1));
+    std::swap(order[i], order[j]);
+  }
+  MLPTRY(m, hipMemcpyAsync(m->xs, x, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice, m->stream));
+  MLPTRY(m, hipMemcpyAsync(m->ys, t, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+  MLPTRY(m, hipMemcpyAsync(m->order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                           m->stream));
+  MLPTRY(m, hipStreamSynchronize(m->stream));
+  m->nData = n;
+  m->cursor = 0;
+  return VPD_OK;
+}
+
+int vpd_mlp_train(vpd_mlp *m, int steps, int batch, float lr, float *lastLoss) {
+  if (!m || batch <= 0 || batch > m->maxB || steps < 0)
+    return mfail(m, VPD_ERR_ARGUMENT, "Batch exceeds the network's capacity.");
+  if (m->nData <= 0)
+    return mfail(m, VPD_ERR_STATE, "No training set.");
+  for (int s = 0; s < steps; ++s) {
+    k_mlp_gather<<<(int)ceil_div(batch, 256), 256, 0, m->stream>>>(batch, m->nData, m->cursor,
+                                                                    m->order, m->xs, m->ys, m->xb,
+                                                                    m->yb);
+    m->cursor = (m->cursor + batch) % m->nData;
+    mlp_forward_backward(m, batch);
+    ++m->adamStep;
+    const float c1 = 1.f - powf(0.9f, (float)m->adamStep);
+    const float c2 = 1.f - powf(0.999f, (float)m->adamStep);
+    k_mlp_adam<<<(int)ceil_div(MLP_PARAMS, 256), 256, 0, m->stream>>>(MLP_PARAMS, m->theta,
+                                                                       m->grad, m->m, m->v, lr, c1,
+                                                                       c2);
+  }
+  MLPTRY(m, hipGetLastError());
+  return mlp_batch_loss(m, batch, lastLoss);
+}
+
+int vpd_mlp_predict(vpd_mlp *m, const float *x, int64_t n, float *y) {
+  if (!m || !x || !y || n < 0)
+    return mfail(m, VPD_ERR_ARGUMENT, "Invalid prediction arguments.");
+  if (n == 0)
+    return VPD_OK;
+  float *dx = nullptr, *dy = nullptr;
+  if (malloc_n(m, dx, n * MLP_IN) || malloc_n(m, dy, n))
+    return VPD_ERR_DEVICE;
+  MLPTRY(m, hipMemcpyAsync(dx, x, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice, m->stream));
+  k_mlp_infer_raw<<<(int)ceil_div(n, 64), 256, 0, m->stream>>>(n, dx, m->theta, dy);
+  MLPTRY(m, hipGetLastError());
+  MLPTRY(m, hipMemcpyAsync(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
+  MLPTRY(m, hipStreamSynchronize(m->stream));
+  dfree(dx);
+  dfree(dy);
+  return VPD_OK;
+}
+
+int vpd_mlp_seed_device(vpd_mlp *m, const double *inc, int64_t incStride, const double *state,
+                        int64_t stateStride, int64_t n, double *seed, double *elapsedMs) {
+  if (!m || !inc || !state || !seed || n < 0)
+    return mfail(m, VPD_ERR_ARGUMENT, "Invalid seed arguments.");
+  if (n == 0)
+    return VPD_OK;
+  hipEvent_t a, b;
+  MLPTRY(m, hipEventCreate(&a));
+  MLPTRY(m, hipEventCreate(&b));
+  MLPTRY(m, hipEventRecord(a, m->stream));
+  k_mlp_seed<<<(int)ceil_div(n, 64), 256, 0, m->stream>>>(n, inc, incStride, state, stateStride,
+                                                           m->theta, m->norm, seed);
+  MLPTRY(m, hipGetLastError());
+  MLPTRY(m, hipEventRecord(b, m->stream));
+  MLPTRY(m, hipEventSynchronize(b));
+  float ms = 0.f;
+  MLPTRY(m, hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (elapsedMs)
+    *elapsedMs = ms;
   return VPD_OK;
 }
 

@@ -60,6 +60,19 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
   return mapped < nblocks ? mapped : -1;
 }
 
+// The bond passes walk the particle range in `rounds` consecutive sweeps. Within a sweep
+// every XCD works on one contiguous stretch, so the neighbour data an XCD touches at once is
+// that stretch plus one horizon either side: with enough rounds it fits the XCD's 4 MB L2,
+// while every sweep still fills the chip. The autotuner picks the number of rounds.
+struct Rounds {
+  int64_t n;
+  int64_t blocksPerRound; // 256-particle workgroups per sweep
+  int rounds;
+  __device__ __forceinline__ int64_t particle(int r, int64_t blk) const {
+    return ((int64_t)r * blocksPerRound + blk) * 256 + threadIdx.x;
+  }
+};
+
 // Per-component constraint codes: 2 bits per axis, 0 free, 1 held at zero, 2 follows the
 // ramp (BoundaryConditions.cpp:81-115 collapsed to what its sequence of writes leaves).
 struct Ramp {
@@ -174,16 +187,11 @@ __global__ void __launch_bounds__(256) k_shape(int64_t n, Planes x, const double
 // positions: a bond breaks (irreversibly) when its stretch exceeds `initiation` or, with a
 // compression threshold armed, falls below −failure. Damage does not enter F (:181-196).
 template <bool DAMAGE>
-__global__ void __launch_bounds__(256)
-    k_defgrad(int64_t n, int64_t nblocks, Planes x, Planes u, const double *__restrict__ vol,
-              Sell bonds, Planes kinv, Planes fout, uint32_t *broken, double initiation,
-              double failure, int useCompression) {
-  const int64_t blk = xcd_block(nblocks);
-  if (blk < 0)
-    return;
-  const int64_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= n)
-    return;
+__device__ __forceinline__ void defgrad_one(int64_t i, Planes x, Planes u,
+                                            const double *__restrict__ vol, Sell bonds,
+                                            Planes kinv, Planes fout, uint32_t *broken,
+                                            double initiation, double failure,
+                                            int useCompression) {
   const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
   const double yi0 = x0 + u(0, i), yi1 = x1 + u(1, i), yi2 = x2 + u(2, i);
   double num[9];
@@ -227,6 +235,23 @@ __global__ void __launch_bounds__(256)
     fout(c, i) = f.m[c];
 }
 
+template <bool DAMAGE>
+__global__ void __launch_bounds__(256)
+    k_defgrad(Rounds rd, Planes x, Planes u, const double *__restrict__ vol, Sell bonds,
+              Planes kinv, Planes fout, uint32_t *broken, double initiation, double failure,
+              int useCompression) {
+  const int64_t blk = xcd_block(rd.blocksPerRound);
+  if (blk < 0)
+    return;
+  for (int r = 0; r < rd.rounds; ++r) {
+    const int64_t i = rd.particle(r, blk);
+    if (i >= rd.n)
+      break;
+    defgrad_one<DAMAGE>(i, x, u, vol, bonds, kinv, fout, broken, initiation, failure,
+                        useCompression);
+  }
+}
+
 // Material point update. VumatPdCorrespondenceMaterial.cpp:110-173 for every point at
 // once (nblock = N; the 256-row padding of the CPU host leaves results unchanged because
 // the routine is point-wise): Flanagan-Taylor kinematics, strainInc = D̄·dt and stressOld
@@ -239,51 +264,22 @@ struct MaterialArgs {
   int64_t nblocks;
   Planes fnew, fN, vN, rN, sig, statev, kinv, amap;
   double *temp, *eint, *einel;
-  Planes sigOut, strainOut;
-  int *iterOut;
+  Planes sigOut, strainOut, incOut;
+  int *iterOut, *fallbackOut;
+  const double *seed;
   JcParams jc;
-  double dt;
+  double dt, rtol;
   int *err;
 };
 
-__global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
-  const int64_t blk = xcd_block(g.nblocks);
-  if (blk < 0)
-    return;
-  const int64_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= g.n)
-    return;
-  T3 f1, f0, v0, r0;
-#pragma unroll
-  for (int c = 0; c < 9; ++c) {
-    f1.m[c] = g.fnew(c, i);
-    f0.m[c] = g.fN(c, i);
-    v0.m[c] = g.vN(c, i);
-    r0.m[c] = g.rN(c, i);
-  }
-  Corot k;
-  if (!corotational(f0, f1, v0, r0, g.dt, k)) {
-    raise_error(g.err, ERR_SINGULAR_COROT);
-    return;
-  }
-  double inc[6], sOld[6], sNew[6], st[6];
-  to_voigt(k.Dbar, inc);
-#pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    inc[c] *= g.dt;
-    sOld[c] = g.sig(c, i);
-    st[c] = g.statev(c, i);
-  }
-  double eint = g.eint[i], einel = g.einel[i];
-  const int it = jc_point(g.jc, g.dt, inc, sOld, st, eint, einel, sNew);
+// The stages after the return map, shared by the fused and the split material kernels:
+// history commit, σ = R σ̄ Rᵀ, optional diagnostics, A = P K⁻¹.
+__device__ __forceinline__ void material_finish(const MaterialArgs &g, int64_t i, const T3 &f1,
+                                                const T3 &r1, const double sNew[6],
+                                                const double st[6], double eint, double einel,
+                                                const ReturnMapStats &stats) {
   const T3 sFull = from_voigt(sNew);
-  const T3 sigma = t3_mul(k.R1, t3_mul(sFull, t3_transpose(k.R1)));
-#pragma unroll
-  for (int c = 0; c < 9; ++c) {
-    g.fN(c, i) = f1.m[c];
-    g.vN(c, i) = k.V1.m[c];
-    g.rN(c, i) = k.R1.m[c];
-  }
+  const T3 sigma = t3_mul(r1, t3_mul(sFull, t3_transpose(r1)));
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     g.sig(c, i) = sNew[c];
@@ -293,7 +289,9 @@ __global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
   g.eint[i] = eint;
   g.einel[i] = einel;
   if (g.iterOut)
-    g.iterOut[i] = it;
+    g.iterOut[i] = stats.iterations;
+  if (g.fallbackOut)
+    g.fallbackOut[i] = stats.fallbacks;
   if (g.sigOut.p) {
 #pragma unroll
     for (int c = 0; c < 9; ++c)
@@ -323,6 +321,95 @@ __global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
     g.amap(c, i) = A.m[c];
 }
 
+// Material point update, fused: VumatPdCorrespondenceMaterial.cpp:110-173 for every point at
+// once (nblock = N; the 256-row padding of the CPU host leaves results unchanged because
+// the routine is point-wise): Flanagan-Taylor kinematics, strainInc = D̄·dt and stressOld
+// packed as Voigt (VumatMaterialAdapter.cpp:290-330), the Johnson-Cook return map,
+// σ = R σ̄ Rᵀ, history commit; then A_i = (J σ F⁻ᵀ) K_i⁻¹ for the force pass
+// (PdCorrespondenceSkeleton.cpp:209-210). With FT_ONLY the kernel stops after the
+// kinematics: it commits F, V, R and writes the strain increments to incOut, for a
+// return-map stage that needs them first (the learned seed).
+template <int MODE, bool FT_ONLY = false>
+__global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
+  const int64_t blk = xcd_block(g.nblocks);
+  if (blk < 0)
+    return;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= g.n)
+    return;
+  T3 f1, f0, v0, r0;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    f1.m[c] = g.fnew(c, i);
+    f0.m[c] = g.fN(c, i);
+    v0.m[c] = g.vN(c, i);
+    r0.m[c] = g.rN(c, i);
+  }
+  Corot k;
+  if (!corotational(f0, f1, v0, r0, g.dt, k)) {
+    raise_error(g.err, ERR_SINGULAR_COROT);
+    return;
+  }
+  double inc[6];
+  to_voigt(k.Dbar, inc);
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+    inc[c] *= g.dt;
+  if (g.incOut.p) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      g.incOut(c, i) = inc[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    g.fN(c, i) = f1.m[c];
+    g.vN(c, i) = k.V1.m[c];
+    g.rN(c, i) = k.R1.m[c];
+  }
+  if (FT_ONLY)
+    return;
+  double sOld[6], sNew[6], st[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    sOld[c] = g.sig(c, i);
+    st[c] = g.statev(c, i);
+  }
+  double eint = g.eint[i], einel = g.einel[i];
+  ReturnMapStats stats;
+  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
+  material_finish(g, i, f1, k.R1, sNew, st, eint, einel, stats);
+}
+
+// The return-map half of the split material update: reads the strain increments written by
+// k_material<…, true>, the committed F and R, and the per-point seed.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
+  const int64_t blk = xcd_block(g.nblocks);
+  if (blk < 0)
+    return;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= g.n)
+    return;
+  double inc[6], sOld[6], sNew[6], st[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    inc[c] = g.incOut(c, i);
+    sOld[c] = g.sig(c, i);
+    st[c] = g.statev(c, i);
+  }
+  double eint = g.eint[i], einel = g.einel[i];
+  ReturnMapStats stats;
+  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[i] : 0.0, g.rtol,
+                 &stats);
+  T3 f1, r1;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    f1.m[c] = g.fN(c, i);
+    r1.m[c] = g.rN(c, i);
+  }
+  material_finish(g, i, f1, r1, sNew, st, eint, einel, stats);
+}
+
 // Bond pass 2. PdCorrespondenceSkeleton.cpp:213-227: f_i = Σ_k (1 − d_ik)(A_i ξ − A_j(−ξ)) V_j
 // in CSR order (a broken bond contributes exactly zero, so it is skipped), then the
 // velocity-Verlet corrector (ExplicitTimeIntegrator.cpp:36-52) and the constraints at t;
@@ -331,6 +418,7 @@ __global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
 struct ForceArgs {
   int64_t n;
   int64_t nblocks;
+  Rounds rd;
   Planes x, u, v, a, amap, forceOut;
   const double *vol;
   Sell bonds;
@@ -340,14 +428,8 @@ struct ForceArgs {
   Ramp rampNow, rampNext;
 };
 
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true>
-__global__ void __launch_bounds__(256) k_force(ForceArgs g) {
-  const int64_t blk = xcd_block(g.nblocks);
-  if (blk < 0)
-    return;
-  const int64_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= g.n)
-    return;
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE>
+__device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i) {
   const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
   T3 ai;
 #pragma unroll
@@ -411,6 +493,18 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
   }
 }
 
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true>
+__global__ void __launch_bounds__(256) k_force(ForceArgs g) {
+  const int64_t blk = xcd_block(g.rd.blocksPerRound);
+  if (blk < 0)
+    return;
+  for (int r = 0; r < g.rd.rounds; ++r) {
+    const int64_t i = g.rd.particle(r, blk);
+    if (i >= g.rd.n)
+      break;
+    force_one<DAMAGE, FUSE_NEXT, INTEGRATE>(g, i);
+  }
+}
 
 // cauchyToFirstPiola · K⁻¹ for a caller-supplied spatial stress (PdCorrespondenceSkeleton.cpp:
 // 83-91, 209-210): A_i = (J σ F⁻ᵀ) K_i⁻¹.
@@ -572,18 +666,21 @@ __global__ void __launch_bounds__(256) k_damage_index(int64_t n, const double *_
 
 // The Johnson-Cook VUMAT on a caller's column-major device block ([k + nblock*c]),
 // JohnsonCook.f90:74-156, one lane per point. Unread VUMAT outputs are copied through as
-// the routine does (stretch, defgrad, field).
+// the routine does (stretch, defgrad, field). MODE selects the return-map solver; seed,
+// iterations and fallbacks are optional per-point arrays.
 struct VumatArgs {
   int64_t nblock;
   int nstatev, nfieldv;
   JcParams jc;
-  double dt;
+  double dt, rtol;
   const double *strainInc, *tempOld, *stretchOld, *defgradOld, *fieldOld, *stressOld, *stateOld,
-      *enerInternOld, *enerInelasOld;
+      *enerInternOld, *enerInelasOld, *seed;
   double *tempNew, *stretchNew, *defgradNew, *fieldNew, *stressNew, *stateNew, *enerInternNew,
       *enerInelasNew;
+  int *iterations, *fallbacks;
 };
 
+template <int MODE>
 __global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= g.nblock)
@@ -595,25 +692,35 @@ __global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
     inc[c] = g.strainInc[k + nb * c];
     sOld[c] = g.stressOld[k + nb * c];
     st[c] = g.stateOld[k + nb * c];
-    g.stretchNew[k + nb * c] = g.stretchOld[k + nb * c];
   }
+  if (g.stretchNew)
+    for (int c = 0; c < 6; ++c)
+      g.stretchNew[k + nb * c] = g.stretchOld[k + nb * c];
   for (int c = 6; c < g.nstatev; ++c)
     g.stateNew[k + nb * c] = g.stateOld[k + nb * c];
-#pragma unroll
-  for (int c = 0; c < 9; ++c)
-    g.defgradNew[k + nb * c] = g.defgradOld[k + nb * c];
-  for (int c = 0; c < g.nfieldv; ++c)
-    g.fieldNew[k + nb * c] = g.fieldOld[k + nb * c];
+  if (g.defgradNew)
+    for (int c = 0; c < 9; ++c)
+      g.defgradNew[k + nb * c] = g.defgradOld[k + nb * c];
+  if (g.fieldNew)
+    for (int c = 0; c < g.nfieldv; ++c)
+      g.fieldNew[k + nb * c] = g.fieldOld[k + nb * c];
   double eint = g.enerInternOld[k], einel = g.enerInelasOld[k];
-  jc_point(g.jc, g.dt, inc, sOld, st, eint, einel, sNew);
+  ReturnMapStats stats;
+  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[k] : 0.0, g.rtol,
+                 &stats);
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     g.stressNew[k + nb * c] = sNew[c];
     g.stateNew[k + nb * c] = st[c];
   }
-  g.tempNew[k] = st[2];
+  if (g.tempNew)
+    g.tempNew[k] = st[2];
   g.enerInternNew[k] = eint;
   g.enerInelasNew[k] = einel;
+  if (g.iterations)
+    g.iterations[k] = stats.iterations;
+  if (g.fallbacks)
+    g.fallbacks[k] = stats.fallbacks;
 }
 
 } // namespace vpd

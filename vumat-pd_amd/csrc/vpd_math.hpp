@@ -262,12 +262,49 @@ __device__ __forceinline__ double jc_yield(const JcParams &p, double ep, double 
          (1.0 + p.rateC * log(er / p.rate0));
 }
 
-// One material point of JohnsonCook.f90:74-156 in the exact bracketed-bisection form.
-// in: stressOld[6], strainInc[6], state[6] (in/out), energies (in/out). Writes
-// stressNew[6]; returns the iteration count (0 for the init branch).
+// Return-map solvers of the Johnson-Cook stage.
+//   RM_BISECTION  the exact bracketed bisection of JohnsonCook.f90:114-136 (parity mode):
+//                 tolerance E·1e-9, 50 iterations, iteration-1 seed = state 5.
+//   RM_BISECTION_TIGHT  the same scheme run to |r| <= rtol·q_trial (200 iterations): the
+//                 plain return map at the fast modes' accuracy.
+//   RM_NEWTON     safeguarded Newton on r(Δ) = (q − 3μΔ) − max(Y(ε̄+Δ, Δ/dt, T), Y_old)
+//                 inside the bracket [0, q/3μ], seeded with state 5 (the reference's seed),
+//                 falling back to bisection whenever a step leaves the bracket; stops at
+//                 |r| <= rtol·q_trial.
+//   RM_LEARNED    RM_NEWTON seeded with an external Δε̄ᵖ estimate (the learned warm start).
+enum ReturnMap { RM_BISECTION = 0, RM_BISECTION_TIGHT = 1, RM_NEWTON = 2, RM_LEARNED = 3 };
+
+struct ReturnMapStats {
+  int iterations = 0;
+  int fallbacks = 0; // Newton steps replaced by bisection
+};
+
+// Y and dY/dΔ at Δ > 0 (JohnsonCook.f90:160-170 differentiated in Δ).
+__device__ __forceinline__ void jc_yield_slope(const JcParams &p, double epOld, double dep,
+                                               double dt, double temp, double &Y, double &dY) {
+  double theta = (temp - p.tref) / (p.tmelt - p.tref);
+  theta = fmin(1.0, fmax(0.0, theta));
+  const double soft = 1.0 - pow(theta, p.thermalM);
+  const double ep = epOld + dep;
+  const double epn = pow(ep, p.hardN);
+  const double hard = p.a0 + p.b0 * epn;
+  const double rate = dep / dt;
+  const double er = rate == 0.0 ? p.rate0 : rate;
+  const double rfac = 1.0 + p.rateC * log(er / p.rate0);
+  Y = hard * soft * rfac;
+  const double dHard = ep > 0.0 ? p.b0 * p.hardN * epn / ep : 0.0;
+  dY = soft * (dHard * rfac + (dep > 0.0 ? hard * p.rateC / dep : 0.0));
+}
+
+// One material point of JohnsonCook.f90:74-156. in: stressOld[6], strainInc[6], state[6]
+// (in/out), energies (in/out); writes stressNew[6]; returns the iteration count (0 for the
+// init branch). The bookkeeping after the solve is the reference's for every solver.
+template <int MODE>
 __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const double inc[6],
                                         const double stressOld[6], double state[6],
-                                        double &eint, double &einel, double stressNew[6]) {
+                                        double &eint, double &einel, double stressNew[6],
+                                        double seed = 0.0, double rtol = 1.0e-12,
+                                        ReturnMapStats *stats = nullptr) {
   double trial[6];
   const double traceInc = inc[0] + inc[1] + inc[2];
 #pragma unroll
@@ -302,28 +339,73 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
       sqrt(1.5 * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
                   2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5])));
 
-  double epInc = 0.0, epLower = 0.0, epUpper = qTrial / (2.0 * p.mu);
-  double yield = oldYield;
+  double epInc = 0.0, yield = oldYield;
   int iteration = 0;
-  while (iteration < 50) { // JohnsonCook.f90:119-136
-    iteration = iteration + 1;
-    const double epRate = epInc / dt;
-    yield = jc_yield(p, epOld + epInc, epRate, temperature);
-    yield = fmax(yield, oldYield);
-    const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
-    const double residual = qNew - yield;
-    if (fabs(residual) < p.tolerance)
-      break;
-    if (epInc == 0.0 && residual < 0.0)
-      break;
-    if (residual >= 0.0)
-      epLower = fmax(epLower, epInc);
-    else
-      epUpper = fmin(epUpper, epInc);
-    epInc = 0.5 * (epLower + epUpper);
-    if (iteration == 1)
-      epInc = fmin(fmax(state[4], epLower), epUpper);
+  if (MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT) {
+    const double tol = MODE == RM_BISECTION ? p.tolerance : rtol * qTrial;
+    const int maxIt = MODE == RM_BISECTION ? 50 : 200;
+    double epLower = 0.0, epUpper = qTrial / (2.0 * p.mu);
+    while (iteration < maxIt) { // JohnsonCook.f90:119-136
+      iteration = iteration + 1;
+      const double epRate = epInc / dt;
+      yield = jc_yield(p, epOld + epInc, epRate, temperature);
+      yield = fmax(yield, oldYield);
+      const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
+      const double residual = qNew - yield;
+      if (fabs(residual) < tol)
+        break;
+      if (epInc == 0.0 && residual < 0.0)
+        break;
+      if (residual >= 0.0)
+        epLower = fmax(epLower, epInc);
+      else
+        epUpper = fmin(epUpper, epInc);
+      epInc = 0.5 * (epLower + epUpper);
+      if (iteration == 1)
+        epInc = fmin(fmax(state[4], epLower), epUpper);
+    }
+  } else {
+    // Iteration 1 tests Δ = 0 exactly as the reference does (elastic exit).
+    iteration = 1;
+    yield = fmax(jc_yield(p, epOld, 0.0, temperature), oldYield);
+    const double r0 = fmax(0.0, qTrial) - yield;
+    if (r0 > 0.0 && fabs(r0) >= rtol * qTrial) {
+      double lo = 0.0, hi = qTrial / (3.0 * p.mu);
+      double d = MODE == RM_LEARNED ? seed : state[4];
+      if (!(d > lo && d < hi))
+        d = 0.5 * (lo + hi);
+      int fallbacks = 0;
+      for (int it = 0; it < 60; ++it) {
+        ++iteration;
+        double Y, dY;
+        jc_yield_slope(p, epOld, d, dt, temperature, Y, dY);
+        const bool capped = Y < oldYield;
+        const double Yc = capped ? oldYield : Y;
+        const double r = (qTrial - 3.0 * p.mu * d) - Yc;
+        yield = Yc;
+        epInc = d;
+        if (fabs(r) <= rtol * qTrial)
+          break;
+        if (r > 0.0)
+          lo = d;
+        else
+          hi = d;
+        const double slope = -3.0 * p.mu - (capped ? 0.0 : dY);
+        double next = d - r / slope;
+        if (!(next > lo && next < hi)) {
+          next = 0.5 * (lo + hi);
+          ++fallbacks;
+        }
+        if (next == d)
+          break;
+        d = next;
+      }
+      if (stats)
+        stats->fallbacks += fallbacks;
+    }
   }
+  if (stats)
+    stats->iterations += iteration;
 
   double upd[6];
 #pragma unroll

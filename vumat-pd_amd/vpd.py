@@ -23,9 +23,10 @@ _U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 FIELD = dict(x=0, volume=1, u=2, v=3, a=4, fN=5, vN=6, rN=7, stressN=8, statev=9, temp=10,
              eint=11, einel=12, sigma=13, strain=14, force=15, damage=16, damage_index=17,
-             kinv=18, F=19, iterations=20)
+             kinv=18, F=19, iterations=20, strain_inc=21, fallbacks=22)
 DAMAGE_NONE, DAMAGE_CRITICAL_STRETCH = 0, 1
-RETURN_MAP_BISECTION = 0
+RETURN_MAP_BISECTION, RETURN_MAP_BISECTION_TIGHT, RETURN_MAP_NEWTON, RETURN_MAP_LEARNED = 0, 1, 2, 3
+MLP_PARAMS, MLP_NORM = 8961, 18
 
 
 class VpdError(RuntimeError):
@@ -89,6 +90,9 @@ def lib():
         L.vpd_compute_force_and_correct.argtypes = [P, C.c_double, _D]
         L.vpd_step.argtypes = [P, C.POINTER(StepArgs)]
         L.vpd_set_timing.argtypes = [P, C.c_int]
+        L.vpd_set_tuning.argtypes = [P, C.c_int, C.c_int]
+        L.vpd_get_tuning.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
+        L.vpd_autotune.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
         L.vpd_collect_timing.argtypes = [P, C.POINTER(KernelTime), C.c_int,
                                          C.POINTER(C.c_int)]
         L.vpd_make_patch.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _D, _D]
@@ -97,6 +101,25 @@ def lib():
                                                     C.POINTER(C.c_int64)]
         L.vpd_smooth_ramp.argtypes = [C.c_double, C.c_double, C.c_double, _D]
         L.vpd_smooth_ramp.restype = None
+        L.vpd_set_return_map.argtypes = [P, C.c_int, C.c_double]
+        L.vpd_attach_warm_start.argtypes = [P, P]
+        _F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.vpd_mlp_create.argtypes = [C.c_int, C.POINTER(P)]
+        L.vpd_mlp_destroy.argtypes = [P]
+        L.vpd_mlp_destroy.restype = None
+        L.vpd_mlp_last_error.argtypes = [P]
+        L.vpd_mlp_last_error.restype = C.c_char_p
+        L.vpd_mlp_set_params.argtypes = [P, C.c_void_p, C.c_void_p]
+        L.vpd_mlp_get_params.argtypes = [P, _F, _F]
+        L.vpd_mlp_loss_and_gradient.argtypes = [P, _F, _F, C.c_int, C.POINTER(C.c_float), _F]
+        L.vpd_mlp_set_dataset.argtypes = [P, _F, _F, C.c_int64, C.c_uint64]
+        L.vpd_mlp_train.argtypes = [P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
+        L.vpd_mlp_predict.argtypes = [P, _F, C.c_int64, _F]
+        L.vpd_mlp_seed_device.argtypes = [P, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                          C.c_int64, C.c_void_p, C.POINTER(C.c_double)]
+        L.vpd_return_map_device.argtypes = ([C.c_int, C.c_int64, C.c_int, _D, C.c_int, C.c_double,
+                                             C.c_double] + [C.c_void_p] * 13 +
+                                            [C.POINTER(C.c_double)])
         L.vpd_device_malloc.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
         L.vpd_device_free.argtypes = [C.c_void_p]
         L.vpd_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
@@ -240,8 +263,10 @@ class Context:
     def _shape(self, name):
         if name == "damage":
             return (self.nnz,), np.float64
-        if name == "iterations":
+        if name in ("iterations", "fallbacks"):
             return (self.n,), np.int32
+        if name == "strain_inc":
+            return (self.n, 6), np.float64
         w = {"x": 3, "u": 3, "v": 3, "a": 3, "force": 3, "volume": 1, "temp": 1, "eint": 1,
              "einel": 1, "damage_index": 1, "statev": self.nstatev}.get(name, 9)
         return ((self.n,) if w == 1 else (self.n, w)), np.float64
@@ -284,6 +309,31 @@ class Context:
 
     def synchronize(self):
         self._check(lib().vpd_synchronize(self.h))
+
+    def set_return_map(self, mode, rtol=1e-12, warm_start=None):
+        self._check(lib().vpd_set_return_map(self.h, mode, rtol))
+        if warm_start is not None:
+            self._warm = warm_start  # keep the network alive while attached
+            self._check(lib().vpd_attach_warm_start(self.h, warm_start.h))
+
+    TUNE_KEYS = ("defgrad_lds", "force_lds", "defgrad_rounds", "force_rounds")
+
+    def autotune(self, reps=3):
+        out = (C.c_int * 4)()
+        self._check(lib().vpd_autotune(self.h, reps, out))
+        return dict(zip(self.TUNE_KEYS, list(out)))
+
+    def set_tuning(self, **knobs):
+        for k, v in knobs.items():
+            self._check(lib().vpd_set_tuning(self.h, self.TUNE_KEYS.index(k), int(v)))
+
+    def get_tuning(self):
+        out = {}
+        for i, k in enumerate(self.TUNE_KEYS):
+            v = C.c_int()
+            self._check(lib().vpd_get_tuning(self.h, i, C.byref(v)))
+            out[k] = v.value
+        return out
 
     def set_timing(self, on=True):
         self._check(lib().vpd_set_timing(self.h, int(on)))
@@ -350,6 +400,90 @@ def vumat_device(props, dt, strain_inc, stress_old, state_old, temp_old, eint_ol
     return {k: v.read() for k, v in out.items()}
 
 
+class WarmStart:
+    """The return-map warm-start network (vpd_mlp_*): FP32 training on the GPU, bf16 MFMA
+    inference. theta has the 8961-value layout W1, b1, W2, b2, W3, b3, W4, b4."""
+
+    def __init__(self, max_batch=4096):
+        self.h = C.c_void_p()
+        self.max_batch = max_batch
+        if lib().vpd_mlp_create(max_batch, C.byref(self.h)):
+            raise VpdError(lib().vpd_mlp_last_error(None).decode())
+
+    def _check(self, rc):
+        if rc:
+            raise VpdError(lib().vpd_mlp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().vpd_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, theta=None, norm=None):
+        t = None if theta is None else np.ascontiguousarray(theta, np.float32)
+        nm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        self._check(lib().vpd_mlp_set_params(
+            self.h, None if t is None else t.ctypes.data_as(C.c_void_p),
+            None if nm is None else nm.ctypes.data_as(C.c_void_p)))
+
+    def get_params(self):
+        theta = np.zeros(MLP_PARAMS, np.float32)
+        norm = np.zeros(MLP_NORM, np.float32)
+        self._check(lib().vpd_mlp_get_params(self.h, theta, norm))
+        return theta, norm
+
+    def loss_and_gradient(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        self._check(lib().vpd_mlp_loss_and_gradient(self.h, x, t, len(t), C.byref(loss), g))
+        return loss.value, g
+
+    def set_dataset(self, x, t, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        self._check(lib().vpd_mlp_set_dataset(self.h, x, t, len(t), seed))
+
+    def train(self, steps, batch=4096, lr=1e-3):
+        loss = C.c_float()
+        self._check(lib().vpd_mlp_train(self.h, steps, batch, lr, C.byref(loss)))
+        return loss.value
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(len(x), np.float32)
+        self._check(lib().vpd_mlp_predict(self.h, x, len(x), y))
+        return y
+
+
+def return_map(mode, props, dt, strain_inc, stress_old, state_old, eint_old, einel_old,
+               seed=None, rtol=1e-12):
+    """vpd_return_map_device on device copies of column-major blocks (6, nb) / (nstatev, nb).
+    Returns outputs, per-point iterations and fallbacks, and the kernel time in ms."""
+    nb = strain_inc.shape[1]
+    nst = state_old.shape[0]
+    D = DeviceBuffer
+    ins = [D(strain_inc), D(stress_old), D(state_old), D(eint_old), D(einel_old)]
+    sd = D(seed) if seed is not None else None
+    outs = [D(np.zeros((6, nb))), D(np.zeros((nst, nb))), D(np.zeros(nb)), D(np.zeros(nb))]
+    it = D(np.zeros(nb))  # 8 bytes per point: room for the int32 counts
+    fb = D(np.zeros(nb))
+    ms = C.c_double()
+    rc = lib().vpd_return_map_device(mode, nb, nst, _f64(props), len(props), dt, rtol,
+                                     *[b.ptr for b in ins], sd.ptr if sd else None,
+                                     *[b.ptr for b in outs], it.ptr, fb.ptr, C.byref(ms))
+    if rc:
+        raise VpdError(lib().vpd_last_error(None).decode())
+    iters = it.read().view(np.int32)[:nb]
+    fbs = fb.read().view(np.int32)[:nb]
+    return dict(stress=outs[0].read(), state=outs[1].read(), eint=outs[2].read(),
+                einel=outs[3].read(), iterations=iters, fallbacks=fbs, ms=ms.value)
+
+
 # ------------------------------------------------------------------ a bar case
 class Bar:
     """The reference's regional-constraint bar (BoundaryValueCaseRunner.cpp:142-229) on the GPU:
@@ -361,10 +495,11 @@ class Bar:
                  rise_time=1.0, component=0, region_layers=3, plane_strain=False, nstatev=6,
                  tref=293.0, initial_state=None, damage=None, velocity=None,
                  material_density=7850.0, return_map=RETURN_MAP_BISECTION, rowptr=None,
-                 col=None):
+                 col=None, rtol=1e-12, warm_start=None):
         self.dt, self.amplitude, self.rise_time, self.velocity = dt, amplitude, rise_time, velocity
         self.ctx = Context(horizon, density, props, nstatev, initial_state, tref, spacing,
-                           material_density, damage, return_map)
+                           material_density, damage)
+        self.ctx.set_return_map(return_map, rtol, warm_start)
         self.x, self.vol = make_patch(nx, ny, nz, spacing)
         self.n = self.x.shape[0]
         self.ctx.set_particles(self.x, self.vol)

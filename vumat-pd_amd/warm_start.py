@@ -1,0 +1,136 @@
+"""Training pipeline of the learned return-map warm start (no reference counterpart).
+
+1. Log (input, Δε̄ᵖ) pairs from seed-fixed plain GPU runs of the 30×8×8 JC tensile bar
+   (Δx = 1 mm, δ = 3.015Δx, dt = 0.5·CFL, end velocity drawn per run from a seeded
+   generator): for every plastic point of every step, the network input is the strain
+   increment the return map received (6 Voigt components), the equivalent plastic strain and
+   the temperature before the update, and the target is the converged Δε̄ᵖ.
+2. Fit the input/output normalisation, then train the 8 → 64 → 64 → 64 → 1 network on the GPU
+   with the project's FP32 forward/backward kernels and Adam.
+3. Save θ and the normalisation (an .npz of FP32 arrays).
+
+    python vumat-pd_amd/warm_start.py --pairs 2000000 --out vumat-pd_amd/warm_start_jc.npz
+"""
+from __future__ import annotations
+
+import argparse
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import vpd  # noqa: E402
+
+JC_PROPS = [200000.0, 0.3, 350.0, 275.0, 0.022, 0.36, 1.0, 1.0e-3, 293.0, 1793.0, 7850.0, 0.9,
+            500.0, 1.0]
+RHO = 7.85e-3
+
+
+def cfl_dt(h, fraction=0.5, props=JC_PROPS, rho=RHO):
+    E, nu = props[0], props[1]
+    return fraction * h / math.sqrt(E * (1 - nu) / (rho * (1 + nu) * (1 - 2 * nu)))
+
+
+def initial_params(seed=0):
+    """Glorot-uniform weights, zero biases, in the flat θ layout."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((8, 64), (64, 64), (64, 64), (64, 1)):
+        lim = math.sqrt(6.0 / (fan_in + fan_out))
+        parts.append(rng.uniform(-lim, lim, fan_out * fan_in))
+        parts.append(np.zeros(fan_out))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def log_pairs(target_pairs, seed=1234, steps=2000, nx=30, ny=8, nz=8, spacing=1.0,
+              return_map=vpd.RETURN_MAP_BISECTION, max_runs=64, verbose=True):
+    """Runs bars with end velocities drawn from U[0.5, 4] mm/ms until target_pairs plastic
+    (input, Δε̄ᵖ) pairs are logged. Returns x (m, 8) raw inputs and y (m,) targets."""
+    rng = np.random.default_rng(seed)
+    xs, ys, total = [], [], 0
+    dt = cfl_dt(spacing)
+    for run in range(max_runs):
+        v = float(rng.uniform(0.5, 4.0))
+        bar = vpd.Bar(nx, ny, nz, spacing, 3.015 * spacing, JC_PROPS, dt, RHO, velocity=v,
+                      return_map=return_map)
+        for _ in range(steps):
+            before = bar.get("statev")
+            bar.step(1, fuse=False, want_fields=True)
+            after = bar.get("statev")
+            plastic = after[:, 4] > 0.0
+            if plastic.any():
+                inc = bar.get("strain_inc")[plastic]
+                x = np.concatenate([inc, before[plastic, 1:2], before[plastic, 2:3]], 1)
+                xs.append(x)
+                ys.append(after[plastic, 4])
+                total += int(plastic.sum())
+        if verbose:
+            print("[warm-start] run %d v=%.3f mm/ms: %d pairs so far" % (run, v, total),
+                  flush=True)
+        bar.ctx.close()
+        if total >= target_pairs:
+            break
+    x = np.concatenate(xs)[:target_pairs]
+    y = np.concatenate(ys)[:target_pairs]
+    return x, y
+
+
+def normalisation(x, y):
+    """Input mean/std per feature (std floored) and output mean/std: 18 FP32 values."""
+    mx, sx = x.mean(0), x.std(0)
+    sx = np.where(sx > 0, sx, 1.0)
+    return np.concatenate([mx, sx, [y.mean(), max(y.std(), 1e-30)]]).astype(np.float32)
+
+
+def train(x, y, steps=20000, batch=4096, lr=1e-3, seed=7, verbose=True):
+    norm = normalisation(x, y)
+    xn = ((x - norm[:8]) / norm[8:16]).astype(np.float32)
+    yn = ((y - norm[16]) / norm[17]).astype(np.float32)
+    net = vpd.WarmStart(batch)
+    net.set_params(initial_params(seed), norm)
+    net.set_dataset(xn, yn, seed=seed)
+    t0 = time.perf_counter()
+    chunk = max(1, steps // 10)
+    loss = float("nan")
+    for s in range(0, steps, chunk):
+        # Step decay of the learning rate over the last half of training.
+        rate = lr if s < steps // 2 else lr * 0.1
+        loss = net.train(min(chunk, steps - s), batch, rate)
+        if verbose:
+            print("[warm-start] step %d loss %.4e" % (s + chunk, loss), flush=True)
+    if verbose:
+        print("[warm-start] trained %d steps in %.1f s" % (steps, time.perf_counter() - t0))
+    return net, norm, loss
+
+
+def save(path, net):
+    theta, norm = net.get_params()
+    np.savez(path, theta=theta, norm=norm)
+
+
+def load(path, max_batch=4096):
+    data = np.load(path)
+    net = vpd.WarmStart(max_batch)
+    net.set_params(data["theta"], data["norm"])
+    return net
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pairs", type=int, default=2000000)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--out", default=os.path.join(HERE, "warm_start_jc.npz"))
+    ap.add_argument("--seed", type=int, default=1234)
+    a = ap.parse_args()
+    x, y = log_pairs(a.pairs, seed=a.seed)
+    net, norm, loss = train(x, y, steps=a.steps)
+    save(a.out, net)
+    print("[warm-start] saved %s (final batch loss %.4e, %d pairs)" % (a.out, loss, len(y)))
+
+
+if __name__ == "__main__":
+    main()
