@@ -267,6 +267,28 @@ def test_fused_equals_unfused(vpd, orc):
         assert np.array_equal(a.get(name), b.get(name)), name
 
 
+@pytest.mark.parametrize("knobs", [
+    dict(defgrad_pipeline=1, force_pipeline=1),
+    dict(defgrad_pipeline=1, force_pipeline=1, defgrad_rounds=3, force_rounds=5),
+    dict(defgrad_lds=36408, force_lds=36408, defgrad_rounds=2, force_rounds=2),
+])
+def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
+    # Launch knobs change how the bond passes are scheduled, never what they sum or in which
+    # order: every field equals the default launch's bit for bit, damage included.
+    banner("launch tuning %s" % knobs)
+    kw = dict(nx=40, ny=10, nz=10, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
+              density=7.85e-3, velocity=400.0, damage=(0.002, 0.0))
+    a = vpd.Bar(**kw)
+    b = vpd.Bar(**kw)
+    b.ctx.set_tuning(**knobs)
+    assert all(b.ctx.get_tuning()[k] == v for k, v in knobs.items())
+    a.step(60)
+    b.step(60)
+    assert a.get("damage").sum() > 0
+    for name in ["u", "v", "a", "stressN", "statev", "damage"]:
+        assert np.array_equal(a.get(name), b.get(name)), name
+
+
 def test_full_size_balance_and_determinism(vpd):
     # 500k-particle bar, damage on: Σ f_i V_i = 0 up to rounding (pairwise cancellation),
     # two runs bit-identical, state finite.
@@ -285,3 +307,30 @@ def test_full_size_balance_and_determinism(vpd):
     b = vpd.Bar(**kw)
     b.step(20)
     assert np.array_equal(b.get("u"), u1)
+
+
+@pytest.mark.parametrize("failure", [0.0, 0.01, 1.5])
+def test_damage_near_threshold_bitwise(vpd, orc, failure):
+    # Bonds stretched to within 1e-13 … 1e-6 of the critical ratio (and of the compression
+    # threshold) on both sides: the GPU's filtered test must break exactly the oracle's set.
+    banner(f"damage near threshold failure={failure}")
+    rng = np.random.default_rng(21)
+    x, vol = orc.make_patch(10, 6, 5, 1.0)
+    rp, col = orc.build_neighbors(x, 1.5)
+    n = len(x)
+    sc = 0.02
+    u = np.zeros((n, 3))
+    # Give every particle a displacement that scales its bonds to (1+sc)(1+η) along x.
+    eta = rng.choice([-1e-6, -1e-10, -1e-13, 0.0, 1e-13, 1e-10, 1e-6], n)
+    u[:, 0] = x[:, 0] * sc * (1 + eta)
+    u[:, 1] = -x[:, 1] * failure * (1 + eta) * 0.5
+    c = ctx_for(vpd, x, vol, horizon=1.5, damage=(sc, failure), rowptr=rp, col=col)
+    c.initialize_shape()
+    c.initialize_dynamics()
+    c.upload("u", u)
+    c.compute_deformation_gradient()
+    got = c.download("damage")
+    ref = np.zeros(len(col))
+    orc.damage_update(x, u, vol, rp, col, sc, failure, ref)
+    assert np.array_equal(got, ref)
+    assert 0 < got.sum() < len(col)

@@ -98,6 +98,7 @@ struct vpd_context {
   int ldsDefgrad = 0, ldsForce = 0;
   // Sweeps per bond pass (see Rounds in vpd_kernels.hpp). Chosen by vpd_autotune.
   int roundsDefgrad = 1, roundsForce = 1;
+  int pipeDefgrad = 0, pipeForce = 0;
   // Return map of the material stage; the learned mode adds the network, the strain
   // increments it reads and the seeds it writes.
   double rtol = 1.0e-12;
@@ -431,25 +432,31 @@ int launch_predict(vpd_context *c, double dt, const double ramp[3]) {
   });
 }
 
-int launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, int *err = nullptr) {
-  (void)err;
+template <bool DAMAGE, bool PIPE>
+void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression) {
+  k_defgrad<DAMAGE, PIPE><<<grid_rounds(rd), 256, lds, c->stream>>>(
+      rd, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c), planes(c->kinv, c->n),
+      planes(c->fnew, c->n), c->broken, c->initiation, c->failure, useCompression);
+}
+
+void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, bool pipe) {
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
   const Rounds rd = rounds_of(c->n, rounds);
-  if (dmg)
-    k_defgrad<true><<<grid_rounds(rd), 256, lds, c->stream>>>(
-        rd, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c), planes(c->kinv, c->n),
-        planes(c->fnew, c->n), c->broken, c->initiation, c->failure, useCompression);
+  if (dmg && pipe)
+    launch_defgrad_t<true, true>(c, lds, rd, useCompression);
+  else if (dmg)
+    launch_defgrad_t<true, false>(c, lds, rd, useCompression);
+  else if (pipe)
+    launch_defgrad_t<false, true>(c, lds, rd, useCompression);
   else
-    k_defgrad<false><<<grid_rounds(rd), 256, lds, c->stream>>>(
-        rd, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c), planes(c->kinv, c->n),
-        planes(c->fnew, c->n), c->broken, c->initiation, c->failure, useCompression);
-  return VPD_OK;
+    launch_defgrad_t<false, false>(c, lds, rd, useCompression);
 }
 
 int launch_defgrad(vpd_context *c) {
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
-  return timed(c, K_DEFGRAD,
-               [&] { launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad); });
+  return timed(c, K_DEFGRAD, [&] {
+    launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad, c->pipeDefgrad != 0);
+  });
 }
 
 int launch_material(vpd_context *c, double dt, bool fields) {
@@ -505,6 +512,35 @@ int launch_material(vpd_context *c, double dt, bool fields) {
   }
 }
 
+template <bool DAMAGE, bool FUSE, bool INTEGRATE>
+void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, bool pipe) {
+  if (pipe)
+    k_force<DAMAGE, FUSE, INTEGRATE, true><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+  else
+    k_force<DAMAGE, FUSE, INTEGRATE, false><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+}
+
+// The force pass in one of its forms: with or without damage flags, the next step's predictor
+// fused in, the integrator (INTEGRATE false leaves only the force density), the pipelined
+// bond loop.
+void launch_force_with(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, bool integrate,
+                       int lds, bool pipe) {
+  if (!integrate) {
+    if (dmg)
+      launch_force_t<true, false, false>(c, g, lds, pipe);
+    else
+      launch_force_t<false, false, false>(c, g, lds, pipe);
+  } else if (dmg && fuse) {
+    launch_force_t<true, true, true>(c, g, lds, pipe);
+  } else if (dmg) {
+    launch_force_t<true, false, true>(c, g, lds, pipe);
+  } else if (fuse) {
+    launch_force_t<false, true, true>(c, g, lds, pipe);
+  } else {
+    launch_force_t<false, false, true>(c, g, lds, pipe);
+  }
+}
+
 int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
                  const double *rampNext, bool fields) {
   ForceArgs g;
@@ -526,16 +562,8 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.rampNext = fuse ? ramp_of(rampNext) : Ramp{0, 0, 0};
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   g.rd = rounds_of(c->n, c->roundsForce);
-  const int grid = grid_rounds(g.rd);
   return timed(c, K_FORCE, [&] {
-    if (dmg && fuse)
-      k_force<true, true><<<grid, 256, c->ldsForce, c->stream>>>(g);
-    else if (dmg)
-      k_force<true, false><<<grid, 256, c->ldsForce, c->stream>>>(g);
-    else if (fuse)
-      k_force<false, true><<<grid, 256, c->ldsForce, c->stream>>>(g);
-    else
-      k_force<false, false><<<grid, 256, c->ldsForce, c->stream>>>(g);
+    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, c->pipeForce != 0);
   });
 }
 
@@ -1200,10 +1228,7 @@ int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *forc
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   g.rd = rounds_of(c->n, c->roundsForce);
   if (int s = timed(c, K_FORCE, [&] {
-        if (dmg)
-          k_force<true, false, false><<<grid_rounds(g.rd), 256, 0, c->stream>>>(g);
-        else
-          k_force<false, false, false><<<grid_rounds(g.rd), 256, 0, c->stream>>>(g);
+        launch_force_with(c, g, dmg, false, false, c->ldsForce, c->pipeForce != 0);
       }))
     return s;
   c->haveForce = true;
@@ -1356,6 +1381,10 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
     if (value < 1 || value > 1024)
       return fail(c, VPD_ERR_ARGUMENT, "Sweeps per bond pass must lie in [1, 1024].");
     (knob == VPD_TUNE_DEFGRAD_ROUNDS ? c->roundsDefgrad : c->roundsForce) = value;
+  } else if (knob == VPD_TUNE_DEFGRAD_PIPELINE || knob == VPD_TUNE_FORCE_PIPELINE) {
+    if (value != 0 && value != 1)
+      return fail(c, VPD_ERR_ARGUMENT, "Bond-loop pipelining is 0 (off) or 1 (on).");
+    (knob == VPD_TUNE_DEFGRAD_PIPELINE ? c->pipeDefgrad : c->pipeForce) = value;
   } else {
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
   }
@@ -1378,13 +1407,19 @@ int vpd_get_tuning(vpd_context *c, int knob, int *value) {
   case VPD_TUNE_FORCE_ROUNDS:
     *value = c->roundsForce;
     return VPD_OK;
+  case VPD_TUNE_DEFGRAD_PIPELINE:
+    *value = c->pipeDefgrad;
+    return VPD_OK;
+  case VPD_TUNE_FORCE_PIPELINE:
+    *value = c->pipeForce;
+    return VPD_OK;
   default:
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
   }
 }
 
-// Times both bond passes over a grid of (sweeps per pass) × (workgroups per CU) and keeps
-// the fastest for each. The timing runs use the passes' side-effect-free forms: F into its
+// Times both bond passes over a grid of (bond-loop pipelining) × (sweeps per pass) ×
+// (workgroups per CU) and keeps the fastest for each. The timing runs use the passes' side-effect-free forms: F into its
 // scratch plane, and the force gather without the integrator.
 int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   if (int s = require_ready(c))
@@ -1409,7 +1444,8 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   g.broken = c->broken;
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   float best[2] = {1e30f, 1e30f};
-  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1};
+  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1}, pickPipe[2] = {0, 0};
+  for (int pipe = 0; pipe < 2; ++pipe)
   for (int lds : ldsChoices)
     for (int rounds : roundChoices) {
       if (rounds > 1 && rounds > ceil_div(c->n, 256))
@@ -1420,11 +1456,9 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
         for (int r = 0; r <= reps; ++r) {
           HIPTRY(c, hipEventRecord(a, c->stream));
           if (pass == 0)
-            launch_defgrad_with(c, false, lds, rounds);
-          else if (dmg)
-            k_force<true, false, false><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+            launch_defgrad_with(c, false, lds, rounds, pipe != 0);
           else
-            k_force<false, false, false><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+            launch_force_with(c, g, dmg, false, false, lds, pipe != 0);
           HIPTRY(c, hipGetLastError());
           HIPTRY(c, hipEventRecord(b, c->stream));
           HIPTRY(c, hipEventSynchronize(b));
@@ -1437,6 +1471,7 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
           best[pass] = total;
           pickLds[pass] = lds;
           pickRounds[pass] = rounds;
+          pickPipe[pass] = pipe;
         }
       }
     }
@@ -1446,12 +1481,16 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   c->roundsDefgrad = pickRounds[0];
   c->ldsForce = pickLds[1];
   c->roundsForce = pickRounds[1];
+  c->pipeDefgrad = pickPipe[0];
+  c->pipeForce = pickPipe[1];
   c->haveFnew = false; // F scratch now holds the tuning runs' values
   if (tuned) {
     tuned[VPD_TUNE_DEFGRAD_LDS] = c->ldsDefgrad;
     tuned[VPD_TUNE_FORCE_LDS] = c->ldsForce;
     tuned[VPD_TUNE_DEFGRAD_ROUNDS] = c->roundsDefgrad;
     tuned[VPD_TUNE_FORCE_ROUNDS] = c->roundsForce;
+    tuned[VPD_TUNE_DEFGRAD_PIPELINE] = c->pipeDefgrad;
+    tuned[VPD_TUNE_FORCE_PIPELINE] = c->pipeForce;
   }
   return check_device(c);
 }

@@ -96,6 +96,55 @@ __device__ __forceinline__ void constrain(uint8_t code, const Ramp &r, double u[
   }
 }
 
+
+// Critical-stretch test of PdSimpleDamage.cpp:72-77 with a cheap exact filter.
+// The reference decides stretch = (‖y‖ − ‖ξ‖)/‖ξ‖ > s_c (or < −f), with two square roots and a
+// division per bond. With a = ‖ξ‖², b = ‖y‖² (the same sums the reference takes the roots
+// of), that test lands on the same side as b ⋛ a(1+s_c)² whenever b/a is more than 1e-9
+// away from the threshold ratio: the rounded stretch differs from the exact one by a few
+// ulps (≤ 1e-15 relative), far inside that margin. Only bonds within the margin, and the
+// degenerate a = 0, take the reference's exact expression.
+struct StretchTest {
+  double tLo, tHi; // (1+s_c)² (1 ∓ 1e-9)
+  double cLo, cHi; // (1−f)² (1 ∓ 1e-9), used when the compression test is armed and f < 1
+  double initiation, failure;
+  int compression;
+};
+
+__host__ __device__ inline StretchTest make_stretch_test(double initiation, double failure,
+                                                         int useCompression) {
+  StretchTest t;
+  const double up = (1.0 + initiation) * (1.0 + initiation);
+  t.tLo = up * (1.0 - 1e-9);
+  t.tHi = up * (1.0 + 1e-9);
+  const double dn = (1.0 - failure) * (1.0 - failure);
+  t.cLo = dn * (1.0 - 1e-9);
+  t.cHi = dn * (1.0 + 1e-9);
+  t.initiation = initiation;
+  t.failure = failure;
+  t.compression = useCompression ? (failure < 1.0 ? 1 : 2) : 0; // 2: armed but unreachable
+  return t;
+}
+
+__device__ __forceinline__ bool stretch_breaks(const StretchTest &t, const double xi[3],
+                                               const double yij[3]) {
+  const double a = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+  const double b = yij[0] * yij[0] + yij[1] * yij[1] + yij[2] * yij[2];
+  if (a > 0.0) {
+    const bool tensionDecided = b <= a * t.tLo || b >= a * t.tHi;
+    const bool compressionDecided =
+        t.compression != 1 || b >= a * t.cHi || b <= a * t.cLo;
+    if (tensionDecided && compressionDecided) {
+      if (b >= a * t.tHi)
+        return true;
+      return t.compression == 1 && b <= a * t.cLo;
+    }
+  }
+  const double initial = sqrt(a);
+  const double stretch = (sqrt(b) - initial) / initial;
+  return stretch > t.initiation || (t.compression != 0 && stretch < -t.failure);
+}
+
 // ExplicitTimeIntegrator.cpp:23-34 then the constraints at t (BoundaryConditions.cpp:81).
 __global__ void __launch_bounds__(256) k_predict(int64_t n, Planes u, Planes v, Planes a,
                                                  const uint8_t *__restrict__ codes, double dt,
@@ -186,12 +235,49 @@ __global__ void __launch_bounds__(256) k_shape(int64_t n, Planes x, const double
 // and, with DAMAGE, PdSimpleDamage.cpp:63-84 on the same bond and the same predicted
 // positions: a bond breaks (irreversibly) when its stretch exceeds `initiation` or, with a
 // compression threshold armed, falls below −failure. Damage does not enter F (:181-196).
+// Neighbour data one F-pass bond needs: X_j, u_j, V_j.
+struct DefgradBond {
+  double x[3], u[3], vol;
+};
+
+__device__ __forceinline__ void load_defgrad_bond(Planes x, Planes u,
+                                                  const double *__restrict__ vol, int j,
+                                                  DefgradBond &b) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    b.x[d] = x(d, j);
+    b.u[d] = u(d, j);
+  }
+  b.vol = vol[j];
+}
+
 template <bool DAMAGE>
+__device__ __forceinline__ void add_defgrad_bond(double x0, double x1, double x2, double yi0,
+                                                 double yi1, double yi2, const DefgradBond &b,
+                                                 double num[9], const StretchTest &test,
+                                                 uint32_t *broken, int64_t p) {
+  const double xi[3] = {b.x[0] - x0, b.x[1] - x1, b.x[2] - x2};
+  const double yij[3] = {(b.x[0] + b.u[0]) - yi0, (b.x[1] + b.u[1]) - yi1,
+                         (b.x[2] + b.u[2]) - yi2};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      num[3 * a + c] += b.vol * yij[a] * xi[c];
+  if (DAMAGE) {
+    const uint32_t bit = 1u << (p & 31);
+    if (!(broken[p >> 5] & bit) && stretch_breaks(test, xi, yij))
+      atomicOr(&broken[p >> 5], bit);
+  }
+}
+
+template <bool DAMAGE, bool PIPE>
 __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, Planes u,
                                             const double *__restrict__ vol, Sell bonds,
                                             Planes kinv, Planes fout, uint32_t *broken,
                                             double initiation, double failure,
                                             int useCompression) {
+  const StretchTest test = make_stretch_test(initiation, failure, useCompression);
   const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
   const double yi0 = x0 + u(0, i), yi1 = x1 + u(1, i), yi2 = x2 + u(2, i);
   double num[9];
@@ -200,27 +286,27 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, Planes u,
     num[c] = 0.0;
   const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
   const int64_t p0 = sell_pos(bonds, i, 0);
-  for (int k = 0; k < len; ++k) {
-    const int64_t p = p0 + 64 * (int64_t)k;
-    const int j = bonds.col[p];
-    const double xj0 = x(0, j), xj1 = x(1, j), xj2 = x(2, j);
-    const double xi[3] = {xj0 - x0, xj1 - x1, xj2 - x2};
-    const double yij[3] = {(xj0 + u(0, j)) - yi0, (xj1 + u(1, j)) - yi1, (xj2 + u(2, j)) - yi2};
-    const double vj = vol[j];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b)
-        num[3 * a + b] += vj * yij[a] * xi[b];
-    if (DAMAGE) {
-      const uint32_t bit = 1u << (p & 31);
-      if (!(broken[p >> 5] & bit)) {
-        const double initial = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
-        const double stretch =
-            (sqrt(yij[0] * yij[0] + yij[1] * yij[1] + yij[2] * yij[2]) - initial) / initial;
-        if (stretch > initiation || (useCompression && stretch < -failure))
-          atomicOr(&broken[p >> 5], bit);
-      }
+  if (!PIPE) {
+    for (int k = 0; k < len; ++k) {
+      const int64_t p = p0 + 64 * (int64_t)k;
+      DefgradBond b;
+      load_defgrad_bond(x, u, vol, bonds.col[p], b);
+      add_defgrad_bond<DAMAGE>(x0, x1, x2, yi0, yi1, yi2, b, num, test, broken, p);
+    }
+  } else if (len > 0) {
+    // Software pipeline as in the force pass; the sum runs in CSR order.
+    DefgradBond cur;
+    load_defgrad_bond(x, u, vol, bonds.col[p0], cur);
+    int jNext = len > 1 ? bonds.col[p0 + 64] : 0;
+    for (int k = 0; k < len; ++k) {
+      const int64_t p = p0 + 64 * (int64_t)k;
+      DefgradBond nxt;
+      if (k + 1 < len)
+        load_defgrad_bond(x, u, vol, jNext, nxt);
+      if (k + 2 < len)
+        jNext = bonds.col[p + 128];
+      add_defgrad_bond<DAMAGE>(x0, x1, x2, yi0, yi1, yi2, cur, num, test, broken, p);
+      cur = nxt;
     }
   }
   T3 nm, ki;
@@ -235,7 +321,7 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, Planes u,
     fout(c, i) = f.m[c];
 }
 
-template <bool DAMAGE>
+template <bool DAMAGE, bool PIPE = false>
 __global__ void __launch_bounds__(256)
     k_defgrad(Rounds rd, Planes x, Planes u, const double *__restrict__ vol, Sell bonds,
               Planes kinv, Planes fout, uint32_t *broken, double initiation, double failure,
@@ -247,8 +333,8 @@ __global__ void __launch_bounds__(256)
     const int64_t i = rd.particle(r, blk);
     if (i >= rd.n)
       break;
-    defgrad_one<DAMAGE>(i, x, u, vol, bonds, kinv, fout, broken, initiation, failure,
-                        useCompression);
+    defgrad_one<DAMAGE, PIPE>(i, x, u, vol, bonds, kinv, fout, broken, initiation, failure,
+                              useCompression);
   }
 }
 
@@ -428,7 +514,42 @@ struct ForceArgs {
   Ramp rampNow, rampNext;
 };
 
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE>
+// Neighbour data one force-pass bond needs: X_j, V_j, A_j.
+struct ForceBond {
+  double x[3], vol;
+  T3 a;
+};
+
+__device__ __forceinline__ void load_force_bond(const ForceArgs &g, int j, ForceBond &b) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    b.x[d] = g.x(d, j);
+  b.vol = g.vol[j];
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    b.a.m[c] = g.amap(c, j);
+}
+
+// One bond's term of PdCorrespondenceSkeleton.cpp:219-225 added to acc, in the reference's
+// operation order: t_i = A_i ξ, t_j = A_j (−ξ), acc += (t_i − t_j) V_j.
+__device__ __forceinline__ void add_force_bond(const T3 &ai, double x0, double x1, double x2,
+                                               const ForceBond &b, double acc[3]) {
+  V3 xi;
+  xi.v[0] = b.x[0] - x0;
+  xi.v[1] = b.x[1] - x1;
+  xi.v[2] = b.x[2] - x2;
+  const V3 ti = t3_mulv(ai, xi);
+  V3 nxi;
+  nxi.v[0] = -xi.v[0];
+  nxi.v[1] = -xi.v[1];
+  nxi.v[2] = -xi.v[2];
+  const V3 tj = t3_mulv(b.a, nxi);
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    acc[d] += (ti.v[d] - tj.v[d]) * b.vol;
+}
+
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool PIPE>
 __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i) {
   const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
   T3 ai;
@@ -438,29 +559,33 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i) {
   double acc[3] = {0.0, 0.0, 0.0};
   const int len = g.bonds.rowptr[i + 1] - g.bonds.rowptr[i];
   const int64_t p0 = sell_pos(g.bonds, i, 0);
-  for (int k = 0; k < len; ++k) {
-    const int64_t p = p0 + 64 * (int64_t)k;
-    if (DAMAGE && (g.broken[p >> 5] & (1u << (p & 31))))
-      continue;
-    const int j = g.bonds.col[p];
-    V3 xi;
-    xi.v[0] = g.x(0, j) - x0;
-    xi.v[1] = g.x(1, j) - x1;
-    xi.v[2] = g.x(2, j) - x2;
-    const V3 ti = t3_mulv(ai, xi);
-    V3 nxi;
-    nxi.v[0] = -xi.v[0];
-    nxi.v[1] = -xi.v[1];
-    nxi.v[2] = -xi.v[2];
-    T3 aj;
-#pragma unroll
-    for (int c = 0; c < 9; ++c)
-      aj.m[c] = g.amap(c, j);
-    const V3 tj = t3_mulv(aj, nxi);
-    const double vj = g.vol[j];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-      acc[d] += (ti.v[d] - tj.v[d]) * vj;
+  if (!PIPE) {
+    for (int k = 0; k < len; ++k) {
+      const int64_t p = p0 + 64 * (int64_t)k;
+      if (DAMAGE && (g.broken[p >> 5] & (1u << (p & 31))))
+        continue;
+      ForceBond b;
+      load_force_bond(g, g.bonds.col[p], b);
+      add_force_bond(ai, x0, x1, x2, b, acc);
+    }
+  } else if (len > 0) {
+    // Software pipeline: bond k+1's data and bond k+2's column index are in flight while
+    // bond k is summed, so each wave keeps twice the independent loads outstanding. The
+    // summation order is unchanged.
+    ForceBond cur;
+    load_force_bond(g, g.bonds.col[p0], cur);
+    int jNext = len > 1 ? g.bonds.col[p0 + 64] : 0;
+    for (int k = 0; k < len; ++k) {
+      const int64_t p = p0 + 64 * (int64_t)k;
+      ForceBond nxt;
+      if (k + 1 < len)
+        load_force_bond(g, jNext, nxt);
+      if (k + 2 < len)
+        jNext = g.bonds.col[p + 128];
+      if (!DAMAGE || !(g.broken[p >> 5] & (1u << (p & 31))))
+        add_force_bond(ai, x0, x1, x2, cur, acc);
+      cur = nxt;
+    }
   }
   if (g.forceOut.p) {
 #pragma unroll
@@ -493,7 +618,7 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i) {
   }
 }
 
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true>
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true, bool PIPE = false>
 __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
   if (blk < 0)
@@ -502,7 +627,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
     const int64_t i = g.rd.particle(r, blk);
     if (i >= g.rd.n)
       break;
-    force_one<DAMAGE, FUSE_NEXT, INTEGRATE>(g, i);
+    force_one<DAMAGE, FUSE_NEXT, INTEGRATE, PIPE>(g, i);
   }
 }
 
@@ -624,6 +749,7 @@ __global__ void __launch_bounds__(256) k_bond_damage(int64_t n, Planes x, Planes
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
+  const StretchTest test = make_stretch_test(initiation, failure, useCompression);
   const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
   const double yi0 = x0 + u(0, i), yi1 = x1 + u(1, i), yi2 = x2 + u(2, i);
   const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
@@ -636,10 +762,7 @@ __global__ void __launch_bounds__(256) k_bond_damage(int64_t n, Planes x, Planes
     const double xj0 = x(0, j), xj1 = x(1, j), xj2 = x(2, j);
     const double xi[3] = {xj0 - x0, xj1 - x1, xj2 - x2};
     const double yij[3] = {(xj0 + u(0, j)) - yi0, (xj1 + u(1, j)) - yi1, (xj2 + u(2, j)) - yi2};
-    const double initial = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
-    const double stretch =
-        (sqrt(yij[0] * yij[0] + yij[1] * yij[1] + yij[2] * yij[2]) - initial) / initial;
-    if (stretch > initiation || (useCompression && stretch < -failure))
+    if (stretch_breaks(test, xi, yij))
       atomicOr(&broken[p >> 5], bit);
   }
 }

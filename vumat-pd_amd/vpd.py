@@ -118,7 +118,7 @@ def lib():
         L.vpd_mlp_seed_device.argtypes = [P, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                           C.c_int64, C.c_void_p, C.POINTER(C.c_double)]
         L.vpd_return_map_device.argtypes = ([C.c_int, C.c_int64, C.c_int, _D, C.c_int, C.c_double,
-                                             C.c_double] + [C.c_void_p] * 13 +
+                                             C.c_double] + [C.c_void_p] * 12 +
                                             [C.POINTER(C.c_double)])
         L.vpd_device_malloc.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
         L.vpd_device_free.argtypes = [C.c_void_p]
@@ -316,10 +316,11 @@ class Context:
             self._warm = warm_start  # keep the network alive while attached
             self._check(lib().vpd_attach_warm_start(self.h, warm_start.h))
 
-    TUNE_KEYS = ("defgrad_lds", "force_lds", "defgrad_rounds", "force_rounds")
+    TUNE_KEYS = ("defgrad_lds", "force_lds", "defgrad_rounds", "force_rounds",
+                 "defgrad_pipeline", "force_pipeline")
 
     def autotune(self, reps=3):
-        out = (C.c_int * 4)()
+        out = (C.c_int * len(self.TUNE_KEYS))()
         self._check(lib().vpd_autotune(self.h, reps, out))
         return dict(zip(self.TUNE_KEYS, list(out)))
 
