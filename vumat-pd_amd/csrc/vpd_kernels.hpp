@@ -6,19 +6,13 @@
 // per row, which keeps the reference's summation order and needs no atomics.
 #pragma once
 
+#include "vpd_diag.hpp"
 #include "vpd_math.hpp"
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace vpd {
-
-// Planes of an SoA field with stride N.
-struct Planes {
-  double *p;
-  int64_t n;
-  __device__ __forceinline__ double &operator()(int c, int64_t i) const { return p[c * n + i]; }
-};
 
 __device__ __forceinline__ void raise_error(int *err, int bit) { atomicOr(err, bit); }
 
@@ -356,14 +350,18 @@ struct MaterialArgs {
   JcParams jc;
   double dt, rtol;
   int *err;
+  DiagArgs diag;
 };
 
 // The stages after the return map, shared by the fused and the split material kernels:
-// history commit, σ = R σ̄ Rᵀ, optional diagnostics, A = P K⁻¹.
+// history commit, σ = R σ̄ Rᵀ, optional field outputs, the run diagnostics' material terms
+// (DIAG: into dv, from the old stress sOld and the strain increment dE), A = P K⁻¹.
+template <bool DIAG>
 __device__ __forceinline__ void material_finish(const MaterialArgs &g, int64_t i, const T3 &f1,
                                                 const T3 &r1, const double sNew[6],
                                                 const double st[6], double eint, double einel,
-                                                const ReturnMapStats &stats) {
+                                                const ReturnMapStats &stats, const double sOld[6],
+                                                const double dE[6], double *dv) {
   const T3 sFull = from_voigt(sNew);
   const T3 sigma = t3_mul(r1, t3_mul(sFull, t3_transpose(r1)));
 #pragma unroll
@@ -383,14 +381,15 @@ __device__ __forceinline__ void material_finish(const MaterialArgs &g, int64_t i
     for (int c = 0; c < 9; ++c)
       g.sigOut(c, i) = sigma.m[c];
   }
-  if (g.strainOut.p) { // CorotationalKinematics.cpp:141-149
-    T3 e = t3_mul(t3_transpose(f1), f1);
-    e.m[0] -= 1.0;
-    e.m[4] -= 1.0;
-    e.m[8] -= 1.0;
+  if (g.strainOut.p || DIAG) {
+    const T3 e = gl_strain(f1);
+    if (g.strainOut.p) {
 #pragma unroll
-    for (int c = 0; c < 9; ++c)
-      g.strainOut(c, i) = e.m[c] * 0.5;
+      for (int c = 0; c < 9; ++c)
+        g.strainOut(c, i) = e.m[c];
+    }
+    if (DIAG)
+      material_diag(g.diag, i, sOld, sigma, e, dE, st, dv);
   }
   T3 P;
   if (!first_piola(sigma, f1, P)) {
@@ -415,14 +414,9 @@ __device__ __forceinline__ void material_finish(const MaterialArgs &g, int64_t i
 // (PdCorrespondenceSkeleton.cpp:209-210). With FT_ONLY the kernel stops after the
 // kinematics: it commits F, V, R and writes the strain increments to incOut, for a
 // return-map stage that needs them first (the learned seed).
-template <int MODE, bool FT_ONLY = false>
-__global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
-  const int64_t blk = xcd_block(g.nblocks);
-  if (blk < 0)
-    return;
-  const int64_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= g.n)
-    return;
+// One material point of k_material (see there); DIAG adds the diagnostics terms to dv.
+template <int MODE, bool FT_ONLY, bool DIAG>
+__device__ __forceinline__ void material_one(const MaterialArgs &g, int64_t i, double *dv) {
   T3 f1, f0, v0, r0;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
@@ -446,14 +440,23 @@ __global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
     for (int c = 0; c < 6; ++c)
       g.incOut(c, i) = inc[c];
   }
+  double dE[6];
+  if (DIAG) // E_n of the committed F_n equals the stored strainN of the reference bit for bit
+    strain_increment(gl_strain(f1), gl_strain(f0), dE);
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
     g.fN(c, i) = f1.m[c];
     g.vN(c, i) = k.V1.m[c];
     g.rN(c, i) = k.R1.m[c];
   }
-  if (FT_ONLY)
+  if (FT_ONLY) {
+    if (DIAG) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        g.diag.dstrain(c, i) = dE[c];
+    }
     return;
+  }
   double sOld[6], sNew[6], st[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -463,25 +466,47 @@ __global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
   jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
-  material_finish(g, i, f1, k.R1, sNew, st, eint, einel, stats);
+  material_finish<DIAG>(g, i, f1, k.R1, sNew, st, eint, einel, stats, sOld, dE, dv);
 }
 
-// The return-map half of the split material update: reads the strain increments written by
-// k_material<…, true>, the committed F and R, and the per-point seed.
-template <int MODE>
-__global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
+// Material point update, fused: VumatPdCorrespondenceMaterial.cpp:110-173 for every point at
+// once (nblock = N; the 256-row padding of the CPU host leaves results unchanged because
+// the routine is point-wise): Flanagan-Taylor kinematics, strainInc = D̄·dt and stressOld
+// packed as Voigt (VumatMaterialAdapter.cpp:290-330), the Johnson-Cook return map,
+// σ = R σ̄ Rᵀ, history commit; then A_i = (J σ F⁻ᵀ) K_i⁻¹ for the force pass
+// (PdCorrespondenceSkeleton.cpp:209-210). With FT_ONLY the kernel stops after the
+// kinematics: it commits F, V, R and writes the strain increments to incOut, for a
+// return-map stage that needs them first (the learned seed). With DIAG (and not FT_ONLY)
+// each 256-point chunk also writes its row of the run diagnostics' material terms.
+template <int MODE, bool FT_ONLY = false, bool DIAG = false>
+__global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
   const int64_t blk = xcd_block(g.nblocks);
   if (blk < 0)
     return;
   const int64_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= g.n)
+  if (!DIAG || FT_ONLY) {
+    if (i < g.n)
+      material_one<MODE, FT_ONLY, DIAG>(g, i, nullptr);
     return;
-  double inc[6], sOld[6], sNew[6], st[6];
+  }
+  double dv[D_NET_X] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < g.n)
+    material_one<MODE, false, true>(g, i, dv);
+  diag_block_store<D_NET_X, 0>(dv, g.diag.partials + blk * DIAG_STRIDE);
+}
+
+// The return-map half of the split material update: reads the strain increments written by
+// k_material<…, true>, the committed F and R, and the per-point seed.
+template <int MODE, bool DIAG>
+__device__ __forceinline__ void material_return_one(const MaterialArgs &g, int64_t i,
+                                                    double *dv) {
+  double inc[6], sOld[6], sNew[6], st[6], dE[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     inc[c] = g.incOut(c, i);
     sOld[c] = g.sig(c, i);
     st[c] = g.statev(c, i);
+    dE[c] = DIAG ? g.diag.dstrain(c, i) : 0.0;
   }
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
@@ -493,7 +518,24 @@ __global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
     f1.m[c] = g.fN(c, i);
     r1.m[c] = g.rN(c, i);
   }
-  material_finish(g, i, f1, r1, sNew, st, eint, einel, stats);
+  material_finish<DIAG>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, dE, dv);
+}
+
+template <int MODE, bool DIAG = false>
+__global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
+  const int64_t blk = xcd_block(g.nblocks);
+  if (blk < 0)
+    return;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  if (!DIAG) {
+    if (i < g.n)
+      material_return_one<MODE, false>(g, i, nullptr);
+    return;
+  }
+  double dv[D_NET_X] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (i < g.n)
+    material_return_one<MODE, true>(g, i, dv);
+  diag_block_store<D_NET_X, 0>(dv, g.diag.partials + blk * DIAG_STRIDE);
 }
 
 // Bond pass 2. PdCorrespondenceSkeleton.cpp:213-227: f_i = Σ_k (1 − d_ik)(A_i ξ − A_j(−ξ)) V_j
@@ -512,6 +554,7 @@ struct ForceArgs {
   const uint8_t *codes;
   double dt, density;
   Ramp rampNow, rampNext;
+  DiagArgs diag;
 };
 
 // Neighbour data one force-pass bond needs: X_j, V_j, A_j.
@@ -549,8 +592,8 @@ __device__ __forceinline__ void add_force_bond(const T3 &ai, double x0, double x
     acc[d] += (ti.v[d] - tj.v[d]) * b.vol;
 }
 
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool PIPE>
-__device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i) {
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool PIPE, bool DIAG>
+__device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double *dv) {
   const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
   T3 ai;
 #pragma unroll
@@ -604,6 +647,24 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i) {
   }
   const uint8_t code = g.codes[i];
   constrain(code, g.rampNow, uu, vv, aa);
+  if (DIAG) { // the force-side terms of BoundaryValueCaseRunner.cpp:244-285, in its expressions
+    const double vi = g.vol[i];
+    const int ax = g.diag.axis;
+    dv[D_NET_X - D_NET_X] = acc[0] * vi;
+    dv[D_NET_Y - D_NET_X] = acc[1] * vi;
+    dv[D_NET_Z - D_NET_X] = acc[2] * vi;
+    dv[D_REACTION - D_NET_X] =
+        ((code >> (2 * ax)) & 3) == 2 /* load band */ ? (g.density * g.rampNow.a - acc[ax]) * vi : 0.0;
+    dv[D_KINETIC - D_NET_X] =
+        0.5 * g.density * vi * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
+    const uint8_t lat = g.diag.latSet ? g.diag.latSet[i] : 0;
+    dv[D_LATERAL_NEG - D_NET_X] = (lat & 1) ? uu[g.diag.lateral] : 0.0;
+    dv[D_LATERAL_POS - D_NET_X] = (lat & 2) ? uu[g.diag.lateral] : 0.0;
+    dv[D_NONFINITE_FORCE - D_NET_X] =
+        (__builtin_isfinite(acc[0]) && __builtin_isfinite(acc[1]) && __builtin_isfinite(acc[2]))
+            ? 0.0
+            : 1.0;
+  }
   if (FUSE_NEXT) {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -618,16 +679,34 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i) {
   }
 }
 
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true, bool PIPE = false>
+// With DIAG (and INTEGRATE) each 256-particle chunk also writes its row of the run
+// diagnostics' force terms; the chunk loop then runs block-uniformly so that every thread
+// reaches the chunk's reduction.
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true, bool PIPE = false, bool DIAG = false>
 __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
+  static_assert(!DIAG || INTEGRATE, "the diagnostics read the corrected velocity");
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
   if (blk < 0)
     return;
   for (int r = 0; r < g.rd.rounds; ++r) {
     const int64_t i = g.rd.particle(r, blk);
-    if (i >= g.rd.n)
+    if (!DIAG) {
+      if (i >= g.rd.n)
+        break;
+      force_one<DAMAGE, FUSE_NEXT, INTEGRATE, PIPE, false>(g, i, nullptr);
+      continue;
+    }
+    const int64_t chunk = (int64_t)r * g.rd.blocksPerRound + blk;
+    if (chunk * 256 >= g.rd.n) // the same for every thread of the block
       break;
-    force_one<DAMAGE, FUSE_NEXT, INTEGRATE, PIPE>(g, i);
+    constexpr int NS = D_COUNT - D_NET_X;
+    double dv[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      dv[s] = 0.0;
+    if (i < g.rd.n)
+      force_one<DAMAGE, FUSE_NEXT, INTEGRATE, PIPE, true>(g, i, dv);
+    diag_block_store<NS, D_NET_X>(dv, g.diag.partials + chunk * DIAG_STRIDE + D_NET_X);
   }
 }
 

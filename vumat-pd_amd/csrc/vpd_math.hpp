@@ -12,6 +12,13 @@
 
 namespace vpd {
 
+// Planes of an SoA field with stride N.
+struct Planes {
+  double *p;
+  int64_t n;
+  __device__ __forceinline__ double &operator()(int c, int64_t i) const { return p[c * n + i]; }
+};
+
 struct T3 {
   double m[9];
 };
