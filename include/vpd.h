@@ -197,8 +197,8 @@ int vpd_rotate_and_strain(const double *stress, const double *rotation, const do
                           int64_t n, double *spatial_stress, double *strain);
 
 /* ---- the fused increment ---------------------------------------------------------- */
-/* One step of the loop body in src/host/BoundaryValueCaseRunner.cpp:207-229 (without its
- * diagnostics): [predict + constrain unless the previous step fused it] → F (+ damage) →
+/* One step of the loop body in src/host/BoundaryValueCaseRunner.cpp:207-229 (plus its
+ * diagnostics when enabled, see vpd_enable_diagnostics): [predict + constrain unless the previous step fused it] → F (+ damage) →
  * material → force + correct + constrain [+ next predict + constrain]. Asynchronous on the
  * context stream; numeric errors surface at the next vpd_synchronize / download. */
 int vpd_step(vpd_context *ctx, const vpd_step_args *args);
@@ -236,6 +236,49 @@ enum vpd_tuning {
 int vpd_set_tuning(vpd_context *ctx, int knob, int value);
 int vpd_get_tuning(vpd_context *ctx, int knob, int *value);
 int vpd_autotune(vpd_context *ctx, int reps, int *tuned);
+
+/* ---- run diagnostics ------------------------------------------------------------- */
+/* The per-step observables of runMaterial (src/host/BoundaryValueCaseRunner.cpp:231-305),
+ * reduced on the device inside the material and force kernels of every vpd_step while
+ * enabled. Each step appends one record of VPD_DIAG_STRIDE doubles (slots below) to a
+ * device buffer of `capacity` records; vpd_fetch_diagnostics copies the pending records out
+ * in step order and empties the buffer (vpd_step fails while it is full). Sums run in a
+ * fixed order: identical inputs and particle count give identical records, whatever the
+ * launch tuning; they differ from the CPU's serial sums by rounding only. */
+enum vpd_diag_slot {
+  VPD_DIAG_INTERNAL_ENERGY = 0,  /* Σ contraction(½(σ̄_n + σ_{n+1}), E_{n+1} − E_n)·V (:231-237) */
+  VPD_DIAG_OBSERVED_STRESS = 1,  /* Σ σ[axis][axis]·w over the observation weights (:250-253) */
+  VPD_DIAG_OBSERVED_STRAIN = 2,  /* Σ E[axis][axis]·w */
+  VPD_DIAG_MAX_EQPS = 3,         /* max(0, statev[eqps_index]); 0 if eqps_index < 0 */
+  VPD_DIAG_MAX_INDICATOR = 4,    /* max(0, internalIndicator(statev)) (:95-110) */
+  VPD_DIAG_NONFINITE_MATERIAL = 5, /* 1 if any stress, strain or state value is not finite */
+  VPD_DIAG_NET_FORCE_X = 6,      /* Σ f·V (:248-249) */
+  VPD_DIAG_NET_FORCE_Y = 7,
+  VPD_DIAG_NET_FORCE_Z = 8,
+  VPD_DIAG_REACTION = 9,         /* Σ over the load band (code RAMP on load_axis) of
+                                    (density·ramp[2] − f[axis])·V (:268-271) */
+  VPD_DIAG_KINETIC = 10,         /* Σ ½·density·V·|v|² after the step (:75-83) */
+  VPD_DIAG_LATERAL_NEG = 11,     /* Σ u[lateral_axis] over particles with lateral_sets bit 0 */
+  VPD_DIAG_LATERAL_POS = 12,     /* Σ u[lateral_axis] over particles with lateral_sets bit 1 */
+  VPD_DIAG_NONFINITE_FORCE = 13, /* 1 if any force component is not finite */
+  VPD_DIAG_STRIDE = 16
+};
+typedef struct vpd_diag_config {
+  int load_axis;                     /* 0 x, 1 y, 2 z */
+  int lateral_axis;
+  const double *observation_weight;  /* host, n; NULL: no observation window */
+  const uint8_t *lateral_sets;       /* host, n; bit 0 negative face, bit 1 positive face */
+  int eqps_index;                    /* state index of ε̄ᵖ, or -1 */
+  int indicator_count;               /* 0..8 state indices of the internal indicator */
+  int indicator_indices[8];
+  int indicator_min;                 /* 1: min over the indices; 0: max */
+  int indicator_one_minus;           /* 1: report 1 − value */
+  int capacity;                      /* step records held between fetches */
+} vpd_diag_config;
+/* NULL cfg disables. The configuration is copied; the arrays may be freed afterwards. */
+int vpd_enable_diagnostics(vpd_context *ctx, const vpd_diag_config *cfg);
+/* Synchronises; writes the pending records (count × VPD_DIAG_STRIDE doubles). */
+int vpd_fetch_diagnostics(vpd_context *ctx, double *records, int capacity, int *count);
 
 /* ---- per-kernel timing ------------------------------------------------------------ */
 int vpd_set_timing(vpd_context *ctx, int enable);
@@ -316,6 +359,21 @@ int vpd_regional_constraint_codes(const double *x, int64_t n, int component, int
                                   int64_t *anchor, int64_t *rotation_pin);
 /* RegionalConstraintSystem::ramp (src/host/BoundaryConditions.cpp:70-79). */
 void vpd_smooth_ramp(double amplitude, double rise_time, double time, double out[3]);
+/* selectBoundaryRegion (src/host/ParticleSets.cpp:12-50): mask[i] = 1 for the particles within
+ * `width` of the "left"/"right" (x) or "bottom"/"top" (y) extreme. Returns the count, or
+ * -VPD_ERR_ARGUMENT for an unknown side, a width <= 0 or an empty selection. */
+int64_t vpd_select_boundary_region(const double *x, int64_t n, const char *side, double width,
+                                   uint8_t *mask);
+
+/* ---- the JSON case runner ----------------------------------------------------------- */
+/* runDemoCaseFromJson (src/host/DemoCaseRunner.cpp:435-458) for "vumat_pd_boundary_case"
+ * files (runBoundaryValueCase, src/host/BoundaryValueCaseRunner.cpp:309-440) on the GPU: the
+ * same case and material JSON schema, the same timeseries.csv, field_snapshots.csv and
+ * metadata.json in output.directory, the same stdout summary line. Materials must be the
+ * Johnson-Cook VUMAT (cmname JOHNSONCOOK); damage "none" or "critical_stretch". Returns 0
+ * when every material passes its acceptance checks, 2 when one fails, 1 on an error (the
+ * message goes to stderr as the reference prints it). */
+int vpd_run_case_json(const char *case_path, const char *plugin_directory);
 
 #ifdef __cplusplus
 }

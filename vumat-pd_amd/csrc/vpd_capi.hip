@@ -107,6 +107,15 @@ struct vpd_context {
   int *fallbacks = nullptr;
   std::vector<TimingRec> recs;
   std::vector<hipEvent_t> eventPool;
+  // Run diagnostics (vpd_enable_diagnostics): per-chunk partial rows, the step records kept
+  // until fetched, the observation weights and lateral face flags, and the strain-increment
+  // scratch of the split (learned) material update.
+  bool diagOn = false;
+  DiagArgs diag{};
+  double *diagPartials = nullptr, *diagRecords = nullptr, *diagW = nullptr;
+  double *diagDstrain = nullptr;
+  uint8_t *diagLat = nullptr;
+  int diagCapacity = 0, diagCount = 0;
 };
 
 namespace {
@@ -459,7 +468,41 @@ int launch_defgrad(vpd_context *c) {
   });
 }
 
-int launch_material(vpd_context *c, double dt, bool fields) {
+template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
+  const int grid = grid8_of(c->n);
+  switch (c->returnMap) {
+  case RM_BISECTION:
+    return timed(c, K_MATERIAL,
+                 [&] { k_material<RM_BISECTION, false, DIAG><<<grid, 256, 0, c->stream>>>(g); });
+  case RM_BISECTION_TIGHT:
+    return timed(c, K_MATERIAL, [&] {
+      k_material<RM_BISECTION_TIGHT, false, DIAG><<<grid, 256, 0, c->stream>>>(g);
+    });
+  case RM_NEWTON:
+    return timed(c, K_MATERIAL,
+                 [&] { k_material<RM_NEWTON, false, DIAG><<<grid, 256, 0, c->stream>>>(g); });
+  default: {
+    if (!c->mlp)
+      return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
+    if (int s = timed(c, K_MATERIAL,
+                      [&] { k_material<RM_LEARNED, true, DIAG><<<grid, 256, 0, c->stream>>>(g); }))
+      return s;
+    if (int s = timed(c, K_MLP_SEED, [&] {
+          k_mlp_seed<<<(int)ceil_div(c->n, 64), 256, 0, c->stream>>>(
+              c->n, c->inc, c->n, c->statev, c->n, c->mlp->theta, c->mlp->norm, c->seed);
+        }))
+      return s;
+    g.seed = c->seed;
+    return timed(c, K_RETURN_MAP, [&] {
+      k_material_return<RM_LEARNED, DIAG><<<grid, 256, 0, c->stream>>>(g);
+    });
+  }
+  }
+}
+
+// The material stage in the context's return-map mode; with `diag`, the run diagnostics'
+// material terms are reduced as well (vpd_step with diagnostics enabled).
+int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
   MaterialArgs g;
   g.n = c->n;
   g.nblocks = ceil_div(c->n, 256);
@@ -484,65 +527,52 @@ int launch_material(vpd_context *c, double dt, bool fields) {
   g.dt = dt;
   g.rtol = c->rtol;
   g.err = c->derr;
-  const int grid = grid8_of(c->n);
-  switch (c->returnMap) {
-  case RM_BISECTION:
-    return timed(c, K_MATERIAL,
-                 [&] { k_material<RM_BISECTION><<<grid, 256, 0, c->stream>>>(g); });
-  case RM_BISECTION_TIGHT:
-    return timed(c, K_MATERIAL,
-                 [&] { k_material<RM_BISECTION_TIGHT><<<grid, 256, 0, c->stream>>>(g); });
-  case RM_NEWTON:
-    return timed(c, K_MATERIAL, [&] { k_material<RM_NEWTON><<<grid, 256, 0, c->stream>>>(g); });
-  default: {
-    if (!c->mlp)
-      return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
-    if (int s = timed(c, K_MATERIAL,
-                      [&] { k_material<RM_LEARNED, true><<<grid, 256, 0, c->stream>>>(g); }))
-      return s;
-    if (int s = timed(c, K_MLP_SEED, [&] {
-          k_mlp_seed<<<(int)ceil_div(c->n, 64), 256, 0, c->stream>>>(
-              c->n, c->inc, c->n, c->statev, c->n, c->mlp->theta, c->mlp->norm, c->seed);
-        }))
-      return s;
-    g.seed = c->seed;
-    return timed(c, K_RETURN_MAP,
-                 [&] { k_material_return<RM_LEARNED><<<grid, 256, 0, c->stream>>>(g); });
-  }
-  }
+  g.diag = c->diag;
+  if (diag && !c->diagOn)
+    return fail(c, VPD_ERR_STATE, "Run diagnostics are not enabled.");
+  return diag ? launch_material_t<true>(c, g) : launch_material_t<false>(c, g);
 }
 
-template <bool DAMAGE, bool FUSE, bool INTEGRATE>
+template <bool DAMAGE, bool FUSE, bool INTEGRATE, bool DIAG = false>
 void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, bool pipe) {
   if (pipe)
-    k_force<DAMAGE, FUSE, INTEGRATE, true><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+    k_force<DAMAGE, FUSE, INTEGRATE, true, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
   else
-    k_force<DAMAGE, FUSE, INTEGRATE, false><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+    k_force<DAMAGE, FUSE, INTEGRATE, false, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+}
+
+template <bool DIAG>
+void launch_force_integrating(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, int lds,
+                              bool pipe) {
+  if (dmg && fuse)
+    launch_force_t<true, true, true, DIAG>(c, g, lds, pipe);
+  else if (dmg)
+    launch_force_t<true, false, true, DIAG>(c, g, lds, pipe);
+  else if (fuse)
+    launch_force_t<false, true, true, DIAG>(c, g, lds, pipe);
+  else
+    launch_force_t<false, false, true, DIAG>(c, g, lds, pipe);
 }
 
 // The force pass in one of its forms: with or without damage flags, the next step's predictor
 // fused in, the integrator (INTEGRATE false leaves only the force density), the pipelined
-// bond loop.
+// bond loop, the run diagnostics' force terms (integrating forms only).
 void launch_force_with(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, bool integrate,
-                       int lds, bool pipe) {
+                       int lds, bool pipe, bool diag = false) {
   if (!integrate) {
     if (dmg)
       launch_force_t<true, false, false>(c, g, lds, pipe);
     else
       launch_force_t<false, false, false>(c, g, lds, pipe);
-  } else if (dmg && fuse) {
-    launch_force_t<true, true, true>(c, g, lds, pipe);
-  } else if (dmg) {
-    launch_force_t<true, false, true>(c, g, lds, pipe);
-  } else if (fuse) {
-    launch_force_t<false, true, true>(c, g, lds, pipe);
+  } else if (diag) {
+    launch_force_integrating<true>(c, g, dmg, fuse, lds, pipe);
   } else {
-    launch_force_t<false, false, true>(c, g, lds, pipe);
+    launch_force_integrating<false>(c, g, dmg, fuse, lds, pipe);
   }
 }
 
 int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
-                 const double *rampNext, bool fields) {
+                 const double *rampNext, bool fields, bool diag = false) {
   ForceArgs g;
   g.n = c->n;
   g.nblocks = ceil_div(c->n, 256);
@@ -562,8 +592,9 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.rampNext = fuse ? ramp_of(rampNext) : Ramp{0, 0, 0};
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   g.rd = rounds_of(c->n, c->roundsForce);
+  g.diag = c->diag;
   return timed(c, K_FORCE, [&] {
-    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, c->pipeForce != 0);
+    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, c->pipeForce != 0, diag);
   });
 }
 
@@ -700,6 +731,11 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->broken);
   dfree(c->colS);
   dfree(c->sliceBase);
+  dfree(c->diagPartials);
+  dfree(c->diagRecords);
+  dfree(c->diagW);
+  dfree(c->diagDstrain);
+  dfree(c->diagLat);
   dfree(c->derr);
   if (c->stream)
     (void)hipStreamDestroy(c->stream);
@@ -714,6 +750,7 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   if (n > INT_MAX)
     return fail(c, VPD_ERR_UNSUPPORTED, "Particle count exceeds the int32 index range.");
   c->n = n;
+  c->diagOn = false; // sized for the previous block
   const int64_t sz[] = {3 * n, n, 9 * n, 3 * n, 3 * n, 3 * n, 9 * n, 9 * n,
                         9 * n, 6 * n, (int64_t)c->nstatev * n, n, n, n, 9 * n, 9 * n};
   double **bufs[] = {&c->x,  &c->vol, &c->kinv, &c->u,    &c->v,   &c->a,
@@ -1183,13 +1220,22 @@ int vpd_step(vpd_context *c, const vpd_step_args *g) {
   if (!c->predicted)
     if (int s = launch_predict(c, g->dt, g->ramp))
       return s;
+  if (c->diagOn && c->diagCount >= c->diagCapacity)
+    return fail(c, VPD_ERR_STATE,
+                "The diagnostics record buffer is full; fetch the records first.");
   if (int s = launch_defgrad(c))
     return s;
-  if (int s = launch_material(c, g->dt, g->want_fields != 0))
+  if (int s = launch_material(c, g->dt, g->want_fields != 0, c->diagOn))
     return s;
   if (int s = launch_force(c, g->dt, g->ramp, g->fuse_next != 0, g->ramp_next,
-                           g->want_fields != 0))
+                           g->want_fields != 0, c->diagOn))
     return s;
+  if (c->diagOn) {
+    k_diag_reduce<<<1, 256, 0, c->stream>>>(c->diagPartials, ceil_div(c->n, 256),
+                                            c->diagRecords + (int64_t)c->diagCount * DIAG_STRIDE);
+    HIPTRY(c, hipGetLastError());
+    ++c->diagCount;
+  }
   c->predicted = g->fuse_next != 0;
   c->haveFnew = true;
   c->haveFields = g->want_fields != 0;
@@ -1367,6 +1413,90 @@ int vpd_update_damage(vpd_context *c) {
                                                       c->initiation, c->failure, useCompression);
   HIPTRY(c, hipGetLastError());
   HIPTRY(c, hipStreamSynchronize(c->stream));
+  return VPD_OK;
+}
+
+int vpd_enable_diagnostics(vpd_context *c, const vpd_diag_config *cfg) {
+  if (int s = require_block(c))
+    return s;
+  if (!cfg) {
+    c->diagOn = false;
+    c->diagCount = 0;
+    return VPD_OK;
+  }
+  const int ax = cfg->load_axis, lat = cfg->lateral_axis;
+  if (ax < 0 || ax > 2 || lat < 0 || lat > 2)
+    return fail(c, VPD_ERR_ARGUMENT, "Diagnostics axes must lie in [0, 2].");
+  if (cfg->capacity < 1)
+    return fail(c, VPD_ERR_ARGUMENT, "Diagnostics need room for at least one step record.");
+  // The material kernels hold the Johnson-Cook model's 6 state variables in registers;
+  // entries beyond them never change from their initial values.
+  const int held = std::min(c->nstatev, 6);
+  if (cfg->eqps_index < -1 || cfg->eqps_index >= held)
+    return fail(c, VPD_ERR_ARGUMENT,
+                "equivalentPlasticStrainIndex must name one of the first 6 state variables.");
+  if (cfg->indicator_count < 0 || cfg->indicator_count > 8)
+    return fail(c, VPD_ERR_ARGUMENT, "At most 8 internal-indicator state indices.");
+  for (int k = 0; k < cfg->indicator_count; ++k)
+    if (cfg->indicator_indices[k] < 0 || cfg->indicator_indices[k] >= held)
+      return fail(c, VPD_ERR_ARGUMENT,
+                  "internalIndicator indices must name the first 6 state variables.");
+  const int64_t chunks = ceil_div(c->n, 256);
+  if (dalloc(c, c->diagPartials, chunks * DIAG_STRIDE) ||
+      dalloc(c, c->diagRecords, (int64_t)cfg->capacity * DIAG_STRIDE) ||
+      dalloc(c, c->diagDstrain, 6 * c->n))
+    return VPD_ERR_DEVICE;
+  DiagArgs d{};
+  d.partials = c->diagPartials;
+  d.dstrain = planes(c->diagDstrain, c->n);
+  d.vol = c->vol;
+  if (cfg->observation_weight) {
+    if (dalloc(c, c->diagW, c->n))
+      return VPD_ERR_DEVICE;
+    HIPTRY(c, hipMemcpyAsync(c->diagW, cfg->observation_weight, sizeof(double) * c->n,
+                             hipMemcpyHostToDevice, c->stream));
+    d.obsW = c->diagW;
+  }
+  if (cfg->lateral_sets) {
+    if (dalloc(c, c->diagLat, c->n))
+      return VPD_ERR_DEVICE;
+    HIPTRY(c, hipMemcpyAsync(c->diagLat, cfg->lateral_sets, c->n, hipMemcpyHostToDevice,
+                             c->stream));
+    d.latSet = c->diagLat;
+  }
+  d.axis = ax;
+  d.lateral = lat;
+  d.eqpsIndex = cfg->eqps_index;
+  d.nstatev = held;
+  d.indCount = cfg->indicator_count;
+  d.indMin = cfg->indicator_min != 0;
+  d.indOneMinus = cfg->indicator_one_minus != 0;
+  for (int k = 0; k < 8; ++k)
+    d.ind[k] = k < cfg->indicator_count ? cfg->indicator_indices[k] : 0;
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  c->diag = d;
+  c->diagCapacity = cfg->capacity;
+  c->diagCount = 0;
+  c->diagOn = true;
+  return VPD_OK;
+}
+
+int vpd_fetch_diagnostics(vpd_context *c, double *records, int capacity, int *count) {
+  if (!c || !count || (capacity > 0 && !records))
+    return fail(c, VPD_ERR_ARGUMENT, "Null argument.");
+  if (!c->diagOn)
+    return fail(c, VPD_ERR_STATE, "Run diagnostics are not enabled.");
+  if (capacity < c->diagCount)
+    return fail(c, VPD_ERR_ARGUMENT, "The record array is smaller than the pending records.");
+  if (int s = check_device(c))
+    return s;
+  if (c->diagCount > 0)
+    HIPTRY(c, hipMemcpyAsync(records, c->diagRecords,
+                             sizeof(double) * DIAG_STRIDE * (size_t)c->diagCount,
+                             hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  *count = c->diagCount;
+  c->diagCount = 0;
   return VPD_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <string>
 #include <vector>
 
 namespace {
@@ -39,6 +40,26 @@ void setCode(uint8_t &code, int axis, int value) {
 } // namespace
 
 extern "C" {
+
+// selectBoundaryRegion (ParticleSets.cpp:12-50): side "left", "right", "bottom" or "top".
+int64_t vpd_select_boundary_region(const double *x, int64_t n, const char *side, double width,
+                                   uint8_t *mask) {
+  if (!x || !side || !mask || n <= 0 || !(width > 0.0))
+    return -VPD_ERR_ARGUMENT;
+  const std::string s(side);
+  const int code = s == "left" ? 0 : s == "right" ? 1 : s == "bottom" ? 2 : s == "top" ? 3 : -1;
+  if (code < 0)
+    return -VPD_ERR_ARGUMENT;
+  std::vector<char> m;
+  if (!selectBand(x, n, code, width, m))
+    return -VPD_ERR_ARGUMENT;
+  int64_t count = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    mask[i] = m[i] ? 1 : 0;
+    count += m[i] ? 1 : 0;
+  }
+  return count;
+}
 
 // Geometry.cpp:10-34
 int vpd_make_patch(int nx, int ny, int nz, double h, double *x, double *volume) {

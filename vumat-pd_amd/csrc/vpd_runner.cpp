@@ -1,0 +1,778 @@
+// The JSON boundary-value case runner on the GPU: reads the reference's case and material
+// files (src/config/DemoConfig.cpp, src/config/MaterialConfigJson.cpp), runs every material
+// of the case through vpd_step with the fused run diagnostics, and writes the reference's
+// outputs (src/host/BoundaryValueCaseRunner.cpp). Only host code; every particle-sized
+// computation runs on the device, and per step only one 16-double diagnostics record
+// crosses to the host (in batches).
+#include "vpd.h"
+#include "vpd_json.hpp"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using vpdjson::Value;
+
+// ---- configuration (the reference's DemoCaseConfig and VumatMaterialConfig fields) --------
+
+struct MaterialFile { // VumatMaterialConfig (include/VumatTypes.hpp), MaterialConfigJson.cpp:62-110
+  std::string name, cmname, symbolName, stressFrame, pluginPath;
+  int nstatev = 0;
+  std::vector<double> props;
+  double density = 0.0;
+};
+
+struct CaseMaterial { // DemoMaterialConfig (include/DemoConfig.hpp:83-96)
+  std::string label;
+  MaterialFile material;
+  double elasticE = 0.0, elasticNu = 0.0, referenceTemperature = 293.0;
+  std::vector<double> initialState;
+  int eqpsIndex = -1;
+  std::vector<int> indicatorIndices;
+  bool indicatorMin = false, indicatorOneMinus = false;
+};
+
+struct CaseConfig { // DemoCaseConfig (include/DemoConfig.hpp:98-113), boundary cases only
+  std::string name, type, scope, classification;
+  std::string unitLength = "mm", unitTime = "ms", unitForce = "N", unitStress = "MPa",
+              unitMassDensity = "N*ms^2/mm^4";
+  std::string geometryType = "patch";
+  int nx = 1, ny = 1, nz = 1;
+  double spacing = 1.0, horizon = 1.0;
+  int steps = 1;
+  double dt = 1.0e-6, totalTime = 0.0;
+  std::string bcType = "affine_displacement", supportRegion = "left", loadRegion = "right",
+              component = "x";
+  double amplitude = 0.0, riseTime = 0.0;
+  int regionLayers = 1;
+  bool planeStrain = false, rigidBodySuppression = false;
+  double density = 0.0, energyTol = 1.0e-3, netForceTol = 1.0e-8, minPlastic = 0.0,
+         kineticLimit = 1.0;
+  int outputEvery = 1, snapshotEvery = 0;
+  std::string damageMode = "none";
+  double damageInitiation = 0.0, damageFailure = 1.0;
+  std::string outputDir;
+  double axialMin = std::numeric_limits<double>::quiet_NaN();
+  double axialMax = std::numeric_limits<double>::quiet_NaN();
+  std::vector<CaseMaterial> materials;
+};
+
+std::string join_path(const std::string &dir, const std::string &file) {
+  if (dir.empty())
+    return file;
+  return (dir.back() == '/' || dir.back() == '\\') ? dir + file : dir + "/" + file;
+}
+
+// MaterialConfigJson.cpp:62-110.
+MaterialFile load_material_file(const std::string &path, const std::string &pluginDir) {
+  using namespace vpdjson;
+  const Value root = load_file(path);
+  if (get_string(root, "schemaVersion", false, "1.0") != "1.0")
+    throw std::runtime_error("Unsupported VUMAT material schemaVersion: " +
+                             get_string(root, "schemaVersion", false, "1.0"));
+  MaterialFile m;
+  m.name = get_string(root, "name");
+  m.cmname = get_string(root, "cmname");
+  const std::string convention = get_string(root, "tensorConvention", false, "legacy_vumat_pd");
+  m.stressFrame = get_string(root, "stressFrame", false, "corotational");
+  if (convention != "legacy_vumat_pd")
+    throw std::runtime_error("Unsupported tensorConvention; explicit mapping is required.");
+  if (m.stressFrame != "corotational" && m.stressFrame != "spatial")
+    throw std::runtime_error("stressFrame must be corotational or spatial.");
+  m.symbolName =
+      get_string(root, "symbolName", false, get_string(root, "symbol", false, "vumat_adapter_call"));
+  const int ndir = get_int(root, "ndir", false, 3), nshr = get_int(root, "nshr", false, 3);
+  m.nstatev = get_int(root, "nstatev");
+  const int nfieldv = get_int(root, "nfieldv", false, 1);
+  m.props = get_double_array(root, "props");
+  m.density = get_double(root, "density");
+  if (ndir != 3 || nshr != 3)
+    throw std::runtime_error("The current VUMAT adapter requires ndir=3 and nshr=3.");
+  if (m.nstatev < 0 || nfieldv < 0 || m.density <= 0.0)
+    throw std::runtime_error("Invalid VUMAT dimensions or density in: " + path);
+  const std::string explicitPath = get_string(root, "pluginPath", false, "");
+  std::string plugin = explicitPath;
+  if (plugin.empty()) {
+    plugin = get_string(root, "plugin", false, "");
+    if (plugin.empty())
+      throw std::runtime_error("Material config requires either pluginPath or plugin.");
+    if (plugin.find('/') == std::string::npos && plugin.find('\\') == std::string::npos &&
+        plugin.find('.') == std::string::npos)
+      plugin += ".so";
+  }
+  m.pluginPath = (!plugin.empty() && plugin[0] == '/') ? plugin : join_path(pluginDir, plugin);
+  return m;
+}
+
+std::vector<int> int_array(const Value &o, const std::string &name) {
+  const Value *v = vpdjson::find(o, name);
+  if (!v)
+    return {};
+  if (v->kind != Value::Array)
+    throw std::runtime_error("Expected integer array: " + name);
+  std::vector<int> out;
+  for (const Value &e : v->array) {
+    if (e.kind != Value::Number || !std::isfinite(e.number) || std::trunc(e.number) != e.number ||
+        e.number < 0 || e.number > std::numeric_limits<int>::max())
+      throw std::runtime_error("Expected nonnegative integer entries in " + name);
+    out.push_back(static_cast<int>(e.number));
+  }
+  return out;
+}
+
+// DemoConfig.cpp:206-281.
+CaseMaterial load_case_material(const Value &e, const std::string &pluginDir) {
+  using namespace vpdjson;
+  CaseMaterial m;
+  m.label = get_string(e, "name");
+  const std::string path = get_string(
+      e, "materialConfigPath", false,
+      get_string(e, "configPath", false, get_string(e, "material_config_path", false, "")));
+  if (path.empty())
+    throw std::runtime_error("Material entry requires materialConfigPath or configPath.");
+  m.material = load_material_file(path, pluginDir);
+  if (m.material.stressFrame != "corotational")
+    throw std::runtime_error(
+        "JSON case hosts require corotational stress; use the dedicated spatial demo.");
+  (void)get_bool(e, "reference", false, false);
+  m.elasticE = get_double(e, "elasticE", false, 0.0);
+  m.elasticNu = get_double(e, "elasticNu", false, 0.0);
+  m.referenceTemperature = get_double(e, "referenceTemperature", false, m.referenceTemperature);
+  if (get_string(e, "kinematicsMode", false, "corotational") != "corotational")
+    throw std::runtime_error("Unsupported kinematicsMode for material: " + m.label);
+  if (const Value *init = find(e, "initialState")) {
+    if (init->kind != Value::Array)
+      throw std::runtime_error("Expected numeric array: initialState");
+    for (const Value &x : init->array) {
+      if (x.kind != Value::Number)
+        throw std::runtime_error("Expected numeric entries in initialState");
+      m.initialState.push_back(x.number);
+    }
+  }
+  const int nstatev = m.material.nstatev;
+  if (!m.initialState.empty() && m.initialState.size() != static_cast<size_t>(nstatev))
+    throw std::runtime_error("initialState length does not match nstatev.");
+  m.eqpsIndex = get_int(e, "equivalentPlasticStrainIndex", false, -1);
+  if (m.eqpsIndex < -1 || m.eqpsIndex >= nstatev)
+    throw std::runtime_error("equivalentPlasticStrainIndex is outside nstatev for material: " +
+                             m.label);
+  const Value *over = find(e, "initialStateOverrides");
+  if (over && over->kind == Value::Object) {
+    if (m.initialState.empty())
+      m.initialState.assign(static_cast<size_t>(nstatev), 0.0);
+    for (const auto &item : over->object) {
+      const std::string &key = item.first;
+      const bool digits = !key.empty() && std::all_of(key.begin(), key.end(), [](char c) {
+        return c >= '0' && c <= '9';
+      });
+      if (!digits)
+        throw std::runtime_error("Invalid initialStateOverrides index for material: " + m.label);
+      const long index = std::stol(key);
+      if (index < 0 || index >= nstatev || item.second.kind != Value::Number)
+        throw std::runtime_error("Invalid initialStateOverrides entry for material: " + m.label);
+      m.initialState[static_cast<size_t>(index)] = item.second.number;
+    }
+  }
+  const Value *ind = find(e, "internalIndicator");
+  if (ind && ind->kind == Value::Object) {
+    m.indicatorIndices = int_array(*ind, "indices");
+    const std::string red = get_string(*ind, "reduction", false, "max");
+    const std::string tr = get_string(*ind, "transform", false, "identity");
+    if (red != "min" && red != "max")
+      throw std::runtime_error("internalIndicator.reduction must be min or max.");
+    if (tr != "identity" && tr != "one_minus")
+      throw std::runtime_error("internalIndicator.transform must be identity or one_minus.");
+    for (int k : m.indicatorIndices)
+      if (k < 0 || k >= nstatev)
+        throw std::runtime_error("internalIndicator index is outside nstatev for material: " +
+                                 m.label);
+    m.indicatorMin = red == "min";
+    m.indicatorOneMinus = tr == "one_minus";
+  }
+  return m;
+}
+
+// DemoConfig.cpp:17-197 and the checks of :285-380 that apply to boundary cases.
+CaseConfig load_case(const std::string &path, const std::string &pluginDir) {
+  using namespace vpdjson;
+  const Value root = load_file(path);
+  const std::string schema = get_string(root, "schemaVersion", false, "1.0");
+  if (schema != "1.0")
+    throw std::runtime_error("Unsupported VUMAT-PD case schemaVersion: " + schema);
+  CaseConfig c;
+  c.name = get_string(root, "name");
+  c.type = get_string(root, "type", false, "vumat_pd_demo");
+  c.scope = get_string(root, "scope", false, "");
+  c.classification = get_string(root, "classification", false, "verification");
+  if (const Value *u = find(root, "units"); u && u->kind == Value::Object) {
+    c.unitLength = get_string(*u, "length", false, c.unitLength);
+    c.unitTime = get_string(*u, "time", false, c.unitTime);
+    c.unitForce = get_string(*u, "force", false, c.unitForce);
+    c.unitStress = get_string(*u, "stress", false, c.unitStress);
+    c.unitMassDensity = get_string(*u, "massDensity", false, c.unitMassDensity);
+  }
+  const Value *geo = find(root, "geometry");
+  if (!geo || geo->kind != Value::Object)
+    geo = find(root, "grid");
+  if (geo && geo->kind == Value::Object) {
+    c.geometryType = get_string(*geo, "type", false, c.geometryType);
+    c.nx = get_int(*geo, "nx", false, c.nx);
+    c.ny = get_int(*geo, "ny", false, c.ny);
+    c.nz = get_int(*geo, "nz", false, c.nz);
+    c.spacing = get_double(*geo, "spacing", false, c.spacing);
+    c.horizon = get_double(*geo, "horizon", false, c.horizon);
+  }
+  const Value *ti = find(root, "time_integration");
+  const Value &tsrc = (ti && ti->kind == Value::Object) ? *ti : root;
+  c.steps = get_int(tsrc, "steps", false, c.steps);
+  c.dt = get_double(tsrc, "dt", false, c.dt);
+  c.totalTime = get_double(tsrc, "totalTime", false, c.totalTime);
+  if (const Value *bc = find(root, "boundary_conditions"); bc && bc->kind == Value::Object) {
+    c.bcType = get_string(*bc, "type", false, c.bcType);
+    (void)get_string(*bc, "mode", false, "");
+    (void)get_double(*bc, "maxStrain", false, 0.0);
+    c.amplitude = get_double(*bc, "amplitude", false, c.amplitude);
+    c.riseTime = get_double(*bc, "riseTime", false, c.riseTime);
+    (void)get_bool(*bc, "transverseConstraint", false, false);
+    c.supportRegion = get_string(*bc, "supportRegion", false, c.supportRegion);
+    c.loadRegion = get_string(*bc, "loadRegion", false, c.loadRegion);
+    c.component = get_string(*bc, "component", false, c.component);
+    c.regionLayers = get_int(*bc, "regionLayers", false, c.regionLayers);
+    c.planeStrain = get_bool(*bc, "planeStrain", false, c.planeStrain);
+    c.rigidBodySuppression = get_bool(*bc, "rigidBodySuppression", false, false);
+  }
+  if (const Value *d = find(root, "dynamics"); d && d->kind == Value::Object) {
+    c.density = get_double(*d, "density", false, c.density);
+    c.energyTol = get_double(*d, "energyToleranceRelative", false, c.energyTol);
+    c.netForceTol = get_double(*d, "netForceTolerance", false, c.netForceTol);
+    c.outputEvery = get_int(*d, "outputEvery", false, c.outputEvery);
+    c.snapshotEvery = get_int(*d, "snapshotEvery", false, c.snapshotEvery);
+    c.minPlastic = get_double(*d, "minimumPlasticStrain", false, c.minPlastic);
+    c.kineticLimit = get_double(*d, "kineticToInternalLimit", false, c.kineticLimit);
+  }
+  if (const Value *d = find(root, "damage"); d && d->kind == Value::Object) {
+    const std::string model = get_string(*d, "model", false, "");
+    if (model == "critical_stretch" || model == "Critical Stretch")
+      c.damageMode = "critical_stretch";
+    else if (!model.empty())
+      c.damageMode = model;
+    c.damageInitiation = get_double(*d, "initiation", false, c.damageInitiation);
+    if (c.damageInitiation <= 0.0)
+      c.damageInitiation = get_double(*d, "criticalStretch", false, c.damageInitiation);
+    c.damageFailure = get_double(*d, "failure", false, c.damageFailure);
+    (void)get_bool(*d, "deletionEnabled", false, false);
+  }
+  c.outputDir = "validation_cases/output/" + c.name;
+  if (const Value *o = find(root, "output"); o && o->kind == Value::Object)
+    c.outputDir = get_string(*o, "directory", false, c.outputDir);
+  if (const Value *ob = find(root, "observation")) {
+    if (ob->kind != Value::Object)
+      throw std::runtime_error("observation must be a JSON object.");
+    c.axialMin = get_double(*ob, "axialMin", false, c.axialMin);
+    c.axialMax = get_double(*ob, "axialMax", false, c.axialMax);
+  }
+  const Value *mats = find(root, "materials");
+  if (!mats || mats->kind != Value::Array)
+    throw std::runtime_error("Demo case requires a materials array.");
+  for (const Value &e : mats->array) {
+    if (e.kind != Value::Object)
+      throw std::runtime_error("Each materials entry must be a JSON object.");
+    c.materials.push_back(load_case_material(e, pluginDir));
+  }
+  if (c.totalTime <= 0.0)
+    c.totalTime = c.dt * static_cast<double>(c.steps);
+  if (c.type != "vumat_pd_demo_case" && c.type != "vumat_pd_boundary_case" &&
+      c.type != "vumat_material_point_case")
+    throw std::runtime_error("Unsupported case type: " + c.type);
+  if (c.type != "vumat_pd_boundary_case")
+    throw std::runtime_error("The GPU case runner runs vumat_pd_boundary_case files; got: " +
+                             c.type);
+  if (c.spacing <= 0.0 || c.horizon <= 0.0)
+    throw std::runtime_error("PD geometry requires positive spacing and horizon.");
+  if (c.geometryType != "patch")
+    throw std::runtime_error("Unsupported geometry type: " + c.geometryType);
+  if (c.nx < 2 || c.ny < 2 || c.nz < 2)
+    throw std::runtime_error("PD patch geometry requires nx/ny/nz >= 2.");
+  if (c.steps < 1 || !std::isfinite(c.dt) || c.dt <= 0.0)
+    throw std::runtime_error("time_integration requires steps >= 1 and dt > 0.");
+  const double expected = c.dt * c.steps;
+  if (!std::isfinite(expected) || !std::isfinite(c.totalTime))
+    throw std::runtime_error("time_integration duration must be finite.");
+  if (std::abs(c.totalTime - expected) > 1.0e-12 * std::max(1.0, expected))
+    throw std::runtime_error("time_integration.totalTime must equal steps * dt.");
+  if (c.bcType != "regional_smooth_displacement" || c.amplitude == 0.0 || c.riseTime <= 0.0 ||
+      c.regionLayers < 1 || !c.rigidBodySuppression)
+    throw std::runtime_error(
+        "Boundary-value case requires regional smooth loading and rigid-body suppression.");
+  const bool xCase = c.component == "x" && c.supportRegion == "left" && c.loadRegion == "right";
+  const bool yCase = c.component == "y" && c.supportRegion == "bottom" && c.loadRegion == "top";
+  if (!xCase && !yCase)
+    throw std::runtime_error(
+        "Regional boundary case supports left/right-x or bottom/top-y loading.");
+  if (c.nz < 3 || c.horizon / c.spacing < 1.5)
+    throw std::runtime_error("PD boundary cases require nz>=3 and horizon/spacing>=1.5.");
+  if (c.density <= 0.0 || c.outputEvery < 1 || c.snapshotEvery < 1 || c.energyTol <= 0.0 ||
+      c.netForceTol <= 0.0 || c.kineticLimit <= 0.0)
+    throw std::runtime_error("Invalid boundary-value dynamics configuration.");
+  if (c.unitLength != "mm" || c.unitTime != "ms" || c.unitForce != "N" || c.unitStress != "MPa" ||
+      c.unitMassDensity != "N*ms^2/mm^4")
+    throw std::runtime_error(
+        "Boundary-value cases currently require the documented mm-ms-N-MPa unit system.");
+  if (c.materials.empty())
+    throw std::runtime_error("Demo case requires at least one material.");
+  return c;
+}
+
+// ---- output formatting (iostream's default float format at the stream's precision) -------
+
+std::string num(double x, int precision) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.*g", precision, x);
+  return buf;
+}
+
+std::string json_number(double x) { return std::isfinite(x) ? num(x, 15) : "null"; }
+
+struct RunSummary { // BoundaryValueCaseRunner.cpp:32-52
+  std::string material;
+  bool pass = false, finite = true;
+  double runtimeSeconds = 0, maxNetInternalForce = 0, finalEnergyError = 0, maxEnergyError = 0,
+         maxKineticToInternal = 0, finalKineticToInternal = 0, maxEqps = 0,
+         maxInternalIndicator = 0, earlyElasticSlope = 0, analyticalPlaneStrainSlope = 0,
+         earlyElasticRelativeError = 0, finalReaction = 0, finalNominalStress = 0,
+         finalEngineeringStrain = 0, observationMin = 0, observationMax = 0;
+};
+
+void check(vpd_context *ctx, int status) {
+  if (status != VPD_OK)
+    throw std::runtime_error(vpd_last_error(ctx));
+}
+
+struct ContextDeleter {
+  void operator()(vpd_context *c) const { vpd_destroy(c); }
+};
+
+double internal_indicator(const CaseMaterial &m, const double *state) {
+  if (m.indicatorIndices.empty())
+    return 0.0;
+  double value = m.indicatorMin ? std::numeric_limits<double>::infinity()
+                                : -std::numeric_limits<double>::infinity();
+  for (int k : m.indicatorIndices)
+    value = m.indicatorMin ? std::min(value, state[k]) : std::max(value, state[k]);
+  return m.indicatorOneMinus ? 1.0 - value : value;
+}
+
+std::vector<char> region(const std::vector<double> &x, int64_t n, const char *side,
+                         double width) {
+  std::vector<uint8_t> mask(n);
+  const int64_t count = vpd_select_boundary_region(x.data(), n, side, width, mask.data());
+  if (count == -VPD_ERR_ARGUMENT && width > 0.0)
+    throw std::runtime_error("Boundary-region selection returned no particles.");
+  if (count < 0)
+    throw std::runtime_error("Boundary-region width and particle set must be nonzero.");
+  return std::vector<char>(mask.begin(), mask.end());
+}
+
+// meanCoordinate (BoundaryValueCaseRunner.cpp:85-93) over a mask, in index order.
+double mean_of(const std::vector<double> &field, const std::vector<char> &mask, int comp) {
+  double sum = 0.0;
+  int64_t count = 0;
+  for (size_t i = 0; i < mask.size(); ++i)
+    if (mask[i]) {
+      sum += field[3 * i + comp];
+      ++count;
+    }
+  return sum / static_cast<double>(count);
+}
+
+// runMaterial (BoundaryValueCaseRunner.cpp:139-307) with the particle work on the GPU.
+RunSummary run_material(const CaseConfig &cfg, const CaseMaterial &mat, std::ostream &ts,
+                        std::ostream &snap) {
+  const auto started = std::chrono::steady_clock::now();
+  const int64_t n = static_cast<int64_t>(cfg.nx) * cfg.ny * cfg.nz;
+  std::vector<double> x(3 * n), vol(n);
+  if (vpd_make_patch(cfg.nx, cfg.ny, cfg.nz, cfg.spacing, x.data(), vol.data()) != VPD_OK)
+    throw std::runtime_error("Particle patch requires positive dimensions and spacing.");
+  const int axis = cfg.component == "y" ? 1 : 0;
+  const int lateral = axis == 0 ? 1 : 0;
+  const double width = static_cast<double>(cfg.regionLayers - 1) * cfg.spacing + 1.0e-9 * cfg.spacing;
+  const auto support = region(x, n, cfg.supportRegion.c_str(), width);
+  const auto loaded = region(x, n, cfg.loadRegion.c_str(), width);
+  std::vector<uint8_t> codes(n);
+  int64_t anchor = 0, pin = 0;
+  if (vpd_regional_constraint_codes(x.data(), n, axis, cfg.regionLayers, cfg.spacing,
+                                    cfg.planeStrain ? 1 : 0, codes.data(), &anchor,
+                                    &pin) != VPD_OK)
+    throw std::runtime_error("Boundary-region selection returned no particles.");
+  const double loadingSign = cfg.amplitude > 0.0 ? 1.0 : -1.0;
+  const double gaugeLength = std::abs(mean_of(x, loaded, axis) - mean_of(x, support, axis));
+  const auto latNeg = region(x, n, lateral == 0 ? "left" : "bottom", 1.0e-9 * cfg.spacing);
+  const auto latPos = region(x, n, lateral == 0 ? "right" : "top", 1.0e-9 * cfg.spacing);
+  const double lateralGauge = std::abs(mean_of(x, latPos, lateral) - mean_of(x, latNeg, lateral));
+  int64_t countNeg = 0, countPos = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    countNeg += latNeg[i] ? 1 : 0;
+    countPos += latPos[i] ? 1 : 0;
+  }
+
+  // observationBounds / observationWeights (include/ObservationWindow.hpp:13-58).
+  double lo = x[axis], hi = lo;
+  for (int64_t i = 0; i < n; ++i) {
+    lo = std::min(lo, x[3 * i + axis]);
+    hi = std::max(hi, x[3 * i + axis]);
+  }
+  const double band = (cfg.regionLayers - 1) * cfg.spacing;
+  const double obsA = std::isnan(cfg.axialMin) ? lo + band : cfg.axialMin;
+  const double obsB = std::isnan(cfg.axialMax) ? hi - band : cfg.axialMax;
+  if (!std::isfinite(obsA) || !std::isfinite(obsB) || obsA < lo || obsB > hi || obsA >= obsB)
+    throw std::runtime_error(
+        "Observation window must lie within the body and have positive length.");
+  const double tol = 1e-8 * cfg.spacing;
+  std::vector<double> weight(n, 0.0);
+  bool foundA = false, foundB = false;
+  for (int64_t i = 0; i < n; ++i) {
+    const double xi = x[3 * i + axis];
+    const bool atA = std::abs(xi - obsA) < tol, atB = std::abs(xi - obsB) < tol;
+    foundA = foundA || atA;
+    foundB = foundB || atB;
+    if (xi < obsA - tol || xi > obsB + tol)
+      continue;
+    const bool interiorEnd = (atA && obsA > lo + tol) || (atB && obsB < hi - tol);
+    weight[i] = vol[i] * (interiorEnd ? 0.5 : 1.0);
+  }
+  if (!foundA || !foundB)
+    throw std::runtime_error("Observation bounds must coincide with reference nodal planes.");
+  double interiorVolume = 0.0;
+  for (int64_t i = 0; i < n; ++i)
+    if (weight[i] > 0.0)
+      interiorVolume += weight[i];
+  const int crossCount = axis == 0 ? cfg.ny : cfg.nx;
+  const double crossSectionArea =
+      static_cast<double>((crossCount - 1) * (cfg.nz - 1)) * cfg.spacing * cfg.spacing;
+
+  // The material must be the device's Johnson-Cook VUMAT.
+  const std::string plugin = std::filesystem::path(mat.material.pluginPath).filename().string();
+  std::string cm = mat.material.cmname;
+  std::transform(cm.begin(), cm.end(), cm.begin(), [](unsigned char ch) { return std::toupper(ch); });
+  if (cm.rfind("JOHNSONCOOK", 0) != 0 && plugin.rfind("jc_vumat_plugin", 0) != 0)
+    throw std::runtime_error("Only the Johnson-Cook VUMAT runs on the device; material " +
+                             mat.label + " uses " + plugin);
+  vpd_config vc{};
+  vc.horizon = cfg.horizon;
+  vc.density = cfg.density;
+  vc.nprops = static_cast<int>(mat.material.props.size());
+  vc.props = mat.material.props.data();
+  vc.nstatev = mat.material.nstatev;
+  vc.initial_state = mat.initialState.empty() ? nullptr : mat.initialState.data();
+  vc.reference_temperature = mat.referenceTemperature;
+  vc.char_length = cfg.spacing;
+  vc.material_density = mat.material.density;
+  if (cfg.damageMode == "none") {
+    vc.damage_mode = VPD_DAMAGE_NONE;
+  } else if (cfg.damageMode == "critical_stretch") {
+    if (cfg.damageInitiation <= 0.0)
+      throw std::runtime_error("critical_stretch requires initiation > 0.");
+    vc.damage_mode = VPD_DAMAGE_CRITICAL_STRETCH;
+  } else {
+    throw std::runtime_error("Unsupported damage model on the device: " + cfg.damageMode);
+  }
+  vc.damage_initiation = cfg.damageInitiation;
+  vc.damage_failure = cfg.damageFailure;
+  vc.return_map = VPD_RETURN_MAP_BISECTION;
+  vpd_context *raw = nullptr;
+  if (vpd_create(&vc, &raw) != VPD_OK)
+    throw std::runtime_error(vpd_last_error(nullptr));
+  std::unique_ptr<vpd_context, ContextDeleter> owner(raw);
+  vpd_context *ctx = raw;
+  check(ctx, vpd_set_particles(ctx, x.data(), vol.data(), n));
+  int64_t nnz = 0;
+  check(ctx, vpd_build_neighbors(ctx, &nnz));
+  check(ctx, vpd_set_constraints(ctx, codes.data()));
+  check(ctx, vpd_initialize_shape(ctx));
+  check(ctx, vpd_initialize_material(ctx));
+  check(ctx, vpd_initialize_dynamics(ctx));
+  double ramp0[3];
+  vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, 0.0, ramp0);
+  check(ctx, vpd_apply_constraints(ctx, ramp0));
+
+  std::vector<uint8_t> latSets(n);
+  for (int64_t i = 0; i < n; ++i)
+    latSets[i] = (latNeg[i] ? 1 : 0) | (latPos[i] ? 2 : 0);
+  vpd_diag_config dc{};
+  dc.load_axis = axis;
+  dc.lateral_axis = lateral;
+  dc.observation_weight = weight.data();
+  dc.lateral_sets = latSets.data();
+  dc.eqps_index = mat.eqpsIndex;
+  dc.indicator_count = static_cast<int>(mat.indicatorIndices.size());
+  if (dc.indicator_count > 8)
+    throw std::runtime_error("At most 8 internal-indicator state indices run on the device.");
+  for (int k = 0; k < dc.indicator_count; ++k)
+    dc.indicator_indices[k] = mat.indicatorIndices[k];
+  dc.indicator_min = mat.indicatorMin ? 1 : 0;
+  dc.indicator_one_minus = mat.indicatorOneMinus ? 1 : 0;
+  dc.capacity = std::min(cfg.steps, 2048);
+  check(ctx, vpd_enable_diagnostics(ctx, &dc));
+
+  RunSummary summary;
+  summary.material = mat.label;
+  summary.observationMin = obsA;
+  summary.observationMax = obsB;
+  summary.analyticalPlaneStrainSlope = mat.elasticE / (1.0 - mat.elasticNu * mat.elasticNu);
+  double internalEnergy = 0.0, externalWork = 0.0, previousPower = 0.0;
+  double slopeX2 = 0.0, slopeXY = 0.0;
+  const double dt = cfg.dt;
+  std::vector<double> records(static_cast<size_t>(dc.capacity) * VPD_DIAG_STRIDE);
+  int processed = 0; // steps whose record has been consumed
+
+  // Host half of the per-step diagnostics (BoundaryValueCaseRunner.cpp:240-305).
+  auto consume = [&](const double *r, int step) {
+    const double time = static_cast<double>(step) * dt;
+    double ramp[3];
+    vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, time, ramp);
+    internalEnergy += r[VPD_DIAG_INTERNAL_ENERGY];
+    const double net = std::sqrt(r[VPD_DIAG_NET_FORCE_X] * r[VPD_DIAG_NET_FORCE_X] +
+                                 r[VPD_DIAG_NET_FORCE_Y] * r[VPD_DIAG_NET_FORCE_Y] +
+                                 r[VPD_DIAG_NET_FORCE_Z] * r[VPD_DIAG_NET_FORCE_Z]);
+    const double reaction = r[VPD_DIAG_REACTION];
+    const double averageStress = loadingSign * r[VPD_DIAG_OBSERVED_STRESS] / interiorVolume;
+    const double averageAxialStrain = loadingSign * r[VPD_DIAG_OBSERVED_STRAIN] / interiorVolume;
+    const double maxEqps = r[VPD_DIAG_MAX_EQPS];
+    const double maxIndicator = r[VPD_DIAG_MAX_INDICATOR];
+    summary.finite = summary.finite && r[VPD_DIAG_NONFINITE_MATERIAL] == 0.0 &&
+                     r[VPD_DIAG_NONFINITE_FORCE] == 0.0;
+    const double power = reaction * ramp[1];
+    externalWork += 0.5 * dt * (previousPower + power);
+    previousPower = power;
+    const double kinetic = r[VPD_DIAG_KINETIC];
+    const double scale =
+        std::max({std::abs(externalWork), std::abs(internalEnergy) + kinetic, 1.0e-12});
+    const double energyError = std::abs(kinetic + internalEnergy - externalWork) / scale;
+    const double kineticRatio = kinetic / std::max(std::abs(internalEnergy), 1.0e-12);
+    const double engineeringStrain = loadingSign * ramp[0] / gaugeLength;
+    const double lateralStrain = (r[VPD_DIAG_LATERAL_POS] / static_cast<double>(countPos) -
+                                  r[VPD_DIAG_LATERAL_NEG] / static_cast<double>(countNeg)) /
+                                 lateralGauge;
+    const double nominalStress = loadingSign * reaction / crossSectionArea;
+    summary.maxNetInternalForce = std::max(summary.maxNetInternalForce, net);
+    summary.maxEnergyError = std::max(summary.maxEnergyError, energyError);
+    summary.maxKineticToInternal = std::max(summary.maxKineticToInternal, kineticRatio);
+    summary.finalKineticToInternal = kineticRatio;
+    summary.maxEqps = std::max(summary.maxEqps, maxEqps);
+    summary.maxInternalIndicator = std::max(summary.maxInternalIndicator, maxIndicator);
+    summary.finalEnergyError = energyError;
+    summary.finalReaction = reaction;
+    summary.finalNominalStress = nominalStress;
+    summary.finalEngineeringStrain = engineeringStrain;
+    summary.finite = summary.finite && std::isfinite(kinetic) && std::isfinite(internalEnergy) &&
+                     std::isfinite(externalWork) && std::isfinite(energyError);
+    if (engineeringStrain > 1.0e-6 && engineeringStrain <= 1.0e-3) {
+      slopeX2 += engineeringStrain * engineeringStrain;
+      slopeXY += engineeringStrain * averageStress;
+    }
+    if (step % cfg.outputEvery == 0 || step == 1 || step == cfg.steps) {
+      const double vals[] = {time, ramp[0], ramp[1], reaction, engineeringStrain, lateralStrain,
+                             nominalStress, averageAxialStrain, averageStress, maxEqps,
+                             maxIndicator, kinetic, internalEnergy, externalWork, energyError,
+                             net};
+      ts << mat.label << ',' << step;
+      for (double v : vals)
+        ts << ',' << num(v, 16);
+      ts << ',' << (summary.finite ? 1 : 0) << '\n';
+    }
+  };
+  auto drain = [&] {
+    int count = 0;
+    check(ctx, vpd_fetch_diagnostics(ctx, records.data(), dc.capacity, &count));
+    for (int k = 0; k < count; ++k)
+      consume(&records[static_cast<size_t>(k) * VPD_DIAG_STRIDE], ++processed);
+  };
+
+  const int nstatev = mat.material.nstatev;
+  std::vector<double> u(3 * n), v(3 * n), stress(9 * n), strain(9 * n), dmg(n),
+      statev(static_cast<size_t>(nstatev) * n);
+  int pending = 0;
+  for (int step = 1; step <= cfg.steps; ++step) {
+    const double time = static_cast<double>(step) * dt;
+    const bool snapshot = step % cfg.snapshotEvery == 0 || step == cfg.steps;
+    vpd_step_args a{};
+    a.dt = dt;
+    a.time = time;
+    a.step_time = (step - 1) * dt;
+    vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, time, a.ramp);
+    // The next predictor is fused into this step unless the state after this step is read.
+    a.fuse_next = snapshot ? 0 : 1;
+    if (a.fuse_next)
+      vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, time + dt, a.ramp_next);
+    a.want_fields = snapshot ? 1 : 0;
+    if (pending == dc.capacity) {
+      drain();
+      pending = 0;
+    }
+    check(ctx, vpd_step(ctx, &a));
+    ++pending;
+    if (!snapshot)
+      continue;
+    drain();
+    pending = 0;
+    check(ctx, vpd_download(ctx, VPD_FIELD_DISPLACEMENT, u.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_VELOCITY, v.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_STRESS, stress.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_STRAIN, strain.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_STATEV, statev.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_DAMAGE_INDEX, dmg.data()));
+    // writeSnapshot (BoundaryValueCaseRunner.cpp:114-136).
+    std::string line;
+    for (int64_t i = 0; i < n; ++i) {
+      const double *x0 = &x[3 * i], *ui = &u[3 * i], *vi = &v[3 * i];
+      const double *st = &statev[static_cast<size_t>(i) * nstatev];
+      const double eqps = mat.eqpsIndex >= 0 ? st[mat.eqpsIndex] : 0.0;
+      const double vals[] = {x0[0], x0[1], x0[2], ui[0], ui[1], ui[2],
+                             x0[0] + ui[0], x0[1] + ui[1], x0[2] + ui[2], vi[0], vi[1], vi[2],
+                             strain[9 * i + 4 * axis], stress[9 * i + 4 * axis], eqps,
+                             internal_indicator(mat, st), dmg[i]};
+      line = mat.label + ',' + std::to_string(step) + ',' + num(time, 16) + ',' +
+             std::to_string(i);
+      for (double val : vals)
+        line += ',' + num(val, 16);
+      line += '\n';
+      snap << line;
+    }
+  }
+  if (pending)
+    drain();
+
+  summary.earlyElasticSlope = slopeX2 > 0.0 ? slopeXY / slopeX2 : 0.0;
+  summary.earlyElasticRelativeError =
+      summary.analyticalPlaneStrainSlope > 0.0
+          ? std::abs(summary.earlyElasticSlope - summary.analyticalPlaneStrainSlope) /
+                summary.analyticalPlaneStrainSlope
+          : 0.0;
+  summary.runtimeSeconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+  summary.pass = summary.finite && summary.maxNetInternalForce <= cfg.netForceTol &&
+                 summary.finalEnergyError <= cfg.energyTol &&
+                 summary.finalKineticToInternal <= cfg.kineticLimit &&
+                 summary.maxEqps >= cfg.minPlastic;
+  return summary;
+}
+
+// runBoundaryValueCase (BoundaryValueCaseRunner.cpp:309-440).
+int run_boundary_case(const CaseConfig &cfg) {
+  std::filesystem::create_directories(cfg.outputDir);
+  std::ofstream ts(cfg.outputDir + "/timeseries.csv");
+  std::ofstream snap(cfg.outputDir + "/field_snapshots.csv");
+  if (!ts || !snap)
+    throw std::runtime_error("Failed to open boundary-value output files.");
+  ts << "material,step,time_ms,load_displacement_mm,load_velocity_mm_per_ms,"
+        "reaction_force_N,engineering_strain,lateral_strain,nominal_stress_MPa,"
+        "average_axial_strain,average_axial_stress_MPa,max_eqps,max_internal_indicator,"
+        "kinetic_energy_Nmm,internal_energy_Nmm,external_work_Nmm,relative_energy_error,"
+        "net_internal_force_N,finite\n";
+  snap << "material,step,time_ms,particle,x0_mm,y0_mm,z0_mm,ux_mm,uy_mm,uz_mm,"
+          "x_mm,y_mm,z_mm,vx_mm_per_ms,vy_mm_per_ms,vz_mm_per_ms,axial_strain,"
+          "axial_stress_MPa,eqps,internal_indicator,simple_damage\n";
+  std::vector<RunSummary> summaries;
+  const long particles = static_cast<long>(cfg.nx) * cfg.ny * cfg.nz;
+  for (const auto &m : cfg.materials) {
+    RunSummary s = run_material(cfg, m, ts, snap);
+    std::cout << cfg.name << '/' << s.material << ": particles=" << particles
+              << " runtime_s=" << s.runtimeSeconds << " energy_error=" << s.finalEnergyError
+              << " max_eqps=" << s.maxEqps << " pass=" << (s.pass ? "true" : "false") << '\n';
+    summaries.push_back(s);
+  }
+  const bool pass = std::all_of(summaries.begin(), summaries.end(),
+                                [](const RunSummary &s) { return s.pass; });
+  std::ofstream md(cfg.outputDir + "/metadata.json");
+  md << "{\n  \"name\": \"" << cfg.name << "\",\n"
+     << "  \"solver\": \"correspondence peridynamics\",\n"
+     << "  \"status\": \"" << (pass ? "pass" : "fail") << "\",\n"
+     << "  \"classification\": \"" << cfg.classification << "\",\n"
+     << "  \"scope\": \"" << cfg.scope << "\",\n"
+     << "  \"units\": {\"length\": \"mm\", \"time\": \"ms\", \"force\": \"N\", "
+        "\"stress\": \"MPa\", \"solverMassDensity\": \"N*ms^2/mm^4\", "
+        "\"sourceDensity\": \"kg/m^3\"},\n"
+     << "  \"densityConversion\": \"rho_solver = rho_kg_per_m3 * 1e-6\",\n"
+     << "  \"particles\": " << particles << ",\n"
+     << "  \"grid\": [" << cfg.nx << ',' << cfg.ny << ',' << cfg.nz
+     << "],\n  \"spacing_mm\": " << num(cfg.spacing, 6)
+     << ",\n  \"horizon_mm\": " << num(cfg.horizon, 6)
+     << ",\n  \"horizonSpacingRatio\": " << num(cfg.horizon / cfg.spacing, 6)
+     << ",\n  \"neighborSearch\": \"cell-linked-list\",\n"
+     << "  \"planeStrain\": " << (cfg.planeStrain ? "true" : "false")
+     << ",\n  \"outerSimpleDamage\": " << (cfg.damageMode != "none" ? "true" : "false") << ",\n"
+     << "  \"zeroEnergyStabilization\": \"none\",\n"
+     << "  \"timeStepPolicy\": \"user supplied; no computed stability bound\",\n"
+     << "  \"boundarySets\": {\"support\": \"" << cfg.supportRegion << "\", \"loaded\": \""
+     << cfg.loadRegion << "\", \"regionLayers\": " << cfg.regionLayers
+     << ", \"rigidBodySuppression\": \"two lateral components at support centre and z at "
+        "support upper midplane point\"},\n"
+     << "  \"dt_ms\": " << num(cfg.dt, 6) << ",\n  \"steps\": " << cfg.steps
+     << ",\n  \"materials\": [\n";
+  for (size_t i = 0; i < summaries.size(); ++i) {
+    const auto &s = summaries[i];
+    const auto &m = cfg.materials[i];
+    md << "    {\"name\": \"" << s.material << "\", \"status\": \"" << (s.pass ? "pass" : "fail")
+       << "\", \"plugin_file\": \""
+       << std::filesystem::path(m.material.pluginPath).filename().string()
+       << "\", \"sourceDensity_kg_per_m3\": " << json_number(m.material.density)
+       << ", \"solverDensity\": " << json_number(cfg.density)
+       << ", \"runtimeSeconds\": " << json_number(s.runtimeSeconds)
+       << ", \"observationAxis\": \"" << cfg.component << "\""
+       << ", \"observationMin_mm\": " << json_number(s.observationMin)
+       << ", \"observationMax_mm\": " << json_number(s.observationMax)
+       << ", \"maxNetInternalForce_N\": " << json_number(s.maxNetInternalForce)
+       << ", \"finalEnergyError\": " << json_number(s.finalEnergyError)
+       << ", \"maxEnergyError\": " << json_number(s.maxEnergyError)
+       << ", \"maxKineticToInternal\": " << json_number(s.maxKineticToInternal)
+       << ", \"finalKineticToInternal\": " << json_number(s.finalKineticToInternal)
+       << ", \"maxEqps\": " << json_number(s.maxEqps)
+       << ", \"maxInternalIndicator\": " << json_number(s.maxInternalIndicator)
+       << ", \"earlyElasticSlope_MPa\": " << json_number(s.earlyElasticSlope)
+       << ", \"analyticalPlaneStrainSlope_MPa\": " << json_number(s.analyticalPlaneStrainSlope)
+       << ", \"earlyElasticRelativeError\": " << json_number(s.earlyElasticRelativeError)
+       << ", \"finalReaction_N\": " << json_number(s.finalReaction)
+       << ", \"finalNominalStress_MPa\": " << json_number(s.finalNominalStress)
+       << ", \"finalEngineeringStrain\": " << json_number(s.finalEngineeringStrain)
+       << ", \"finite\": " << (s.finite ? "true" : "false") << "}"
+       << (i + 1 == summaries.size() ? "\n" : ",\n");
+  }
+  md << "  ],\n  \"acceptance\": {\"energyToleranceRelative\": " << num(cfg.energyTol, 6)
+     << ", \"netForceTolerance_N\": " << num(cfg.netForceTol, 6)
+     << ", \"minimumPlasticStrain\": " << num(cfg.minPlastic, 6)
+     << ", \"kineticToInternalLimit\": " << num(cfg.kineticLimit, 6) << "},\n"
+     << "  \"materialComparisonCaveat\": \"Internal state variables are model-specific observers "
+        "and are not mapped to a common bond-breaking rule.\",\n"
+     << "  \"parameterMatchingPolicy\": \"Only identifiable macroscopic properties are matched; "
+        "formulation-specific or hard-coded parameters are not claimed to be equivalent.\",\n"
+     << "  \"legacyBlockPolicy\": \"Material-independent fixed-width blocks of 256 points; a "
+        "non-full final block duplicates its last valid point and discards padded outputs.\"\n}\n";
+  return pass ? 0 : 2;
+}
+
+} // namespace
+
+extern "C" int vpd_run_case_json(const char *case_path, const char *plugin_directory) {
+  try {
+    if (!case_path)
+      throw std::runtime_error("Failed to open JSON file: (null)");
+    const CaseConfig cfg = load_case(case_path, plugin_directory ? plugin_directory : "");
+    return run_boundary_case(cfg);
+  } catch (const std::exception &error) {
+    std::cerr << "VUMAT-PD demo case failed: " << error.what() << "\n";
+    return 1;
+  }
+}

@@ -52,6 +52,21 @@ class KernelTime(C.Structure):
     _fields_ = [("name", C.c_char_p), ("total_ms", C.c_double), ("launches", C.c_int64)]
 
 
+class DiagConfig(C.Structure):
+    _fields_ = [("load_axis", C.c_int), ("lateral_axis", C.c_int),
+                ("observation_weight", C.c_void_p), ("lateral_sets", C.c_void_p),
+                ("eqps_index", C.c_int), ("indicator_count", C.c_int),
+                ("indicator_indices", C.c_int * 8), ("indicator_min", C.c_int),
+                ("indicator_one_minus", C.c_int), ("capacity", C.c_int)]
+
+
+# Slots of a run-diagnostics record (enum vpd_diag_slot).
+DIAG_SLOTS = ("internal_energy", "observed_stress", "observed_strain", "max_eqps",
+              "max_indicator", "nonfinite_material", "net_force_x", "net_force_y", "net_force_z",
+              "reaction", "kinetic", "lateral_neg", "lateral_pos", "nonfinite_force")
+DIAG_STRIDE = 16
+
+
 _lib = None
 
 
@@ -101,6 +116,11 @@ def lib():
                                                     C.POINTER(C.c_int64)]
         L.vpd_smooth_ramp.argtypes = [C.c_double, C.c_double, C.c_double, _D]
         L.vpd_smooth_ramp.restype = None
+        L.vpd_select_boundary_region.argtypes = [_D, C.c_int64, C.c_char_p, C.c_double, _U8]
+        L.vpd_select_boundary_region.restype = C.c_int64
+        L.vpd_run_case_json.argtypes = [C.c_char_p, C.c_char_p]
+        L.vpd_enable_diagnostics.argtypes = [P, C.POINTER(DiagConfig)]
+        L.vpd_fetch_diagnostics.argtypes = [P, _D, C.c_int, C.POINTER(C.c_int)]
         L.vpd_set_return_map.argtypes = [P, C.c_int, C.c_double]
         L.vpd_attach_warm_start.argtypes = [P, P]
         _F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -176,6 +196,22 @@ def smooth_ramp(amplitude, rise_time, t):
     out = np.zeros(3)
     lib().vpd_smooth_ramp(amplitude, rise_time, t, out)
     return out
+
+
+def select_boundary_region(x, side, width):
+    """selectBoundaryRegion (ParticleSets.cpp:12-50) as a boolean mask."""
+    x = _f64(x)
+    mask = np.zeros(x.shape[0], np.uint8)
+    count = lib().vpd_select_boundary_region(x, x.shape[0], side.encode(), width, mask)
+    if count < 0:
+        raise VpdError("boundary-region selection failed for side %r, width %r" % (side, width))
+    return mask.astype(bool)
+
+
+def run_case_json(case_path, plugin_directory=""):
+    """runDemoCaseFromJson on the GPU for a vumat_pd_boundary_case file: 0 pass, 2 fail,
+    1 error (message on stderr)."""
+    return lib().vpd_run_case_json(os.fsencode(case_path), os.fsencode(plugin_directory))
 
 
 # ------------------------------------------------------------------ context
@@ -338,6 +374,40 @@ class Context:
 
     def set_timing(self, on=True):
         self._check(lib().vpd_set_timing(self.h, int(on)))
+
+    def enable_diagnostics(self, load_axis=0, lateral_axis=1, observation_weight=None,
+                           lateral_sets=None, eqps_index=1, indicator=(), indicator_min=False,
+                           indicator_one_minus=False, capacity=1024):
+        """Per-step run diagnostics (vpd_enable_diagnostics); None disables."""
+        if load_axis is None:
+            self._check(lib().vpd_enable_diagnostics(self.h, None))
+            return
+        cfg = DiagConfig()
+        cfg.load_axis, cfg.lateral_axis = load_axis, lateral_axis
+        keep = []
+        if observation_weight is not None:
+            w = _f64(observation_weight)
+            keep.append(w)
+            cfg.observation_weight = w.ctypes.data
+        if lateral_sets is not None:
+            ls = np.ascontiguousarray(lateral_sets, np.uint8)
+            keep.append(ls)
+            cfg.lateral_sets = ls.ctypes.data
+        cfg.eqps_index = eqps_index
+        cfg.indicator_count = len(indicator)
+        for k, v in enumerate(indicator):
+            cfg.indicator_indices[k] = v
+        cfg.indicator_min, cfg.indicator_one_minus = int(indicator_min), int(indicator_one_minus)
+        cfg.capacity = capacity
+        self._diag_capacity = capacity
+        self._check(lib().vpd_enable_diagnostics(self.h, C.byref(cfg)))
+
+    def fetch_diagnostics(self):
+        """The pending step records as an array (steps, DIAG_STRIDE)."""
+        buf = np.zeros((self._diag_capacity, DIAG_STRIDE))
+        cnt = C.c_int()
+        self._check(lib().vpd_fetch_diagnostics(self.h, buf, self._diag_capacity, C.byref(cnt)))
+        return buf[:cnt.value].copy()
 
     def collect_timing(self):
         buf = (KernelTime * 16)()
