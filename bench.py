@@ -92,6 +92,75 @@ def cpu_baseline(sample_steps, w):
                 % (sample_steps, n, threads), seconds=dt)
 
 
+def timed_run(bar, steps, warmup):
+    """Warmup steps, reset to the state before step 1, then `steps` timed steps. Returns
+    (seconds, per-kernel HIP-event times)."""
+    if warmup:
+        bar.step(warmup)
+        bar.ctx.synchronize()
+        bar.reset()
+    bar.ctx.set_timing(True)
+    bar.ctx.synchronize()
+    t0 = time.perf_counter()
+    bar.step(steps)
+    bar.ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    return elapsed, bar.ctx.collect_timing()
+
+
+def return_map_study(points, seed=4321):
+    """The return-map solvers on `points` plastic material points logged from seed-fixed plain
+    runs of the 30×8×8 JC bar (the warm start's training workload, a held-out seed): kernel
+    time, mean iterations and the largest stress deviation from the FP64 tight bisection
+    (|r| ≤ 1e-14·q), per solver. The learned time includes the network's inference."""
+    import numpy as np
+    import vpd
+    import warm_start as ws
+
+    rec = ws.log_pairs(points, seed=seed, verbose=False)
+    m = len(rec["y"])
+    inc = np.ascontiguousarray(rec["inc"].T)
+    stress = np.ascontiguousarray(rec["stress"].T)
+    state = np.ascontiguousarray(rec["state"].T)
+    z = np.zeros(m)
+    dt = ws.cfl_dt(1.0)
+    props = ws.JC_PROPS
+    judge = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, props, dt, inc, stress, state, z, z,
+                           rtol=1e-14)
+    scale = np.abs(judge["stress"]).max()
+    net = vpd.load_warm_start()
+    out = {"points": m, "source": "plastic points of 30x8x8 JC bar plain runs, seed %d" % seed}
+    for name, mode in (("bisection_parity", vpd.RETURN_MAP_BISECTION),
+                       ("bisection_tight", vpd.RETURN_MAP_BISECTION_TIGHT),
+                       ("newton_state5_seed", vpd.RETURN_MAP_NEWTON),
+                       ("newton_learned_seed", vpd.RETURN_MAP_LEARNED)):
+        best = None
+        for _ in range(3):  # the fastest of three launches
+            if mode == vpd.RETURN_MAP_LEARNED:
+                r = vpd.return_map_learned(net, props, dt, inc, stress, state, z, z)
+                total = r["ms"] + r["seed_ms"]
+            else:
+                r = vpd.return_map(mode, props, dt, inc, stress, state, z, z)
+                total = r["ms"]
+            if best is None or total < best[0]:
+                best = (total, r)
+        total, r = best
+        row = {"ms": total, "mean_iterations": float(r["iterations"].mean()),
+               "max_stress_err_rel": float(np.abs(r["stress"] - judge["stress"]).max() / scale),
+               "max_eqps_inc_err": float(np.abs(r["state"][4] - judge["state"][4]).max()),
+               "newton_fallbacks_per_point": float(r["fallbacks"].mean())}
+        if mode == vpd.RETURN_MAP_LEARNED:
+            row["seed_ms"] = r["seed_ms"]
+            row["seed_median_rel_err"] = float(np.median(
+                np.abs(r["seed"] - judge["state"][4]) / np.maximum(judge["state"][4], 1e-30)))
+        out[name] = row
+    learned = out["newton_learned_seed"]["ms"]
+    out["speedup_learned_vs_tight_bisection"] = out["bisection_tight"]["ms"] / learned
+    out["speedup_learned_vs_parity_bisection"] = out["bisection_parity"]["ms"] / learned
+    out["speedup_learned_vs_newton_state5"] = out["newton_state5_seed"]["ms"] / learned
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +169,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true")
+    ap.add_argument("--return-map-points", type=int, default=1000000)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this path is not sharded across GPUs "
@@ -118,18 +189,8 @@ def main():
     bbar = nnz / n
     tuned = bar.ctx.get_tuning() if args.no_autotune else bar.ctx.autotune(3)
     print("[bench] launch tuning %s" % tuned, file=sys.stderr, flush=True)
-    if args.warmup:
-        bar.step(args.warmup)
-        bar.ctx.synchronize()
-        bar.reset()
-    bar.ctx.set_timing(True)
-    bar.ctx.synchronize()
-    print("[bench] timing %d steps" % args.steps, file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    bar.step(args.steps)
-    bar.ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    kt = bar.ctx.collect_timing()
+    print("[bench] timing %d plain steps" % args.steps, file=sys.stderr, flush=True)
+    elapsed, kt = timed_run(bar, args.steps, args.warmup)
     u = bar.get("u")
     eqps = bar.get("statev")[:, 1]
     broken = bar.get("damage").sum()
@@ -143,7 +204,7 @@ def main():
     achieved = per_launch / avg_s
     step_bps = step_bytes_per_particle(bbar, w["damage"] is not None)
     out = {
-        "metric": "particle-steps/s (500k JC bar, δ=3.015Δx) GPU plain; %HBM peak",
+        "metric": "particle-steps/s (500k JC bar, \u03b4=3.015\u0394x) GPU plain vs warm-start; %HBM peak",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -170,6 +231,24 @@ def main():
     out["roofline"]["achieved"] /= 1e9
     out["roofline"]["peak"] = HBM_PEAK / 1e9
     out["roofline"]["unit"] = "GB/s"
+    if not args.no_warm_start:
+        # The same bar and launch tuning with the learned return map (compacted: network and
+        # Newton on the plastic points only).
+        print("[bench] timing %d warm-start steps" % args.steps, file=sys.stderr, flush=True)
+        net = vpd.load_warm_start()
+        bar.ctx.set_return_map(vpd.RETURN_MAP_LEARNED, 1e-12, net)
+        ws_elapsed, ws_kt = timed_run(bar, args.steps, args.warmup)
+        ws_eqps = bar.get("statev")[:, 1]
+        out["warm_start"] = {
+            "value": n * args.steps / ws_elapsed, "unit": "particle-steps/s",
+            "ms_per_step": ws_elapsed / args.steps * 1e3,
+            "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in ws_kt.items()},
+            "max_eqps": float(ws_eqps.max())}
+        bar.ctx.set_return_map(vpd.RETURN_MAP_BISECTION, 1e-12)
+        if args.return_map_points:
+            print("[bench] return-map study on %d plastic points" % args.return_map_points,
+                  file=sys.stderr, flush=True)
+            out["return_map"] = return_map_study(args.return_map_points)
     if not args.no_cpu_baseline:
         del bar
         print("[bench] CPU baseline", file=sys.stderr, flush=True)

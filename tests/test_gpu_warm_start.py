@@ -145,3 +145,84 @@ def test_mlp_training_deterministic_and_descends(vpd):
         params.append(net.get_params()[0])
     assert losses[0][1] < 0.25 * losses[0][0]
     assert np.array_equal(params[0], params[1])  # a training run is deterministic
+
+
+def plastic_bar(vpd, mode, net=None, diag=False):
+    kw = dict(nx=30, ny=8, nz=8, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
+              density=7.85e-3, velocity=40.0, return_map=mode, warm_start=net)
+    bar = vpd.Bar(**kw)
+    if diag:
+        bar.ctx.enable_diagnostics(0, 1, capacity=200)
+    bar.step(150, want_fields=True)
+    return bar
+
+
+def any_network(vpd):
+    import mlp
+    net = vpd.WarmStart(4096)
+    norm = np.zeros(18, np.float32)
+    norm[8:16] = 1e-3
+    norm[16], norm[17] = 1e-4, 1e-4
+    net.set_params(mlp.init(9).astype(np.float32), norm)
+    return net
+
+
+def test_learned_step_compacted_equals_split(vpd):
+    # The learned update runs compacted (screen → network and Newton on the plastic list)
+    # unless run diagnostics are on, which take the uncompacted split path. Both give the
+    # same bits: every point's arithmetic is the same, whatever its slot in the list.
+    banner("learned return map: compacted == split")
+    net = any_network(vpd)
+    a = plastic_bar(vpd, vpd.RETURN_MAP_LEARNED, net, diag=True)
+    b = plastic_bar(vpd, vpd.RETURN_MAP_LEARNED, net)
+    assert (b.get("statev")[:, 4] > 0).sum() > 100  # plastic flow on this step
+    for name in ("u", "v", "a", "stressN", "statev", "temp", "eint", "einel"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+    assert np.array_equal(a.get("iterations"), b.get("iterations"))
+    # Against the state-5-seeded Newton: the same converged states within the tolerance.
+    c = plastic_bar(vpd, vpd.RETURN_MAP_NEWTON)
+    for name in ("stressN", "u"):
+        x, y = b.get(name), c.get(name)
+        assert np.abs(x - y).max() <= 1e-9 * np.abs(y).max(), name
+    # State 1-5 (state 6 counts iterations, which differ with the seed).
+    x, y = b.get("statev")[:, :5], c.get("statev")[:, :5]
+    assert (np.abs(x - y) <= 1e-9 * np.abs(y).max(0)).all()
+
+
+def test_trained_warm_start_gain(vpd):
+    # The shipped network (warm_start_jc.npz, trained on the GPU from plain runs of the
+    # 30×8×8 JC bar) on plastic points of held-out runs (other velocities): every solver lands
+    # on the tight FP64 solution; the learned seed needs fewer iterations than the same
+    # Newton solve started without it (bracket midpoint) and far fewer than bisection.
+    banner("trained warm start: gain on held-out plastic points")
+    import warm_start as ws
+    net = vpd.load_warm_start()
+    rec = ws.log_pairs(40000, seed=777, runs=2, steps=1500, verbose=False)
+    m = len(rec["y"])
+    assert m >= 20000
+    inc = np.ascontiguousarray(rec["inc"].T)
+    stress = np.ascontiguousarray(rec["stress"].T)
+    state = np.ascontiguousarray(rec["state"].T)
+    z = np.zeros(m)
+    dt = ws.cfl_dt(1.0)
+    judge = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, ws.JC_PROPS, dt, inc, stress, state,
+                           z, z, rtol=1e-14)
+    scale = np.abs(judge["stress"]).max()
+    runs = {
+        "learned": vpd.return_map_learned(net, ws.JC_PROPS, dt, inc, stress, state, z, z),
+        "no_seed": vpd.return_map(vpd.RETURN_MAP_LEARNED, ws.JC_PROPS, dt, inc, stress, state,
+                                  z, z, seed=np.zeros(m)),
+        "state5_seed": vpd.return_map(vpd.RETURN_MAP_NEWTON, ws.JC_PROPS, dt, inc, stress, state,
+                                      z, z),
+        "bisection_tight": vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, ws.JC_PROPS, dt, inc,
+                                          stress, state, z, z),
+    }
+    it = {k: float(r["iterations"].mean()) for k, r in runs.items()}
+    seed_err = np.median(np.abs(runs["learned"]["seed"] - judge["state"][4]) /
+                         judge["state"][4])
+    print("mean iterations:", it, "median seed error %.3g" % seed_err)
+    for k, r in runs.items():
+        assert np.abs(r["stress"] - judge["stress"]).max() <= 1e-10 * scale, k
+    assert seed_err < 0.05
+    assert it["learned"] < 0.8 * it["no_seed"]
+    assert it["learned"] < 0.25 * it["bisection_tight"]

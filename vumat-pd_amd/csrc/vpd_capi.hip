@@ -105,6 +105,9 @@ struct vpd_context {
   vpd_mlp *mlp = nullptr;
   double *inc = nullptr, *seed = nullptr;
   int *fallbacks = nullptr;
+  // The compacted learned update: list length, particle per slot, strain increments per slot.
+  int *listCount = nullptr, *list = nullptr;
+  double *incList = nullptr;
   std::vector<TimingRec> recs;
   std::vector<hipEvent_t> eventPool;
   // Run diagnostics (vpd_enable_diagnostics): per-chunk partial rows, the step records kept
@@ -484,6 +487,27 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
   default: {
     if (!c->mlp)
       return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
+    if (!DIAG) { // screen, then the network and the Newton solve on the plastic points only
+      if (!c->list && (dalloc(c, c->listCount, 1) || dalloc(c, c->list, c->n) ||
+                       dalloc(c, c->incList, 6 * c->n)))
+        return VPD_ERR_DEVICE;
+      const CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n)};
+      if (int s = timed(c, K_MATERIAL, [&] {
+            k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
+            k_material_screen<RM_LEARNED><<<grid, 256, 0, c->stream>>>(g, cl);
+          }))
+        return s;
+      if (int s = timed(c, K_MLP_SEED, [&] {
+            k_mlp_seed_list<<<(int)ceil_div(c->n, 64), 256, 0, c->stream>>>(
+                c->listCount, c->list, c->incList, c->n, c->statev, c->n, c->mlp->theta,
+                c->mlp->norm, c->seed);
+          }))
+        return s;
+      g.seed = c->seed;
+      return timed(c, K_RETURN_MAP, [&] {
+        k_material_return_list<RM_LEARNED><<<grid_of(c->n), 256, 0, c->stream>>>(g, cl);
+      });
+    }
     if (int s = timed(c, K_MATERIAL,
                       [&] { k_material<RM_LEARNED, true, DIAG><<<grid, 256, 0, c->stream>>>(g); }))
       return s;
@@ -728,6 +752,9 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->fallbacks);
   dfree(c->inc);
   dfree(c->seed);
+  dfree(c->listCount);
+  dfree(c->list);
+  dfree(c->incList);
   dfree(c->broken);
   dfree(c->colS);
   dfree(c->sliceBase);
@@ -769,6 +796,9 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   dfree(c->dmgIndex);
   dfree(c->iters);
   dfree(c->fallbacks);
+  dfree(c->listCount); // sized for the previous block
+  dfree(c->list);
+  dfree(c->incList);
   c->haveNeighbors = c->haveShape = c->haveMaterial = c->haveDynamics = false;
   c->haveFnew = c->haveFields = c->haveForce = c->predicted = false;
   if (int s = put_planes(c, c->x, x, 3))

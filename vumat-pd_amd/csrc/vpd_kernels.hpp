@@ -288,19 +288,51 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, Planes u,
       add_defgrad_bond<DAMAGE>(x0, x1, x2, yi0, yi1, yi2, b, num, test, broken, p);
     }
   } else if (len > 0) {
-    // Software pipeline as in the force pass; the sum runs in CSR order.
+    // Software pipeline as in the force pass: while bond k is summed, bond k+1's data and
+    // flag word and bond k+2's column index are in flight. Every load is unconditional
+    // (indices clamped to the row's last bond), so the compiler can wait for exactly the
+    // loads each use needs. New breaks are gathered in a 64-bit mask and written out every
+    // 64 bonds; the row's flags are read only by this wavefront before that, and F does not
+    // depend on them, so deferring the writes changes nothing.
+    const int last = len - 1;
+    int jNext = bonds.col[p0 + 64 * (int64_t)(last < 1 ? last : 1)];
     DefgradBond cur;
     load_defgrad_bond(x, u, vol, bonds.col[p0], cur);
-    int jNext = len > 1 ? bonds.col[p0 + 64] : 0;
+    uint32_t flagCur = DAMAGE ? broken[p0 >> 5] : 0u;
+    uint64_t fresh = 0;
     for (int k = 0; k < len; ++k) {
       const int64_t p = p0 + 64 * (int64_t)k;
+      const int k1 = k + 1 < last ? k + 1 : last;
+      const int k2 = k + 2 < last ? k + 2 : last;
+      const int jAfter = bonds.col[p0 + 64 * (int64_t)k2];
       DefgradBond nxt;
-      if (k + 1 < len)
-        load_defgrad_bond(x, u, vol, jNext, nxt);
-      if (k + 2 < len)
-        jNext = bonds.col[p + 128];
-      add_defgrad_bond<DAMAGE>(x0, x1, x2, yi0, yi1, yi2, cur, num, test, broken, p);
+      load_defgrad_bond(x, u, vol, jNext, nxt);
+      const int64_t pn = p0 + 64 * (int64_t)k1;
+      const uint32_t flagNext = DAMAGE ? broken[pn >> 5] : 0u;
+      const double xi[3] = {cur.x[0] - x0, cur.x[1] - x1, cur.x[2] - x2};
+      const double yij[3] = {(cur.x[0] + cur.u[0]) - yi0, (cur.x[1] + cur.u[1]) - yi1,
+                             (cur.x[2] + cur.u[2]) - yi2};
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          num[3 * a + c] += cur.vol * yij[a] * xi[c];
+      if (DAMAGE) {
+        const bool brk = !(flagCur & (1u << (p & 31))) && stretch_breaks(test, xi, yij);
+        fresh |= (uint64_t)brk << (k & 63);
+        if ((k & 63) == 63 || k == last) {
+          const int kBase = k & ~63;
+          while (fresh) {
+            const int b = __ffsll((unsigned long long)fresh) - 1;
+            fresh &= fresh - 1;
+            const int64_t q = p0 + 64 * (int64_t)(kBase + b);
+            atomicOr(&broken[q >> 5], 1u << (q & 31));
+          }
+        }
+      }
       cur = nxt;
+      flagCur = flagNext;
+      jNext = jAfter;
     }
   }
   T3 nm, ki;
@@ -538,6 +570,114 @@ __global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
   diag_block_store<D_NET_X, 0>(dv, g.diag.partials + blk * DIAG_STRIDE);
 }
 
+// ---- the compacted material update of the learned return map ----------------------------
+// Most points of a step are elastic, and the warm-start network only helps the plastic ones.
+// k_material_screen runs the fused update for every point whose return map would end at its
+// first iteration and appends the others to a list (one atomic per wavefront); the network
+// and the Newton solve then run on that dense list only (k_mlp_seed_list,
+// k_material_return_list). Each point's arithmetic is that of the fused kernel, so the
+// result does not depend on the list order.
+struct CompactArgs {
+  int *count;   // number of listed points (zeroed before the screen)
+  int *list;    // particle index per slot
+  Planes inc;   // strain increments per slot (6 planes of the particle-count capacity)
+};
+
+__global__ void k_zero_count(int *count) { *count = 0; }
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, CompactArgs cl) {
+  const int64_t blk = xcd_block(g.nblocks);
+  if (blk < 0)
+    return;
+  const int64_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= g.n)
+    return;
+  T3 f1, f0, v0, r0;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    f1.m[c] = g.fnew(c, i);
+    f0.m[c] = g.fN(c, i);
+    v0.m[c] = g.vN(c, i);
+    r0.m[c] = g.rN(c, i);
+  }
+  Corot k;
+  if (!corotational(f0, f1, v0, r0, g.dt, k)) {
+    raise_error(g.err, ERR_SINGULAR_COROT);
+    return;
+  }
+  double inc[6];
+  to_voigt(k.Dbar, inc);
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+    inc[c] *= g.dt;
+  if (g.incOut.p) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      g.incOut(c, i) = inc[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    g.fN(c, i) = f1.m[c];
+    g.vN(c, i) = k.V1.m[c];
+    g.rN(c, i) = k.R1.m[c];
+  }
+  double sOld[6], st[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    sOld[c] = g.sig(c, i);
+    st[c] = g.statev(c, i);
+  }
+  const bool plastic = !jc_elastic(g.jc, inc, sOld, st, g.rtol);
+  const uint64_t mask = __ballot(plastic);
+  if (plastic) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    int base = 0;
+    if (lane == leader)
+      base = atomicAdd(cl.count, __popcll(mask));
+    base = __shfl(base, leader);
+    const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+    cl.list[slot] = (int)i;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      cl.inc(c, slot) = inc[c];
+    return;
+  }
+  double sNew[6];
+  double eint = g.eint[i], einel = g.einel[i];
+  ReturnMapStats stats;
+  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
+  material_finish<false>(g, i, f1, k.R1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
+}
+
+// The Newton solve and the rest of the update for the listed points, seeded per slot.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_material_return_list(MaterialArgs g, CompactArgs cl) {
+  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= *cl.count)
+    return;
+  const int64_t i = cl.list[slot];
+  double inc[6], sOld[6], sNew[6], st[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    inc[c] = cl.inc(c, slot);
+    sOld[c] = g.sig(c, i);
+    st[c] = g.statev(c, i);
+  }
+  double eint = g.eint[i], einel = g.einel[i];
+  ReturnMapStats stats;
+  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[slot] : 0.0,
+                 g.rtol, &stats);
+  T3 f1, r1;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    f1.m[c] = g.fN(c, i);
+    r1.m[c] = g.rN(c, i);
+  }
+  material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
+}
+
 // Bond pass 2. PdCorrespondenceSkeleton.cpp:213-227: f_i = Σ_k (1 − d_ik)(A_i ξ − A_j(−ξ)) V_j
 // in CSR order (a broken bond contributes exactly zero, so it is skipped), then the
 // velocity-Verlet corrector (ExplicitTimeIntegrator.cpp:36-52) and the constraints at t;
@@ -573,10 +713,10 @@ __device__ __forceinline__ void load_force_bond(const ForceArgs &g, int j, Force
     b.a.m[c] = g.amap(c, j);
 }
 
-// One bond's term of PdCorrespondenceSkeleton.cpp:219-225 added to acc, in the reference's
-// operation order: t_i = A_i ξ, t_j = A_j (−ξ), acc += (t_i − t_j) V_j.
-__device__ __forceinline__ void add_force_bond(const T3 &ai, double x0, double x1, double x2,
-                                               const ForceBond &b, double acc[3]) {
+// One bond's term of PdCorrespondenceSkeleton.cpp:219-225 in the reference's operation
+// order: t_i = A_i ξ, t_j = A_j (−ξ), term = (t_i − t_j) V_j (the caller adds it).
+__device__ __forceinline__ void force_bond_term(const T3 &ai, double x0, double x1, double x2,
+                                                const ForceBond &b, double term[3]) {
   V3 xi;
   xi.v[0] = b.x[0] - x0;
   xi.v[1] = b.x[1] - x1;
@@ -589,7 +729,7 @@ __device__ __forceinline__ void add_force_bond(const T3 &ai, double x0, double x
   const V3 tj = t3_mulv(b.a, nxi);
 #pragma unroll
   for (int d = 0; d < 3; ++d)
-    acc[d] += (ti.v[d] - tj.v[d]) * b.vol;
+    term[d] = (ti.v[d] - tj.v[d]) * b.vol;
 }
 
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool PIPE, bool DIAG>
@@ -609,25 +749,41 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
         continue;
       ForceBond b;
       load_force_bond(g, g.bonds.col[p], b);
-      add_force_bond(ai, x0, x1, x2, b, acc);
+      double term[3];
+      force_bond_term(ai, x0, x1, x2, b, term);
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        acc[d] += term[d];
     }
   } else if (len > 0) {
-    // Software pipeline: bond k+1's data and bond k+2's column index are in flight while
-    // bond k is summed, so each wave keeps twice the independent loads outstanding. The
-    // summation order is unchanged.
+    // Software pipeline: while bond k is summed, bond k+1's data and flag word and bond
+    // k+2's column index are in flight, so each wavefront keeps two bonds of independent
+    // loads outstanding. Every load is unconditional (indices clamped to the row's last
+    // bond) and the flag test is a select, so the compiler waits only for the loads each
+    // use needs. The summation order is unchanged.
+    const int last = len - 1;
+    int jNext = g.bonds.col[p0 + 64 * (int64_t)(last < 1 ? last : 1)];
     ForceBond cur;
     load_force_bond(g, g.bonds.col[p0], cur);
-    int jNext = len > 1 ? g.bonds.col[p0 + 64] : 0;
+    uint32_t flagCur = DAMAGE ? g.broken[p0 >> 5] : 0u;
     for (int k = 0; k < len; ++k) {
       const int64_t p = p0 + 64 * (int64_t)k;
+      const int k1 = k + 1 < last ? k + 1 : last;
+      const int k2 = k + 2 < last ? k + 2 : last;
+      const int jAfter = g.bonds.col[p0 + 64 * (int64_t)k2];
       ForceBond nxt;
-      if (k + 1 < len)
-        load_force_bond(g, jNext, nxt);
-      if (k + 2 < len)
-        jNext = g.bonds.col[p + 128];
-      if (!DAMAGE || !(g.broken[p >> 5] & (1u << (p & 31))))
-        add_force_bond(ai, x0, x1, x2, cur, acc);
+      load_force_bond(g, jNext, nxt);
+      const int64_t pn = p0 + 64 * (int64_t)k1;
+      const uint32_t flagNext = DAMAGE ? g.broken[pn >> 5] : 0u;
+      double term[3];
+      force_bond_term(ai, x0, x1, x2, cur, term);
+      const bool skip = DAMAGE && (flagCur & (1u << (p & 31)));
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        acc[d] = skip ? acc[d] : acc[d] + term[d];
       cur = nxt;
+      flagCur = flagNext;
+      jNext = jAfter;
     }
   }
   if (g.forceOut.p) {

@@ -303,6 +303,38 @@ __device__ __forceinline__ void jc_yield_slope(const JcParams &p, double epOld, 
   dY = soft * (dHard * rfac + (dep > 0.0 ? hard * p.rateC / dep : 0.0));
 }
 
+// Whether jc_point<MODE> (MODE ≠ RM_BISECTION) ends at its first iteration, Δε̄ᵖ = 0: the
+// init branch, or a trial stress that does not exceed the current yield stress by more than
+// the solve tolerance. Evaluates exactly the expressions jc_point's first iteration does.
+__device__ __forceinline__ bool jc_elastic(const JcParams &p, const double inc[6],
+                                           const double stressOld[6], const double state[6],
+                                           double rtol) {
+  if (state[0] == 0.0)
+    return true;
+  double trial[6];
+  const double traceInc = inc[0] + inc[1] + inc[2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    trial[i] = stressOld[i] + p.lambda * traceInc + 2.0 * p.mu * inc[i];
+#pragma unroll
+  for (int i = 3; i < 6; ++i)
+    trial[i] = stressOld[i] + 2.0 * p.mu * inc[i];
+  const double pressure = (trial[0] + trial[1] + trial[2]) / 3.0;
+  double dev[6];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    dev[i] = trial[i] - pressure;
+#pragma unroll
+  for (int i = 3; i < 6; ++i)
+    dev[i] = trial[i];
+  const double qTrial =
+      sqrt(1.5 * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
+                  2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5])));
+  const double yield = fmax(jc_yield(p, state[1], 0.0, state[2]), state[3]);
+  const double r0 = fmax(0.0, qTrial) - yield;
+  return !(r0 > 0.0 && fabs(r0) >= rtol * qTrial);
+}
+
 // One material point of JohnsonCook.f90:74-156. in: stressOld[6], strainInc[6], state[6]
 // (in/out), energies (in/out); writes stressNew[6]; returns the iteration count (0 for the
 // init branch). The bookkeeping after the solve is the reference's for every solver.

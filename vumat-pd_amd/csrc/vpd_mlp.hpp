@@ -198,6 +198,16 @@ __device__ __forceinline__ void mlp_layer_mfma(const float *h_in, const float *_
   }
 }
 
+// (x − mean) / std in FP64, then rounded to FP32: a feature whose spread is far below its
+// magnitude (the temperature moves by micro-kelvins at 293 K) keeps its information, which
+// an FP32 subtraction would round away.
+__device__ __forceinline__ void normalise_features(const double x[MLP_IN],
+                                                   const float *__restrict__ norm, float *out) {
+#pragma unroll
+  for (int c = 0; c < MLP_IN; ++c)
+    out[c] = (float)((x[c] - (double)norm[c]) / (double)norm[MLP_IN + c]);
+}
+
 // Network seeds for n points. Inputs per point: strain increments (6 planes, stride
 // incStride), ε̄ᵖ and T (planes of `state`, stride stateStride: components 1 and 2 of the
 // JC state). Writes seed[i] = max(0, y·std + mean). One wavefront per 16 points.
@@ -216,17 +226,68 @@ __global__ void __launch_bounds__(256) k_mlp_seed(int64_t n, const double *__res
   float *h0 = buf[wave][0], *h1 = buf[wave][1];
   if (lane < 16) {
     const int64_t i = base + lane;
-    float x[MLP_IN] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double x[MLP_IN];
+#pragma unroll
+    for (int c = 0; c < MLP_IN; ++c)
+      x[c] = norm[c]; // padding rows normalise to 0
     if (i < n) {
 #pragma unroll
       for (int c = 0; c < 6; ++c)
-        x[c] = (float)inc[c * incStride + i];
-      x[6] = (float)state[1 * stateStride + i];
-      x[7] = (float)state[2 * stateStride + i];
+        x[c] = inc[c * incStride + i];
+      x[6] = state[1 * stateStride + i];
+      x[7] = state[2 * stateStride + i];
     }
+    normalise_features(x, norm, h0 + lane * MLP_IN);
+  }
+  __syncthreads();
+  mlp_layer_mfma<MLP_IN, MLP_H, true>(h0, theta + MLP_OFF_W1, theta + MLP_OFF_B1, h1, lane);
+  __syncthreads();
+  mlp_layer_mfma<MLP_H, MLP_H, true>(h1, theta + MLP_OFF_W2, theta + MLP_OFF_B2, h0, lane);
+  __syncthreads();
+  mlp_layer_mfma<MLP_H, MLP_H, true>(h0, theta + MLP_OFF_W3, theta + MLP_OFF_B3, h1, lane);
+  __syncthreads();
+  mlp_layer_mfma<MLP_H, 1, false>(h1, theta + MLP_OFF_W4, theta + MLP_OFF_B4, h0, lane);
+  __syncthreads();
+  if (lane < 16 && base + lane < n) {
+    const float y = h0[lane] * norm[2 * MLP_IN + 1] + norm[2 * MLP_IN];
+    seed[base + lane] = y > 0.f ? (double)y : 0.0;
+  }
+}
+
+// k_mlp_seed over a compacted point list: slot s of the list holds particle list[s], its
+// strain increments at inc[c·incStride + s]; the state planes are indexed by particle. The
+// list length is read on the device; blocks past it exit whole.
+__global__ void __launch_bounds__(256) k_mlp_seed_list(const int *__restrict__ count,
+                                                       const int *__restrict__ list,
+                                                       const double *__restrict__ inc,
+                                                       int64_t incStride,
+                                                       const double *__restrict__ state,
+                                                       int64_t stateStride,
+                                                       const float *__restrict__ theta,
+                                                       const float *__restrict__ norm,
+                                                       double *__restrict__ seed) {
+  const int64_t n = *count;
+  if ((int64_t)blockIdx.x * 64 >= n) // the same for every thread of the block
+    return;
+  __shared__ float buf[4][2][16 * MLP_H];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16;
+  float *h0 = buf[wave][0], *h1 = buf[wave][1];
+  if (lane < 16) {
+    const int64_t s = base + lane;
+    double x[MLP_IN];
 #pragma unroll
     for (int c = 0; c < MLP_IN; ++c)
-      h0[lane * MLP_IN + c] = (x[c] - norm[c]) / norm[MLP_IN + c];
+      x[c] = norm[c]; // padding rows normalise to 0
+    if (s < n) {
+      const int64_t i = list[s];
+#pragma unroll
+      for (int c = 0; c < 6; ++c)
+        x[c] = inc[c * incStride + s];
+      x[6] = state[1 * stateStride + i];
+      x[7] = state[2 * stateStride + i];
+    }
+    normalise_features(x, norm, h0 + lane * MLP_IN);
   }
   __syncthreads();
   mlp_layer_mfma<MLP_IN, MLP_H, true>(h0, theta + MLP_OFF_W1, theta + MLP_OFF_B1, h1, lane);

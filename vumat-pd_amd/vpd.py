@@ -531,21 +531,14 @@ class WarmStart:
         return y
 
 
-def return_map(mode, props, dt, strain_inc, stress_old, state_old, eint_old, einel_old,
-               seed=None, rtol=1e-12):
-    """vpd_return_map_device on device copies of column-major blocks (6, nb) / (nstatev, nb).
-    Returns outputs, per-point iterations and fallbacks, and the kernel time in ms."""
-    nb = strain_inc.shape[1]
-    nst = state_old.shape[0]
+def _return_map_on_device(mode, props, dt, nb, nst, ins, seed_ptr, rtol):
     D = DeviceBuffer
-    ins = [D(strain_inc), D(stress_old), D(state_old), D(eint_old), D(einel_old)]
-    sd = D(seed) if seed is not None else None
     outs = [D(np.zeros((6, nb))), D(np.zeros((nst, nb))), D(np.zeros(nb)), D(np.zeros(nb))]
     it = D(np.zeros(nb))  # 8 bytes per point: room for the int32 counts
     fb = D(np.zeros(nb))
     ms = C.c_double()
     rc = lib().vpd_return_map_device(mode, nb, nst, _f64(props), len(props), dt, rtol,
-                                     *[b.ptr for b in ins], sd.ptr if sd else None,
+                                     *[b.ptr for b in ins], seed_ptr,
                                      *[b.ptr for b in outs], it.ptr, fb.ptr, C.byref(ms))
     if rc:
         raise VpdError(lib().vpd_last_error(None).decode())
@@ -553,6 +546,47 @@ def return_map(mode, props, dt, strain_inc, stress_old, state_old, eint_old, ein
     fbs = fb.read().view(np.int32)[:nb]
     return dict(stress=outs[0].read(), state=outs[1].read(), eint=outs[2].read(),
                 einel=outs[3].read(), iterations=iters, fallbacks=fbs, ms=ms.value)
+
+
+def return_map(mode, props, dt, strain_inc, stress_old, state_old, eint_old, einel_old,
+               seed=None, rtol=1e-12):
+    """vpd_return_map_device on device copies of column-major blocks (6, nb) / (nstatev, nb).
+    Returns outputs, per-point iterations and fallbacks, and the kernel time in ms."""
+    nb = strain_inc.shape[1]
+    D = DeviceBuffer
+    ins = [D(strain_inc), D(stress_old), D(state_old), D(eint_old), D(einel_old)]
+    sd = D(seed) if seed is not None else None
+    return _return_map_on_device(mode, props, dt, nb, state_old.shape[0], ins,
+                                 sd.ptr if sd else None, rtol)
+
+
+def return_map_learned(net, props, dt, strain_inc, stress_old, state_old, eint_old, einel_old,
+                       rtol=1e-12):
+    """The learned return map on a device block: the network's bf16 MFMA seeds
+    (vpd_mlp_seed_device, timed) feed vpd_return_map_device's Newton solve (timed). Returns
+    return_map's dict plus seed_ms and the seeds."""
+    nb = strain_inc.shape[1]
+    D = DeviceBuffer
+    ins = [D(strain_inc), D(stress_old), D(state_old), D(eint_old), D(einel_old)]
+    seed = D(np.zeros(nb))
+    seed_ms = C.c_double()
+    if lib().vpd_mlp_seed_device(net.h, ins[0].ptr, nb, ins[2].ptr, nb, nb, seed.ptr,
+                                 C.byref(seed_ms)):
+        raise VpdError(lib().vpd_mlp_last_error(net.h).decode())
+    out = _return_map_on_device(RETURN_MAP_LEARNED, props, dt, nb, state_old.shape[0], ins,
+                                seed.ptr, rtol)
+    out["seed_ms"] = seed_ms.value
+    out["seed"] = seed.read()
+    return out
+
+
+def load_warm_start(path=None, max_batch=4096):
+    """The trained warm-start network shipped with the package (warm_start_jc.npz)."""
+    path = path or os.path.join(HERE, "warm_start_jc.npz")
+    data = np.load(path)
+    net = WarmStart(max_batch)
+    net.set_params(data["theta"], data["norm"])
+    return net
 
 
 # ------------------------------------------------------------------ a bar case

@@ -46,42 +46,91 @@ def initial_params(seed=0):
     return np.concatenate(parts).astype(np.float32)
 
 
+VOIGT_OF_TENSOR = [0, 4, 8, 5, 2, 1]  # Voigt [11,22,33,23,13,12] entries of a row-major 3×3
+
+
 def log_pairs(target_pairs, seed=1234, steps=2000, nx=30, ny=8, nz=8, spacing=1.0,
-              return_map=vpd.RETURN_MAP_BISECTION, max_runs=64, verbose=True):
-    """Runs bars with end velocities drawn from U[0.5, 4] mm/ms until target_pairs plastic
-    (input, Δε̄ᵖ) pairs are logged. Returns x (m, 8) raw inputs and y (m,) targets."""
+              return_map=vpd.RETURN_MAP_BISECTION, runs=8, verbose=True):
+    """Runs `runs` bars with end velocities drawn from U[0.5, 4] mm/ms and keeps an even,
+    seeded random share of each run's plastic points, target_pairs in all. Returns a dict
+    of per-point records: x (m, 8) raw network inputs, y (m,) converged Δε̄ᵖ, and the full
+    return-map inputs inc (m, 6), stress (m, 6) (σ̄_n, Voigt), state (m, 6) (before the
+    update), for evaluation."""
     rng = np.random.default_rng(seed)
-    xs, ys, total = [], [], 0
+    out = {k: [] for k in ("x", "y", "inc", "stress", "state")}
+    total = 0
     dt = cfl_dt(spacing)
-    for run in range(max_runs):
+    for run in range(runs):
         v = float(rng.uniform(0.5, 4.0))
+        rec = {k: [] for k in out}
         bar = vpd.Bar(nx, ny, nz, spacing, 3.015 * spacing, JC_PROPS, dt, RHO, velocity=v,
                       return_map=return_map)
         for _ in range(steps):
             before = bar.get("statev")
+            stress = bar.get("stressN")[:, VOIGT_OF_TENSOR]
             bar.step(1, fuse=False, want_fields=True)
             after = bar.get("statev")
             plastic = after[:, 4] > 0.0
             if plastic.any():
                 inc = bar.get("strain_inc")[plastic]
-                x = np.concatenate([inc, before[plastic, 1:2], before[plastic, 2:3]], 1)
-                xs.append(x)
-                ys.append(after[plastic, 4])
-                total += int(plastic.sum())
-        if verbose:
-            print("[warm-start] run %d v=%.3f mm/ms: %d pairs so far" % (run, v, total),
-                  flush=True)
+                rec["x"].append(np.concatenate([inc, before[plastic, 1:3]], 1))
+                rec["y"].append(after[plastic, 4])
+                rec["inc"].append(inc)
+                rec["stress"].append(stress[plastic])
+                rec["state"].append(before[plastic])
         bar.ctx.close()
-        if total >= target_pairs:
-            break
-    x = np.concatenate(xs)[:target_pairs]
-    y = np.concatenate(ys)[:target_pairs]
-    return x, y
+        got = sum(len(y) for y in rec["y"])
+        want = min(got, (target_pairs - total) // (runs - run))
+        keep = np.sort(rng.choice(got, size=want, replace=False)) if got else []
+        for k in out:
+            if got:
+                out[k].append(np.concatenate(rec[k])[keep])
+        total += want
+        if verbose:
+            print("[warm-start] run %d v=%.3f mm/ms: %d plastic points, kept %d (%d so far)"
+                  % (run, v, got, want, total), flush=True)
+    return {k: np.concatenate(v) for k, v in out.items()}
+
+
+def evaluate(net, records, dt=None, limit=200000):
+    """Return-map solvers on logged plastic points: mean iterations, kernel time and the
+    largest stress deviation from the tight bisection (the accuracy judge), per mode."""
+    dt = cfl_dt(1.0) if dt is None else dt
+    m = min(limit, len(records["y"]))
+    inc = np.ascontiguousarray(records["inc"][:m].T)
+    stress = np.ascontiguousarray(records["stress"][:m].T)
+    state = np.ascontiguousarray(records["state"][:m].T)
+    z = np.zeros(m)
+    seed = np.zeros(m)
+    theta, norm = net.get_params()
+    norm = norm.astype(np.float64)
+    xn = (records["x"][:m] - norm[:8]) / norm[8:16]  # FP64, as the seed kernels do
+    seed = np.maximum(net.predict(xn).astype(np.float64) * norm[17] + norm[16], 0.0)
+    out = {}
+    judge = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, JC_PROPS, dt, inc, stress, state, z,
+                           z, rtol=1e-14)
+    scale = np.abs(judge["stress"]).max()
+    for name, mode, sd in (("bisection", vpd.RETURN_MAP_BISECTION, None),
+                           ("bisection_tight", vpd.RETURN_MAP_BISECTION_TIGHT, None),
+                           ("newton", vpd.RETURN_MAP_NEWTON, None),
+                           ("learned", vpd.RETURN_MAP_LEARNED, seed)):
+        r = vpd.return_map(mode, JC_PROPS, dt, inc, stress, state, z, z, seed=sd)
+        out[name] = dict(mean_iterations=float(r["iterations"].mean()), ms=r["ms"],
+                         max_stress_err_rel=float(np.abs(r["stress"] - judge["stress"]).max()
+                                                  / scale),
+                         fallbacks=float(r["fallbacks"].mean()))
+    out["seed_rel_err_median"] = float(np.median(np.abs(seed - records["y"][:m]) /
+                                                 np.maximum(records["y"][:m], 1e-30)))
+    return out
 
 
 def normalisation(x, y):
-    """Input mean/std per feature (std floored) and output mean/std: 18 FP32 values."""
+    """Input mean/std per feature and output mean/std: 18 FP32 values. An input's std is
+    floored at 1e-3 of its mean's magnitude, so a feature that barely moves in the training
+    runs (the temperature, heated by micro-kelvins) maps to ~0 instead of amplifying its
+    noise, and the network stays bounded where it does move (adiabatic heating)."""
     mx, sx = x.mean(0), x.std(0)
+    sx = np.maximum(sx, 1e-3 * np.abs(mx))
     sx = np.where(sx > 0, sx, 1.0)
     return np.concatenate([mx, sx, [y.mean(), max(y.std(), 1e-30)]]).astype(np.float32)
 
@@ -125,11 +174,15 @@ def main():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--out", default=os.path.join(HERE, "warm_start_jc.npz"))
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--eval-seed", type=int, default=98765)
     a = ap.parse_args()
-    x, y = log_pairs(a.pairs, seed=a.seed)
-    net, norm, loss = train(x, y, steps=a.steps)
+    rec = log_pairs(a.pairs, seed=a.seed)
+    net, norm, loss = train(rec["x"], rec["y"], steps=a.steps)
     save(a.out, net)
-    print("[warm-start] saved %s (final batch loss %.4e, %d pairs)" % (a.out, loss, len(y)))
+    print("[warm-start] saved %s (final batch loss %.4e, %d pairs)" % (a.out, loss, len(rec["y"])))
+    held = log_pairs(200000, seed=a.eval_seed, verbose=False)
+    import json
+    print("[warm-start] held-out evaluation " + json.dumps(evaluate(net, held)), flush=True)
 
 
 if __name__ == "__main__":
