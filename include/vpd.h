@@ -214,24 +214,20 @@ typedef struct vpd_mlp vpd_mlp;
 int vpd_attach_warm_start(vpd_context *ctx, vpd_mlp *mlp);
 
 /* ---- launch tuning ---------------------------------------------------------------- */
-/* Three knobs per bond pass (no reference counterpart):
+/* Two knobs per bond pass (no reference counterpart):
  *  - sweeps: the pass walks the particle range in this many consecutive sweeps, so the
  *    neighbour data an XCD touches at once (its stretch of the sweep plus a horizon either
  *    side) can fit its L2;
- *  - LDS bytes reserved per 256-thread workgroup, capping workgroups per CU;
- *  - pipelining (0/1): the bond loop loads bond k+1 while it sums bond k. The sum order, and
- *    so every result bit, is the same either way.
- * vpd_autotune times both passes over the grid of the three on the context's own block and
- * keeps the fastest; tuned[] (VPD_TUNE_COUNT ints) receives the values indexed by enum
- * vpd_tuning. */
+ *  - LDS bytes reserved per 256-thread workgroup, capping workgroups per CU.
+ * The sum order, and so every result bit, is the same for every setting. vpd_autotune times
+ * both passes over the grid of the two on the context's own block and keeps the fastest;
+ * tuned[] (VPD_TUNE_COUNT ints) receives the values indexed by enum vpd_tuning. */
 enum vpd_tuning {
   VPD_TUNE_DEFGRAD_LDS = 0,
   VPD_TUNE_FORCE_LDS = 1,
   VPD_TUNE_DEFGRAD_ROUNDS = 2,
   VPD_TUNE_FORCE_ROUNDS = 3,
-  VPD_TUNE_DEFGRAD_PIPELINE = 4,
-  VPD_TUNE_FORCE_PIPELINE = 5,
-  VPD_TUNE_COUNT = 6
+  VPD_TUNE_COUNT = 4
 };
 int vpd_set_tuning(vpd_context *ctx, int knob, int value);
 int vpd_get_tuning(vpd_context *ctx, int knob, int *value);

@@ -204,8 +204,7 @@ def test_diagnostics_change_no_state_and_ignore_tuning(vpd):
     plain = diag_bar(vpd)
     plain.step(80)
     recs = []
-    for knobs in ({}, dict(force_rounds=3, defgrad_rounds=2, force_pipeline=1,
-                           defgrad_pipeline=1)):
+    for knobs in ({}, dict(force_rounds=3, defgrad_rounds=2, force_lds=50000)):
         bar = diag_bar(vpd, **knobs)
         w = bar.vol * (bar.x[:, 0] > 3.5) * (bar.x[:, 0] < 25.5)
         lat = (bar.x[:, 1] == bar.x[:, 1].min()) * 1 + (bar.x[:, 1] == bar.x[:, 1].max()) * 2

@@ -268,9 +268,8 @@ def test_fused_equals_unfused(vpd, orc):
 
 
 @pytest.mark.parametrize("knobs", [
-    dict(defgrad_pipeline=1, force_pipeline=1),
-    dict(defgrad_pipeline=1, force_pipeline=1, defgrad_rounds=3, force_rounds=5),
-    dict(defgrad_lds=36408, force_lds=36408, defgrad_rounds=2, force_rounds=2),
+    dict(defgrad_rounds=3, force_rounds=5),
+    dict(defgrad_lds=36408, force_lds=64000, defgrad_rounds=2, force_rounds=2),
 ])
 def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
     # Launch knobs change how the bond passes are scheduled, never what they sum or in which

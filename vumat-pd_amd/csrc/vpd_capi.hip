@@ -78,7 +78,8 @@ struct vpd_context {
   double *u = nullptr, *v = nullptr, *a = nullptr;
   double *fN = nullptr, *vN = nullptr, *rN = nullptr, *sig = nullptr, *statev = nullptr;
   double *temp = nullptr, *eint = nullptr, *einel = nullptr;
-  double *fnew = nullptr, *amap = nullptr;
+  double *fnew = nullptr, *amap = nullptr; // amap: A = P K⁻¹ as 5 component pairs (10 n)
+  double *xv = nullptr;                     // (X_x, X_y), (X_z, V) pairs (4 n), static
   double *sigOut = nullptr, *strainOut = nullptr, *force = nullptr, *dmgIndex = nullptr;
   int *iters = nullptr;
   uint32_t *broken = nullptr; // one bit per sliced bond slot
@@ -98,7 +99,6 @@ struct vpd_context {
   int ldsDefgrad = 0, ldsForce = 0;
   // Sweeps per bond pass (see Rounds in vpd_kernels.hpp). Chosen by vpd_autotune.
   int roundsDefgrad = 1, roundsForce = 1;
-  int pipeDefgrad = 0, pipeForce = 0;
   // Return map of the material stage; the learned mode adds the network, the strain
   // increments it reads and the seeds it writes.
   double rtol = 1.0e-12;
@@ -234,6 +234,7 @@ int check_device(vpd_context *c, int mask = ~0) {
 }
 
 Planes planes(double *p, int64_t n) { return Planes{p, n}; }
+PairPlanes pairs(double *p, int64_t n) { return PairPlanes{reinterpret_cast<d2v *>(p), n}; }
 
 int grid_of(int64_t n) { return (int)ceil_div(n, 256); }
 // Grids of the XCD-remapped kernels are rounded up to whole rounds of 8 workgroups.
@@ -407,7 +408,7 @@ int build_sell(vpd_context *c) {
     int m = 0;
     for (int64_t i = 64 * s; i < std::min<int64_t>(n, 64 * s + 64); ++i)
       m = std::max(m, c->hostRowptr[i + 1] - c->hostRowptr[i]);
-    c->hostBase[s + 1] = c->hostBase[s] + 64 * (int64_t)m;
+    c->hostBase[s + 1] = c->hostBase[s] + 64 * (int64_t)((m + 3) / 4 * 4);
   }
   c->sellTotal = c->hostBase[ns];
   if (dalloc(c, c->sliceBase, ns + 1) || dalloc(c, c->colS, c->sellTotal))
@@ -429,7 +430,7 @@ int build_sell(vpd_context *c) {
 
 // Sliced slot of CSR entry p of row i.
 int64_t sell_slot(const vpd_context *c, int64_t i, int64_t p) {
-  return c->hostBase[i >> 6] + 64 * (p - c->hostRowptr[i]) + (i & 63);
+  return vpd::sell_slot(c->hostBase[i >> 6], i, p - c->hostRowptr[i]);
 }
 
 Sell sell_of(const vpd_context *c) { return Sell{c->sliceBase, c->colS, c->rowptr}; }
@@ -444,31 +445,27 @@ int launch_predict(vpd_context *c, double dt, const double ramp[3]) {
   });
 }
 
-template <bool DAMAGE, bool PIPE>
+template <bool DAMAGE>
 void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression) {
-  k_defgrad<DAMAGE, PIPE><<<grid_rounds(rd), 256, lds, c->stream>>>(
-      rd, planes(c->x, c->n), planes(c->u, c->n), c->vol, sell_of(c), planes(c->kinv, c->n),
-      planes(c->fnew, c->n), c->broken, c->initiation, c->failure, useCompression);
+  k_defgrad<DAMAGE><<<grid_rounds(rd), 256, lds, c->stream>>>(
+      rd, planes(c->x, c->n), pairs(c->xv, c->n), planes(c->u, c->n), sell_of(c),
+      planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
+      useCompression);
 }
 
-void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, bool pipe) {
+void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds) {
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
   const Rounds rd = rounds_of(c->n, rounds);
-  if (dmg && pipe)
-    launch_defgrad_t<true, true>(c, lds, rd, useCompression);
-  else if (dmg)
-    launch_defgrad_t<true, false>(c, lds, rd, useCompression);
-  else if (pipe)
-    launch_defgrad_t<false, true>(c, lds, rd, useCompression);
+  if (dmg)
+    launch_defgrad_t<true>(c, lds, rd, useCompression);
   else
-    launch_defgrad_t<false, false>(c, lds, rd, useCompression);
+    launch_defgrad_t<false>(c, lds, rd, useCompression);
 }
 
 int launch_defgrad(vpd_context *c) {
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
-  return timed(c, K_DEFGRAD, [&] {
-    launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad, c->pipeDefgrad != 0);
-  });
+  return timed(c, K_DEFGRAD,
+               [&] { launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad); });
 }
 
 template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
@@ -498,8 +495,8 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
           }))
         return s;
       if (int s = timed(c, K_MLP_SEED, [&] {
-            k_mlp_seed_list<<<(int)ceil_div(c->n, 64), 256, 0, c->stream>>>(
-                c->listCount, c->list, c->incList, c->n, c->statev, c->n, c->mlp->theta,
+            k_mlp_seed<true><<<mlp_grid(c->n), 256, 0, c->stream>>>(
+                0, c->listCount, c->list, c->incList, c->n, c->statev, c->n, c->mlp->theta,
                 c->mlp->norm, c->seed);
           }))
         return s;
@@ -512,8 +509,9 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
                       [&] { k_material<RM_LEARNED, true, DIAG><<<grid, 256, 0, c->stream>>>(g); }))
       return s;
     if (int s = timed(c, K_MLP_SEED, [&] {
-          k_mlp_seed<<<(int)ceil_div(c->n, 64), 256, 0, c->stream>>>(
-              c->n, c->inc, c->n, c->statev, c->n, c->mlp->theta, c->mlp->norm, c->seed);
+          k_mlp_seed<false><<<mlp_grid(c->n), 256, 0, c->stream>>>(
+              c->n, nullptr, nullptr, c->inc, c->n, c->statev, c->n, c->mlp->theta,
+              c->mlp->norm, c->seed);
         }))
       return s;
     g.seed = c->seed;
@@ -537,7 +535,7 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
   g.sig = planes(c->sig, c->n);
   g.statev = planes(c->statev, c->n);
   g.kinv = planes(c->kinv, c->n);
-  g.amap = planes(c->amap, c->n);
+  g.amap = pairs(c->amap, c->n);
   g.temp = c->temp;
   g.eint = c->eint;
   g.einel = c->einel;
@@ -558,40 +556,36 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
 }
 
 template <bool DAMAGE, bool FUSE, bool INTEGRATE, bool DIAG = false>
-void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, bool pipe) {
-  if (pipe)
-    k_force<DAMAGE, FUSE, INTEGRATE, true, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
-  else
-    k_force<DAMAGE, FUSE, INTEGRATE, false, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+void launch_force_t(vpd_context *c, const ForceArgs &g, int lds) {
+  k_force<DAMAGE, FUSE, INTEGRATE, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
 }
 
 template <bool DIAG>
-void launch_force_integrating(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, int lds,
-                              bool pipe) {
+void launch_force_integrating(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, int lds) {
   if (dmg && fuse)
-    launch_force_t<true, true, true, DIAG>(c, g, lds, pipe);
+    launch_force_t<true, true, true, DIAG>(c, g, lds);
   else if (dmg)
-    launch_force_t<true, false, true, DIAG>(c, g, lds, pipe);
+    launch_force_t<true, false, true, DIAG>(c, g, lds);
   else if (fuse)
-    launch_force_t<false, true, true, DIAG>(c, g, lds, pipe);
+    launch_force_t<false, true, true, DIAG>(c, g, lds);
   else
-    launch_force_t<false, false, true, DIAG>(c, g, lds, pipe);
+    launch_force_t<false, false, true, DIAG>(c, g, lds);
 }
 
 // The force pass in one of its forms: with or without damage flags, the next step's predictor
-// fused in, the integrator (INTEGRATE false leaves only the force density), the pipelined
-// bond loop, the run diagnostics' force terms (integrating forms only).
+// fused in, the integrator (INTEGRATE false leaves only the force density), the run
+// diagnostics' force terms (integrating forms only).
 void launch_force_with(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, bool integrate,
-                       int lds, bool pipe, bool diag = false) {
+                       int lds, bool diag = false) {
   if (!integrate) {
     if (dmg)
-      launch_force_t<true, false, false>(c, g, lds, pipe);
+      launch_force_t<true, false, false>(c, g, lds);
     else
-      launch_force_t<false, false, false>(c, g, lds, pipe);
+      launch_force_t<false, false, false>(c, g, lds);
   } else if (diag) {
-    launch_force_integrating<true>(c, g, dmg, fuse, lds, pipe);
+    launch_force_integrating<true>(c, g, dmg, fuse, lds);
   } else {
-    launch_force_integrating<false>(c, g, dmg, fuse, lds, pipe);
+    launch_force_integrating<false>(c, g, dmg, fuse, lds);
   }
 }
 
@@ -604,7 +598,9 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.u = planes(c->u, c->n);
   g.v = planes(c->v, c->n);
   g.a = planes(c->a, c->n);
-  g.amap = planes(c->amap, c->n);
+  g.amap = pairs(c->amap, c->n);
+  g.xv = pairs(c->xv, c->n);
+  g.xv = pairs(c->xv, c->n);
   g.forceOut = planes(fields ? c->force : nullptr, c->n);
   g.vol = c->vol;
   g.bonds = sell_of(c);
@@ -618,7 +614,7 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.rd = rounds_of(c->n, c->roundsForce);
   g.diag = c->diag;
   return timed(c, K_FORCE, [&] {
-    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, c->pipeForce != 0, diag);
+    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag);
   });
 }
 
@@ -744,6 +740,7 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->einel);
   dfree(c->fnew);
   dfree(c->amap);
+  dfree(c->xv);
   dfree(c->sigOut);
   dfree(c->strainOut);
   dfree(c->force);
@@ -779,10 +776,10 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   c->n = n;
   c->diagOn = false; // sized for the previous block
   const int64_t sz[] = {3 * n, n, 9 * n, 3 * n, 3 * n, 3 * n, 9 * n, 9 * n,
-                        9 * n, 6 * n, (int64_t)c->nstatev * n, n, n, n, 9 * n, 9 * n};
+                        9 * n, 6 * n, (int64_t)c->nstatev * n, n, n, n, 9 * n, 10 * n, 4 * n};
   double **bufs[] = {&c->x,  &c->vol, &c->kinv, &c->u,    &c->v,   &c->a,
                      &c->fN, &c->vN,  &c->rN,   &c->sig,  &c->statev, &c->temp,
-                     &c->eint, &c->einel, &c->fnew, &c->amap};
+                     &c->eint, &c->einel, &c->fnew, &c->amap, &c->xv};
   for (size_t k = 0; k < sizeof(sz) / sizeof(sz[0]); ++k)
     if (dalloc(c, *bufs[k], sz[k]))
       return VPD_ERR_DEVICE;
@@ -804,6 +801,8 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   if (int s = put_planes(c, c->x, x, 3))
     return s;
   HIPTRY(c, hipMemcpyAsync(c->vol, vol, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  k_pack_xv<<<grid_of(n), 256, 0, c->stream>>>(n, planes(c->x, n), c->vol, pairs(c->xv, n));
+  HIPTRY(c, hipGetLastError());
   HIPTRY(c, hipStreamSynchronize(c->stream));
   return VPD_OK;
 }
@@ -1288,7 +1287,7 @@ int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *forc
     return s;
   k_piola<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->sigOut, c->n),
                                                 planes(c->fnew, c->n), planes(c->kinv, c->n),
-                                                planes(c->amap, c->n), c->derr);
+                                                pairs(c->amap, c->n), c->derr);
   HIPTRY(c, hipGetLastError());
   if (int s = check_device(c))
     return s;
@@ -1296,7 +1295,8 @@ int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *forc
   g.n = c->n;
   g.nblocks = ceil_div(c->n, 256);
   g.x = planes(c->x, c->n);
-  g.amap = planes(c->amap, c->n);
+  g.amap = pairs(c->amap, c->n);
+  g.xv = pairs(c->xv, c->n);
   g.forceOut = planes(c->force, c->n);
   g.vol = c->vol;
   g.bonds = sell_of(c);
@@ -1304,7 +1304,7 @@ int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *forc
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   g.rd = rounds_of(c->n, c->roundsForce);
   if (int s = timed(c, K_FORCE, [&] {
-        launch_force_with(c, g, dmg, false, false, c->ldsForce, c->pipeForce != 0);
+        launch_force_with(c, g, dmg, false, false, c->ldsForce);
       }))
     return s;
   c->haveForce = true;
@@ -1541,10 +1541,6 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
     if (value < 1 || value > 1024)
       return fail(c, VPD_ERR_ARGUMENT, "Sweeps per bond pass must lie in [1, 1024].");
     (knob == VPD_TUNE_DEFGRAD_ROUNDS ? c->roundsDefgrad : c->roundsForce) = value;
-  } else if (knob == VPD_TUNE_DEFGRAD_PIPELINE || knob == VPD_TUNE_FORCE_PIPELINE) {
-    if (value != 0 && value != 1)
-      return fail(c, VPD_ERR_ARGUMENT, "Bond-loop pipelining is 0 (off) or 1 (on).");
-    (knob == VPD_TUNE_DEFGRAD_PIPELINE ? c->pipeDefgrad : c->pipeForce) = value;
   } else {
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
   }
@@ -1567,19 +1563,13 @@ int vpd_get_tuning(vpd_context *c, int knob, int *value) {
   case VPD_TUNE_FORCE_ROUNDS:
     *value = c->roundsForce;
     return VPD_OK;
-  case VPD_TUNE_DEFGRAD_PIPELINE:
-    *value = c->pipeDefgrad;
-    return VPD_OK;
-  case VPD_TUNE_FORCE_PIPELINE:
-    *value = c->pipeForce;
-    return VPD_OK;
   default:
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
   }
 }
 
-// Times both bond passes over a grid of (bond-loop pipelining) × (sweeps per pass) ×
-// (workgroups per CU) and keeps the fastest for each. The timing runs use the passes' side-effect-free forms: F into its
+// Times both bond passes over a grid of (sweeps per pass) × (workgroups per CU) and keeps the
+// fastest for each. The timing runs use the passes' side-effect-free forms: F into its
 // scratch plane, and the force gather without the integrator.
 int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   if (int s = require_ready(c))
@@ -1588,7 +1578,8 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
     reps = 3;
   if (int s = ensure_outputs(c))
     return s;
-  const int ldsChoices[] = {0, 36408}; // ≤ 8 and 4 workgroups per CU
+  // LDS reservations giving at most 8 (no limit), 4, 3 and 2 workgroups per CU.
+  const int ldsChoices[] = {0, 36408, 50000, 64000};
   const int roundChoices[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
   hipEvent_t a, b;
   HIPTRY(c, hipEventCreate(&a));
@@ -1597,15 +1588,15 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   g.n = c->n;
   g.nblocks = ceil_div(c->n, 256);
   g.x = planes(c->x, c->n);
-  g.amap = planes(c->amap, c->n);
+  g.amap = pairs(c->amap, c->n);
+  g.xv = pairs(c->xv, c->n);
   g.forceOut = planes(c->force, c->n);
   g.vol = c->vol;
   g.bonds = sell_of(c);
   g.broken = c->broken;
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   float best[2] = {1e30f, 1e30f};
-  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1}, pickPipe[2] = {0, 0};
-  for (int pipe = 0; pipe < 2; ++pipe)
+  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1};
   for (int lds : ldsChoices)
     for (int rounds : roundChoices) {
       if (rounds > 1 && rounds > ceil_div(c->n, 256))
@@ -1616,9 +1607,9 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
         for (int r = 0; r <= reps; ++r) {
           HIPTRY(c, hipEventRecord(a, c->stream));
           if (pass == 0)
-            launch_defgrad_with(c, false, lds, rounds, pipe != 0);
+            launch_defgrad_with(c, false, lds, rounds);
           else
-            launch_force_with(c, g, dmg, false, false, lds, pipe != 0);
+            launch_force_with(c, g, dmg, false, false, lds);
           HIPTRY(c, hipGetLastError());
           HIPTRY(c, hipEventRecord(b, c->stream));
           HIPTRY(c, hipEventSynchronize(b));
@@ -1631,7 +1622,6 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
           best[pass] = total;
           pickLds[pass] = lds;
           pickRounds[pass] = rounds;
-          pickPipe[pass] = pipe;
         }
       }
     }
@@ -1641,16 +1631,12 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   c->roundsDefgrad = pickRounds[0];
   c->ldsForce = pickLds[1];
   c->roundsForce = pickRounds[1];
-  c->pipeDefgrad = pickPipe[0];
-  c->pipeForce = pickPipe[1];
   c->haveFnew = false; // F scratch now holds the tuning runs' values
   if (tuned) {
     tuned[VPD_TUNE_DEFGRAD_LDS] = c->ldsDefgrad;
     tuned[VPD_TUNE_FORCE_LDS] = c->ldsForce;
     tuned[VPD_TUNE_DEFGRAD_ROUNDS] = c->roundsDefgrad;
     tuned[VPD_TUNE_FORCE_ROUNDS] = c->roundsForce;
-    tuned[VPD_TUNE_DEFGRAD_PIPELINE] = c->pipeDefgrad;
-    tuned[VPD_TUNE_FORCE_PIPELINE] = c->pipeForce;
   }
   return check_device(c);
 }
@@ -2116,7 +2102,7 @@ int vpd_mlp_predict(vpd_mlp *m, const float *x, int64_t n, float *y) {
   if (malloc_n(m, dx, n * MLP_IN) || malloc_n(m, dy, n))
     return VPD_ERR_DEVICE;
   MLPTRY(m, hipMemcpyAsync(dx, x, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice, m->stream));
-  k_mlp_infer_raw<<<(int)ceil_div(n, 64), 256, 0, m->stream>>>(n, dx, m->theta, dy);
+  k_mlp_infer_raw<<<mlp_grid(n), 256, 0, m->stream>>>(n, dx, m->theta, dy);
   MLPTRY(m, hipGetLastError());
   MLPTRY(m, hipMemcpyAsync(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
   MLPTRY(m, hipStreamSynchronize(m->stream));
@@ -2135,8 +2121,9 @@ int vpd_mlp_seed_device(vpd_mlp *m, const double *inc, int64_t incStride, const 
   MLPTRY(m, hipEventCreate(&a));
   MLPTRY(m, hipEventCreate(&b));
   MLPTRY(m, hipEventRecord(a, m->stream));
-  k_mlp_seed<<<(int)ceil_div(n, 64), 256, 0, m->stream>>>(n, inc, incStride, state, stateStride,
-                                                           m->theta, m->norm, seed);
+  k_mlp_seed<false><<<mlp_grid(n), 256, 0, m->stream>>>(n, nullptr, nullptr, inc, incStride,
+                                                         state, stateStride, m->theta, m->norm,
+                                                         seed);
   MLPTRY(m, hipGetLastError());
   MLPTRY(m, hipEventRecord(b, m->stream));
   MLPTRY(m, hipEventSynchronize(b));

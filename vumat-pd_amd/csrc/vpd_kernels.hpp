@@ -16,18 +16,25 @@ namespace vpd {
 
 __device__ __forceinline__ void raise_error(int *err, int bit) { atomicOr(err, bit); }
 
-// Bond lists for the per-step passes in a 64-wide sliced layout: the k-th bond of particle
-// i = 64 s + L sits at base[s] + 64 k + L, so at every k the lanes of a wavefront read 64
-// consecutive column indices, and bond flags for lanes 0-31 / 32-63 share one word each.
-// Within a row the bonds keep their CSR (ascending) order.
+// Bond lists for the per-step passes in a 64-wide sliced layout grouped by 4 bonds: bond k
+// of particle i = 64 s + L sits at base[s] + 256 (k / 4) + 4 L + k % 4. One 16-byte load gives
+// a lane the column indices of 4 consecutive bonds (the lanes of a wavefront read 1 KiB
+// contiguous), so a lane issues the data loads of 4 bonds at once; the flag bits of a 4-bond
+// group are contiguous, and a flag word holds the groups of 8 lanes. A slice is padded to a
+// multiple of 4 of its longest row with index 0 (a valid particle whose data is loaded and
+// not used). Within a row the bonds keep their CSR (ascending) order.
 struct Sell {
   const int64_t *base;
   const int *col;
   const int *rowptr;
 };
-__device__ __forceinline__ int64_t sell_pos(const Sell &s, int64_t i, int k) {
-  return s.base[i >> 6] + 64 * (int64_t)k + (i & 63);
+__host__ __device__ __forceinline__ int64_t sell_slot(int64_t sliceBase, int64_t i, int64_t k) {
+  return sliceBase + 256 * (k >> 2) + 4 * (i & 63) + (k & 3);
 }
+__device__ __forceinline__ int64_t sell_pos(const Sell &s, int64_t i, int k) {
+  return sell_slot(s.base[i >> 6], i, k);
+}
+typedef int i4v __attribute__((ext_vector_type(4)));
 
 // Fills the sliced layout from CSR (one lane per particle).
 __global__ void __launch_bounds__(256) k_sell_fill(int64_t n, const int *__restrict__ rowptr,
@@ -38,9 +45,23 @@ __global__ void __launch_bounds__(256) k_sell_fill(int64_t n, const int *__restr
   if (i >= n)
     return;
   const int b = rowptr[i], e = rowptr[i + 1];
-  const int64_t o = base[i >> 6] + (i & 63);
   for (int k = 0; k < e - b; ++k)
-    colS[o + 64 * (int64_t)k] = col[b + k];
+    colS[sell_slot(base[i >> 6], i, k)] = col[b + k];
+}
+
+// The static neighbour data of both bond passes as component pairs: (X_x, X_y), (X_z, V).
+__global__ void __launch_bounds__(256) k_pack_xv(int64_t n, Planes x,
+                                                 const double *__restrict__ vol, PairPlanes xv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  d2v a, b;
+  a.x = x(0, i);
+  a.y = x(1, i);
+  b.x = x(2, i);
+  b.y = vol[i];
+  xv(0, i) = a;
+  xv(1, i) = b;
 }
 
 // Workgroups are handed out to the 8 XCDs round-robin by workgroup id. Remapping the id so
@@ -229,49 +250,13 @@ __global__ void __launch_bounds__(256) k_shape(int64_t n, Planes x, const double
 // and, with DAMAGE, PdSimpleDamage.cpp:63-84 on the same bond and the same predicted
 // positions: a bond breaks (irreversibly) when its stretch exceeds `initiation` or, with a
 // compression threshold armed, falls below −failure. Damage does not enter F (:181-196).
-// Neighbour data one F-pass bond needs: X_j, u_j, V_j.
-struct DefgradBond {
-  double x[3], u[3], vol;
-};
-
-__device__ __forceinline__ void load_defgrad_bond(Planes x, Planes u,
-                                                  const double *__restrict__ vol, int j,
-                                                  DefgradBond &b) {
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    b.x[d] = x(d, j);
-    b.u[d] = u(d, j);
-  }
-  b.vol = vol[j];
-}
-
+// Bonds run in groups of 4 (one 16-byte column load); each bond's X_j and V_j come as two
+// component pairs, u_j from its planes. A group's new breaks are set with one atomic OR on
+// the flag word the lane shares with 7 others.
 template <bool DAMAGE>
-__device__ __forceinline__ void add_defgrad_bond(double x0, double x1, double x2, double yi0,
-                                                 double yi1, double yi2, const DefgradBond &b,
-                                                 double num[9], const StretchTest &test,
-                                                 uint32_t *broken, int64_t p) {
-  const double xi[3] = {b.x[0] - x0, b.x[1] - x1, b.x[2] - x2};
-  const double yij[3] = {(b.x[0] + b.u[0]) - yi0, (b.x[1] + b.u[1]) - yi1,
-                         (b.x[2] + b.u[2]) - yi2};
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      num[3 * a + c] += b.vol * yij[a] * xi[c];
-  if (DAMAGE) {
-    const uint32_t bit = 1u << (p & 31);
-    if (!(broken[p >> 5] & bit) && stretch_breaks(test, xi, yij))
-      atomicOr(&broken[p >> 5], bit);
-  }
-}
-
-template <bool DAMAGE, bool PIPE>
-__device__ __forceinline__ void defgrad_one(int64_t i, Planes x, Planes u,
-                                            const double *__restrict__ vol, Sell bonds,
-                                            Planes kinv, Planes fout, uint32_t *broken,
-                                            double initiation, double failure,
-                                            int useCompression) {
-  const StretchTest test = make_stretch_test(initiation, failure, useCompression);
+__device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, Planes u,
+                                            Sell bonds, Planes kinv, Planes fout,
+                                            uint32_t *broken, const StretchTest &test) {
   const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
   const double yi0 = x0 + u(0, i), yi1 = x1 + u(1, i), yi2 = x2 + u(2, i);
   double num[9];
@@ -279,61 +264,34 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, Planes u,
   for (int c = 0; c < 9; ++c)
     num[c] = 0.0;
   const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
-  const int64_t p0 = sell_pos(bonds, i, 0);
-  if (!PIPE) {
-    for (int k = 0; k < len; ++k) {
-      const int64_t p = p0 + 64 * (int64_t)k;
-      DefgradBond b;
-      load_defgrad_bond(x, u, vol, bonds.col[p], b);
-      add_defgrad_bond<DAMAGE>(x0, x1, x2, yi0, yi1, yi2, b, num, test, broken, p);
-    }
-  } else if (len > 0) {
-    // Software pipeline as in the force pass: while bond k is summed, bond k+1's data and
-    // flag word and bond k+2's column index are in flight. Every load is unconditional
-    // (indices clamped to the row's last bond), so the compiler can wait for exactly the
-    // loads each use needs. New breaks are gathered in a 64-bit mask and written out every
-    // 64 bonds; the row's flags are read only by this wavefront before that, and F does not
-    // depend on them, so deferring the writes changes nothing.
-    const int last = len - 1;
-    int jNext = bonds.col[p0 + 64 * (int64_t)(last < 1 ? last : 1)];
-    DefgradBond cur;
-    load_defgrad_bond(x, u, vol, bonds.col[p0], cur);
-    uint32_t flagCur = DAMAGE ? broken[p0 >> 5] : 0u;
-    uint64_t fresh = 0;
-    for (int k = 0; k < len; ++k) {
-      const int64_t p = p0 + 64 * (int64_t)k;
-      const int k1 = k + 1 < last ? k + 1 : last;
-      const int k2 = k + 2 < last ? k + 2 : last;
-      const int jAfter = bonds.col[p0 + 64 * (int64_t)k2];
-      DefgradBond nxt;
-      load_defgrad_bond(x, u, vol, jNext, nxt);
-      const int64_t pn = p0 + 64 * (int64_t)k1;
-      const uint32_t flagNext = DAMAGE ? broken[pn >> 5] : 0u;
-      const double xi[3] = {cur.x[0] - x0, cur.x[1] - x1, cur.x[2] - x2};
-      const double yij[3] = {(cur.x[0] + cur.u[0]) - yi0, (cur.x[1] + cur.u[1]) - yi1,
-                             (cur.x[2] + cur.u[2]) - yi2};
+  const int64_t g0 = sell_pos(bonds, i, 0);
+  for (int k0 = 0; k0 < len; k0 += 4) {
+    const int64_t pg = g0 + 64 * (int64_t)k0; // slot of bond k0 (k0 is a multiple of 4)
+    const i4v jj = *(const i4v *)(bonds.col + pg);
+    const uint32_t flags = DAMAGE ? broken[pg >> 5] : 0u;
+    uint32_t fresh = 0;
 #pragma unroll
-      for (int a = 0; a < 3; ++a)
+    for (int q = 0; q < 4; ++q) {
+      const int j = jj[q];
+      const d2v a = xv(0, j), b = xv(1, j);
+      const double uj0 = u(0, j), uj1 = u(1, j), uj2 = u(2, j);
+      if (k0 + q < len) {
+        const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
+        const double yij[3] = {(a.x + uj0) - yi0, (a.y + uj1) - yi1, (b.x + uj2) - yi2};
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          num[3 * a + c] += cur.vol * yij[a] * xi[c];
-      if (DAMAGE) {
-        const bool brk = !(flagCur & (1u << (p & 31))) && stretch_breaks(test, xi, yij);
-        fresh |= (uint64_t)brk << (k & 63);
-        if ((k & 63) == 63 || k == last) {
-          const int kBase = k & ~63;
-          while (fresh) {
-            const int b = __ffsll((unsigned long long)fresh) - 1;
-            fresh &= fresh - 1;
-            const int64_t q = p0 + 64 * (int64_t)(kBase + b);
-            atomicOr(&broken[q >> 5], 1u << (q & 31));
-          }
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            num[3 * r + c] += b.y * yij[r] * xi[c];
+        if (DAMAGE) {
+          const uint32_t bit = 1u << ((pg + q) & 31);
+          if (!(flags & bit) && stretch_breaks(test, xi, yij))
+            fresh |= bit;
         }
       }
-      cur = nxt;
-      flagCur = flagNext;
-      jNext = jAfter;
     }
+    if (DAMAGE && fresh)
+      atomicOr(&broken[pg >> 5], fresh);
   }
   T3 nm, ki;
 #pragma unroll
@@ -347,21 +305,35 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, Planes u,
     fout(c, i) = f.m[c];
 }
 
-template <bool DAMAGE, bool PIPE = false>
+template <bool DAMAGE>
 __global__ void __launch_bounds__(256)
-    k_defgrad(Rounds rd, Planes x, Planes u, const double *__restrict__ vol, Sell bonds,
-              Planes kinv, Planes fout, uint32_t *broken, double initiation, double failure,
-              int useCompression) {
+    k_defgrad(Rounds rd, Planes x, PairPlanes xv, Planes u, Sell bonds, Planes kinv, Planes fout,
+              uint32_t *broken, double initiation, double failure, int useCompression) {
   const int64_t blk = xcd_block(rd.blocksPerRound);
   if (blk < 0)
     return;
+  const StretchTest test = make_stretch_test(initiation, failure, useCompression);
   for (int r = 0; r < rd.rounds; ++r) {
     const int64_t i = rd.particle(r, blk);
     if (i >= rd.n)
       break;
-    defgrad_one<DAMAGE, PIPE>(i, x, u, vol, bonds, kinv, fout, broken, initiation, failure,
-                              useCompression);
+    defgrad_one<DAMAGE>(i, x, xv, u, bonds, kinv, fout, broken, test);
   }
+}
+
+// A = P K⁻¹ of particle i into its 5 component pairs.
+__device__ __forceinline__ void store_amap(PairPlanes amap, int64_t i, const T3 &A) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    d2v v;
+    v.x = A.m[2 * c];
+    v.y = A.m[2 * c + 1];
+    amap(c, i) = v;
+  }
+  d2v last;
+  last.x = A.m[8];
+  last.y = 0.0;
+  amap(4, i) = last;
 }
 
 // Material point update. VumatPdCorrespondenceMaterial.cpp:110-173 for every point at
@@ -374,7 +346,8 @@ __global__ void __launch_bounds__(256)
 struct MaterialArgs {
   int64_t n;
   int64_t nblocks;
-  Planes fnew, fN, vN, rN, sig, statev, kinv, amap;
+  Planes fnew, fN, vN, rN, sig, statev, kinv;
+  PairPlanes amap; // A = P K⁻¹ as 5 component pairs (the last one (A_8, 0))
   double *temp, *eint, *einel;
   Planes sigOut, strainOut, incOut;
   int *iterOut, *fallbackOut;
@@ -432,10 +405,7 @@ __device__ __forceinline__ void material_finish(const MaterialArgs &g, int64_t i
 #pragma unroll
   for (int c = 0; c < 9; ++c)
     ki.m[c] = g.kinv(c, i);
-  const T3 A = t3_mul(P, ki);
-#pragma unroll
-  for (int c = 0; c < 9; ++c)
-    g.amap(c, i) = A.m[c];
+  store_amap(g.amap, i, t3_mul(P, ki));
 }
 
 // Material point update, fused: VumatPdCorrespondenceMaterial.cpp:110-173 for every point at
@@ -687,7 +657,8 @@ struct ForceArgs {
   int64_t n;
   int64_t nblocks;
   Rounds rd;
-  Planes x, u, v, a, amap, forceOut;
+  Planes x, u, v, a, forceOut;
+  PairPlanes xv, amap; // neighbour data as component pairs: (X_x, X_y), (X_z, V); A
   const double *vol;
   Sell bonds;
   const uint32_t *broken;
@@ -697,93 +668,66 @@ struct ForceArgs {
   DiagArgs diag;
 };
 
-// Neighbour data one force-pass bond needs: X_j, V_j, A_j.
-struct ForceBond {
-  double x[3], vol;
-  T3 a;
-};
-
-__device__ __forceinline__ void load_force_bond(const ForceArgs &g, int j, ForceBond &b) {
+// A of particle j from its 5 component pairs.
+__device__ __forceinline__ T3 load_amap(PairPlanes amap, int64_t j) {
+  T3 A;
 #pragma unroll
-  for (int d = 0; d < 3; ++d)
-    b.x[d] = g.x(d, j);
-  b.vol = g.vol[j];
-#pragma unroll
-  for (int c = 0; c < 9; ++c)
-    b.a.m[c] = g.amap(c, j);
+  for (int c = 0; c < 4; ++c) {
+    const d2v v = amap(c, j);
+    A.m[2 * c] = v.x;
+    A.m[2 * c + 1] = v.y;
+  }
+  A.m[8] = amap(4, j).x;
+  return A;
 }
 
 // One bond's term of PdCorrespondenceSkeleton.cpp:219-225 in the reference's operation
 // order: t_i = A_i ξ, t_j = A_j (−ξ), term = (t_i − t_j) V_j (the caller adds it).
-__device__ __forceinline__ void force_bond_term(const T3 &ai, double x0, double x1, double x2,
-                                                const ForceBond &b, double term[3]) {
+__device__ __forceinline__ void force_bond_term(const T3 &ai, const T3 &aj, const double xi3[3],
+                                                double vj, double term[3]) {
   V3 xi;
-  xi.v[0] = b.x[0] - x0;
-  xi.v[1] = b.x[1] - x1;
-  xi.v[2] = b.x[2] - x2;
+  xi.v[0] = xi3[0];
+  xi.v[1] = xi3[1];
+  xi.v[2] = xi3[2];
   const V3 ti = t3_mulv(ai, xi);
   V3 nxi;
   nxi.v[0] = -xi.v[0];
   nxi.v[1] = -xi.v[1];
   nxi.v[2] = -xi.v[2];
-  const V3 tj = t3_mulv(b.a, nxi);
+  const V3 tj = t3_mulv(aj, nxi);
 #pragma unroll
   for (int d = 0; d < 3; ++d)
-    term[d] = (ti.v[d] - tj.v[d]) * b.vol;
+    term[d] = (ti.v[d] - tj.v[d]) * vj;
 }
 
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool PIPE, bool DIAG>
+// Bond pass 2 for particle i: the bonds in groups of 4 (one 16-byte column load, one flag
+// word), each bond's X_j, V_j and A_j as 7 component-pair loads; a broken bond contributes
+// exactly zero, so it is skipped. The sum runs in CSR order.
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool DIAG>
 __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double *dv) {
   const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
-  T3 ai;
-#pragma unroll
-  for (int c = 0; c < 9; ++c)
-    ai.m[c] = g.amap(c, i);
+  const T3 ai = load_amap(g.amap, i);
   double acc[3] = {0.0, 0.0, 0.0};
   const int len = g.bonds.rowptr[i + 1] - g.bonds.rowptr[i];
-  const int64_t p0 = sell_pos(g.bonds, i, 0);
-  if (!PIPE) {
-    for (int k = 0; k < len; ++k) {
-      const int64_t p = p0 + 64 * (int64_t)k;
-      if (DAMAGE && (g.broken[p >> 5] & (1u << (p & 31))))
-        continue;
-      ForceBond b;
-      load_force_bond(g, g.bonds.col[p], b);
-      double term[3];
-      force_bond_term(ai, x0, x1, x2, b, term);
+  const int64_t g0 = sell_pos(g.bonds, i, 0);
+  for (int k0 = 0; k0 < len; k0 += 4) {
+    const int64_t pg = g0 + 64 * (int64_t)k0; // slot of bond k0 (k0 is a multiple of 4)
+    const i4v jj = *(const i4v *)(g.bonds.col + pg);
+    const uint32_t flags = DAMAGE ? g.broken[pg >> 5] : 0u;
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        acc[d] += term[d];
-    }
-  } else if (len > 0) {
-    // Software pipeline: while bond k is summed, bond k+1's data and flag word and bond
-    // k+2's column index are in flight, so each wavefront keeps two bonds of independent
-    // loads outstanding. Every load is unconditional (indices clamped to the row's last
-    // bond) and the flag test is a select, so the compiler waits only for the loads each
-    // use needs. The summation order is unchanged.
-    const int last = len - 1;
-    int jNext = g.bonds.col[p0 + 64 * (int64_t)(last < 1 ? last : 1)];
-    ForceBond cur;
-    load_force_bond(g, g.bonds.col[p0], cur);
-    uint32_t flagCur = DAMAGE ? g.broken[p0 >> 5] : 0u;
-    for (int k = 0; k < len; ++k) {
-      const int64_t p = p0 + 64 * (int64_t)k;
-      const int k1 = k + 1 < last ? k + 1 : last;
-      const int k2 = k + 2 < last ? k + 2 : last;
-      const int jAfter = g.bonds.col[p0 + 64 * (int64_t)k2];
-      ForceBond nxt;
-      load_force_bond(g, jNext, nxt);
-      const int64_t pn = p0 + 64 * (int64_t)k1;
-      const uint32_t flagNext = DAMAGE ? g.broken[pn >> 5] : 0u;
-      double term[3];
-      force_bond_term(ai, x0, x1, x2, cur, term);
-      const bool skip = DAMAGE && (flagCur & (1u << (p & 31)));
+    for (int q = 0; q < 4; ++q) {
+      const int j = jj[q];
+      const d2v a = g.xv(0, j), b = g.xv(1, j);
+      const T3 aj = load_amap(g.amap, j);
+      const bool live = k0 + q < len && !(DAMAGE && (flags & (1u << ((pg + q) & 31))));
+      if (live) {
+        const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
+        double term[3];
+        force_bond_term(ai, aj, xi, b.y, term);
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        acc[d] = skip ? acc[d] : acc[d] + term[d];
-      cur = nxt;
-      flagCur = flagNext;
-      jNext = jAfter;
+        for (int d = 0; d < 3; ++d)
+          acc[d] += term[d];
+      }
     }
   }
   if (g.forceOut.p) {
@@ -838,7 +782,7 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
 // With DIAG (and INTEGRATE) each 256-particle chunk also writes its row of the run
 // diagnostics' force terms; the chunk loop then runs block-uniformly so that every thread
 // reaches the chunk's reduction.
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true, bool PIPE = false, bool DIAG = false>
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true, bool DIAG = false>
 __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
   static_assert(!DIAG || INTEGRATE, "the diagnostics read the corrected velocity");
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
@@ -849,7 +793,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
     if (!DIAG) {
       if (i >= g.rd.n)
         break;
-      force_one<DAMAGE, FUSE_NEXT, INTEGRATE, PIPE, false>(g, i, nullptr);
+      force_one<DAMAGE, FUSE_NEXT, INTEGRATE, false>(g, i, nullptr);
       continue;
     }
     const int64_t chunk = (int64_t)r * g.rd.blocksPerRound + blk;
@@ -861,7 +805,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
     for (int s = 0; s < NS; ++s)
       dv[s] = 0.0;
     if (i < g.rd.n)
-      force_one<DAMAGE, FUSE_NEXT, INTEGRATE, PIPE, true>(g, i, dv);
+      force_one<DAMAGE, FUSE_NEXT, INTEGRATE, true>(g, i, dv);
     diag_block_store<NS, D_NET_X>(dv, g.diag.partials + chunk * DIAG_STRIDE + D_NET_X);
   }
 }
@@ -869,7 +813,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
 // cauchyToFirstPiola · K⁻¹ for a caller-supplied spatial stress (PdCorrespondenceSkeleton.cpp:
 // 83-91, 209-210): A_i = (J σ F⁻ᵀ) K_i⁻¹.
 __global__ void __launch_bounds__(256) k_piola(int64_t n, Planes sigma, Planes f, Planes kinv,
-                                               Planes amap, int *err) {
+                                               PairPlanes amap, int *err) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
@@ -884,10 +828,7 @@ __global__ void __launch_bounds__(256) k_piola(int64_t n, Planes sigma, Planes f
     raise_error(err, ERR_PIOLA_DET);
     return;
   }
-  const T3 A = t3_mul(P, ki);
-#pragma unroll
-  for (int c = 0; c < 9; ++c)
-    amap(c, i) = A.m[c];
+  store_amap(amap, i, t3_mul(P, ki));
 }
 
 // The full Flanagan-Taylor update of CorotationalKinematics.cpp:61-135 with every output,

@@ -19,6 +19,17 @@ struct Planes {
   __device__ __forceinline__ double &operator()(int c, int64_t i) const { return p[c * n + i]; }
 };
 
+// Planes of component pairs: pair c of particle i is the 16-byte (component 2c, 2c+1) at
+// p[c·n + i]. The bond passes gather neighbour data in this form: one 16-byte load per lane
+// brings two components, and the 4 lanes of a quad, reading 4 consecutive particles, touch
+// 64 contiguous bytes.
+typedef double d2v __attribute__((ext_vector_type(2)));
+struct PairPlanes {
+  d2v *p;
+  int64_t n;
+  __device__ __forceinline__ d2v &operator()(int c, int64_t i) const { return p[c * n + i]; }
+};
+
 struct T3 {
   double m[9];
 };

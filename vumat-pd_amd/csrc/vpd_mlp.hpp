@@ -160,175 +160,197 @@ __device__ __forceinline__ short to_bf16(float f) {
   return (short)(u >> 16);
 }
 
-// One hidden or output layer for the wavefront's 16 points. Input activations h_in
-// [16][K] (FP32, LDS), weights W [N][K] row-major (global), output → h_out [16][N] (LDS)
-// after bias and activation. v_mfma_f32_16x16x16_bf16 operand layouts (lane l):
-//   A[m][k]: m = l % 16, k = 4 (l / 16) + i    (i = 0..3 in the 4-vector)
-//   B[k][n]: n = l % 16, k = 4 (l / 16) + i
+// Inference keeps the whole network in the wavefront's registers and computes each layer as
+// D[m][n] = Σ_k W[m][k] X[n][k] with the output feature as the MFMA M dimension and the point
+// as N. v_mfma_f32_16x16x16_bf16 operand layouts (lane l, i = 0..3 in the 4-vector):
+//   A[m][k]: m = l % 16, k = 4 (l / 16) + i    (a 16×16 weight tile)
+//   B[k][n]: n = l % 16, k = 4 (l / 16) + i    (16 points' activations)
 //   D[m][n]: n = l % 16, m = 4 (l / 16) + i
-template <int K, int N, bool TANH>
-__device__ __forceinline__ void mlp_layer_mfma(const float *h_in, const float *__restrict__ W,
-                                               const float *__restrict__ bias, float *h_out,
-                                               int lane) {
+// A layer's D tile for features 16t..16t+15 is then exactly the B operand of the next layer's
+// K-slice t, so activations stay in registers from input to output, and a wavefront loads
+// the weights once for all the 16-point tiles it processes.
+struct MlpWeights {
+  bf16x4 w1[4];    // [M-tile]: features 16t.., inputs 0..15 (8 real)
+  bf16x4 w2[4][4]; // [M-tile][K-slice]
+  bf16x4 w3[4][4];
+  bf16x4 w4[4];    // output 0 (tile row 0) × K-slices
+};
+
+__device__ __forceinline__ bf16x4 bf16_of(f32x4 v) {
+  bf16x4 o;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    o[i] = to_bf16(v[i]);
+  return o;
+}
+
+__device__ __forceinline__ void mlp_load_weights(const float *__restrict__ theta, int lane,
+                                                 MlpWeights &w) {
   const int r = lane & 15, q = lane >> 4;
-  constexpr int KT = (K + 15) / 16, NT = (N + 15) / 16;
 #pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < 4; ++t) {
+    f32x4 v = {0.f, 0.f, 0.f, 0.f};
+    if (q < 2)
+      v = *(const f32x4 *)(theta + MLP_OFF_W1 + (16 * t + r) * MLP_IN + 4 * q);
+    w.w1[t] = bf16_of(v);
+  }
 #pragma unroll
-    for (int kt = 0; kt < KT; ++kt) {
-      bf16x4 a, b;
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int k = kt * 16 + 4 * q + i;
-        const int n = nt * 16 + r;
-        a[i] = to_bf16(k < K ? h_in[r * K + k] : 0.f);
-        b[i] = to_bf16(k < K && n < N ? W[n * K + k] : 0.f);
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    for (int s = 0; s < 4; ++s) {
+      const int off = (16 * t + r) * MLP_H + 16 * s + 4 * q;
+      w.w2[t][s] = bf16_of(*(const f32x4 *)(theta + MLP_OFF_W2 + off));
+      w.w3[t][s] = bf16_of(*(const f32x4 *)(theta + MLP_OFF_W3 + off));
     }
-    const int n = nt * 16 + r;
-    if (n < N) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float z = acc[i] + bias[n];
-        h_out[(4 * q + i) * N + n] = TANH ? tanhf(z) : z;
-      }
-    }
+  for (int s = 0; s < 4; ++s) {
+    f32x4 v = {0.f, 0.f, 0.f, 0.f};
+    if (r == 0)
+      v = *(const f32x4 *)(theta + MLP_OFF_W4 + 16 * s + 4 * q);
+    w.w4[s] = bf16_of(v);
   }
 }
 
-// (x − mean) / std in FP64, then rounded to FP32: a feature whose spread is far below its
-// magnitude (the temperature moves by micro-kelvins at 293 K) keeps its information, which
-// an FP32 subtraction would round away.
-__device__ __forceinline__ void normalise_features(const double x[MLP_IN],
-                                                   const float *__restrict__ norm, float *out) {
-#pragma unroll
-  for (int c = 0; c < MLP_IN; ++c)
-    out[c] = (float)((x[c] - (double)norm[c]) / (double)norm[MLP_IN + c]);
+// tanh from one exponential: (1 − e)/(1 + e) with e = exp(−2|z|), sign restored. About 1e-7
+// absolute error, far below the bf16 rounding the result goes through next.
+__device__ __forceinline__ float fast_tanh(float z) {
+  const float e = __expf(-2.f * fabsf(z));
+  return copysignf(__fdividef(1.f - e, 1.f + e), z);
 }
 
-// Network seeds for n points. Inputs per point: strain increments (6 planes, stride
-// incStride), ε̄ᵖ and T (planes of `state`, stride stateStride: components 1 and 2 of the
-// JC state). Writes seed[i] = max(0, y·std + mean). One wavefront per 16 points.
-__global__ void __launch_bounds__(256) k_mlp_seed(int64_t n, const double *__restrict__ inc,
+__device__ __forceinline__ bf16x4 tanh_bf16(f32x4 v) {
+  bf16x4 o;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    o[i] = to_bf16(fast_tanh(v[i]));
+  return o;
+}
+
+// The forward pass for one tile of 16 points. x: the input tile in the B layout (lanes
+// 0-15 hold inputs 0-3 of point l % 16, lanes 16-31 inputs 4-7, the rest zero). Returns the
+// raw output of point l in lanes l < 16.
+__device__ __forceinline__ float mlp_tile(const MlpWeights &w, const float *__restrict__ theta,
+                                          bf16x4 x, int q) {
+  bf16x4 h[4], g[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x4 acc = *(const f32x4 *)(theta + MLP_OFF_B1 + 16 * t + 4 * q);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w1[t], x, acc, 0, 0, 0);
+    h[t] = tanh_bf16(acc);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x4 acc = *(const f32x4 *)(theta + MLP_OFF_B2 + 16 * t + 4 * q);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w2[t][s], h[s], acc, 0, 0, 0);
+    g[t] = tanh_bf16(acc);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x4 acc = *(const f32x4 *)(theta + MLP_OFF_B3 + 16 * t + 4 * q);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w3[t][s], g[s], acc, 0, 0, 0);
+    h[t] = tanh_bf16(acc);
+  }
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w4[s], h[s], acc, 0, 0, 0);
+  return acc[0] + theta[MLP_OFF_B4];
+}
+
+// (x − mean) / std in FP64, then bf16: a feature whose spread is far below its magnitude (the
+// temperature moves by micro-kelvins at 293 K) keeps its information, which an FP32
+// subtraction would round away.
+__device__ __forceinline__ short normalised_bf16(double x, const float *__restrict__ norm, int c) {
+  return to_bf16((float)((x - (double)norm[c]) / (double)norm[MLP_IN + c]));
+}
+
+// Network seeds for n points (LIST: for the `*count` points of a compacted list; slot s holds
+// particle list[s] and its strain increments at inc[c·incStride + s]). Inputs per point: the 6
+// strain increments, ε̄ᵖ and T (components 1 and 2 of the JC state planes, stride
+// stateStride). Writes seed[s] = max(0, y·std + mean). Each wavefront loads the weights once
+// and walks 16-point tiles with a grid stride; wavefronts with no tile exit at once, and the
+// tile loop is uniform per wavefront, so every MFMA runs with all lanes active.
+template <bool LIST>
+__global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__restrict__ count,
+                                                  const int *__restrict__ list,
+                                                  const double *__restrict__ inc,
                                                   int64_t incStride,
                                                   const double *__restrict__ state,
                                                   int64_t stateStride,
                                                   const float *__restrict__ theta,
                                                   const float *__restrict__ norm,
                                                   double *__restrict__ seed) {
-  __shared__ float buf[4][2][16 * MLP_H];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16;
-  // Every wave runs every layer (the MFMAs need the full wavefront and the barriers the
-  // whole workgroup); waves past the end work on zeros and store nothing.
-  float *h0 = buf[wave][0], *h1 = buf[wave][1];
-  if (lane < 16) {
-    const int64_t i = base + lane;
-    double x[MLP_IN];
-#pragma unroll
-    for (int c = 0; c < MLP_IN; ++c)
-      x[c] = norm[c]; // padding rows normalise to 0
-    if (i < n) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-        x[c] = inc[c * incStride + i];
-      x[6] = state[1 * stateStride + i];
-      x[7] = state[2 * stateStride + i];
-    }
-    normalise_features(x, norm, h0 + lane * MLP_IN);
-  }
-  __syncthreads();
-  mlp_layer_mfma<MLP_IN, MLP_H, true>(h0, theta + MLP_OFF_W1, theta + MLP_OFF_B1, h1, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, MLP_H, true>(h1, theta + MLP_OFF_W2, theta + MLP_OFF_B2, h0, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, MLP_H, true>(h0, theta + MLP_OFF_W3, theta + MLP_OFF_B3, h1, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, 1, false>(h1, theta + MLP_OFF_W4, theta + MLP_OFF_B4, h0, lane);
-  __syncthreads();
-  if (lane < 16 && base + lane < n) {
-    const float y = h0[lane] * norm[2 * MLP_IN + 1] + norm[2 * MLP_IN];
-    seed[base + lane] = y > 0.f ? (double)y : 0.0;
-  }
-}
-
-// k_mlp_seed over a compacted point list: slot s of the list holds particle list[s], its
-// strain increments at inc[c·incStride + s]; the state planes are indexed by particle. The
-// list length is read on the device; blocks past it exit whole.
-__global__ void __launch_bounds__(256) k_mlp_seed_list(const int *__restrict__ count,
-                                                       const int *__restrict__ list,
-                                                       const double *__restrict__ inc,
-                                                       int64_t incStride,
-                                                       const double *__restrict__ state,
-                                                       int64_t stateStride,
-                                                       const float *__restrict__ theta,
-                                                       const float *__restrict__ norm,
-                                                       double *__restrict__ seed) {
-  const int64_t n = *count;
-  if ((int64_t)blockIdx.x * 64 >= n) // the same for every thread of the block
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int64_t n = LIST ? (int64_t)*count : nPoints;
+  const int64_t tiles = (n + 15) / 16;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  if (wave >= tiles)
     return;
-  __shared__ float buf[4][2][16 * MLP_H];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16;
-  float *h0 = buf[wave][0], *h1 = buf[wave][1];
-  if (lane < 16) {
-    const int64_t s = base + lane;
-    double x[MLP_IN];
+  MlpWeights w;
+  mlp_load_weights(theta, lane, w);
+  const double mean = norm[2 * MLP_IN], sdev = norm[2 * MLP_IN + 1];
+  for (int64_t tile = wave; tile < tiles; tile += waves) {
+    const int64_t s = tile * 16 + r;
+    bf16x4 x = {0, 0, 0, 0};
+    if (q < 2 && s < n) {
+      const int64_t i = LIST ? (int64_t)list[s] : s;
+      if (q == 0) {
 #pragma unroll
-    for (int c = 0; c < MLP_IN; ++c)
-      x[c] = norm[c]; // padding rows normalise to 0
-    if (s < n) {
-      const int64_t i = list[s];
-#pragma unroll
-      for (int c = 0; c < 6; ++c)
-        x[c] = inc[c * incStride + s];
-      x[6] = state[1 * stateStride + i];
-      x[7] = state[2 * stateStride + i];
+        for (int c = 0; c < 4; ++c)
+          x[c] = normalised_bf16(inc[c * incStride + s], norm, c);
+      } else {
+        x[0] = normalised_bf16(inc[4 * incStride + s], norm, 4);
+        x[1] = normalised_bf16(inc[5 * incStride + s], norm, 5);
+        x[2] = normalised_bf16(state[1 * stateStride + i], norm, 6);
+        x[3] = normalised_bf16(state[2 * stateStride + i], norm, 7);
+      }
     }
-    normalise_features(x, norm, h0 + lane * MLP_IN);
-  }
-  __syncthreads();
-  mlp_layer_mfma<MLP_IN, MLP_H, true>(h0, theta + MLP_OFF_W1, theta + MLP_OFF_B1, h1, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, MLP_H, true>(h1, theta + MLP_OFF_W2, theta + MLP_OFF_B2, h0, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, MLP_H, true>(h0, theta + MLP_OFF_W3, theta + MLP_OFF_B3, h1, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, 1, false>(h1, theta + MLP_OFF_W4, theta + MLP_OFF_B4, h0, lane);
-  __syncthreads();
-  if (lane < 16 && base + lane < n) {
-    const float y = h0[lane] * norm[2 * MLP_IN + 1] + norm[2 * MLP_IN];
-    seed[base + lane] = y > 0.f ? (double)y : 0.0;
+    const float y = mlp_tile(w, theta, x, q);
+    if (q == 0 && s < n) {
+      const double v = (double)y * sdev + mean;
+      seed[s] = v > 0.0 ? v : 0.0;
+    }
   }
 }
 
-// Raw network outputs (normalised units) for a batch of already-normalised inputs
-// x[b][8]: the inference kernel's arithmetic, exposed for checking against the FP64 net.
+// Raw network outputs (normalised units) for already-normalised FP32 inputs x[b][8]: the
+// inference arithmetic of k_mlp_seed, exposed for checking against the FP64 network.
 __global__ void __launch_bounds__(256) k_mlp_infer_raw(int64_t n, const float *__restrict__ x,
                                                        const float *__restrict__ theta,
                                                        float *__restrict__ y) {
-  __shared__ float buf[4][2][16 * MLP_H];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16;
-  float *h0 = buf[wave][0], *h1 = buf[wave][1];
-  if (lane < 16) {
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int64_t tiles = (n + 15) / 16;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  if (wave >= tiles)
+    return;
+  MlpWeights w;
+  mlp_load_weights(theta, lane, w);
+  for (int64_t tile = wave; tile < tiles; tile += waves) {
+    const int64_t s = tile * 16 + r;
+    bf16x4 xt = {0, 0, 0, 0};
+    if (q < 2 && s < n) {
 #pragma unroll
-    for (int c = 0; c < MLP_IN; ++c)
-      h0[lane * MLP_IN + c] = base + lane < n ? x[(base + lane) * MLP_IN + c] : 0.f;
+      for (int c = 0; c < 4; ++c)
+        xt[c] = to_bf16(x[s * MLP_IN + 4 * q + c]);
+    }
+    const float out = mlp_tile(w, theta, xt, q);
+    if (q == 0 && s < n)
+      y[s] = out;
   }
-  __syncthreads();
-  mlp_layer_mfma<MLP_IN, MLP_H, true>(h0, theta + MLP_OFF_W1, theta + MLP_OFF_B1, h1, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, MLP_H, true>(h1, theta + MLP_OFF_W2, theta + MLP_OFF_B2, h0, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, MLP_H, true>(h0, theta + MLP_OFF_W3, theta + MLP_OFF_B3, h1, lane);
-  __syncthreads();
-  mlp_layer_mfma<MLP_H, 1, false>(h1, theta + MLP_OFF_W4, theta + MLP_OFF_B4, h0, lane);
-  __syncthreads();
-  if (lane < 16 && base + lane < n)
-    y[base + lane] = h0[lane];
+}
+
+// Grid of the inference kernels: enough 4-wavefront workgroups to fill the chip, at most one
+// wavefront per 16-point tile.
+inline int mlp_grid(int64_t points) {
+  const int64_t tiles = (points + 15) / 16;
+  const int64_t blocks = (tiles + 3) / 4;
+  return (int)(blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048);
 }
 
 } // namespace vpd

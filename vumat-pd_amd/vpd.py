@@ -352,8 +352,7 @@ class Context:
             self._warm = warm_start  # keep the network alive while attached
             self._check(lib().vpd_attach_warm_start(self.h, warm_start.h))
 
-    TUNE_KEYS = ("defgrad_lds", "force_lds", "defgrad_rounds", "force_rounds",
-                 "defgrad_pipeline", "force_pipeline")
+    TUNE_KEYS = ("defgrad_lds", "force_lds", "defgrad_rounds", "force_rounds")
 
     def autotune(self, reps=3):
         out = (C.c_int * len(self.TUNE_KEYS))()
