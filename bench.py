@@ -64,6 +64,19 @@ def step_bytes_per_particle(bbar, damage):
     return 816 + 136 + 144 + 24 + 8 + 8 * bbar + (bbar / 4.0 if damage else 0.0)
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (calibrated against a device copy), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    if data.get("kernel") != kernel or data.get("workload") != workload:
+        return None
+    return data["hbm_bytes_per_launch"]
+
+
 def cpu_baseline(sample_steps, w):
     """The reference's own OpenMP increment (oracle/_ref) on this host, or the restated
     oracle when the reference build is absent, on the same bar for `sample_steps` steps."""
@@ -221,7 +234,8 @@ def main():
                    "end_velocity_mm_per_ms": w["velocity"], "damage": "critical_stretch s_c=0.02",
                    "return_map": "bisection (parity)", "steps_from": 1, "launch_tuning": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9 * 1e9,
-                     "unit": "B/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "B/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(dom, w["name"]),
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_s * 1e3},
         "step_model": {"bytes_per_particle_step": step_bps,
                        "hbm_fraction": step_bps * value / HBM_PEAK},

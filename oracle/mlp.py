@@ -1,6 +1,6 @@
 """ORACLE — test infrastructure only.
 
-FP64 numpy statement of the return-map warm-start network: 8 → 64 → 64 → 64 → 1, tanh on
+FP64 numpy statement of the return-map warm-start network: 8 → 64 → 64 → 64 → 1, ReLU on
 the hidden layers, linear output, mean-squared-error loss, Adam. Tests use it to check the
 GPU training kernels' forward pass, gradients and optimiser step, and the bf16 MFMA
 inference. It has no counterpart in the reference (the warm start is an addition); it is
@@ -49,9 +49,9 @@ def init(seed=0):
 def forward(theta, x):
     """x (B, 8) normalised inputs -> (y (B,), cache)."""
     W1, b1, W2, b2, W3, b3, W4, b4 = unflatten(np.asarray(theta, np.float64))
-    h1 = np.tanh(x @ W1.T + b1)
-    h2 = np.tanh(h1 @ W2.T + b2)
-    h3 = np.tanh(h2 @ W3.T + b3)
+    h1 = np.maximum(x @ W1.T + b1, 0.0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0.0)
+    h3 = np.maximum(h2 @ W3.T + b3, 0.0)
     y = (h3 @ W4.T + b4)[:, 0]
     return y, (x, h1, h2, h3)
 
@@ -66,13 +66,13 @@ def loss_and_grad(theta, x, t):
     dy = (2.0 / B) * r[:, None]
     gW4 = dy.T @ h3
     gb4 = dy.sum(0)
-    dz3 = (dy @ W4) * (1 - h3 * h3)
+    dz3 = (dy @ W4) * (h3 > 0)
     gW3 = dz3.T @ h2
     gb3 = dz3.sum(0)
-    dz2 = (dz3 @ W3) * (1 - h2 * h2)
+    dz2 = (dz3 @ W3) * (h2 > 0)
     gW2 = dz2.T @ h1
     gb2 = dz2.sum(0)
-    dz1 = (dz2 @ W2) * (1 - h1 * h1)
+    dz1 = (dz2 @ W2) * (h1 > 0)
     gW1 = dz1.T @ x
     gb1 = dz1.sum(0)
     g = np.concatenate([a.ravel() for a in (gW1, gb1, gW2, gb2, gW3, gb3, gW4, gb4)])

@@ -109,8 +109,10 @@ def test_mlp_gradient_vs_fp64_oracle(vpd):
                                    t.astype(np.float64))
     assert abs(loss - lref) <= 1e-5 * lref
     assert np.abs(g - gref).max() <= 1e-4 * np.abs(gref).max()
+    # FP32 against FP64: a pre-activation within rounding of a ReLU kink may fall on the other
+    # side, which moves a few components by more than rounding.
     cos = float(g @ gref / (np.linalg.norm(g) * np.linalg.norm(gref)))
-    assert cos > 1 - 1e-8
+    assert cos > 1 - 1e-6
 
 
 def test_mlp_mfma_inference_vs_fp64(vpd):

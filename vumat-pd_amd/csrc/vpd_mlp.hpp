@@ -1,13 +1,14 @@
-// The return-map warm start: an 8 → 64 → 64 → 64 → 1 tanh network that maps a material
+// The return-map warm start: an 8 → 64 → 64 → 64 → 1 ReLU network that maps a material
 // point's (6 Voigt strain increments, ε̄ᵖ, T) to an estimate of its plastic increment Δε̄ᵖ,
 // which seeds the safeguarded Newton solve of the Johnson-Cook return map.
 //
 //  * Training runs on the GPU in FP32 with these kernels: per-layer dense forward, the
 //    backward pass (input gradients and per-chunk weight-gradient partials reduced in a
 //    fixed order, so a training run is deterministic) and Adam.
-//  * Inference runs as bf16 MFMA tiles (v_mfma_f32_16x16x16_bf16) with FP32 accumulation:
-//    one wavefront pushes 16 points through the four layers, staging each hidden layer in
-//    LDS between the MFMA output layout and the next layer's input layout.
+//  * Inference runs as bf16 MFMA tiles (v_mfma_f32_16x16x16_bf16) with FP32 accumulation,
+//    the whole network resident in each wavefront's registers (see MlpWeights). ReLU keeps
+//    the per-element work between the MFMAs to a max and a packed bf16 conversion, so the
+//    inference is bound by the matrix cores rather than by transcendental VALU work.
 //
 // Flat parameter layout (FP32): W1 (64×8), b1 (64), W2 (64×64), b2 (64), W3 (64×64), b3 (64),
 // W4 (1×64), b4 (1): 8961 values. Normalisation (FP32, 18 values): input mean[8], std[8],
@@ -28,7 +29,7 @@ constexpr int MLP_NORM = 18;
 
 // ------------------------------------------------------------------ training (FP32)
 
-// Y[b][n] = act(Σ_k X[b][k] W[n][k] + bias[n]); act 1 = tanh, 0 = identity.
+// Y[b][n] = act(Σ_k X[b][k] W[n][k] + bias[n]); act 1 = ReLU, 0 = identity.
 __global__ void __launch_bounds__(256) k_mlp_dense(int B, int K, int N, const float *__restrict__ X,
                                                    const float *__restrict__ W,
                                                    const float *__restrict__ bias,
@@ -40,7 +41,7 @@ __global__ void __launch_bounds__(256) k_mlp_dense(int B, int K, int N, const fl
   float acc = bias[n];
   for (int k = 0; k < K; ++k)
     acc += X[(int64_t)b * K + k] * W[n * K + k];
-  Y[t] = act ? tanhf(acc) : acc;
+  Y[t] = act ? fmaxf(acc, 0.f) : acc;
 }
 
 // dy = 2 (y − target) / B and per-sample squared error (for the batch loss).
@@ -56,7 +57,7 @@ __global__ void __launch_bounds__(256) k_mlp_loss_grad(int B, const float *__res
   sq[b] = r * r;
 }
 
-// dX[b][k] = (Σ_n dZ[b][n] W[n][k]) · (tanh' at H[b][k] when H is given).
+// dX[b][k] = (Σ_n dZ[b][n] W[n][k]) · (ReLU' at H[b][k] when H is given: 1 where H > 0).
 __global__ void __launch_bounds__(256) k_mlp_back_input(int B, int K, int N,
                                                         const float *__restrict__ dZ,
                                                         const float *__restrict__ W,
@@ -69,10 +70,8 @@ __global__ void __launch_bounds__(256) k_mlp_back_input(int B, int K, int N,
   float acc = 0.f;
   for (int n = 0; n < N; ++n)
     acc += dZ[(int64_t)b * N + n] * W[n * K + k];
-  if (H) {
-    const float h = H[t];
-    acc *= 1.0f - h * h;
-  }
+  if (H && !(H[t] > 0.f))
+    acc = 0.f;
   dX[t] = acc;
 }
 
@@ -153,13 +152,6 @@ __global__ void __launch_bounds__(256) k_mlp_gather(int B, int64_t n, int64_t st
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Round-to-nearest-even FP32 → bf16 bit pattern.
-__device__ __forceinline__ short to_bf16(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
-
 // Inference keeps the whole network in the wavefront's registers and computes each layer as
 // D[m][n] = Σ_k W[m][k] X[n][k] with the output feature as the MFMA M dimension and the point
 // as N. v_mfma_f32_16x16x16_bf16 operand layouts (lane l, i = 0..3 in the 4-vector):
@@ -174,14 +166,14 @@ struct MlpWeights {
   bf16x4 w2[4][4]; // [M-tile][K-slice]
   bf16x4 w3[4][4];
   bf16x4 w4[4];    // output 0 (tile row 0) × K-slices
+  f32x4 b1[4], b2[4], b3[4]; // biases in the D layout: feature 16t + 4 (l / 16) + i
+  float b4;
 };
 
+// FP32 → bf16 (round to nearest even) with the hardware's packed conversion.
+typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ bf16x4 bf16_of(f32x4 v) {
-  bf16x4 o;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    o[i] = to_bf16(v[i]);
-  return o;
+  return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16v4));
 }
 
 __device__ __forceinline__ void mlp_load_weights(const float *__restrict__ theta, int lane,
@@ -209,63 +201,69 @@ __device__ __forceinline__ void mlp_load_weights(const float *__restrict__ theta
       v = *(const f32x4 *)(theta + MLP_OFF_W4 + 16 * s + 4 * q);
     w.w4[s] = bf16_of(v);
   }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    w.b1[t] = *(const f32x4 *)(theta + MLP_OFF_B1 + 16 * t + 4 * q);
+    w.b2[t] = *(const f32x4 *)(theta + MLP_OFF_B2 + 16 * t + 4 * q);
+    w.b3[t] = *(const f32x4 *)(theta + MLP_OFF_B3 + 16 * t + 4 * q);
+  }
+  w.b4 = theta[MLP_OFF_B4];
 }
 
-// tanh from one exponential: (1 − e)/(1 + e) with e = exp(−2|z|), sign restored. About 1e-7
-// absolute error, far below the bf16 rounding the result goes through next.
-__device__ __forceinline__ float fast_tanh(float z) {
-  const float e = __expf(-2.f * fabsf(z));
-  return copysignf(__fdividef(1.f - e, 1.f + e), z);
-}
-
-__device__ __forceinline__ bf16x4 tanh_bf16(f32x4 v) {
-  bf16x4 o;
+__device__ __forceinline__ bf16x4 relu_bf16(f32x4 v) {
 #pragma unroll
   for (int i = 0; i < 4; ++i)
-    o[i] = to_bf16(fast_tanh(v[i]));
-  return o;
+    v[i] = fmaxf(v[i], 0.f);
+  return bf16_of(v);
 }
 
 // The forward pass for one tile of 16 points. x: the input tile in the B layout (lanes
 // 0-15 hold inputs 0-3 of point l % 16, lanes 16-31 inputs 4-7, the rest zero). Returns the
 // raw output of point l in lanes l < 16.
-__device__ __forceinline__ float mlp_tile(const MlpWeights &w, const float *__restrict__ theta,
-                                          bf16x4 x, int q) {
+// The K-slices of a layer run outermost so that the 4 M-tiles' accumulation chains are
+// independent at every step and the MFMA pipeline stays full.
+__device__ __forceinline__ void mlp_hidden(const bf16x4 (&w)[4][4], const f32x4 (&b)[4],
+                                           const bf16x4 (&in)[4], bf16x4 (&out)[4]) {
+  f32x4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    acc[t] = b[t];
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w[t][s], in[s], acc[t], 0, 0, 0);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    out[t] = relu_bf16(acc[t]);
+}
+
+__device__ __forceinline__ float mlp_tile(const MlpWeights &w, bf16x4 x) {
   bf16x4 h[4], g[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    f32x4 acc = *(const f32x4 *)(theta + MLP_OFF_B1 + 16 * t + 4 * q);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w1[t], x, acc, 0, 0, 0);
-    h[t] = tanh_bf16(acc);
+    const f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w1[t], x, w.b1[t], 0, 0, 0);
+    h[t] = relu_bf16(acc);
   }
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    f32x4 acc = *(const f32x4 *)(theta + MLP_OFF_B2 + 16 * t + 4 * q);
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w2[t][s], h[s], acc, 0, 0, 0);
-    g[t] = tanh_bf16(acc);
-  }
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    f32x4 acc = *(const f32x4 *)(theta + MLP_OFF_B3 + 16 * t + 4 * q);
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w3[t][s], g[s], acc, 0, 0, 0);
-    h[t] = tanh_bf16(acc);
-  }
+  mlp_hidden(w.w2, w.b2, h, g);
+  mlp_hidden(w.w3, w.b3, g, h);
   f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int s = 0; s < 4; ++s)
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(w.w4[s], h[s], acc, 0, 0, 0);
-  return acc[0] + theta[MLP_OFF_B4];
+  return acc[0] + w.b4;
 }
 
-// (x − mean) / std in FP64, then bf16: a feature whose spread is far below its magnitude (the
-// temperature moves by micro-kelvins at 293 K) keeps its information, which an FP32
+// (x − mean) · (1/std) in FP64, then bf16: a feature whose spread is far below its magnitude
+// (the temperature moves by micro-kelvins at 293 K) keeps its information, which an FP32
 // subtraction would round away.
-__device__ __forceinline__ short normalised_bf16(double x, const float *__restrict__ norm, int c) {
-  return to_bf16((float)((x - (double)norm[c]) / (double)norm[MLP_IN + c]));
+__device__ __forceinline__ bf16x4 normalised_bf16(const double x[4], const double mean[4],
+                                                  const double invStd[4]) {
+  f32x4 v;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    v[c] = (float)((x[c] - mean[c]) * invStd[c]);
+  return bf16_of(v);
 }
 
 // Network seeds for n points (LIST: for the `*count` points of a compacted list; slot s holds
@@ -294,27 +292,52 @@ __global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__
   MlpWeights w;
   mlp_load_weights(theta, lane, w);
   const double mean = norm[2 * MLP_IN], sdev = norm[2 * MLP_IN + 1];
-  for (int64_t tile = wave; tile < tiles; tile += waves) {
+  // This lane's 4 raw inputs of one tile (lanes 0-15: increments 0-3; lanes 16-31:
+  // increments 4-5, ε̄ᵖ, T); the next tile's are loaded while the current one is computed.
+  auto fetch = [&](int64_t tile, double v[4]) {
     const int64_t s = tile * 16 + r;
-    bf16x4 x = {0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      v[c] = norm[4 * q + c]; // normalises to 0
     if (q < 2 && s < n) {
       const int64_t i = LIST ? (int64_t)list[s] : s;
       if (q == 0) {
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-          x[c] = normalised_bf16(inc[c * incStride + s], norm, c);
+          v[c] = inc[c * incStride + s];
       } else {
-        x[0] = normalised_bf16(inc[4 * incStride + s], norm, 4);
-        x[1] = normalised_bf16(inc[5 * incStride + s], norm, 5);
-        x[2] = normalised_bf16(state[1 * stateStride + i], norm, 6);
-        x[3] = normalised_bf16(state[2 * stateStride + i], norm, 7);
+        v[0] = inc[4 * incStride + s];
+        v[1] = inc[5 * incStride + s];
+        v[2] = state[1 * stateStride + i];
+        v[3] = state[2 * stateStride + i];
       }
     }
-    const float y = mlp_tile(w, theta, x, q);
+  };
+  double mu[4], inv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int f = q < 2 ? 4 * q + c : c;
+    mu[c] = norm[f];
+    inv[c] = 1.0 / (double)norm[MLP_IN + f];
+  }
+  double cur[4];
+  fetch(wave, cur);
+  for (int64_t tile = wave; tile < tiles; tile += waves) {
+    double nxt[4] = {0.0, 0.0, 0.0, 0.0};
+    if (tile + waves < tiles)
+      fetch(tile + waves, nxt);
+    bf16x4 x = {0, 0, 0, 0};
+    if (q < 2)
+      x = normalised_bf16(cur, mu, inv);
+    const float y = mlp_tile(w, x);
+    const int64_t s = tile * 16 + r;
     if (q == 0 && s < n) {
       const double v = (double)y * sdev + mean;
       seed[s] = v > 0.0 ? v : 0.0;
     }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      cur[c] = nxt[c];
   }
 }
 
@@ -334,23 +357,21 @@ __global__ void __launch_bounds__(256) k_mlp_infer_raw(int64_t n, const float *_
   for (int64_t tile = wave; tile < tiles; tile += waves) {
     const int64_t s = tile * 16 + r;
     bf16x4 xt = {0, 0, 0, 0};
-    if (q < 2 && s < n) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        xt[c] = to_bf16(x[s * MLP_IN + 4 * q + c]);
-    }
-    const float out = mlp_tile(w, theta, xt, q);
+    if (q < 2 && s < n)
+      xt = bf16_of(*(const f32x4 *)(x + s * MLP_IN + 4 * q));
+    const float out = mlp_tile(w, xt);
     if (q == 0 && s < n)
       y[s] = out;
   }
 }
 
-// Grid of the inference kernels: enough 4-wavefront workgroups to fill the chip, at most one
-// wavefront per 16-point tile.
+// Grid of the inference kernels: 2 four-wavefront workgroups per CU (the kernels' register
+// use allows 2 wavefronts per SIMD), each wavefront walking many 16-point tiles so that the
+// weights are loaded once per wavefront; at most one wavefront per tile.
 inline int mlp_grid(int64_t points) {
   const int64_t tiles = (points + 15) / 16;
   const int64_t blocks = (tiles + 3) / 4;
-  return (int)(blocks < 2048 ? (blocks > 0 ? blocks : 1) : 2048);
+  return (int)(blocks < 512 ? (blocks > 0 ? blocks : 1) : 512);
 }
 
 } // namespace vpd
