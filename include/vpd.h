@@ -204,6 +204,30 @@ int vpd_rotate_and_strain(const double *stress, const double *rotation, const do
 int vpd_step(vpd_context *ctx, const vpd_step_args *args);
 int vpd_synchronize(vpd_context *ctx);
 
+/* ---- batched specimens ------------------------------------------------------------ */
+/* Many velocity-driven specimens of uneven size in one particle block, stepped by one
+ * launch per pass. This replaces the process-level case sweep of
+ * benchmarks/tensile3d/run_cases.py:95-96 (a thread pool of single-case runs) for specimens
+ * that share the material and damage settings and differ in lattice, horizon, time step,
+ * end velocity and step count.
+ * Specimen s owns particles [first[s], first[s+1]) (first[0] = 0, first[count] = n); its
+ * loaded band moves at u = v·t, v, a = 0 with t = step·dt[s], v = velocity[s]; it takes
+ * steps 1..last_step[s] and then keeps its state while the others go on. Bonds between
+ * specimens are refused (place them more than a horizon apart, or give each its own CSR:
+ * vpd_set_neighbors accepts the concatenated rows). Call after vpd_initialize_dynamics; it
+ * applies every specimen's constraints at t = 0. count 0 returns to one specimen. While a
+ * table is set, the scalar-time entries (vpd_step, vpd_predict, vpd_apply_constraints,
+ * vpd_compute_deformation_gradient, vpd_update_material, vpd_compute_force_and_correct)
+ * and diagnostics are refused; vpd_initialize_material covers every particle.
+ * Each specimen's results equal those of the same specimen run alone through vpd_step with
+ * ramp (v·t, v, 0), bit for bit. */
+int vpd_set_specimens(vpd_context *ctx, int count, const int64_t *first, const double *dt,
+                      const double *velocity, const int32_t *last_step);
+/* Step `step` (1-based, consecutive while fuse_next chains) of every specimen that has not
+ * finished. With fuse_next the next step's predictor runs inside the force pass; a
+ * specimen's final step is never followed by a prediction. */
+int vpd_step_specimens(vpd_context *ctx, int32_t step, int fuse_next, int want_fields);
+
 /* ---- return-map solver of the material stage -------------------------------------- */
 /* Selects the solver (enum vpd_return_map) and the fast modes' relative residual bound
  * rtol (|r| <= rtol·q_trial; 1e-12 by default). Bisection is the parity mode; the others

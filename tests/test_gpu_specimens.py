@@ -1,0 +1,156 @@
+"""Batched specimens (vpd_set_specimens / vpd_step_specimens) on the GPU.
+
+The batch replaces the reference's process-per-case sweep (benchmarks/tensile3d/
+run_cases.py:95-96): specimens of different lattice, horizon, time step, end velocity and
+step count share one particle block and one launch per pass. Each specimen's end state must
+equal the same specimen run alone (vpd.Bar through vpd_step) bit for bit — the same rows in
+the same order, the same per-particle dt and load band, the same arithmetic — and one
+specimen is checked against the oracle as well, at the per-step bound of the bar tests.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import JC_PROPS
+
+pytestmark = pytest.mark.gpu
+
+RHO = 7.85e-3
+FIELDS = ("u", "v", "a", "stressN", "statev", "fN", "rN", "eint", "einel", "temp", "kinv")
+
+
+def banner(msg):
+    print("[gpu-case]", msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def vpd():
+    import vpd as V
+    if V.device_count() == 0:
+        pytest.fail("no HIP device visible to the GPU tests")
+    return V
+
+
+def cfl_dt(h, fraction):
+    E, nu = JC_PROPS[0], JC_PROPS[1]
+    return fraction * h / math.sqrt(E * (1 - nu) / (RHO * (1 + nu) * (1 - 2 * nu)))
+
+
+def specimens(vpd):
+    S = vpd.Specimen
+    return [
+        # lattice, Δx, δ/Δx, dt, end velocity (mm/ms), steps
+        S(30, 8, 8, 1.0, 3.015, cfl_dt(1.0, 0.5), 60.0, 70),
+        S(20, 6, 6, 0.5, 2.015, cfl_dt(0.5, 0.9), 30.0, 45),
+        S(16, 5, 5, 0.7, 4.015, cfl_dt(0.7, 0.25), -40.0, 90),   # compression
+        S(12, 4, 4, 1.0, 3.015, cfl_dt(1.0, 0.5), 80.0, 20),     # finishes early
+        S(9, 9, 3, 0.3, 3.015, cfl_dt(0.3, 0.5), 25.0, 55, component=1),  # loaded along y
+    ]
+
+
+def alone(vpd, sp, damage, return_map=0, steps=None):
+    bar = vpd.Bar(props=JC_PROPS, density=RHO, damage=damage, return_map=return_map,
+                  **sp.bar_kwargs())
+    bar.step(sp.steps if steps is None else steps)
+    return bar
+
+
+def assert_same(batch, s, bar, what):
+    for name in FIELDS + ("damage",) if bar.ctx.nnz else FIELDS:
+        got, want = batch.get(name, s), bar.get(name)
+        assert got.shape == want.shape, (what, name)
+        assert np.array_equal(got, want), (what, name, np.abs(got - want).max())
+
+
+@pytest.mark.parametrize("return_map", [0, 2], ids=["bisection", "newton"])
+def test_batch_equals_specimens_alone(vpd, return_map):
+    banner("batched specimens vs each alone, return map %d" % return_map)
+    damage = (0.003, 0.0)
+    sps = specimens(vpd)
+    batch = vpd.SpecimenBatch(sps, JC_PROPS, RHO, damage=damage, return_map=return_map)
+    assert batch.n == sum(sp.n for sp in sps)
+    batch.step()
+    assert batch.step_index == max(sp.steps for sp in sps)
+    broke = 0
+    for s, sp in enumerate(sps):
+        bar = alone(vpd, sp, damage, return_map)
+        assert np.array_equal(batch.x[batch.first[s]:batch.first[s + 1]], bar.x)
+        assert_same(batch, s, bar, "specimen %d" % s)
+        broke += int(bar.get("damage").sum() > 0)
+        assert bar.get("statev")[:, 1].max() > 0  # every specimen reached the plastic range
+        bar.ctx.close()
+    assert broke >= 2  # the damage path is exercised
+
+
+def test_batch_in_chunks_and_reset(vpd):
+    banner("batched specimens: stepped in chunks, then reset and rerun")
+    sps = specimens(vpd)[:4]
+    batch = vpd.SpecimenBatch(sps, JC_PROPS, RHO, damage=(0.003, 0.0))
+    batch.step(7)           # unfused at 7, fused chain resumes at 8
+    batch.step(13)          # specimen 3 (20 steps) finishes inside this chunk
+    batch.step()
+    first = {s: {n: batch.get(n, s) for n in ("u", "statev", "damage")} for s in range(4)}
+    for s, sp in enumerate(sps):
+        bar = alone(vpd, sp, (0.003, 0.0))
+        assert_same(batch, s, bar, "chunked %d" % s)
+        bar.ctx.close()
+    batch.reset()
+    batch.step()
+    for s in range(4):
+        for n, v in first[s].items():
+            assert np.array_equal(batch.get(n, s), v), (s, n)
+
+
+def test_batch_specimen_vs_oracle(vpd, orc):
+    banner("one batched specimen vs the oracle")
+    sps = specimens(vpd)[:2]
+    batch = vpd.SpecimenBatch(sps, JC_PROPS, RHO, damage=(0.003, 0.0))
+    batch.step()
+    sp = sps[1]
+    o = orc.OracleBar(nx=sp.nx, ny=sp.ny, nz=sp.nz, spacing=sp.spacing, horizon=sp.horizon,
+                      props=JC_PROPS, dt=sp.dt, density=RHO, amplitude=0.0, rise_time=1.0,
+                      damage=(0.003, 0.0), velocity=sp.velocity)
+    o.step(sp.steps)
+    assert np.array_equal(batch.get("damage", 1), o.damage)
+    for name, ov in (("u", o.u), ("v", o.v), ("stressN", o.mat["stressN"]),
+                     ("statev", o.mat["statev"])):
+        gv = batch.get(name, 1)
+        if name == "statev":
+            gv, ov = gv[:, :5], ov[:, :5]
+        err = np.abs(gv - ov).max() / max(np.abs(ov).max(), 1e-300)
+        assert err <= 1e-9 * sp.steps, (name, err)
+
+
+def test_batch_errors(vpd):
+    banner("batched specimens: refused uses")
+    sp = specimens(vpd)[3]
+    bar = vpd.Bar(props=JC_PROPS, density=RHO, **sp.bar_kwargs())
+    n = bar.n
+    with pytest.raises(vpd.VpdError, match="bonded"):  # one bar cut in two shares bonds
+        bar.ctx.set_specimens([0, n // 2, n], [sp.dt] * 2, [1.0] * 2, [5, 5])
+    with pytest.raises(vpd.VpdError, match="cover the particle block"):
+        bar.ctx.set_specimens([0, n - 1], [sp.dt], [1.0], [5])
+    with pytest.raises(vpd.VpdError, match="positive"):
+        bar.ctx.set_specimens([0, n], [0.0], [1.0], [5])
+    with pytest.raises(vpd.VpdError, match="at least one step"):
+        bar.ctx.set_specimens([0, n], [sp.dt], [1.0], [0])
+    with pytest.raises(vpd.VpdError, match="No specimen table"):
+        bar.ctx.step_specimens(1)
+    bar.ctx.set_specimens([0, n], [sp.dt], [sp.velocity], [5])
+    with pytest.raises(vpd.VpdError, match="vpd_step_specimens"):
+        bar.step(1)
+    with pytest.raises(vpd.VpdError, match="one specimen"):
+        bar.ctx.enable_diagnostics(0, 1)
+    bar.ctx.step_specimens(1, fuse_next=True)
+    with pytest.raises(vpd.VpdError, match="prediction was made for step 2"):
+        bar.ctx.step_specimens(3)
+    for k in range(2, 6):
+        bar.ctx.step_specimens(k, fuse_next=True)  # the last step is never followed by one
+    u_batch = bar.get("u")
+    bar.ctx.set_specimens([], [], [], [])  # back to one specimen: the scalar API works again
+    ref = vpd.Bar(props=JC_PROPS, density=RHO, **sp.bar_kwargs())
+    ref.step(5)
+    assert np.array_equal(u_batch, ref.get("u"))

@@ -119,6 +119,15 @@ struct vpd_context {
   double *diagDstrain = nullptr;
   uint8_t *diagLat = nullptr;
   int diagCapacity = 0, diagCount = 0;
+  // Batched specimens (vpd_set_specimens): specimen of each particle, each specimen's time
+  // step, end velocity and last step, and the step being taken. batch.id stays null for a
+  // block that is one specimen.
+  Batch batch{};
+  uint16_t *specId = nullptr;
+  double *specDt = nullptr, *specVel = nullptr;
+  int *specLast = nullptr;
+  int specCount = 0;
+  std::vector<int64_t> specFirst;
 };
 
 namespace {
@@ -441,31 +450,34 @@ int launch_predict(vpd_context *c, double dt, const double ramp[3]) {
   return timed(c, K_PREDICT, [&] {
     k_predict<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->u, c->n), planes(c->v, c->n),
                                                     planes(c->a, c->n), c->codes, dt,
-                                                    ramp_of(ramp));
+                                                    ramp_of(ramp), c->batch);
   });
 }
 
 template <bool DAMAGE>
-void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression) {
+void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression,
+                      const Batch &batch) {
   k_defgrad<DAMAGE><<<grid_rounds(rd), 256, lds, c->stream>>>(
       rd, planes(c->x, c->n), pairs(c->xv, c->n), planes(c->u, c->n), sell_of(c),
       planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
-      useCompression);
+      useCompression, batch);
 }
 
-void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds) {
+void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, const Batch &batch) {
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
   const Rounds rd = rounds_of(c->n, rounds);
   if (dmg)
-    launch_defgrad_t<true>(c, lds, rd, useCompression);
+    launch_defgrad_t<true>(c, lds, rd, useCompression, batch);
   else
-    launch_defgrad_t<false>(c, lds, rd, useCompression);
+    launch_defgrad_t<false>(c, lds, rd, useCompression, batch);
 }
 
+// F of the step being taken: with batched specimens, of those that have not finished.
 int launch_defgrad(vpd_context *c) {
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
-  return timed(c, K_DEFGRAD,
-               [&] { launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad); });
+  return timed(c, K_DEFGRAD, [&] {
+    launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad, c->batch);
+  });
 }
 
 template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
@@ -550,6 +562,7 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
   g.rtol = c->rtol;
   g.err = c->derr;
   g.diag = c->diag;
+  g.batch = c->batch;
   if (diag && !c->diagOn)
     return fail(c, VPD_ERR_STATE, "Run diagnostics are not enabled.");
   return diag ? launch_material_t<true>(c, g) : launch_material_t<false>(c, g);
@@ -600,7 +613,6 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.a = planes(c->a, c->n);
   g.amap = pairs(c->amap, c->n);
   g.xv = pairs(c->xv, c->n);
-  g.xv = pairs(c->xv, c->n);
   g.forceOut = planes(fields ? c->force : nullptr, c->n);
   g.vol = c->vol;
   g.bonds = sell_of(c);
@@ -613,6 +625,7 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   g.rd = rounds_of(c->n, c->roundsForce);
   g.diag = c->diag;
+  g.batch = c->batch;
   return timed(c, K_FORCE, [&] {
     launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag);
   });
@@ -623,6 +636,14 @@ int require_block(vpd_context *c) {
     return fail(c, VPD_ERR_ARGUMENT, "Null vpd_context.");
   if (c->n <= 0)
     return fail(c, VPD_ERR_STATE, "Invalid or incomplete PD particle block.");
+  return VPD_OK;
+}
+
+// The scalar-time entry points drive a block that is one specimen.
+int require_single(vpd_context *c) {
+  if (c && c->batch.id)
+    return fail(c, VPD_ERR_STATE,
+                "The particle block holds batched specimens; step it with vpd_step_specimens.");
   return VPD_OK;
 }
 
@@ -760,6 +781,10 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->diagW);
   dfree(c->diagDstrain);
   dfree(c->diagLat);
+  dfree(c->specId);
+  dfree(c->specDt);
+  dfree(c->specVel);
+  dfree(c->specLast);
   dfree(c->derr);
   if (c->stream)
     (void)hipStreamDestroy(c->stream);
@@ -775,6 +800,8 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
     return fail(c, VPD_ERR_UNSUPPORTED, "Particle count exceeds the int32 index range.");
   c->n = n;
   c->diagOn = false; // sized for the previous block
+  c->batch = Batch{};
+  c->specCount = 0;
   const int64_t sz[] = {3 * n, n, 9 * n, 3 * n, 3 * n, 3 * n, 9 * n, 9 * n,
                         9 * n, 6 * n, (int64_t)c->nstatev * n, n, n, n, 9 * n, 10 * n, 4 * n};
   double **bufs[] = {&c->x,  &c->vol, &c->kinv, &c->u,    &c->v,   &c->a,
@@ -999,11 +1026,10 @@ int vpd_initialize_material(vpd_context *c) {
   if (!c->haveShape)
     return fail(c, VPD_ERR_STATE, "PD correspondence state is not initialized.");
   // F₀ at the current displacement into fN (VumatPdCorrespondenceMaterial.cpp:54).
-  const int saved = c->damageMode;
-  c->damageMode = VPD_DAMAGE_NONE;
-  int s = launch_defgrad(c);
-  c->damageMode = saved;
-  if (s)
+  // Every particle, whatever step a batched specimen has reached.
+  if (int s = timed(c, K_DEFGRAD, [&] {
+        launch_defgrad_with(c, false, c->ldsDefgrad, c->roundsDefgrad, Batch{});
+      }))
     return s;
   HIPTRY(c, hipMemcpyAsync(c->fN, c->fnew, sizeof(double) * 9 * c->n, hipMemcpyDeviceToDevice,
                            c->stream));
@@ -1174,6 +1200,8 @@ int vpd_download(vpd_context *c, int field, void *host) {
 int vpd_predict(vpd_context *c, double dt, const double ramp[3]) {
   if (int s = require_block(c))
     return s;
+  if (int s = require_single(c))
+    return s;
   if (!c->haveDynamics)
     return fail(c, VPD_ERR_STATE, "Invalid explicit dynamics predictor input.");
   if (!(dt > 0.0) || !ramp)
@@ -1185,17 +1213,21 @@ int vpd_predict(vpd_context *c, double dt, const double ramp[3]) {
 int vpd_apply_constraints(vpd_context *c, const double ramp[3]) {
   if (int s = require_block(c))
     return s;
+  if (int s = require_single(c))
+    return s;
   if (!ramp)
     return fail(c, VPD_ERR_ARGUMENT, "Null ramp.");
   return timed(c, K_CONSTRAIN, [&] {
     k_constrain<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->u, c->n),
                                                       planes(c->v, c->n), planes(c->a, c->n),
-                                                      c->codes, ramp_of(ramp));
+                                                      c->codes, ramp_of(ramp), c->batch);
   });
 }
 
 int vpd_compute_deformation_gradient(vpd_context *c) {
   if (int s = require_block(c))
+    return s;
+  if (int s = require_single(c))
     return s;
   if (!c->haveShape)
     return fail(c, VPD_ERR_STATE, "PD correspondence state is not initialized.");
@@ -1209,6 +1241,8 @@ int vpd_update_material(vpd_context *c, double dt, double stepTime, double total
   (void)stepTime;
   (void)totalTime; // passed through to the VUMAT, which the Johnson-Cook routine ignores
   if (int s = require_block(c))
+    return s;
+  if (int s = require_single(c))
     return s;
   if (!(dt > 0.0))
     return fail(c, VPD_ERR_ARGUMENT, "VUMAT-PD material update requires dt > 0.");
@@ -1228,6 +1262,8 @@ int vpd_update_material(vpd_context *c, double dt, double stepTime, double total
 int vpd_compute_force_and_correct(vpd_context *c, double dt, const double ramp[3]) {
   if (int s = require_ready(c))
     return s;
+  if (int s = require_single(c))
+    return s;
   if (!(dt > 0.0) || !ramp)
     return fail(c, VPD_ERR_ARGUMENT, "Invalid explicit dynamics corrector input.");
   if (int s = ensure_outputs(c))
@@ -1240,6 +1276,8 @@ int vpd_compute_force_and_correct(vpd_context *c, double dt, const double ramp[3
 
 int vpd_step(vpd_context *c, const vpd_step_args *g) {
   if (int s = require_ready(c))
+    return s;
+  if (int s = require_single(c))
     return s;
   if (!g || !(g->dt > 0.0))
     return fail(c, VPD_ERR_ARGUMENT, "Invalid explicit dynamics predictor input.");
@@ -1269,6 +1307,99 @@ int vpd_step(vpd_context *c, const vpd_step_args *g) {
   c->haveFnew = true;
   c->haveFields = g->want_fields != 0;
   c->haveForce = g->want_fields != 0;
+  return VPD_OK;
+}
+
+int vpd_set_specimens(vpd_context *c, int count, const int64_t *first, const double *dt,
+                      const double *velocity, const int32_t *lastStep) {
+  if (int s = require_ready(c))
+    return s;
+  if (count == 0) { // back to one specimen
+    c->batch = Batch{};
+    c->specCount = 0;
+    c->predicted = false;
+    return VPD_OK;
+  }
+  if (c->diagOn)
+    return fail(c, VPD_ERR_UNSUPPORTED,
+                "Run diagnostics cover a block that is one specimen, not batched specimens.");
+  if (count < 0 || count > 65536 || !first || !dt || !velocity || !lastStep)
+    return fail(c, VPD_ERR_ARGUMENT, "Invalid specimen table.");
+  if (first[0] != 0 || first[count] != c->n)
+    return fail(c, VPD_ERR_ARGUMENT, "Specimen ranges must cover the particle block.");
+  for (int s = 0; s < count; ++s) {
+    if (first[s + 1] <= first[s])
+      return fail(c, VPD_ERR_ARGUMENT, "Specimen ranges must be ascending and non-empty.");
+    if (!(dt[s] > 0.0) || !std::isfinite(dt[s]) || !std::isfinite(velocity[s]))
+      return fail(c, VPD_ERR_ARGUMENT, "Specimen time steps must be positive and finite.");
+    if (lastStep[s] < 1)
+      return fail(c, VPD_ERR_ARGUMENT, "Every specimen takes at least one step.");
+  }
+  // A bond between two specimens would couple them: refuse it.
+  std::vector<int> col(std::max<int64_t>(1, c->nnz));
+  if (c->nnz)
+    HIPTRY(c, hipMemcpy(col.data(), c->col, sizeof(int) * c->nnz, hipMemcpyDeviceToHost));
+  std::vector<uint16_t> id(c->n);
+  for (int s = 0; s < count; ++s)
+    for (int64_t i = first[s]; i < first[s + 1]; ++i)
+      id[i] = (uint16_t)s;
+  for (int64_t i = 0; i < c->n; ++i)
+    for (int p = c->hostRowptr[i]; p < c->hostRowptr[i + 1]; ++p)
+      if (id[col[p]] != id[i])
+        return fail(c, VPD_ERR_ARGUMENT,
+                    "Particles " + std::to_string(i) + " and " + std::to_string(col[p]) +
+                        " of different specimens are bonded; keep specimens more than a "
+                        "horizon apart.");
+  if (dalloc(c, c->specId, c->n) || dalloc(c, c->specDt, count) ||
+      dalloc(c, c->specVel, count) || dalloc(c, c->specLast, count))
+    return VPD_ERR_DEVICE;
+  HIPTRY(c, hipMemcpy(c->specId, id.data(), sizeof(uint16_t) * c->n, hipMemcpyHostToDevice));
+  HIPTRY(c, hipMemcpy(c->specDt, dt, sizeof(double) * count, hipMemcpyHostToDevice));
+  HIPTRY(c, hipMemcpy(c->specVel, velocity, sizeof(double) * count, hipMemcpyHostToDevice));
+  HIPTRY(c, hipMemcpy(c->specLast, lastStep, sizeof(int32_t) * count, hipMemcpyHostToDevice));
+  c->specCount = count;
+  c->specFirst.assign(first, first + count + 1);
+  c->batch = Batch{c->specId, c->specDt, c->specVel, c->specLast, 0};
+  c->predicted = false;
+  // Each specimen's loaded band at t = 0 (the single-specimen host's initial constraints).
+  if (int s = timed(c, K_CONSTRAIN, [&] {
+        k_constrain<<<grid_of(c->n), 256, 0, c->stream>>>(
+            c->n, planes(c->u, c->n), planes(c->v, c->n), planes(c->a, c->n), c->codes,
+            Ramp{0, 0, 0}, c->batch);
+      }))
+    return s;
+  return check_device(c);
+}
+
+int vpd_step_specimens(vpd_context *c, int32_t step, int fuseNext, int wantFields) {
+  if (int s = require_ready(c))
+    return s;
+  if (!c->batch.id)
+    return fail(c, VPD_ERR_STATE, "No specimen table; call vpd_set_specimens first.");
+  if (step < 1)
+    return fail(c, VPD_ERR_ARGUMENT, "Specimen steps count from 1.");
+  if (c->predicted && step != c->batch.step + 1)
+    return fail(c, VPD_ERR_STATE, "The fused prediction was made for step " +
+                                      std::to_string(c->batch.step + 1) + ".");
+  if (wantFields)
+    if (int s = ensure_outputs(c))
+      return s;
+  c->batch.step = step;
+  // Scalar arguments are unused: every particle takes its specimen's dt and velocity band.
+  const double none[3] = {0.0, 0.0, 0.0};
+  if (!c->predicted)
+    if (int s = launch_predict(c, 1.0, none))
+      return s;
+  if (int s = launch_defgrad(c))
+    return s;
+  if (int s = launch_material(c, 1.0, wantFields != 0))
+    return s;
+  if (int s = launch_force(c, 1.0, none, fuseNext != 0, none, wantFields != 0))
+    return s;
+  c->predicted = fuseNext != 0;
+  c->haveFnew = true;
+  c->haveFields = wantFields != 0;
+  c->haveForce = wantFields != 0;
   return VPD_OK;
 }
 
@@ -1454,6 +1585,9 @@ int vpd_enable_diagnostics(vpd_context *c, const vpd_diag_config *cfg) {
     c->diagCount = 0;
     return VPD_OK;
   }
+  if (c->batch.id)
+    return fail(c, VPD_ERR_UNSUPPORTED,
+                "Run diagnostics cover a block that is one specimen, not batched specimens.");
   const int ax = cfg->load_axis, lat = cfg->lateral_axis;
   if (ax < 0 || ax > 2 || lat < 0 || lat > 2)
     return fail(c, VPD_ERR_ARGUMENT, "Diagnostics axes must lie in [0, 2].");
@@ -1594,6 +1728,7 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   g.vol = c->vol;
   g.bonds = sell_of(c);
   g.broken = c->broken;
+  g.batch = c->batch;
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   float best[2] = {1e30f, 1e30f};
   int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1};
@@ -1607,7 +1742,7 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
         for (int r = 0; r <= reps; ++r) {
           HIPTRY(c, hipEventRecord(a, c->stream));
           if (pass == 0)
-            launch_defgrad_with(c, false, lds, rounds);
+            launch_defgrad_with(c, false, lds, rounds, c->batch);
           else
             launch_force_with(c, g, dmg, false, false, lds);
           HIPTRY(c, hipGetLastError());

@@ -112,6 +112,44 @@ __device__ __forceinline__ void constrain(uint8_t code, const Ramp &r, double u[
 }
 
 
+// Specimens batched in one particle block (vpd_set_specimens): particle i belongs to specimen
+// id[i], which has its own time step dt[s], its own end-face velocity v[s] (the velocity band
+// u = v·t, v, a = 0 at t = step·dt[s], the expressions the single-specimen host uses) and
+// stops after lastStep[s] steps. With `id` null the block is one specimen and the kernels
+// take their scalar dt and ramps.
+struct Batch {
+  const uint16_t *id;
+  const double *dt, *velocity;
+  const int *lastStep;
+  int step; // the step being taken (1-based; 0 for the initial constraints)
+};
+
+struct PointStep {
+  double dt;
+  Ramp now, next;
+  bool active, last; // last: the specimen's final step, which is not followed by a prediction
+};
+
+__device__ __forceinline__ bool point_active(const Batch &b, int64_t i) {
+  return !b.id || b.step <= b.lastStep[b.id[i]];
+}
+
+__device__ __forceinline__ PointStep point_step(const Batch &b, int64_t i, double dt,
+                                                const Ramp &now, const Ramp &next) {
+  PointStep p{dt, now, next, true, false};
+  if (b.id) {
+    const int s = b.id[i];
+    p.dt = b.dt[s];
+    p.active = b.step <= b.lastStep[s];
+    p.last = b.step >= b.lastStep[s];
+    const double v = b.velocity[s];
+    const double t = (double)b.step * p.dt, t1 = (double)(b.step + 1) * p.dt;
+    p.now = Ramp{v * t, v, 0.0};
+    p.next = Ramp{v * t1, v, 0.0};
+  }
+  return p;
+}
+
 // Critical-stretch test of PdSimpleDamage.cpp:72-77 with a cheap exact filter.
 // The reference decides stretch = (‖y‖ − ‖ξ‖)/‖ξ‖ > s_c (or < −f), with two square roots and a
 // division per bond. With a = ‖ξ‖², b = ‖y‖² (the same sums the reference takes the roots
@@ -162,11 +200,16 @@ __device__ __forceinline__ bool stretch_breaks(const StretchTest &t, const doubl
 
 // ExplicitTimeIntegrator.cpp:23-34 then the constraints at t (BoundaryConditions.cpp:81).
 __global__ void __launch_bounds__(256) k_predict(int64_t n, Planes u, Planes v, Planes a,
-                                                 const uint8_t *__restrict__ codes, double dt,
-                                                 Ramp r) {
+                                                 const uint8_t *__restrict__ codes, double dtIn,
+                                                 Ramp rIn, Batch batch) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
+  const PointStep ps = point_step(batch, i, dtIn, rIn, rIn);
+  if (!ps.active)
+    return;
+  const double dt = ps.dt;
+  const Ramp &r = ps.now;
   double uu[3], vv[3], aa[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -184,13 +227,18 @@ __global__ void __launch_bounds__(256) k_predict(int64_t n, Planes u, Planes v, 
 }
 
 __global__ void __launch_bounds__(256) k_constrain(int64_t n, Planes u, Planes v, Planes a,
-                                                   const uint8_t *__restrict__ codes, Ramp r) {
+                                                   const uint8_t *__restrict__ codes, Ramp rIn,
+                                                   Batch batch) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
   const uint8_t c = codes[i];
   if (c == 0)
     return;
+  const PointStep ps = point_step(batch, i, 0.0, rIn, rIn);
+  if (!ps.active)
+    return;
+  const Ramp &r = ps.now;
   double uu[3], vv[3], aa[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -308,7 +356,8 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, 
 template <bool DAMAGE>
 __global__ void __launch_bounds__(256)
     k_defgrad(Rounds rd, Planes x, PairPlanes xv, Planes u, Sell bonds, Planes kinv, Planes fout,
-              uint32_t *broken, double initiation, double failure, int useCompression) {
+              uint32_t *broken, double initiation, double failure, int useCompression,
+              Batch batch) {
   const int64_t blk = xcd_block(rd.blocksPerRound);
   if (blk < 0)
     return;
@@ -317,6 +366,8 @@ __global__ void __launch_bounds__(256)
     const int64_t i = rd.particle(r, blk);
     if (i >= rd.n)
       break;
+    if (!point_active(batch, i))
+      continue; // a finished specimen keeps its state
     defgrad_one<DAMAGE>(i, x, xv, u, bonds, kinv, fout, broken, test);
   }
 }
@@ -356,6 +407,7 @@ struct MaterialArgs {
   double dt, rtol;
   int *err;
   DiagArgs diag;
+  Batch batch;
 };
 
 // The stages after the return map, shared by the fused and the split material kernels:
@@ -419,6 +471,11 @@ __device__ __forceinline__ void material_finish(const MaterialArgs &g, int64_t i
 // One material point of k_material (see there); DIAG adds the diagnostics terms to dv.
 template <int MODE, bool FT_ONLY, bool DIAG>
 __device__ __forceinline__ void material_one(const MaterialArgs &g, int64_t i, double *dv) {
+  const Ramp none{0.0, 0.0, 0.0};
+  const PointStep ps = point_step(g.batch, i, g.dt, none, none);
+  if (!ps.active)
+    return;
+  const double dt = ps.dt;
   T3 f1, f0, v0, r0;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
@@ -428,7 +485,7 @@ __device__ __forceinline__ void material_one(const MaterialArgs &g, int64_t i, d
     r0.m[c] = g.rN(c, i);
   }
   Corot k;
-  if (!corotational(f0, f1, v0, r0, g.dt, k)) {
+  if (!corotational(f0, f1, v0, r0, dt, k)) {
     raise_error(g.err, ERR_SINGULAR_COROT);
     return;
   }
@@ -436,7 +493,7 @@ __device__ __forceinline__ void material_one(const MaterialArgs &g, int64_t i, d
   to_voigt(k.Dbar, inc);
 #pragma unroll
   for (int c = 0; c < 6; ++c)
-    inc[c] *= g.dt;
+    inc[c] *= dt;
   if (g.incOut.p) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
@@ -467,7 +524,7 @@ __device__ __forceinline__ void material_one(const MaterialArgs &g, int64_t i, d
   }
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
-  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
+  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
   material_finish<DIAG>(g, i, f1, k.R1, sNew, st, eint, einel, stats, sOld, dE, dv);
 }
 
@@ -502,6 +559,10 @@ __global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
 template <int MODE, bool DIAG>
 __device__ __forceinline__ void material_return_one(const MaterialArgs &g, int64_t i,
                                                     double *dv) {
+  const Ramp none{0.0, 0.0, 0.0};
+  const PointStep ps = point_step(g.batch, i, g.dt, none, none);
+  if (!ps.active)
+    return;
   double inc[6], sOld[6], sNew[6], st[6], dE[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -512,7 +573,7 @@ __device__ __forceinline__ void material_return_one(const MaterialArgs &g, int64
   }
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
-  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[i] : 0.0, g.rtol,
+  jc_point<MODE>(g.jc, ps.dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[i] : 0.0, g.rtol,
                  &stats);
   T3 f1, r1;
 #pragma unroll
@@ -563,6 +624,11 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
   const int64_t i = blk * blockDim.x + threadIdx.x;
   if (i >= g.n)
     return;
+  const Ramp none{0.0, 0.0, 0.0};
+  const PointStep ps = point_step(g.batch, i, g.dt, none, none);
+  if (!ps.active)
+    return;
+  const double dt = ps.dt;
   T3 f1, f0, v0, r0;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
@@ -572,7 +638,7 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
     r0.m[c] = g.rN(c, i);
   }
   Corot k;
-  if (!corotational(f0, f1, v0, r0, g.dt, k)) {
+  if (!corotational(f0, f1, v0, r0, dt, k)) {
     raise_error(g.err, ERR_SINGULAR_COROT);
     return;
   }
@@ -580,7 +646,7 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
   to_voigt(k.Dbar, inc);
 #pragma unroll
   for (int c = 0; c < 6; ++c)
-    inc[c] *= g.dt;
+    inc[c] *= dt;
   if (g.incOut.p) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
@@ -617,7 +683,7 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
   double sNew[6];
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
-  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
+  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
   material_finish<false>(g, i, f1, k.R1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
 }
 
@@ -637,7 +703,9 @@ __global__ void __launch_bounds__(256) k_material_return_list(MaterialArgs g, Co
   }
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
-  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[slot] : 0.0,
+  const Ramp none{0.0, 0.0, 0.0};
+  const double dt = point_step(g.batch, i, g.dt, none, none).dt;
+  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[slot] : 0.0,
                  g.rtol, &stats);
   T3 f1, r1;
 #pragma unroll
@@ -666,6 +734,7 @@ struct ForceArgs {
   double dt, density;
   Ramp rampNow, rampNext;
   DiagArgs diag;
+  Batch batch;
 };
 
 // A of particle j from its 5 component pairs.
@@ -705,6 +774,8 @@ __device__ __forceinline__ void force_bond_term(const T3 &ai, const T3 &aj, cons
 // exactly zero, so it is skipped. The sum runs in CSR order.
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool DIAG>
 __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double *dv) {
+  if (!point_active(g.batch, i))
+    return;
   const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
   const T3 ai = load_amap(g.amap, i);
   double acc[3] = {0.0, 0.0, 0.0};
@@ -737,16 +808,17 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
   }
   if (!INTEGRATE)
     return;
+  const PointStep ps = point_step(g.batch, i, g.dt, g.rampNow, g.rampNext);
   double uu[3], vv[3], aa[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double next = acc[d] / g.density;
-    vv[d] = g.v(d, i) + 0.5 * g.dt * (g.a(d, i) + next);
+    vv[d] = g.v(d, i) + 0.5 * ps.dt * (g.a(d, i) + next);
     aa[d] = next;
     uu[d] = g.u(d, i);
   }
   const uint8_t code = g.codes[i];
-  constrain(code, g.rampNow, uu, vv, aa);
+  constrain(code, ps.now, uu, vv, aa);
   if (DIAG) { // the force-side terms of BoundaryValueCaseRunner.cpp:244-285, in its expressions
     const double vi = g.vol[i];
     const int ax = g.diag.axis;
@@ -754,7 +826,7 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
     dv[D_NET_Y - D_NET_X] = acc[1] * vi;
     dv[D_NET_Z - D_NET_X] = acc[2] * vi;
     dv[D_REACTION - D_NET_X] =
-        ((code >> (2 * ax)) & 3) == 2 /* load band */ ? (g.density * g.rampNow.a - acc[ax]) * vi : 0.0;
+        ((code >> (2 * ax)) & 3) == 2 /* load band */ ? (g.density * ps.now.a - acc[ax]) * vi : 0.0;
     dv[D_KINETIC - D_NET_X] =
         0.5 * g.density * vi * (vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
     const uint8_t lat = g.diag.latSet ? g.diag.latSet[i] : 0;
@@ -765,11 +837,11 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
             ? 0.0
             : 1.0;
   }
-  if (FUSE_NEXT) {
+  if (FUSE_NEXT && !ps.last) {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-      uu[d] = uu[d] + (g.dt * vv[d] + 0.5 * g.dt * g.dt * aa[d]);
-    constrain(code, g.rampNext, uu, vv, aa);
+      uu[d] = uu[d] + (ps.dt * vv[d] + 0.5 * ps.dt * ps.dt * aa[d]);
+    constrain(code, ps.next, uu, vv, aa);
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {

@@ -171,9 +171,17 @@ struct MlpWeights {
 };
 
 // FP32 → bf16 (round to nearest even) with the hardware's packed conversion.
-typedef __bf16 bf16v4 __attribute__((ext_vector_type(4)));
+// Converted in pairs (one v_cvt_pk_bf16_f32 each): a 4-wide conversion is lowered to
+// per-element conversions and permutes.
+typedef __bf16 bf16v2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s16x2 bf16_pair(float a, float b) {
+  return __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{a, b}, bf16v2));
+}
 __device__ __forceinline__ bf16x4 bf16_of(f32x4 v) {
-  return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bf16v4));
+  const s16x2 lo = bf16_pair(v[0], v[1]), hi = bf16_pair(v[2], v[3]);
+  return bf16x4{lo[0], lo[1], hi[0], hi[1]};
 }
 
 __device__ __forceinline__ void mlp_load_weights(const float *__restrict__ theta, int lane,
@@ -210,11 +218,15 @@ __device__ __forceinline__ void mlp_load_weights(const float *__restrict__ theta
   w.b4 = theta[MLP_OFF_B4];
 }
 
+// ReLU after the conversion, on the packed bf16 pairs as signed 16-bit integers: a bf16
+// with the sign bit set is a negative int16 and becomes 0, a positive one is kept (the bit
+// patterns of non-negative bf16 values order like their values). Same result as rounding
+// max(v, 0), at a quarter of the instructions of a per-element IEEE max.
 __device__ __forceinline__ bf16x4 relu_bf16(f32x4 v) {
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    v[i] = fmaxf(v[i], 0.f);
-  return bf16_of(v);
+  const s16x2 z = {0, 0};
+  const s16x2 lo = __builtin_elementwise_max(bf16_pair(v[0], v[1]), z);
+  const s16x2 hi = __builtin_elementwise_max(bf16_pair(v[2], v[3]), z);
+  return bf16x4{lo[0], lo[1], hi[0], hi[1]};
 }
 
 // The forward pass for one tile of 16 points. x: the input tile in the B layout (lanes

@@ -616,7 +616,8 @@ RunSummary run_material(const CaseConfig &cfg, const CaseMaterial &mat, std::ost
     // The next predictor is fused into this step unless the state after this step is read.
     a.fuse_next = snapshot ? 0 : 1;
     if (a.fuse_next)
-      vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, time + dt, a.ramp_next);
+      vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, static_cast<double>(step + 1) * dt,
+                      a.ramp_next); // the time the next step recomputes, bit for bit
     a.want_fields = snapshot ? 1 : 0;
     if (pending == dc.capacity) {
       drain();

@@ -104,6 +104,9 @@ def lib():
         L.vpd_update_material.argtypes = [P, C.c_double, C.c_double, C.c_double]
         L.vpd_compute_force_and_correct.argtypes = [P, C.c_double, _D]
         L.vpd_step.argtypes = [P, C.POINTER(StepArgs)]
+        L.vpd_set_specimens.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]
+        L.vpd_step_specimens.argtypes = [P, C.c_int32, C.c_int, C.c_int]
         L.vpd_set_timing.argtypes = [P, C.c_int]
         L.vpd_set_tuning.argtypes = [P, C.c_int, C.c_int]
         L.vpd_get_tuning.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
@@ -345,6 +348,22 @@ class Context:
 
     def synchronize(self):
         self._check(lib().vpd_synchronize(self.h))
+
+    def set_specimens(self, first, dt, velocity, last_step):
+        """Batched specimens: particles [first[s], first[s+1]) with their own dt, end
+        velocity and step count (vpd_set_specimens). Empty arrays return to one specimen."""
+        first = np.ascontiguousarray(first, np.int64)
+        dt, velocity = _f64(dt), _f64(velocity)
+        last_step = np.ascontiguousarray(last_step, np.int32)
+        count = len(dt)
+        self._specimen_tables = (first, dt, velocity, last_step)  # alive during the call
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(lib().vpd_set_specimens(self.h, count, ptr(first), ptr(dt), ptr(velocity),
+                                            ptr(last_step)))
+
+    def step_specimens(self, step, fuse_next=True, want_fields=False):
+        self._check(lib().vpd_step_specimens(self.h, int(step), int(fuse_next),
+                                             int(want_fields)))
 
     def set_return_map(self, mode, rtol=1e-12, warm_start=None):
         self._check(lib().vpd_set_return_map(self.h, mode, rtol))
@@ -640,9 +659,124 @@ class Bar:
             self.step_index += 1
             t = self.step_index * self.dt
             last = s == k - 1
-            nxt = None if (last or not fuse) else self.ramp(t + self.dt)
+            nxt = None if (last or not fuse) else self.ramp((self.step_index + 1) * self.dt)
             self.ctx.step(self.dt, t, (self.step_index - 1) * self.dt, self.ramp(t), nxt,
                           want_fields and last)
 
     def get(self, name):
         return self.ctx.download(name)
+
+
+# ------------------------------------------------------------------ batched specimens
+class Specimen:
+    """One velocity-driven bar of a batch: lattice, horizon δ = horizon_factor·Δx, time step,
+    end velocity (u = v·t on the loaded band) and step count."""
+
+    def __init__(self, nx, ny, nz, spacing, horizon_factor, dt, velocity, steps,
+                 component=0, region_layers=3):
+        self.nx, self.ny, self.nz, self.spacing = nx, ny, nz, spacing
+        self.horizon = horizon_factor * spacing
+        self.dt, self.velocity, self.steps = dt, velocity, int(steps)
+        self.component, self.region_layers = component, region_layers
+
+    @property
+    def n(self):
+        return self.nx * self.ny * self.nz
+
+    def bar_kwargs(self):
+        return dict(nx=self.nx, ny=self.ny, nz=self.nz, spacing=self.spacing,
+                    horizon=self.horizon, dt=self.dt, velocity=self.velocity,
+                    component=self.component, region_layers=self.region_layers)
+
+
+class SpecimenBatch:
+    """Many specimens in one particle block, stepped together (vpd_set_specimens /
+    vpd_step_specimens): the sensitivity sweep that benchmarks/tensile3d/run_cases.py:95-96
+    runs as one process per case. Each specimen keeps its own coordinates, neighbour rows
+    (built on the GPU with its own horizon; the block's CSR is their concatenation with
+    column offsets), constraint codes, dt, velocity and step count; its results equal a
+    `Bar` of the same specimen bit for bit. Material, density and damage are shared."""
+
+    def __init__(self, specimens, props, density, nstatev=6, tref=293.0, damage=None,
+                 material_density=7850.0, return_map=RETURN_MAP_BISECTION, rtol=1e-12,
+                 warm_start=None, neighbor_cache=None):
+        self.specimens = list(specimens)
+        xs, vols, lens, cols, codes = [], [], [], [], []
+        cache = {} if neighbor_cache is None else neighbor_cache
+        offset = 0
+        for sp in self.specimens:
+            x, vol = make_patch(sp.nx, sp.ny, sp.nz, sp.spacing)
+            key = (sp.nx, sp.ny, sp.nz, sp.spacing, sp.horizon)
+            if key not in cache:  # the specimen's own neighbour rows, built on the GPU
+                tmp = Context(sp.horizon, density, props, nstatev, None, tref, sp.spacing,
+                              material_density, damage)
+                tmp.set_particles(x, vol)
+                tmp.build_neighbors()
+                cache[key] = tmp.get_neighbors()
+                tmp.close()
+            rowptr, col = cache[key]
+            xs.append(x)
+            vols.append(vol)
+            lens.append(np.diff(rowptr))
+            cols.append(col.astype(np.int64) + offset)
+            codes.append(regional_constraint_codes(x, sp.component, sp.region_layers,
+                                                   sp.spacing)[0])
+            offset += sp.n
+        self.first = np.cumsum([0] + [sp.n for sp in self.specimens]).astype(np.int64)
+        rowptr = np.zeros(offset + 1, np.int64)
+        rowptr[1:] = np.cumsum(np.concatenate(lens))
+        if rowptr[-1] > np.iinfo(np.int32).max:
+            raise VpdError("The batch holds more bonds than the int32 CSR can index.")
+        self.x = np.concatenate(xs)
+        self.vol = np.concatenate(vols)
+        self.n = offset
+        self.rowptr = rowptr.astype(np.int32)
+        self.codes = np.concatenate(codes)
+        self.ctx = Context(max(sp.horizon for sp in self.specimens), density, props, nstatev,
+                           None, tref, self.specimens[0].spacing, material_density, damage)
+        self.ctx.set_return_map(return_map, rtol, warm_start)
+        self.ctx.set_particles(self.x, self.vol)
+        self.ctx.set_neighbors(self.rowptr, np.concatenate(cols).astype(np.int32))
+        self.ctx.set_constraints(self.codes)
+        self.ctx.initialize_shape()
+        self.ctx.initialize_material()
+        self.ctx.initialize_dynamics()
+        self._table()
+        self.step_index = 0
+
+    def _table(self):
+        sp = self.specimens
+        self.ctx.set_specimens(self.first, [s.dt for s in sp], [s.velocity for s in sp],
+                               [s.steps for s in sp])
+
+    @property
+    def max_steps(self):
+        return max(s.steps for s in self.specimens)
+
+    def reset(self):
+        """Back to the state before step 1 for every specimen."""
+        self.ctx.upload("u", np.zeros((self.n, 3)))
+        self.ctx.initialize_material()
+        self.ctx.initialize_dynamics()
+        self._table()
+        self.step_index = 0
+
+    def step(self, k=None, fuse=True, want_fields=False):
+        """k batch steps (default: until every specimen has finished). Specimens whose count
+        is reached stop and keep their end-of-step state; the batch's last step is left
+        unfused."""
+        k = self.max_steps - self.step_index if k is None else k
+        for s in range(k):
+            self.step_index += 1
+            last = s == k - 1
+            self.ctx.step_specimens(self.step_index, fuse and not last, want_fields and last)
+
+    def get(self, name, specimen=None):
+        out = self.ctx.download(name)
+        if specimen is None:
+            return out
+        a, b = self.first[specimen], self.first[specimen + 1]
+        if name == "damage":
+            return out[self.rowptr[a]:self.rowptr[b]]
+        return out[a:b]
+
