@@ -174,6 +174,56 @@ def return_map_study(points, seed=4321):
     return out
 
 
+def sweep_measure(steps, warmup, end_steps=2000):
+    """The 48-bar sensitivity sweep (Δx × δ/Δx × dt/CFL, 4k-500k particles per bar): `steps`
+    batch steps of all specimens in one particle block (vpd.SpecimenBatch), against the same
+    specimens stepped one after another as separate single-bar contexts (the reference's
+    process-per-case sweep, on the GPU). Both autotuned; setup untimed."""
+    import vpd
+    import sweep
+
+    sps = sweep.specimens(end_steps)
+    if steps > min(sp.steps for sp in sps):
+        raise SystemExit("--steps exceeds the shortest specimen's step count")
+    cache = {}
+    print("[bench] building the %d-specimen batch" % len(sps), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batch = vpd.SpecimenBatch(sps, JC_PROPS, RHO, damage=(0.02, 0.0), neighbor_cache=cache)
+    setup = time.perf_counter() - t0
+    tuned = batch.ctx.autotune(3)
+    batch.reset()
+    batch.step(warmup)
+    batch.ctx.synchronize()
+    batch.reset()
+    batch.ctx.set_timing(True)
+    t0 = time.perf_counter()
+    batch.step(steps)
+    batch.ctx.synchronize()
+    t_batch = time.perf_counter() - t0
+    kt = batch.ctx.collect_timing()
+    n_total, nnz_total = batch.n, int(batch.rowptr[-1])
+    del batch
+    t_seq, per = 0.0, []
+    for sp in sps:
+        rowptr, col = cache[(sp.nx, sp.ny, sp.nz, sp.spacing, sp.horizon)]
+        bar = vpd.Bar(props=JC_PROPS, density=RHO, damage=(0.02, 0.0), rowptr=rowptr, col=col,
+                      **sp.bar_kwargs())
+        bar.ctx.autotune(3)
+        elapsed, _ = timed_run(bar, steps, warmup)
+        t_seq += elapsed
+        per.append(dict(label=sp.label, n=sp.n, ms_per_step=elapsed / steps * 1e3))
+        bar.ctx.close()
+    work = n_total * steps
+    return {"metric": "particle-steps/s (48-bar JC sweep, one block)", "specimens": len(sps),
+            "particles": n_total, "bonds": nnz_total, "steps": steps, "warmup": warmup,
+            "batched": {"value": work / t_batch, "ms_per_step": t_batch / steps * 1e3,
+                        "launch_tuning": tuned,
+                        "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()}},
+            "one_at_a_time": {"value": work / t_seq, "ms_per_step": t_seq / steps * 1e3,
+                              "per_specimen": per},
+            "speedup": t_seq / t_batch, "setup_s": setup}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,11 +234,16 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--return-map-points", type=int, default=1000000)
+    ap.add_argument("--sweep", action="store_true",
+                    help="measure the 48-bar batched sweep instead (JSON on stdout)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this path is not sharded across GPUs "
               "(see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.sweep:
+        print(json.dumps(sweep_measure(min(args.steps, 100), min(args.warmup, 5))))
+        return
 
     import numpy as np
     import vpd

@@ -718,15 +718,22 @@ class SpecimenBatch:
             xs.append(x)
             vols.append(vol)
             lens.append(np.diff(rowptr))
-            cols.append(col.astype(np.int64) + offset)
+            cols.append((col, offset))
             codes.append(regional_constraint_codes(x, sp.component, sp.region_layers,
                                                    sp.spacing)[0])
             offset += sp.n
+        if offset > np.iinfo(np.int32).max:
+            raise VpdError("The batch holds more particles than int32 indices can address.")
         self.first = np.cumsum([0] + [sp.n for sp in self.specimens]).astype(np.int64)
         rowptr = np.zeros(offset + 1, np.int64)
         rowptr[1:] = np.cumsum(np.concatenate(lens))
         if rowptr[-1] > np.iinfo(np.int32).max:
             raise VpdError("The batch holds more bonds than the int32 CSR can index.")
+        col = np.empty(max(int(rowptr[-1]), 1), np.int32)  # rows with column offsets
+        pos = 0
+        for c, off in cols:
+            np.add(c, np.int32(off), out=col[pos:pos + len(c)])
+            pos += len(c)
         self.x = np.concatenate(xs)
         self.vol = np.concatenate(vols)
         self.n = offset
@@ -736,7 +743,8 @@ class SpecimenBatch:
                            None, tref, self.specimens[0].spacing, material_density, damage)
         self.ctx.set_return_map(return_map, rtol, warm_start)
         self.ctx.set_particles(self.x, self.vol)
-        self.ctx.set_neighbors(self.rowptr, np.concatenate(cols).astype(np.int32))
+        self.ctx.set_neighbors(self.rowptr, col[:pos])
+        del col, cols
         self.ctx.set_constraints(self.codes)
         self.ctx.initialize_shape()
         self.ctx.initialize_material()
@@ -771,12 +779,19 @@ class SpecimenBatch:
             last = s == k - 1
             self.ctx.step_specimens(self.step_index, fuse and not last, want_fields and last)
 
-    def get(self, name, specimen=None):
-        out = self.ctx.download(name)
-        if specimen is None:
-            return out
-        a, b = self.first[specimen], self.first[specimen + 1]
+    def _part(self, name, out, s):
+        a, b = self.first[s], self.first[s + 1]
         if name == "damage":
             return out[self.rowptr[a]:self.rowptr[b]]
         return out[a:b]
+
+    def get(self, name, specimen=None):
+        """A field of the whole block, or of one specimen (bonds: its CSR range)."""
+        out = self.ctx.download(name)
+        return out if specimen is None else self._part(name, out, specimen)
+
+    def split(self, name):
+        """One download of a field, as a list of per-specimen views."""
+        out = self.ctx.download(name)
+        return [self._part(name, out, s) for s in range(len(self.specimens))]
 
