@@ -201,6 +201,11 @@ def sweep_measure(steps, warmup, end_steps=2000):
     batch.ctx.synchronize()
     t_batch = time.perf_counter() - t0
     kt = batch.ctx.collect_timing()
+    batch.ctx.set_timing(False)
+    batch.reset()  # the same steps as replays of one captured HIP graph of a step
+    t0 = time.perf_counter()
+    batch.run(steps)
+    t_graph = time.perf_counter() - t0
     n_total, nnz_total = batch.n, int(batch.rowptr[-1])
     del batch
     t_seq, per = 0.0, []
@@ -218,10 +223,44 @@ def sweep_measure(steps, warmup, end_steps=2000):
             "particles": n_total, "bonds": nnz_total, "steps": steps, "warmup": warmup,
             "batched": {"value": work / t_batch, "ms_per_step": t_batch / steps * 1e3,
                         "launch_tuning": tuned,
-                        "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()}},
+                        "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
+                        "graph_value": work / t_graph, "graph_ms_per_step": t_graph / steps * 1e3},
             "one_at_a_time": {"value": work / t_seq, "ms_per_step": t_seq / steps * 1e3,
                               "per_specimen": per},
             "speedup": t_seq / t_batch, "setup_s": setup}
+
+
+def small_bar_graph(steps=2000):
+    """The CPU-sized JC bar (30×8×8, Δx = 1 mm, δ = 3.015Δx, dt = 0.5·CFL, 1 mm/ms, damage
+    off) for `steps` steps: host-issued fused steps (vpd_step) against the same bar as a
+    one-specimen batch replaying one captured HIP graph per step. Launch-bound at this size."""
+    import vpd
+
+    dt = cfl_dt(1.0)
+    bar = vpd.Bar(30, 8, 8, 1.0, 3.015, JC_PROPS, dt, RHO, velocity=1.0)
+    bar.ctx.autotune(3)
+    bar.step(20)
+    bar.reset()
+    bar.ctx.synchronize()
+    t0 = time.perf_counter()
+    bar.step(steps)
+    bar.ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    u_eager = bar.get("u")
+    batch = vpd.SpecimenBatch([vpd.Specimen(30, 8, 8, 1.0, 3.015, dt, 1.0, steps)], JC_PROPS,
+                              RHO)
+    batch.ctx.set_tuning(**bar.ctx.get_tuning())
+    batch.run(20)
+    batch.reset()
+    t0 = time.perf_counter()
+    batch.run()
+    t_graph = time.perf_counter() - t0
+    if not (batch.get("u") == u_eager).all():
+        raise SystemExit("graph-replayed bar differs from the eager bar")
+    return {"workload": "jc_tensile_bar_30x8x8", "particles": bar.n, "steps": steps,
+            "eager_ms_per_step": elapsed / steps * 1e3,
+            "graph_ms_per_step": t_graph / steps * 1e3,
+            "eager_value": bar.n * steps / elapsed, "graph_value": bar.n * steps / t_graph}
 
 
 def main():
@@ -242,7 +281,9 @@ def main():
               "(see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.sweep:
-        print(json.dumps(sweep_measure(min(args.steps, 100), min(args.warmup, 5))))
+        out = sweep_measure(min(args.steps, 100), min(args.warmup, 5))
+        out["small_bar"] = small_bar_graph()
+        print(json.dumps(out))
         return
 
     import numpy as np

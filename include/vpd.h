@@ -227,6 +227,15 @@ int vpd_set_specimens(vpd_context *ctx, int count, const int64_t *first, const d
  * finished. With fuse_next the next step's predictor runs inside the force pass; a
  * specimen's final step is never followed by a prediction. */
 int vpd_step_specimens(vpd_context *ctx, int32_t step, int fuse_next, int want_fields);
+/* Steps first_step .. first_step+count−1 of every unfinished specimen as `count` replays of
+ * one captured HIP graph of a fused step (F + damage, material, force + correct + next
+ * prediction, then a device step counter moves on), so the host issues one launch per step
+ * instead of one per kernel. Results equal vpd_step_specimens with fuse_next, bit for bit.
+ * Specimens still running afterwards hold their prediction for the next step (read state
+ * after a vpd_step_specimens with fuse_next = 0, or once every specimen has finished).
+ * Returns when the steps are done. No counterpart in the reference (its step loop is serial
+ * host code, BoundaryValueCaseRunner.cpp:207-229). */
+int vpd_run_specimens(vpd_context *ctx, int32_t first_step, int32_t count);
 
 /* ---- return-map solver of the material stage -------------------------------------- */
 /* Selects the solver (enum vpd_return_map) and the fast modes' relative residual bound

@@ -154,3 +154,32 @@ def test_batch_errors(vpd):
     ref = vpd.Bar(props=JC_PROPS, density=RHO, **sp.bar_kwargs())
     ref.step(5)
     assert np.array_equal(u_batch, ref.get("u"))
+
+
+@pytest.mark.parametrize("return_map", [0, 2, 3], ids=["bisection", "newton", "learned"])
+def test_graph_replay_equals_eager(vpd, return_map):
+    """Every replay of the captured step against eager steps, with eager steps and host
+    synchronisations between the replays."""
+    banner("batched specimens: graph-replayed steps vs eager, return map %d" % return_map)
+    sps = specimens(vpd)[:4]
+    net = vpd.load_warm_start() if return_map == vpd.RETURN_MAP_LEARNED else None
+    kw = dict(damage=(0.003, 0.0), return_map=return_map, warm_start=net)
+    eager = vpd.SpecimenBatch(sps, JC_PROPS, RHO, **kw)
+    graph = vpd.SpecimenBatch(sps, JC_PROPS, RHO, **kw)
+    # Segments: replayed by the graph (True) or stepped eagerly (False), each ending with an
+    # unfused eager step on both sides so that the states can be compared.
+    for replay, k in ((True, 9), (False, 4), (True, 1), (True, 12), (False, 2), (True, 30)):
+        eager.step(k + 1)
+        if replay:
+            graph.run(k)
+            graph.step(1)
+        else:
+            graph.step(k + 1)
+        graph.ctx.synchronize()
+        for name in ("u", "v", "a", "stressN", "statev", "damage"):
+            assert np.array_equal(graph.get(name), eager.get(name)), (replay, k, name)
+    graph.run()   # the rest, to every specimen's last step
+    eager.step()
+    assert graph.step_index == eager.step_index == max(sp.steps for sp in sps)
+    for name in ("u", "v", "a", "stressN", "statev", "damage", "fN", "rN"):
+        assert np.array_equal(graph.get(name), eager.get(name)), name

@@ -126,6 +126,7 @@ struct vpd_context {
   uint16_t *specId = nullptr;
   double *specDt = nullptr, *specVel = nullptr;
   int *specLast = nullptr;
+  int *stepDev = nullptr; // the step counter read by graph-replayed steps
   int specCount = 0;
   std::vector<int64_t> specFirst;
 };
@@ -480,6 +481,14 @@ int launch_defgrad(vpd_context *c) {
   });
 }
 
+// Buffers of the compacted learned update (plastic-point list), allocated on first use.
+int ensure_list(vpd_context *c) {
+  if (!c->list && (dalloc(c, c->listCount, 1) || dalloc(c, c->list, c->n) ||
+                   dalloc(c, c->incList, 6 * c->n)))
+    return VPD_ERR_DEVICE;
+  return VPD_OK;
+}
+
 template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
   const int grid = grid8_of(c->n);
   switch (c->returnMap) {
@@ -497,9 +506,8 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
     if (!c->mlp)
       return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
     if (!DIAG) { // screen, then the network and the Newton solve on the plastic points only
-      if (!c->list && (dalloc(c, c->listCount, 1) || dalloc(c, c->list, c->n) ||
-                       dalloc(c, c->incList, 6 * c->n)))
-        return VPD_ERR_DEVICE;
+      if (int s = ensure_list(c))
+        return s;
       const CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n)};
       if (int s = timed(c, K_MATERIAL, [&] {
             k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
@@ -785,6 +793,7 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->specDt);
   dfree(c->specVel);
   dfree(c->specLast);
+  dfree(c->stepDev);
   dfree(c->derr);
   if (c->stream)
     (void)hipStreamDestroy(c->stream);
@@ -1351,7 +1360,8 @@ int vpd_set_specimens(vpd_context *c, int count, const int64_t *first, const dou
                         " of different specimens are bonded; keep specimens more than a "
                         "horizon apart.");
   if (dalloc(c, c->specId, c->n) || dalloc(c, c->specDt, count) ||
-      dalloc(c, c->specVel, count) || dalloc(c, c->specLast, count))
+      dalloc(c, c->specVel, count) || dalloc(c, c->specLast, count) ||
+      (!c->stepDev && dalloc(c, c->stepDev, 1)))
     return VPD_ERR_DEVICE;
   HIPTRY(c, hipMemcpy(c->specId, id.data(), sizeof(uint16_t) * c->n, hipMemcpyHostToDevice));
   HIPTRY(c, hipMemcpy(c->specDt, dt, sizeof(double) * count, hipMemcpyHostToDevice));
@@ -1401,6 +1411,70 @@ int vpd_step_specimens(vpd_context *c, int32_t step, int fuseNext, int wantField
   c->haveFields = wantFields != 0;
   c->haveForce = wantFields != 0;
   return VPD_OK;
+}
+
+int vpd_run_specimens(vpd_context *c, int32_t firstStep, int32_t count) {
+  if (int s = require_ready(c))
+    return s;
+  if (!c->batch.id)
+    return fail(c, VPD_ERR_STATE, "No specimen table; call vpd_set_specimens first.");
+  if (firstStep < 1 || count < 1)
+    return fail(c, VPD_ERR_ARGUMENT, "Specimen steps count from 1; count must be positive.");
+  if (c->predicted && firstStep != c->batch.step + 1)
+    return fail(c, VPD_ERR_STATE, "The fused prediction was made for step " +
+                                      std::to_string(c->batch.step + 1) + ".");
+  if (c->returnMap == RM_LEARNED) {
+    if (!c->mlp)
+      return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
+    if (int s = ensure_list(c)) // no allocation may happen while the step is captured
+      return s;
+  }
+  c->batch.step = firstStep;
+  const double none[3] = {0.0, 0.0, 0.0};
+  if (!c->predicted)
+    if (int s = launch_predict(c, 1.0, none))
+      return s;
+  k_set_step<<<1, 64, 0, c->stream>>>(c->stepDev, firstStep);
+  HIPTRY(c, hipGetLastError());
+  // One fused step, reading its step number from the device counter, captured once.
+  const bool timing = c->timing;
+  const Batch eager = c->batch;
+  c->timing = false;
+  c->batch.stepDev = c->stepDev;
+  hipGraph_t graph = nullptr;
+  HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int s = launch_defgrad(c);
+  if (!s)
+    s = launch_material(c, 1.0, false);
+  if (!s)
+    s = launch_force(c, 1.0, none, true, none, false);
+  if (!s)
+    k_advance_step<<<1, 64, 0, c->stream>>>(c->stepDev);
+  const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+  c->batch = eager;
+  c->timing = timing;
+  if (s) {
+    if (ec == hipSuccess && graph)
+      (void)hipGraphDestroy(graph);
+    return s;
+  }
+  HIPTRY(c, ec);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  HIPTRY(c, ei);
+  hipError_t el = hipSuccess;
+  for (int k = 0; k < count && el == hipSuccess; ++k)
+    el = hipGraphLaunch(exec, c->stream);
+  const hipError_t es = hipStreamSynchronize(c->stream);
+  (void)hipGraphExecDestroy(exec);
+  HIPTRY(c, el);
+  HIPTRY(c, es);
+  c->batch.step = firstStep + count - 1;
+  c->predicted = true; // every specimen past its last step, or predicted for the next one
+  c->haveFnew = true;
+  c->haveFields = c->haveForce = false;
+  return check_device(c);
 }
 
 int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *force) {

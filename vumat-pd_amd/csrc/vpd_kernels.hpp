@@ -122,6 +122,10 @@ struct Batch {
   const double *dt, *velocity;
   const int *lastStep;
   int step; // the step being taken (1-based; 0 for the initial constraints)
+  // When set, the step is read from this device counter instead, so that one captured HIP
+  // graph of a step serves every step (k_advance_step moves it on).
+  const int *stepDev;
+  __device__ __forceinline__ int current() const { return stepDev ? *stepDev : step; }
 };
 
 struct PointStep {
@@ -131,7 +135,7 @@ struct PointStep {
 };
 
 __device__ __forceinline__ bool point_active(const Batch &b, int64_t i) {
-  return !b.id || b.step <= b.lastStep[b.id[i]];
+  return !b.id || b.current() <= b.lastStep[b.id[i]];
 }
 
 __device__ __forceinline__ PointStep point_step(const Batch &b, int64_t i, double dt,
@@ -140,14 +144,25 @@ __device__ __forceinline__ PointStep point_step(const Batch &b, int64_t i, doubl
   if (b.id) {
     const int s = b.id[i];
     p.dt = b.dt[s];
-    p.active = b.step <= b.lastStep[s];
-    p.last = b.step >= b.lastStep[s];
+    const int step = b.current();
+    p.active = step <= b.lastStep[s];
+    p.last = step >= b.lastStep[s];
     const double v = b.velocity[s];
-    const double t = (double)b.step * p.dt, t1 = (double)(b.step + 1) * p.dt;
+    const double t = (double)step * p.dt, t1 = (double)(step + 1) * p.dt;
     p.now = Ramp{v * t, v, 0.0};
     p.next = Ramp{v * t1, v, 0.0};
   }
   return p;
+}
+
+__global__ void k_set_step(int *stepDev, int step) {
+  if (threadIdx.x == 0)
+    *stepDev = step;
+}
+
+__global__ void k_advance_step(int *stepDev) {
+  if (threadIdx.x == 0)
+    *stepDev += 1;
 }
 
 // Critical-stretch test of PdSimpleDamage.cpp:72-77 with a cheap exact filter.

@@ -107,6 +107,7 @@ def lib():
         L.vpd_set_specimens.argtypes = [P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p]
         L.vpd_step_specimens.argtypes = [P, C.c_int32, C.c_int, C.c_int]
+        L.vpd_run_specimens.argtypes = [P, C.c_int32, C.c_int32]
         L.vpd_set_timing.argtypes = [P, C.c_int]
         L.vpd_set_tuning.argtypes = [P, C.c_int, C.c_int]
         L.vpd_get_tuning.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
@@ -364,6 +365,10 @@ class Context:
     def step_specimens(self, step, fuse_next=True, want_fields=False):
         self._check(lib().vpd_step_specimens(self.h, int(step), int(fuse_next),
                                              int(want_fields)))
+
+    def run_specimens(self, first_step, count):
+        """`count` fused steps from `first_step` as replays of one captured HIP graph."""
+        self._check(lib().vpd_run_specimens(self.h, int(first_step), int(count)))
 
     def set_return_map(self, mode, rtol=1e-12, warm_start=None):
         self._check(lib().vpd_set_return_map(self.h, mode, rtol))
@@ -784,6 +789,15 @@ class SpecimenBatch:
         if name == "damage":
             return out[self.rowptr[a]:self.rowptr[b]]
         return out[a:b]
+
+    def run(self, k=None):
+        """k batch steps (default: until every specimen has finished) as replays of one
+        captured HIP graph of a fused step. Specimens still running afterwards hold their
+        prediction for the next step; `step` continues from here."""
+        k = self.max_steps - self.step_index if k is None else k
+        if k > 0:
+            self.ctx.run_specimens(self.step_index + 1, k)
+            self.step_index += k
 
     def get(self, name, specimen=None):
         """A field of the whole block, or of one specimen (bonds: its CSR range)."""
