@@ -250,8 +250,9 @@ class OracleBar:
 
     def __init__(self, nx, ny, nz, spacing, horizon, props, dt, density, amplitude, rise_time,
                  component=0, region_layers=3, plane_strain=False, nstatev=6, tref=293.0,
-                 initial_state=None, damage=None, velocity=None, rowptr=None, col=None):
-        self.x, self.vol = make_patch(nx, ny, nz, spacing)
+                 initial_state=None, damage=None, velocity=None, rowptr=None, col=None,
+                 points=None):
+        self.x, self.vol = make_patch(nx, ny, nz, spacing) if points is None else points
         self.n = self.x.shape[0]
         if rowptr is None:
             rowptr, col = build_neighbors(self.x, horizon)

@@ -47,6 +47,23 @@ def test_constraint_codes_match_oracle(orc, component, plane, layers):
     assert (anchor, pin) == (a, p)
 
 
+def test_voided_patch(orc):
+    """Seeded site removal keeps the survivors' lattice order and volumes; the boundary bands
+    of the voided set agree with the oracle's."""
+    import vpd
+    x, vol = vpd.voided_patch(30, 8, 8, 1.0, 0.05, 11)
+    xf, vf = vpd.make_patch(30, 8, 8, 1.0)
+    assert len(x) == 1920 - 96
+    keys = {tuple(p): k for k, p in enumerate(xf)}
+    idx = np.array([keys[tuple(p)] for p in x])
+    assert (np.diff(idx) > 0).all() and np.array_equal(vf[idx], vol)
+    assert np.array_equal(vpd.voided_patch(30, 8, 8, 1.0, 0.05, 11)[0], x)
+    assert not np.array_equal(vpd.voided_patch(30, 8, 8, 1.0, 0.05, 12)[0], x)
+    codes, anchor, pin = vpd.regional_constraint_codes(x, 0, 3, 1.0)
+    c3, a, p = orc.constraint_codes(x, 0, 3, 1.0)
+    assert np.array_equal(vpd.unpack_codes(codes), c3) and (anchor, pin) == (a, p)
+
+
 def test_ramp_matches_oracle(orc):
     import vpd
     for t in (0.0, 1e-5, 0.03, 0.0599, 0.06, 0.2):

@@ -1,0 +1,125 @@
+"""High-rate, impact-like tension of a voided Johnson-Cook bar on one GPU.
+
+400×70×70 lattice at Δx = 0.2 mm (1.96M sites) with 5% of the sites removed at random
+(seeded voids, vpd.voided_patch), δ = 3.015Δx, end-face velocity 20 mm/ms (20 m/s),
+dt = 0.5·CFL, JC with adiabatic heating (Taylor-Quinney β = 0.9, c_p = 500 J/(kg·K), the
+MPa → J/m³ factor 1e6 of the reference's configs/johnson_cook_heat_units.json as
+workToHeat), critical-stretch damage s_c = 0.02, the learned return map (network seed,
+safeguarded Newton finish to |r| ≤ 1e-12·q, bisection fallback), 10000 steps.
+
+Every `log_every` steps one unfused step keeps the fields, and the log records the plastic
+points of that step, their mean return-map iterations, how many needed a bisection fallback
+inside the Newton solve, peak ε̄ᵖ and temperature rise, broken bonds and the particles whose
+bonds are all broken (those `deletionEnabled` would flag, PdSimpleDamage.cpp:81-84).
+
+    python impact.py [--steps 10000] [--log-every 500] [--return-map learned|newton|bisection]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import vpd  # noqa: E402
+
+# The tensile-bar Johnson-Cook material with heating in consistent units: E, ν, A, B, C, n,
+# m, ε̇₀ (1/ms), T_ref, T_melt, ρ, β, c_p, workToHeat.
+JC_HEAT_PROPS = [200000.0, 0.3, 350.0, 275.0, 0.022, 0.36, 1.0, 1.0e-3, 293.0, 1793.0, 7850.0,
+                 0.9, 500.0, 1.0e6]
+RHO = 7.85e-3  # N·ms²/mm⁴
+MODES = {"bisection": vpd.RETURN_MAP_BISECTION, "newton": vpd.RETURN_MAP_NEWTON,
+         "learned": vpd.RETURN_MAP_LEARNED}
+
+
+def cfl_dt(h, fraction=0.5, props=JC_HEAT_PROPS, rho=RHO):
+    E, nu = props[0], props[1]
+    return fraction * h / math.sqrt(E * (1 - nu) / (rho * (1 + nu) * (1 - 2 * nu)))
+
+
+def build(nx=400, ny=70, nz=70, spacing=0.2, void_fraction=0.05, seed=20240607, velocity=20.0,
+          damage=(0.02, 0.0), return_map="learned", net=None, rise=0.0):
+    points = vpd.voided_patch(nx, ny, nz, spacing, void_fraction, seed)
+    mode = MODES[return_map]
+    if mode == vpd.RETURN_MAP_LEARNED and net is None:
+        net = vpd.load_warm_start()
+    return vpd.Bar(nx, ny, nz, spacing, 3.015 * spacing, JC_HEAT_PROPS, cfl_dt(spacing), RHO,
+                   velocity=velocity, damage=damage, return_map=mode, warm_start=net,
+                   points=points, velocity_rise=rise)
+
+
+def log_record(bar, step, tref=293.0):
+    """Observables of the step just taken (an unfused step that kept its fields)."""
+    it = bar.get("iterations")
+    fb = bar.get("fallbacks")
+    statev = bar.get("statev")
+    plastic = statev[:, 4] > 0  # Δε̄ᵖ of this step
+    dmg = bar.get("damage_index")
+    return dict(step=step, plastic_points=int(plastic.sum()),
+                mean_iterations=float(it[plastic].mean()) if plastic.any() else 0.0,
+                fallback_points=int((fb > 0).sum()),
+                fallback_rate=float((fb[plastic] > 0).mean()) if plastic.any() else 0.0,
+                max_eqps=float(statev[:, 1].max()),
+                max_temperature_rise_K=float(statev[:, 2].max() - tref),
+                fully_broken_particles=int((dmg >= 1.0).sum()),
+                mean_damage_index=float(dmg.mean()))
+
+
+def run(bar, steps, log_every):
+    """`steps` steps in chunks of log_every (fused inside a chunk, its last step unfused and
+    logged). Returns (seconds, log)."""
+    log = []
+    bar.ctx.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(log_every, steps - done)
+        bar.step(k, want_fields=True)
+        done += k
+        bar.ctx.synchronize()
+        t_pause = time.perf_counter()
+        log.append(log_record(bar, done))
+        t0 += time.perf_counter() - t_pause  # logging downloads are not timed
+    return time.perf_counter() - t0, log
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--log-every", type=int, default=500)
+    ap.add_argument("--return-map", choices=sorted(MODES), default="learned")
+    ap.add_argument("--nx", type=int, default=400)
+    ap.add_argument("--ny", type=int, default=70)
+    ap.add_argument("--rise", type=float, default=0.0,
+                    help="time (ms) over which the band accelerates to the end velocity")
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    bar = build(a.nx, a.ny, a.ny, return_map=a.return_map, rise=a.rise)
+    setup = time.perf_counter() - t0
+    tuned = bar.ctx.autotune(3)
+    bar.reset()
+    seconds, log = run(bar, a.steps, a.log_every)
+    plastic = sum(r["plastic_points"] for r in log)
+    out = dict(workload="voided_jc_bar_%dx%dx%d" % (a.nx, a.ny, a.ny), particles=bar.n,
+               velocity_rise_ms=a.rise,
+               bonds=bar.ctx.nnz, steps=a.steps, return_map=a.return_map, setup_s=setup,
+               seconds=seconds, particle_steps_per_s=bar.n * a.steps / seconds,
+               launch_tuning=tuned,
+               logged_fallback_rate=(sum(r["fallback_points"] for r in log) / plastic
+                                     if plastic else 0.0),
+               log=log)
+    text = json.dumps(out, indent=1)
+    if a.json:
+        with open(a.json, "w") as f:
+            f.write(text)
+    print(text)
+
+
+if __name__ == "__main__":
+    main()

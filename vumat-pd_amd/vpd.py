@@ -174,6 +174,18 @@ def make_patch(nx, ny, nz, spacing):
     return x, vol
 
 
+def voided_patch(nx, ny, nz, spacing, fraction, seed):
+    """The lattice of make_patch with round(fraction·n) sites removed at random (numpy
+    default_rng(seed), without replacement): the voided bar of the high-rate tension case. The
+    reference builds full patches only (src/host/Geometry.cpp:10-34); kept particles keep
+    their lattice volumes and order."""
+    x, vol = make_patch(nx, ny, nz, spacing)
+    n = x.shape[0]
+    keep = np.ones(n, bool)
+    keep[np.random.default_rng(seed).choice(n, size=int(round(fraction * n)), replace=False)] = False
+    return np.ascontiguousarray(x[keep]), np.ascontiguousarray(vol[keep])
+
+
 def regional_constraint_codes(x, component, region_layers, spacing, plane_strain=False):
     n = x.shape[0]
     codes = np.zeros(n, np.uint8)
@@ -617,18 +629,22 @@ class Bar:
     """The reference's regional-constraint bar (BoundaryValueCaseRunner.cpp:142-229) on the GPU:
     lattice, boundary bands, GPU neighbour CSR, K⁻¹, material and dynamics initialisation,
     then fused steps. `velocity` replaces the half-cosine ramp by a constant end velocity
-    (u = v·t, v, a = 0) for the velocity-driven bars."""
+    (u = v·t, v, a = 0) for the velocity-driven bars; with `velocity_rise` > 0 the band
+    accelerates uniformly to v over that time first (u = v·t²/2t_r, v·t/t_r, v/t_r, then
+    u = v·(t − t_r/2), v, 0)."""
 
     def __init__(self, nx, ny, nz, spacing, horizon, props, dt, density, amplitude=0.0,
                  rise_time=1.0, component=0, region_layers=3, plane_strain=False, nstatev=6,
                  tref=293.0, initial_state=None, damage=None, velocity=None,
                  material_density=7850.0, return_map=RETURN_MAP_BISECTION, rowptr=None,
-                 col=None, rtol=1e-12, warm_start=None):
+                 col=None, rtol=1e-12, warm_start=None, points=None, velocity_rise=0.0):
         self.dt, self.amplitude, self.rise_time, self.velocity = dt, amplitude, rise_time, velocity
+        self.velocity_rise = velocity_rise
         self.ctx = Context(horizon, density, props, nstatev, initial_state, tref, spacing,
                            material_density, damage)
         self.ctx.set_return_map(return_map, rtol, warm_start)
-        self.x, self.vol = make_patch(nx, ny, nz, spacing)
+        # `points` = (x, volume) replaces the full lattice (e.g. voided_patch).
+        self.x, self.vol = make_patch(nx, ny, nz, spacing) if points is None else points
         self.n = self.x.shape[0]
         self.ctx.set_particles(self.x, self.vol)
         if rowptr is None:
@@ -654,7 +670,12 @@ class Bar:
 
     def ramp(self, t):
         if self.velocity is not None:
-            return np.array([self.velocity * t, self.velocity, 0.0])
+            v, tr = self.velocity, self.velocity_rise
+            if tr <= 0.0:
+                return np.array([v * t, v, 0.0])
+            if t < tr:
+                return np.array([v * t * t / (2.0 * tr), v * t / tr, v / tr])
+            return np.array([v * (t - 0.5 * tr), v, 0.0])
         return smooth_ramp(self.amplitude, self.rise_time, t)
 
     def step(self, k=1, fuse=True, want_fields=False):
