@@ -347,6 +347,35 @@ int get_planes(vpd_context *c, const double *src, double *host, int comps) {
   return VPD_OK;
 }
 
+// Host displacement rows <-> the device pair layout (u_x, u_y) planes, then (u_z, 0).
+int put_u(vpd_context *c, const double *host) {
+  const int64_t n = c->n;
+  std::vector<double> stage((size_t)4 * n, 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    stage[2 * i] = host[3 * i];
+    stage[2 * i + 1] = host[3 * i + 1];
+    stage[2 * (n + i)] = host[3 * i + 2];
+  }
+  HIPTRY(c, hipMemcpyAsync(c->u, stage.data(), sizeof(double) * stage.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  return VPD_OK;
+}
+
+int get_u(vpd_context *c, double *host) {
+  const int64_t n = c->n;
+  std::vector<double> stage((size_t)4 * n);
+  HIPTRY(c, hipMemcpyAsync(stage.data(), c->u, sizeof(double) * stage.size(),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < n; ++i) {
+    host[3 * i] = stage[2 * i];
+    host[3 * i + 1] = stage[2 * i + 1];
+    host[3 * i + 2] = stage[2 * (n + i)];
+  }
+  return VPD_OK;
+}
+
 __global__ void k_material_init(int64_t n, Planes vN, Planes rN, Planes sig, Planes statev,
                                 int nstatev, const double *__restrict__ init, double *temp,
                                 double tref, double *eint, double *einel) {
@@ -445,11 +474,13 @@ int64_t sell_slot(const vpd_context *c, int64_t i, int64_t p) {
 
 Sell sell_of(const vpd_context *c) { return Sell{c->sliceBase, c->colS, c->rowptr}; }
 
+UPairs upairs(const vpd_context *c) { return UPairs{c->u, c->n}; }
+
 Ramp ramp_of(const double r[3]) { return Ramp{r[0], r[1], r[2]}; }
 
 int launch_predict(vpd_context *c, double dt, const double ramp[3]) {
   return timed(c, K_PREDICT, [&] {
-    k_predict<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->u, c->n), planes(c->v, c->n),
+    k_predict<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, upairs(c), planes(c->v, c->n),
                                                     planes(c->a, c->n), c->codes, dt,
                                                     ramp_of(ramp), c->batch);
   });
@@ -459,7 +490,7 @@ template <bool DAMAGE>
 void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression,
                       const Batch &batch) {
   k_defgrad<DAMAGE><<<grid_rounds(rd), 256, lds, c->stream>>>(
-      rd, planes(c->x, c->n), pairs(c->xv, c->n), planes(c->u, c->n), sell_of(c),
+      rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
       planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
       useCompression, batch);
 }
@@ -616,7 +647,7 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.n = c->n;
   g.nblocks = ceil_div(c->n, 256);
   g.x = planes(c->x, c->n);
-  g.u = planes(c->u, c->n);
+  g.u = upairs(c);
   g.v = planes(c->v, c->n);
   g.a = planes(c->a, c->n);
   g.amap = pairs(c->amap, c->n);
@@ -811,7 +842,7 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   c->diagOn = false; // sized for the previous block
   c->batch = Batch{};
   c->specCount = 0;
-  const int64_t sz[] = {3 * n, n, 9 * n, 3 * n, 3 * n, 3 * n, 9 * n, 9 * n,
+  const int64_t sz[] = {3 * n, n, 9 * n, 4 * n, 3 * n, 3 * n, 9 * n, 9 * n,
                         9 * n, 6 * n, (int64_t)c->nstatev * n, n, n, n, 9 * n, 10 * n, 4 * n};
   double **bufs[] = {&c->x,  &c->vol, &c->kinv, &c->u,    &c->v,   &c->a,
                      &c->fN, &c->vN,  &c->rN,   &c->sig,  &c->statev, &c->temp,
@@ -822,7 +853,7 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   if (dalloc(c, c->codes, n) || dalloc(c, c->inc, 6 * n) || dalloc(c, c->seed, n))
     return VPD_ERR_DEVICE;
   HIPTRY(c, hipMemsetAsync(c->codes, 0, n, c->stream));
-  HIPTRY(c, hipMemsetAsync(c->u, 0, sizeof(double) * 3 * n, c->stream));
+  HIPTRY(c, hipMemsetAsync(c->u, 0, sizeof(double) * 4 * n, c->stream));
   dfree(c->sigOut);
   dfree(c->strainOut);
   dfree(c->force);
@@ -1131,6 +1162,8 @@ int vpd_upload(vpd_context *c, int field, const void *host) {
   if (field == VPD_FIELD_DISPLACEMENT || field == VPD_FIELD_VELOCITY ||
       field == VPD_FIELD_ACCELERATION)
     c->predicted = false;
+  if (field == VPD_FIELD_DISPLACEMENT)
+    return put_u(c, static_cast<const double *>(host));
   return put_planes(c, dst, static_cast<const double *>(host), components(c, field));
 }
 
@@ -1203,6 +1236,8 @@ int vpd_download(vpd_context *c, int field, void *host) {
   double *src = field_ptr(c, field);
   if (!src)
     return fail(c, VPD_ERR_ARGUMENT, "Unknown field.");
+  if (field == VPD_FIELD_DISPLACEMENT)
+    return get_u(c, static_cast<double *>(host));
   return get_planes(c, src, static_cast<double *>(host), components(c, field));
 }
 
@@ -1227,7 +1262,7 @@ int vpd_apply_constraints(vpd_context *c, const double ramp[3]) {
   if (!ramp)
     return fail(c, VPD_ERR_ARGUMENT, "Null ramp.");
   return timed(c, K_CONSTRAIN, [&] {
-    k_constrain<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->u, c->n),
+    k_constrain<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, upairs(c),
                                                       planes(c->v, c->n), planes(c->a, c->n),
                                                       c->codes, ramp_of(ramp), c->batch);
   });
@@ -1374,7 +1409,7 @@ int vpd_set_specimens(vpd_context *c, int count, const int64_t *first, const dou
   // Each specimen's loaded band at t = 0 (the single-specimen host's initial constraints).
   if (int s = timed(c, K_CONSTRAIN, [&] {
         k_constrain<<<grid_of(c->n), 256, 0, c->stream>>>(
-            c->n, planes(c->u, c->n), planes(c->v, c->n), planes(c->a, c->n), c->codes,
+            c->n, upairs(c), planes(c->v, c->n), planes(c->a, c->n), c->codes,
             Ramp{0, 0, 0}, c->batch);
       }))
     return s;
@@ -1644,7 +1679,7 @@ int vpd_update_damage(vpd_context *c) {
     return VPD_OK;
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
   k_bond_damage<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->x, c->n),
-                                                      planes(c->u, c->n), sell_of(c), c->broken,
+                                                      upairs(c), sell_of(c), c->broken,
                                                       c->initiation, c->failure, useCompression);
   HIPTRY(c, hipGetLastError());
   HIPTRY(c, hipStreamSynchronize(c->stream));

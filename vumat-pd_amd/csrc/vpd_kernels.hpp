@@ -214,7 +214,7 @@ __device__ __forceinline__ bool stretch_breaks(const StretchTest &t, const doubl
 }
 
 // ExplicitTimeIntegrator.cpp:23-34 then the constraints at t (BoundaryConditions.cpp:81).
-__global__ void __launch_bounds__(256) k_predict(int64_t n, Planes u, Planes v, Planes a,
+__global__ void __launch_bounds__(256) k_predict(int64_t n, UPairs u, Planes v, Planes a,
                                                  const uint8_t *__restrict__ codes, double dtIn,
                                                  Ramp rIn, Batch batch) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(256) k_predict(int64_t n, Planes u, Planes v, 
   }
 }
 
-__global__ void __launch_bounds__(256) k_constrain(int64_t n, Planes u, Planes v, Planes a,
+__global__ void __launch_bounds__(256) k_constrain(int64_t n, UPairs u, Planes v, Planes a,
                                                    const uint8_t *__restrict__ codes, Ramp rIn,
                                                    Batch batch) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -314,10 +314,10 @@ __global__ void __launch_bounds__(256) k_shape(int64_t n, Planes x, const double
 // positions: a bond breaks (irreversibly) when its stretch exceeds `initiation` or, with a
 // compression threshold armed, falls below −failure. Damage does not enter F (:181-196).
 // Bonds run in groups of 4 (one 16-byte column load); each bond's X_j and V_j come as two
-// component pairs, u_j from its planes. A group's new breaks are set with one atomic OR on
+// component pairs, u_j as two more. A group's new breaks are set with one atomic OR on
 // the flag word the lane shares with 7 others.
 template <bool DAMAGE>
-__device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, Planes u,
+__device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, UPairs u,
                                             Sell bonds, Planes kinv, Planes fout,
                                             uint32_t *broken, const StretchTest &test) {
   const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
@@ -337,7 +337,8 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, 
     for (int q = 0; q < 4; ++q) {
       const int j = jj[q];
       const d2v a = xv(0, j), b = xv(1, j);
-      const double uj0 = u(0, j), uj1 = u(1, j), uj2 = u(2, j);
+      const d2v ua = u.pair(0, j), ub = u.pair(1, j);
+      const double uj0 = ua.x, uj1 = ua.y, uj2 = ub.x;
       if (k0 + q < len) {
         const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
         const double yij[3] = {(a.x + uj0) - yi0, (a.y + uj1) - yi1, (b.x + uj2) - yi2};
@@ -370,7 +371,7 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, 
 
 template <bool DAMAGE>
 __global__ void __launch_bounds__(256)
-    k_defgrad(Rounds rd, Planes x, PairPlanes xv, Planes u, Sell bonds, Planes kinv, Planes fout,
+    k_defgrad(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv, Planes fout,
               uint32_t *broken, double initiation, double failure, int useCompression,
               Batch batch) {
   const int64_t blk = xcd_block(rd.blocksPerRound);
@@ -740,7 +741,8 @@ struct ForceArgs {
   int64_t n;
   int64_t nblocks;
   Rounds rd;
-  Planes x, u, v, a, forceOut;
+  Planes x, v, a, forceOut;
+  UPairs u;
   PairPlanes xv, amap; // neighbour data as component pairs: (X_x, X_y), (X_z, V); A
   const double *vol;
   Sell bonds;
@@ -1006,7 +1008,7 @@ __global__ void __launch_bounds__(256) k_correct(int64_t n, Planes v, Planes a, 
 }
 
 // The bond loop of PdSimpleDamage.cpp:63-84 on its own (critical stretch).
-__global__ void __launch_bounds__(256) k_bond_damage(int64_t n, Planes x, Planes u, Sell bonds,
+__global__ void __launch_bounds__(256) k_bond_damage(int64_t n, Planes x, UPairs u, Sell bonds,
                                                      uint32_t *broken, double initiation,
                                                      double failure, int useCompression) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -30,6 +30,19 @@ struct PairPlanes {
   __device__ __forceinline__ d2v &operator()(int c, int64_t i) const { return p[c * n + i]; }
 };
 
+// Displacement as two pair planes, (u_x, u_y) and (u_z, 0): the F pass gathers u_j with two
+// 16-byte loads instead of three 8-byte ones. u(c, i) addresses one component.
+struct UPairs {
+  double *p;
+  int64_t n;
+  __device__ __forceinline__ double &operator()(int c, int64_t i) const {
+    return c < 2 ? p[2 * i + c] : p[2 * (n + i)];
+  }
+  __device__ __forceinline__ d2v pair(int h, int64_t i) const {
+    return reinterpret_cast<const d2v *>(p)[h * n + i];
+  }
+};
+
 struct T3 {
   double m[9];
 };
