@@ -247,20 +247,25 @@ typedef struct vpd_mlp vpd_mlp;
 int vpd_attach_warm_start(vpd_context *ctx, vpd_mlp *mlp);
 
 /* ---- launch tuning ---------------------------------------------------------------- */
-/* Two knobs per bond pass (no reference counterpart):
+/* Three knobs per bond pass (no reference counterpart):
  *  - sweeps: the pass walks the particle range in this many consecutive sweeps, so the
  *    neighbour data an XCD touches at once (its stretch of the sweep plus a horizon either
  *    side) can fit its L2;
- *  - LDS bytes reserved per 256-thread workgroup, capping workgroups per CU.
+ *  - LDS bytes reserved per 256-thread workgroup, capping workgroups per CU;
+ *  - gather window: 0 gathers neighbour data straight from memory; W >= 64 stages runs of
+ *    W consecutive particles' data in each wavefront's LDS and serves the bonds that fall in
+ *    them from there (the LDS reservation is then at least the windows').
  * The sum order, and so every result bit, is the same for every setting. vpd_autotune times
- * both passes over the grid of the two on the context's own block and keeps the fastest;
+ * both passes over the grid of the three on the context's own block and keeps the fastest;
  * tuned[] (VPD_TUNE_COUNT ints) receives the values indexed by enum vpd_tuning. */
 enum vpd_tuning {
   VPD_TUNE_DEFGRAD_LDS = 0,
   VPD_TUNE_FORCE_LDS = 1,
   VPD_TUNE_DEFGRAD_ROUNDS = 2,
   VPD_TUNE_FORCE_ROUNDS = 3,
-  VPD_TUNE_COUNT = 4
+  VPD_TUNE_DEFGRAD_WINDOW = 4,
+  VPD_TUNE_FORCE_WINDOW = 5,
+  VPD_TUNE_COUNT = 6
 };
 int vpd_set_tuning(vpd_context *ctx, int knob, int value);
 int vpd_get_tuning(vpd_context *ctx, int knob, int *value);

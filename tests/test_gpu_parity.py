@@ -270,6 +270,9 @@ def test_fused_equals_unfused(vpd, orc):
 @pytest.mark.parametrize("knobs", [
     dict(defgrad_rounds=3, force_rounds=5),
     dict(defgrad_lds=36408, force_lds=64000, defgrad_rounds=2, force_rounds=2),
+    dict(defgrad_window=64, force_window=80),
+    dict(defgrad_window=128, force_window=96, defgrad_rounds=2, force_rounds=3,
+         force_lds=64000),
 ])
 def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
     # Launch knobs change how the bond passes are scheduled, never what they sum or in which
@@ -285,6 +288,49 @@ def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
     b.step(60)
     assert a.get("damage").sum() > 0
     for name in ["u", "v", "a", "stressN", "statev", "damage"]:
+        assert np.array_equal(a.get(name), b.get(name)), name
+
+
+@pytest.mark.parametrize("voids", [0.0, 0.08])
+def test_windowed_gathers_on_irregular_rows(vpd, voids):
+    # Voids make every row irregular: the windows then cover fewer bonds per step, and the
+    # results must still equal the direct gathers bit for bit.
+    banner("windowed gathers, void fraction %g" % voids)
+    points = vpd.voided_patch(24, 9, 7, 1.0, voids, 5)
+    kw = dict(nx=24, ny=9, nz=7, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
+              density=7.85e-3, velocity=300.0, damage=(0.002, 0.0), points=points)
+    a = vpd.Bar(**kw)
+    b = vpd.Bar(**kw)
+    b.ctx.set_tuning(defgrad_window=64, force_window=64)
+    a.step(50, want_fields=True)
+    b.step(50, want_fields=True)
+    assert a.get("damage").sum() > 0
+    for name in ["u", "v", "a", "stressN", "statev", "damage", "force", "F"]:
+        assert np.array_equal(a.get(name), b.get(name)), name
+
+
+def test_window_knob_limits(vpd):
+    b = vpd.Bar(nx=6, ny=4, nz=4, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
+                density=7.85e-3, velocity=1.0)
+    with pytest.raises(vpd.VpdError, match="window"):
+        b.ctx.set_tuning(force_window=32)
+    with pytest.raises(vpd.VpdError, match="window"):
+        b.ctx.set_tuning(force_window=400)  # 4 × 7 pairs × 400 × 16 B > 160 KiB
+    b.ctx.set_tuning(force_window=0, defgrad_window=640)  # 4 pairs × 640 fit
+    assert b.ctx.get_tuning()["defgrad_window"] == 640
+
+
+def test_full_size_windowed_equals_direct(vpd):
+    banner("500k bar: windowed gathers == direct, 12 steps with damage")
+    kw = dict(nx=200, ny=50, nz=50, spacing=0.2, horizon=0.603, props=JC_PROPS, dt=1.7075e-5,
+              density=7.85e-3, velocity=60.0, damage=(0.0005, 0.0))
+    a = vpd.Bar(**kw)
+    b = vpd.Bar(**kw)
+    b.ctx.set_tuning(defgrad_window=80, force_window=80, defgrad_rounds=4, force_rounds=4)
+    a.step(12)
+    b.step(12)
+    assert a.get("damage").sum() > 0
+    for name in ["u", "v", "stressN", "damage"]:
         assert np.array_equal(a.get(name), b.get(name)), name
 
 

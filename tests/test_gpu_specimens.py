@@ -65,12 +65,14 @@ def assert_same(batch, s, bar, what):
         assert np.array_equal(got, want), (what, name, np.abs(got - want).max())
 
 
-@pytest.mark.parametrize("return_map", [0, 2], ids=["bisection", "newton"])
-def test_batch_equals_specimens_alone(vpd, return_map):
-    banner("batched specimens vs each alone, return map %d" % return_map)
+@pytest.mark.parametrize("return_map,windows", [(0, 0), (2, 0), (0, 80)],
+                         ids=["bisection", "newton", "bisection-windowed"])
+def test_batch_equals_specimens_alone(vpd, return_map, windows):
+    banner("batched specimens vs each alone, return map %d, windows %d" % (return_map, windows))
     damage = (0.003, 0.0)
     sps = specimens(vpd)
     batch = vpd.SpecimenBatch(sps, JC_PROPS, RHO, damage=damage, return_map=return_map)
+    batch.ctx.set_tuning(defgrad_window=windows, force_window=windows)
     assert batch.n == sum(sp.n for sp in sps)
     batch.step()
     assert batch.step_index == max(sp.steps for sp in sps)

@@ -99,6 +99,9 @@ struct vpd_context {
   int ldsDefgrad = 0, ldsForce = 0;
   // Sweeps per bond pass (see Rounds in vpd_kernels.hpp). Chosen by vpd_autotune.
   int roundsDefgrad = 1, roundsForce = 1;
+  // Window of the windowed gathers per bond pass (particles staged per window; 0: direct
+  // gathers). Chosen by vpd_autotune.
+  int winDefgrad = 0, winForce = 0;
   // Return map of the material stage; the learned mode adds the network, the strain
   // increments it reads and the seeds it writes.
   double rtol = 1.0e-12;
@@ -486,29 +489,44 @@ int launch_predict(vpd_context *c, double dt, const double ramp[3]) {
   });
 }
 
+// LDS of a windowed bond pass: 4 wavefronts × `pairs` component pairs × W particles.
+constexpr int kMaxLds = 163840;
+int window_lds(int pairs, int window) { return 4 * pairs * window * 16; }
+int launch_lds(int cap, int pairs, int window) {
+  return window > 0 ? std::max(cap, window_lds(pairs, window)) : cap;
+}
+
 template <bool DAMAGE>
 void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression,
-                      const Batch &batch) {
+                      const Batch &batch, int window) {
+  if (window > 0) {
+    k_defgrad_w<DAMAGE><<<grid_rounds(rd), 256, launch_lds(lds, 4, window), c->stream>>>(
+        rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
+        planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
+        useCompression, batch, window);
+    return;
+  }
   k_defgrad<DAMAGE><<<grid_rounds(rd), 256, lds, c->stream>>>(
       rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
       planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
       useCompression, batch);
 }
 
-void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, const Batch &batch) {
+void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, const Batch &batch,
+                         int window) {
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
   const Rounds rd = rounds_of(c->n, rounds);
   if (dmg)
-    launch_defgrad_t<true>(c, lds, rd, useCompression, batch);
+    launch_defgrad_t<true>(c, lds, rd, useCompression, batch, window);
   else
-    launch_defgrad_t<false>(c, lds, rd, useCompression, batch);
+    launch_defgrad_t<false>(c, lds, rd, useCompression, batch, window);
 }
 
 // F of the step being taken: with batched specimens, of those that have not finished.
 int launch_defgrad(vpd_context *c) {
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   return timed(c, K_DEFGRAD, [&] {
-    launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad, c->batch);
+    launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad, c->batch, c->winDefgrad);
   });
 }
 
@@ -608,36 +626,42 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
 }
 
 template <bool DAMAGE, bool FUSE, bool INTEGRATE, bool DIAG = false>
-void launch_force_t(vpd_context *c, const ForceArgs &g, int lds) {
-  k_force<DAMAGE, FUSE, INTEGRATE, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, int window = 0) {
+  if (!DIAG && window > 0)
+    k_force_w<DAMAGE, FUSE, INTEGRATE>
+        <<<grid_rounds(g.rd), 256, launch_lds(lds, 7, window), c->stream>>>(g, window);
+  else
+    k_force<DAMAGE, FUSE, INTEGRATE, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
 }
 
 template <bool DIAG>
-void launch_force_integrating(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, int lds) {
+void launch_force_integrating(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, int lds,
+                              int window) {
   if (dmg && fuse)
-    launch_force_t<true, true, true, DIAG>(c, g, lds);
+    launch_force_t<true, true, true, DIAG>(c, g, lds, window);
   else if (dmg)
-    launch_force_t<true, false, true, DIAG>(c, g, lds);
+    launch_force_t<true, false, true, DIAG>(c, g, lds, window);
   else if (fuse)
-    launch_force_t<false, true, true, DIAG>(c, g, lds);
+    launch_force_t<false, true, true, DIAG>(c, g, lds, window);
   else
-    launch_force_t<false, false, true, DIAG>(c, g, lds);
+    launch_force_t<false, false, true, DIAG>(c, g, lds, window);
 }
 
 // The force pass in one of its forms: with or without damage flags, the next step's predictor
 // fused in, the integrator (INTEGRATE false leaves only the force density), the run
-// diagnostics' force terms (integrating forms only).
+// diagnostics' force terms (integrating forms only, direct gathers), direct or windowed
+// gathers (window > 0).
 void launch_force_with(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, bool integrate,
-                       int lds, bool diag = false) {
+                       int lds, bool diag = false, int window = 0) {
   if (!integrate) {
     if (dmg)
-      launch_force_t<true, false, false>(c, g, lds);
+      launch_force_t<true, false, false>(c, g, lds, window);
     else
-      launch_force_t<false, false, false>(c, g, lds);
+      launch_force_t<false, false, false>(c, g, lds, window);
   } else if (diag) {
-    launch_force_integrating<true>(c, g, dmg, fuse, lds);
+    launch_force_integrating<true>(c, g, dmg, fuse, lds, 0);
   } else {
-    launch_force_integrating<false>(c, g, dmg, fuse, lds);
+    launch_force_integrating<false>(c, g, dmg, fuse, lds, window);
   }
 }
 
@@ -666,7 +690,7 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.diag = c->diag;
   g.batch = c->batch;
   return timed(c, K_FORCE, [&] {
-    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag);
+    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag, c->winForce);
   });
 }
 
@@ -1068,7 +1092,7 @@ int vpd_initialize_material(vpd_context *c) {
   // F₀ at the current displacement into fN (VumatPdCorrespondenceMaterial.cpp:54).
   // Every particle, whatever step a batched specimen has reached.
   if (int s = timed(c, K_DEFGRAD, [&] {
-        launch_defgrad_with(c, false, c->ldsDefgrad, c->roundsDefgrad, Batch{});
+        launch_defgrad_with(c, false, c->ldsDefgrad, c->roundsDefgrad, Batch{}, c->winDefgrad);
       }))
     return s;
   HIPTRY(c, hipMemcpyAsync(c->fN, c->fnew, sizeof(double) * 9 * c->n, hipMemcpyDeviceToDevice,
@@ -1784,6 +1808,12 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
     if (value < 1 || value > 1024)
       return fail(c, VPD_ERR_ARGUMENT, "Sweeps per bond pass must lie in [1, 1024].");
     (knob == VPD_TUNE_DEFGRAD_ROUNDS ? c->roundsDefgrad : c->roundsForce) = value;
+  } else if (knob == VPD_TUNE_DEFGRAD_WINDOW || knob == VPD_TUNE_FORCE_WINDOW) {
+    const int pairsPer = knob == VPD_TUNE_DEFGRAD_WINDOW ? 4 : 7;
+    if (value != 0 && (value < 64 || window_lds(pairsPer, value) > kMaxLds))
+      return fail(c, VPD_ERR_ARGUMENT,
+                  "Gather window must be 0 (direct) or at least 64 particles within the LDS.");
+    (knob == VPD_TUNE_DEFGRAD_WINDOW ? c->winDefgrad : c->winForce) = value;
   } else {
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
   }
@@ -1806,14 +1836,21 @@ int vpd_get_tuning(vpd_context *c, int knob, int *value) {
   case VPD_TUNE_FORCE_ROUNDS:
     *value = c->roundsForce;
     return VPD_OK;
+  case VPD_TUNE_DEFGRAD_WINDOW:
+    *value = c->winDefgrad;
+    return VPD_OK;
+  case VPD_TUNE_FORCE_WINDOW:
+    *value = c->winForce;
+    return VPD_OK;
   default:
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
   }
 }
 
-// Times both bond passes over a grid of (sweeps per pass) × (workgroups per CU) and keeps the
-// fastest for each. The timing runs use the passes' side-effect-free forms: F into its
-// scratch plane, and the force gather without the integrator.
+// Times both bond passes over a grid of (sweeps per pass) × (workgroups per CU) × (gather
+// window) and keeps the fastest for each. The timing runs leave the state as it was: F goes
+// to its scratch plane (with the stretch test when damage is on; the bond flags are restored
+// afterwards), and the force gather runs without the integrator.
 int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   if (int s = require_ready(c))
     return s;
@@ -1839,48 +1876,76 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   g.broken = c->broken;
   g.batch = c->batch;
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
+  // The F pass is timed in the form the steps use (with the stretch test when damage is on);
+  // the flags it may set are restored afterwards.
+  uint32_t *savedFlags = nullptr;
+  const int64_t flagBytes = sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords);
+  if (dmg) {
+    if (dalloc(c, savedFlags, std::max<int64_t>(1, c->brokenWords)))
+      return VPD_ERR_DEVICE;
+    HIPTRY(c, hipMemcpyAsync(savedFlags, c->broken, flagBytes, hipMemcpyDeviceToDevice,
+                             c->stream));
+  }
+  // Gather windows (0: direct gathers); a windowed pass reserves at least the LDS of its
+  // windows, so smaller caps repeat the uncapped setting and are skipped.
+  const int windowChoices[] = {0, 64, 80, 96, 128};
+  const int pairsOf[2] = {4, 7};
   float best[2] = {1e30f, 1e30f};
-  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1};
-  for (int lds : ldsChoices)
-    for (int rounds : roundChoices) {
-      if (rounds > 1 && rounds > ceil_div(c->n, 256))
-        continue;
-      for (int pass = 0; pass < 2; ++pass) {
-        g.rd = rounds_of(c->n, rounds);
-        float total = 0.f;
-        for (int r = 0; r <= reps; ++r) {
-          HIPTRY(c, hipEventRecord(a, c->stream));
-          if (pass == 0)
-            launch_defgrad_with(c, false, lds, rounds, c->batch);
-          else
-            launch_force_with(c, g, dmg, false, false, lds);
-          HIPTRY(c, hipGetLastError());
-          HIPTRY(c, hipEventRecord(b, c->stream));
-          HIPTRY(c, hipEventSynchronize(b));
-          float ms = 0.f;
-          HIPTRY(c, hipEventElapsedTime(&ms, a, b));
-          if (r > 0) // the first run warms caches and code
-            total += ms;
-        }
-        if (total < best[pass]) {
-          best[pass] = total;
-          pickLds[pass] = lds;
-          pickRounds[pass] = rounds;
+  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1}, pickWin[2] = {0, 0};
+  for (int pass = 0; pass < 2; ++pass)
+    for (int win : windowChoices)
+      for (int lds : ldsChoices) {
+        if (win > 0 && lds != 0 && lds <= window_lds(pairsOf[pass], win))
+          continue;
+        for (int rounds : roundChoices) {
+          if (rounds > 1 && rounds > ceil_div(c->n, 256))
+            continue;
+          g.rd = rounds_of(c->n, rounds);
+          float total = 0.f;
+          for (int r = 0; r <= reps; ++r) {
+            HIPTRY(c, hipEventRecord(a, c->stream));
+            if (pass == 0)
+              launch_defgrad_with(c, dmg, lds, rounds, c->batch, win);
+            else
+              launch_force_with(c, g, dmg, false, false, lds, false, win);
+            HIPTRY(c, hipGetLastError());
+            HIPTRY(c, hipEventRecord(b, c->stream));
+            HIPTRY(c, hipEventSynchronize(b));
+            float ms = 0.f;
+            HIPTRY(c, hipEventElapsedTime(&ms, a, b));
+            if (r > 0) // the first run warms caches and code
+              total += ms;
+          }
+          if (total < best[pass]) {
+            best[pass] = total;
+            pickLds[pass] = lds;
+            pickRounds[pass] = rounds;
+            pickWin[pass] = win;
+          }
         }
       }
-    }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
+  if (savedFlags) {
+    HIPTRY(c, hipMemcpyAsync(c->broken, savedFlags, flagBytes, hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    dfree(savedFlags);
+  }
   c->ldsDefgrad = pickLds[0];
   c->roundsDefgrad = pickRounds[0];
   c->ldsForce = pickLds[1];
   c->roundsForce = pickRounds[1];
+  c->winDefgrad = pickWin[0];
+  c->winForce = pickWin[1];
   c->haveFnew = false; // F scratch now holds the tuning runs' values
   if (tuned) {
     tuned[VPD_TUNE_DEFGRAD_LDS] = c->ldsDefgrad;
     tuned[VPD_TUNE_FORCE_LDS] = c->ldsForce;
     tuned[VPD_TUNE_DEFGRAD_ROUNDS] = c->roundsDefgrad;
     tuned[VPD_TUNE_FORCE_ROUNDS] = c->roundsForce;
+    tuned[VPD_TUNE_DEFGRAD_WINDOW] = c->winDefgrad;
+    tuned[VPD_TUNE_FORCE_WINDOW] = c->winForce;
   }
   return check_device(c);
 }

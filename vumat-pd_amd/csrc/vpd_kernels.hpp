@@ -10,6 +10,7 @@
 #include "vpd_math.hpp"
 
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <stdint.h>
 
 namespace vpd {
@@ -85,6 +86,74 @@ struct Rounds {
   int rounds;
   __device__ __forceinline__ int64_t particle(int r, int64_t blk) const {
     return ((int64_t)r * blocksPerRound + blk) * 256 + threadIdx.x;
+  }
+};
+
+// Windowed neighbour gathers (the `_w` bond-pass kernels). A wavefront's 64 consecutive
+// particles have neighbour rows that run over the same index ranges, shifted by one per
+// lane, so most of the neighbour data a wavefront gathers lies in a few short runs of
+// consecutive particles. The windowed kernels stage one such run — W consecutive particles
+// from the smallest neighbour index any lane still needs — in the wavefront's LDS with one
+// coalesced load per lane and component pair, let every lane consume its next bonds whose
+// neighbour lies in the run, and move on. Each lane still walks its own row in CSR order
+// with the arithmetic of the direct kernels, so results are identical bit for bit; only
+// where the neighbour data comes from changes. The window never moves back (every lane's
+// next index is at least its end), and the lane holding the smallest index always advances,
+// so the loop ends.
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    v = min(v, __shfl_xor(v, off));
+  return v;
+}
+
+// The wavefront's LDS writes are complete and ordered before its reads (and the compiler
+// moves no LDS access across this point).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0), vmcnt and expcnt unconstrained
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Cursor over one lane's bonds in CSR order, loading the 16-byte column group (and the flag
+// word) every 4 bonds. `next` is the neighbour index of bond k, or INT_MAX when done.
+struct BondCursor {
+  int64_t g0;
+  int k, len, next;
+  i4v cols;
+  uint32_t flags;
+  __device__ __forceinline__ int64_t group_slot() const { return g0 + 64 * (int64_t)(k & ~3); }
+  __device__ __forceinline__ void load_group(const Sell &b, const uint32_t *broken, bool damage) {
+    const int64_t pg = group_slot();
+    cols = *(const i4v *)(b.col + pg);
+    flags = damage ? broken[pg >> 5] : 0u;
+  }
+  __device__ __forceinline__ void start(const Sell &b, const uint32_t *broken, bool damage,
+                                        int64_t i) {
+    k = 0;
+    len = i >= 0 ? b.rowptr[i + 1] - b.rowptr[i] : 0;
+    g0 = i >= 0 ? sell_pos(b, i, 0) : 0;
+    next = INT_MAX;
+    flags = 0u;
+    if (len > 0) {
+      load_group(b, broken, damage);
+      next = cols[0];
+    }
+  }
+  __device__ __forceinline__ uint32_t bit() const { return 1u << ((group_slot() + (k & 3)) & 31); }
+  // Moves to bond k + 1, loading the next column group when it starts one.
+  __device__ __forceinline__ void advance(const Sell &b, const uint32_t *broken, bool damage) {
+    ++k;
+    if (k >= len) {
+      next = INT_MAX;
+      return;
+    }
+    if ((k & 3) == 0)
+      load_group(b, broken, damage);
+    next = column(k & 3);
+  }
+  // cols[q] by selects: a runtime index into the vector would go through scratch memory.
+  __device__ __forceinline__ int column(int q) const {
+    return q == 0 ? cols.x : q == 1 ? cols.y : q == 2 ? cols.z : cols.w;
   }
 };
 
@@ -401,6 +470,105 @@ __device__ __forceinline__ void store_amap(PairPlanes amap, int64_t i, const T3 
   last.x = A.m[8];
   last.y = 0.0;
   amap(4, i) = last;
+}
+
+// Bond pass 1 with windowed neighbour gathers (see BondCursor): X_j, V_j and u_j of W
+// consecutive particles per window, as 4 component pairs in the wavefront's LDS. New breaks
+// of a group are collected and set with one atomic OR when the lane leaves the group.
+template <bool DAMAGE>
+__device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes xv, UPairs u,
+                                               Sell bonds, uint32_t *broken,
+                                               const StretchTest &test, int W, d2v *win,
+                                               double x0, double x1, double x2, double yi0,
+                                               double yi1, double yi2, double num[9]) {
+  const int lane = threadIdx.x & 63;
+  BondCursor cur;
+  cur.start(bonds, broken, DAMAGE, i);
+  uint32_t fresh = 0;
+  while (true) {
+    const int w0 = wave_min(cur.next);
+    if (w0 == INT_MAX)
+      break;
+    for (int q = lane; q < W; q += 64) {
+      const int64_t p = (int64_t)w0 + q;
+      if (p < n) {
+        win[q] = xv(0, p);
+        win[W + q] = xv(1, p);
+        win[2 * W + q] = u.pair(0, p);
+        win[3 * W + q] = u.pair(1, p);
+      }
+    }
+    wave_lds_sync();
+    while ((int64_t)cur.next < (int64_t)w0 + W) {
+      const int o = cur.next - w0;
+      const d2v a = win[o], b = win[W + o], ua = win[2 * W + o], ub = win[3 * W + o];
+      const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
+      const double yij[3] = {(a.x + ua.x) - yi0, (a.y + ua.y) - yi1, (b.x + ub.x) - yi2};
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          num[3 * r + c] += b.y * yij[r] * xi[c];
+      if (DAMAGE) {
+        const uint32_t bit = cur.bit();
+        if (!(cur.flags & bit) && stretch_breaks(test, xi, yij))
+          fresh |= bit;
+        if (((cur.k + 1) & 3) == 0 || cur.k + 1 >= cur.len) { // leaving the group
+          if (fresh)
+            atomicOr(&broken[cur.group_slot() >> 5], fresh);
+          fresh = 0;
+        }
+      }
+      cur.advance(bonds, broken, DAMAGE);
+    }
+    wave_lds_sync();
+  }
+}
+
+template <bool DAMAGE>
+__global__ void __launch_bounds__(256)
+    k_defgrad_w(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
+                Planes fout, uint32_t *broken, double initiation, double failure,
+                int useCompression, Batch batch, int W) {
+  extern __shared__ d2v wlds[];
+  d2v *win = wlds + (threadIdx.x >> 6) * (4 * W);
+  const int64_t blk = xcd_block(rd.blocksPerRound);
+  if (blk < 0)
+    return;
+  const StretchTest test = make_stretch_test(initiation, failure, useCompression);
+  for (int r = 0; r < rd.rounds; ++r) {
+    const int64_t i = rd.particle(r, blk);
+    if (i - (threadIdx.x & 63) >= rd.n) // the wavefront's first particle: a uniform exit
+      break;
+    const bool mine = i < rd.n && point_active(batch, i);
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0, yi0 = 0.0, yi1 = 0.0, yi2 = 0.0;
+    if (mine) {
+      x0 = x(0, i);
+      x1 = x(1, i);
+      x2 = x(2, i);
+      yi0 = x0 + u(0, i);
+      yi1 = x1 + u(1, i);
+      yi2 = x2 + u(2, i);
+    }
+    double num[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      num[c] = 0.0;
+    defgrad_window<DAMAGE>(rd.n, mine ? i : -1, xv, u, bonds, broken, test, W, win, x0, x1, x2,
+                           yi0, yi1, yi2, num);
+    if (!mine)
+      continue;
+    T3 nm, ki;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      nm.m[c] = num[c];
+      ki.m[c] = kinv(c, i);
+    }
+    const T3 f = t3_mul(nm, ki);
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      fout(c, i) = f.m[c];
+  }
 }
 
 // Material point update. VumatPdCorrespondenceMaterial.cpp:110-173 for every point at
@@ -786,16 +954,13 @@ __device__ __forceinline__ void force_bond_term(const T3 &ai, const T3 &aj, cons
     term[d] = (ti.v[d] - tj.v[d]) * vj;
 }
 
-// Bond pass 2 for particle i: the bonds in groups of 4 (one 16-byte column load, one flag
-// word), each bond's X_j, V_j and A_j as 7 component-pair loads; a broken bond contributes
-// exactly zero, so it is skipped. The sum runs in CSR order.
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool DIAG>
-__device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double *dv) {
-  if (!point_active(g.batch, i))
-    return;
-  const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
-  const T3 ai = load_amap(g.amap, i);
-  double acc[3] = {0.0, 0.0, 0.0};
+// Bond pass 2, direct gathers: the bonds of particle i in groups of 4 (one 16-byte column
+// load, one flag word), each bond's X_j, V_j and A_j as 7 component-pair loads; a broken
+// bond contributes exactly zero, so it is skipped. The sum runs in CSR order.
+template <bool DAMAGE>
+__device__ __forceinline__ void force_sum_direct(const ForceArgs &g, int64_t i, double x0,
+                                                 double x1, double x2, const T3 &ai,
+                                                 double acc[3]) {
   const int len = g.bonds.rowptr[i + 1] - g.bonds.rowptr[i];
   const int64_t g0 = sell_pos(g.bonds, i, 0);
   for (int k0 = 0; k0 < len; k0 += 4) {
@@ -818,6 +983,64 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
       }
     }
   }
+}
+
+// Bond pass 2 with windowed gathers (see BondCursor): X_j, V_j and A_j of W consecutive
+// particles per window as 7 component pairs in the wavefront's LDS. i < 0: a lane with no
+// particle, which still takes part in the wavefront's window steps.
+template <bool DAMAGE>
+__device__ __forceinline__ void force_sum_window(const ForceArgs &g, int64_t i, int W, d2v *win,
+                                                 double x0, double x1, double x2, const T3 &ai,
+                                                 double acc[3]) {
+  const int lane = threadIdx.x & 63;
+  BondCursor cur;
+  cur.start(g.bonds, g.broken, DAMAGE, i);
+  while (true) {
+    const int w0 = wave_min(cur.next);
+    if (w0 == INT_MAX)
+      break;
+    for (int q = lane; q < W; q += 64) {
+      const int64_t p = (int64_t)w0 + q;
+      if (p < g.n) {
+        win[q] = g.xv(0, p);
+        win[W + q] = g.xv(1, p);
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+          win[(2 + c) * W + q] = g.amap(c, p);
+      }
+    }
+    wave_lds_sync();
+    while ((int64_t)cur.next < (int64_t)w0 + W) {
+      const int o = cur.next - w0;
+      if (!(DAMAGE && (cur.flags & cur.bit()))) {
+        const d2v a = win[o], b = win[W + o];
+        T3 aj;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const d2v v = win[(2 + c) * W + o];
+          aj.m[2 * c] = v.x;
+          aj.m[2 * c + 1] = v.y;
+        }
+        aj.m[8] = win[6 * W + o].x;
+        const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
+        double term[3];
+        force_bond_term(ai, aj, xi, b.y, term);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          acc[d] += term[d];
+      }
+      cur.advance(g.bonds, g.broken, DAMAGE);
+    }
+    wave_lds_sync();
+  }
+}
+
+// The rest of the force pass for particle i after its bond sum: the force density output,
+// velocity-Verlet correction and constraints, the diagnostics' force terms and the fused
+// next prediction.
+template <bool FUSE_NEXT, bool INTEGRATE, bool DIAG>
+__device__ __forceinline__ void force_finish(const ForceArgs &g, int64_t i, const double acc[3],
+                                             double *dv) {
   if (g.forceOut.p) {
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -868,6 +1091,17 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
   }
 }
 
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, bool DIAG>
+__device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double *dv) {
+  if (!point_active(g.batch, i))
+    return;
+  const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
+  const T3 ai = load_amap(g.amap, i);
+  double acc[3] = {0.0, 0.0, 0.0};
+  force_sum_direct<DAMAGE>(g, i, x0, x1, x2, ai, acc);
+  force_finish<FUSE_NEXT, INTEGRATE, DIAG>(g, i, acc, dv);
+}
+
 // With DIAG (and INTEGRATE) each 256-particle chunk also writes its row of the run
 // diagnostics' force terms; the chunk loop then runs block-uniformly so that every thread
 // reaches the chunk's reduction.
@@ -896,6 +1130,38 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
     if (i < g.rd.n)
       force_one<DAMAGE, FUSE_NEXT, INTEGRATE, true>(g, i, dv);
     diag_block_store<NS, D_NET_X>(dv, g.diag.partials + chunk * DIAG_STRIDE + D_NET_X);
+  }
+}
+
+// The force pass with windowed gathers (no diagnostics form). Every wavefront keeps all its
+// lanes in the window steps; the rounds loop exits on the wavefront's first particle.
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true>
+__global__ void __launch_bounds__(256) k_force_w(ForceArgs g, int W) {
+  extern __shared__ d2v wlds[];
+  d2v *win = wlds + (threadIdx.x >> 6) * (7 * W);
+  const int64_t blk = xcd_block(g.rd.blocksPerRound);
+  if (blk < 0)
+    return;
+  for (int r = 0; r < g.rd.rounds; ++r) {
+    const int64_t i = g.rd.particle(r, blk);
+    if (i - (threadIdx.x & 63) >= g.rd.n)
+      break;
+    const bool mine = i < g.rd.n && point_active(g.batch, i);
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    T3 ai;
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      ai.m[c] = 0.0;
+    if (mine) {
+      x0 = g.x(0, i);
+      x1 = g.x(1, i);
+      x2 = g.x(2, i);
+      ai = load_amap(g.amap, i);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    force_sum_window<DAMAGE>(g, mine ? i : -1, W, win, x0, x1, x2, ai, acc);
+    if (mine)
+      force_finish<FUSE_NEXT, INTEGRATE, false>(g, i, acc, nullptr);
   }
 }
 

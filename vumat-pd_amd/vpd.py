@@ -388,7 +388,8 @@ class Context:
             self._warm = warm_start  # keep the network alive while attached
             self._check(lib().vpd_attach_warm_start(self.h, warm_start.h))
 
-    TUNE_KEYS = ("defgrad_lds", "force_lds", "defgrad_rounds", "force_rounds")
+    TUNE_KEYS = ("defgrad_lds", "force_lds", "defgrad_rounds", "force_rounds", "defgrad_window",
+                 "force_window")
 
     def autotune(self, reps=3):
         out = (C.c_int * len(self.TUNE_KEYS))()
