@@ -107,25 +107,64 @@ __device__ __forceinline__ int wave_min(int v) {
   return v;
 }
 
-// The wavefront's LDS writes are complete and ordered before its reads (and the compiler
-// moves no LDS access across this point).
+// The wavefront's window stores (LDS-direct loads, vmcnt) and LDS accesses (lgkmcnt) are
+// complete before it goes on, and the compiler moves no LDS access across this point.
 __device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_s_waitcnt(0xc07f); // lgkmcnt(0), vmcnt and expcnt unconstrained
+  __builtin_amdgcn_s_waitcnt(0x0070); // vmcnt(0), lgkmcnt(0); expcnt unconstrained
   __builtin_amdgcn_wave_barrier();
 }
 
-// Cursor over one lane's bonds in CSR order, loading the 16-byte column group (and the flag
-// word) every 4 bonds. `next` is the neighbour index of bond k, or INT_MAX when done.
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Window staging: each lane loads the 16-byte pair c of particle w0 + q0 + lane straight into
+// LDS (no registers), at window slot q0 + lane of plane c. `planes` gives the pair planes.
+template <int PAIRS, class Planeset>
+__device__ __forceinline__ void stage_window(const Planeset &planes, int64_t n, int w0, int W,
+                                             d2v *win) {
+  const int lane = threadIdx.x & 63;
+  for (int q0 = 0; q0 < W; q0 += 64) {
+    const int q = q0 + lane;
+    const int64_t p = (int64_t)w0 + q;
+    if (q < W && p < n) {
+#pragma unroll
+      for (int c = 0; c < PAIRS; ++c)
+        __builtin_amdgcn_global_load_lds((const void *)planes.ptr(c, p),
+                                         (lds_void *)(win + c * W + q0), 16, 0, 0);
+    }
+  }
+}
+
+// The pair planes of the two windowed passes, in window order.
+struct DefgradPlanes {
+  PairPlanes xv;
+  UPairs u;
+  __device__ __forceinline__ const d2v *ptr(int c, int64_t p) const {
+    return c < 2 ? &xv(c, p) : reinterpret_cast<const d2v *>(u.p) + (c - 2) * u.n + p;
+  }
+};
+struct ForcePlanes {
+  PairPlanes xv, amap;
+  __device__ __forceinline__ const d2v *ptr(int c, int64_t p) const {
+    return c < 2 ? &xv(c, p) : &amap(c - 2, p);
+  }
+};
+
+// Cursor over one lane's bonds in CSR order. The 16-byte column group (and flag word) of the
+// next 4 bonds is loaded one group ahead, so that its latency is spent on the current group.
+// `next` is the neighbour index of bond k, or INT_MAX when done.
 struct BondCursor {
   int64_t g0;
   int k, len, next;
-  i4v cols;
-  uint32_t flags;
+  i4v cols, colsAhead;
+  uint32_t flags, flagsAhead;
   __device__ __forceinline__ int64_t group_slot() const { return g0 + 64 * (int64_t)(k & ~3); }
-  __device__ __forceinline__ void load_group(const Sell &b, const uint32_t *broken, bool damage) {
-    const int64_t pg = group_slot();
-    cols = *(const i4v *)(b.col + pg);
-    flags = damage ? broken[pg >> 5] : 0u;
+  __device__ __forceinline__ void fetch_ahead(const Sell &b, const uint32_t *broken, bool damage,
+                                              int groupStart) {
+    if (groupStart < len) {
+      const int64_t pg = g0 + 64 * (int64_t)groupStart;
+      colsAhead = *(const i4v *)(b.col + pg);
+      flagsAhead = damage ? broken[pg >> 5] : 0u;
+    }
   }
   __device__ __forceinline__ void start(const Sell &b, const uint32_t *broken, bool damage,
                                         int64_t i) {
@@ -135,25 +174,31 @@ struct BondCursor {
     next = INT_MAX;
     flags = 0u;
     if (len > 0) {
-      load_group(b, broken, damage);
-      next = cols[0];
+      cols = *(const i4v *)(b.col + g0);
+      flags = damage ? broken[g0 >> 5] : 0u;
+      next = cols.x;
+      fetch_ahead(b, broken, damage, 4);
     }
   }
   __device__ __forceinline__ uint32_t bit() const { return 1u << ((group_slot() + (k & 3)) & 31); }
-  // Moves to bond k + 1, loading the next column group when it starts one.
+  // cols[q] by selects: a runtime index into the vector would go through scratch memory.
+  __device__ __forceinline__ int column(int q) const {
+    return q == 0 ? cols.x : q == 1 ? cols.y : q == 2 ? cols.z : cols.w;
+  }
+  // Moves to bond k + 1; entering a new group takes the group loaded ahead and fetches the
+  // one after it.
   __device__ __forceinline__ void advance(const Sell &b, const uint32_t *broken, bool damage) {
     ++k;
     if (k >= len) {
       next = INT_MAX;
       return;
     }
-    if ((k & 3) == 0)
-      load_group(b, broken, damage);
+    if ((k & 3) == 0) {
+      cols = colsAhead;
+      flags = flagsAhead;
+      fetch_ahead(b, broken, damage, k + 4);
+    }
     next = column(k & 3);
-  }
-  // cols[q] by selects: a runtime index into the vector would go through scratch memory.
-  __device__ __forceinline__ int column(int q) const {
-    return q == 0 ? cols.x : q == 1 ? cols.y : q == 2 ? cols.z : cols.w;
   }
 };
 
@@ -481,7 +526,6 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
                                                const StretchTest &test, int W, d2v *win,
                                                double x0, double x1, double x2, double yi0,
                                                double yi1, double yi2, double num[9]) {
-  const int lane = threadIdx.x & 63;
   BondCursor cur;
   cur.start(bonds, broken, DAMAGE, i);
   uint32_t fresh = 0;
@@ -489,15 +533,7 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
     const int w0 = wave_min(cur.next);
     if (w0 == INT_MAX)
       break;
-    for (int q = lane; q < W; q += 64) {
-      const int64_t p = (int64_t)w0 + q;
-      if (p < n) {
-        win[q] = xv(0, p);
-        win[W + q] = xv(1, p);
-        win[2 * W + q] = u.pair(0, p);
-        win[3 * W + q] = u.pair(1, p);
-      }
-    }
+    stage_window<4>(DefgradPlanes{xv, u}, n, w0, W, win);
     wave_lds_sync();
     while ((int64_t)cur.next < (int64_t)w0 + W) {
       const int o = cur.next - w0;
@@ -992,23 +1028,13 @@ template <bool DAMAGE>
 __device__ __forceinline__ void force_sum_window(const ForceArgs &g, int64_t i, int W, d2v *win,
                                                  double x0, double x1, double x2, const T3 &ai,
                                                  double acc[3]) {
-  const int lane = threadIdx.x & 63;
   BondCursor cur;
   cur.start(g.bonds, g.broken, DAMAGE, i);
   while (true) {
     const int w0 = wave_min(cur.next);
     if (w0 == INT_MAX)
       break;
-    for (int q = lane; q < W; q += 64) {
-      const int64_t p = (int64_t)w0 + q;
-      if (p < g.n) {
-        win[q] = g.xv(0, p);
-        win[W + q] = g.xv(1, p);
-#pragma unroll
-        for (int c = 0; c < 5; ++c)
-          win[(2 + c) * W + q] = g.amap(c, p);
-      }
-    }
+    stage_window<7>(ForcePlanes{g.xv, g.amap}, g.n, w0, W, win);
     wave_lds_sync();
     while ((int64_t)cur.next < (int64_t)w0 + W) {
       const int o = cur.next - w0;
