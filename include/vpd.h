@@ -252,9 +252,10 @@ int vpd_attach_warm_start(vpd_context *ctx, vpd_mlp *mlp);
  *    neighbour data an XCD touches at once (its stretch of the sweep plus a horizon either
  *    side) can fit its L2;
  *  - LDS bytes reserved per 256-thread workgroup, capping workgroups per CU;
- *  - gather window: 0 gathers neighbour data straight from memory; W >= 64 stages runs of
- *    W consecutive particles' data in each wavefront's LDS and serves the bonds that fall in
- *    them from there (the LDS reservation is then at least the windows').
+ *  - gather window: 0 gathers neighbour data straight from memory; W in {64, 80, 96, 128,
+ *    160} stages runs of W consecutive particles' data in each wavefront's LDS and serves
+ *    the bonds that fall in them from there (the LDS reservation is then at least the
+ *    windows').
  * The sum order, and so every result bit, is the same for every setting. vpd_autotune times
  * both passes over the grid of the three on the context's own block and keeps the fastest;
  * tuned[] (VPD_TUNE_COUNT ints) receives the values indexed by enum vpd_tuning. */

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace vpd;
@@ -489,9 +490,38 @@ int launch_predict(vpd_context *c, double dt, const double ramp[3]) {
   });
 }
 
-// LDS of a windowed bond pass: 4 wavefronts × `pairs` component pairs × W particles.
+// LDS of a windowed bond pass: 4 wavefronts × `pairs` component pairs × W particles. The
+// window sizes are compile-time (the LDS offsets of the component planes become immediates).
 constexpr int kMaxLds = 163840;
+constexpr int kWindows[] = {64, 80, 96, 128, 160};
 int window_lds(int pairs, int window) { return 4 * pairs * window * 16; }
+bool window_supported(int window) {
+  for (int w : kWindows)
+    if (w == window)
+      return true;
+  return false;
+}
+
+// f.template operator()<W>() for the compile-time window equal to `window`.
+template <class F> void with_window(int window, F &&f) {
+  switch (window) {
+  case 64:
+    f(std::integral_constant<int, 64>{});
+    break;
+  case 80:
+    f(std::integral_constant<int, 80>{});
+    break;
+  case 96:
+    f(std::integral_constant<int, 96>{});
+    break;
+  case 128:
+    f(std::integral_constant<int, 128>{});
+    break;
+  default:
+    f(std::integral_constant<int, 160>{});
+    break;
+  }
+}
 int launch_lds(int cap, int pairs, int window) {
   return window > 0 ? std::max(cap, window_lds(pairs, window)) : cap;
 }
@@ -500,10 +530,13 @@ template <bool DAMAGE>
 void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression,
                       const Batch &batch, int window) {
   if (window > 0) {
-    k_defgrad_w<DAMAGE><<<grid_rounds(rd), 256, launch_lds(lds, 4, window), c->stream>>>(
-        rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
-        planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
-        useCompression, batch, window);
+    with_window(window, [&](auto w) {
+      k_defgrad_w<DAMAGE, decltype(w)::value>
+          <<<grid_rounds(rd), 256, launch_lds(lds, 4, window), c->stream>>>(
+              rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
+              planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation,
+              c->failure, useCompression, batch);
+    });
     return;
   }
   k_defgrad<DAMAGE><<<grid_rounds(rd), 256, lds, c->stream>>>(
@@ -628,8 +661,10 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
 template <bool DAMAGE, bool FUSE, bool INTEGRATE, bool DIAG = false>
 void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, int window = 0) {
   if (!DIAG && window > 0)
-    k_force_w<DAMAGE, FUSE, INTEGRATE>
-        <<<grid_rounds(g.rd), 256, launch_lds(lds, 7, window), c->stream>>>(g, window);
+    with_window(window, [&](auto w) {
+      k_force_w<DAMAGE, FUSE, INTEGRATE, decltype(w)::value>
+          <<<grid_rounds(g.rd), 256, launch_lds(lds, 7, window), c->stream>>>(g);
+    });
   else
     k_force<DAMAGE, FUSE, INTEGRATE, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
 }
@@ -1809,10 +1844,9 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
       return fail(c, VPD_ERR_ARGUMENT, "Sweeps per bond pass must lie in [1, 1024].");
     (knob == VPD_TUNE_DEFGRAD_ROUNDS ? c->roundsDefgrad : c->roundsForce) = value;
   } else if (knob == VPD_TUNE_DEFGRAD_WINDOW || knob == VPD_TUNE_FORCE_WINDOW) {
-    const int pairsPer = knob == VPD_TUNE_DEFGRAD_WINDOW ? 4 : 7;
-    if (value != 0 && (value < 64 || window_lds(pairsPer, value) > kMaxLds))
+    if (value != 0 && !window_supported(value))
       return fail(c, VPD_ERR_ARGUMENT,
-                  "Gather window must be 0 (direct) or at least 64 particles within the LDS.");
+                  "Gather window must be 0 (direct) or one of 64, 80, 96, 128, 160 particles.");
     (knob == VPD_TUNE_DEFGRAD_WINDOW ? c->winDefgrad : c->winForce) = value;
   } else {
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
@@ -1888,7 +1922,7 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   }
   // Gather windows (0: direct gathers); a windowed pass reserves at least the LDS of its
   // windows, so smaller caps repeat the uncapped setting and are skipped.
-  const int windowChoices[] = {0, 64, 80, 96, 128};
+  const int windowChoices[] = {0, 64, 80, 96, 128, 160};
   const int pairsOf[2] = {4, 7};
   float best[2] = {1e30f, 1e30f};
   int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1}, pickWin[2] = {0, 0};

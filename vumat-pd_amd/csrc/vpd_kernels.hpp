@@ -118,10 +118,10 @@ typedef __attribute__((address_space(3))) void lds_void;
 
 // Window staging: each lane loads the 16-byte pair c of particle w0 + q0 + lane straight into
 // LDS (no registers), at window slot q0 + lane of plane c. `planes` gives the pair planes.
-template <int PAIRS, class Planeset>
-__device__ __forceinline__ void stage_window(const Planeset &planes, int64_t n, int w0, int W,
-                                             d2v *win) {
+template <int PAIRS, int W, class Planeset>
+__device__ __forceinline__ void stage_window(const Planeset &planes, int64_t n, int w0, d2v *win) {
   const int lane = threadIdx.x & 63;
+#pragma unroll
   for (int q0 = 0; q0 < W; q0 += 64) {
     const int q = q0 + lane;
     const int64_t p = (int64_t)w0 + q;
@@ -157,6 +157,7 @@ struct BondCursor {
   int k, len, next;
   i4v cols, colsAhead;
   uint32_t flags, flagsAhead;
+  uint32_t broken4; // the current group's 4 flag bits, bond k & 3 at bit k & 3
   __device__ __forceinline__ int64_t group_slot() const { return g0 + 64 * (int64_t)(k & ~3); }
   __device__ __forceinline__ void fetch_ahead(const Sell &b, const uint32_t *broken, bool damage,
                                               int groupStart) {
@@ -173,13 +174,16 @@ struct BondCursor {
     g0 = i >= 0 ? sell_pos(b, i, 0) : 0;
     next = INT_MAX;
     flags = 0u;
+    broken4 = 0u;
     if (len > 0) {
       cols = *(const i4v *)(b.col + g0);
       flags = damage ? broken[g0 >> 5] : 0u;
+      broken4 = (flags >> (g0 & 31)) & 0xFu;
       next = cols.x;
       fetch_ahead(b, broken, damage, 4);
     }
   }
+  __device__ __forceinline__ bool bond_broken() const { return (broken4 >> (k & 3)) & 1u; }
   __device__ __forceinline__ uint32_t bit() const { return 1u << ((group_slot() + (k & 3)) & 31); }
   // cols[q] by selects: a runtime index into the vector would go through scratch memory.
   __device__ __forceinline__ int column(int q) const {
@@ -196,6 +200,7 @@ struct BondCursor {
     if ((k & 3) == 0) {
       cols = colsAhead;
       flags = flagsAhead;
+      broken4 = (flags >> (group_slot() & 31)) & 0xFu;
       fetch_ahead(b, broken, damage, k + 4);
     }
     next = column(k & 3);
@@ -520,10 +525,10 @@ __device__ __forceinline__ void store_amap(PairPlanes amap, int64_t i, const T3 
 // Bond pass 1 with windowed neighbour gathers (see BondCursor): X_j, V_j and u_j of W
 // consecutive particles per window, as 4 component pairs in the wavefront's LDS. New breaks
 // of a group are collected and set with one atomic OR when the lane leaves the group.
-template <bool DAMAGE>
+template <bool DAMAGE, int W>
 __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes xv, UPairs u,
                                                Sell bonds, uint32_t *broken,
-                                               const StretchTest &test, int W, d2v *win,
+                                               const StretchTest &test, d2v *win,
                                                double x0, double x1, double x2, double yi0,
                                                double yi1, double yi2, double num[9]) {
   BondCursor cur;
@@ -533,9 +538,10 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
     const int w0 = wave_min(cur.next);
     if (w0 == INT_MAX)
       break;
-    stage_window<4>(DefgradPlanes{xv, u}, n, w0, W, win);
+    stage_window<4, W>(DefgradPlanes{xv, u}, n, w0, win);
     wave_lds_sync();
-    while ((int64_t)cur.next < (int64_t)w0 + W) {
+    const int end = w0 > INT_MAX - W ? INT_MAX : w0 + W; // (a finished lane's next is INT_MAX)
+    while (cur.next < end) {
       const int o = cur.next - w0;
       const d2v a = win[o], b = win[W + o], ua = win[2 * W + o], ub = win[3 * W + o];
       const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
@@ -547,7 +553,7 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
           num[3 * r + c] += b.y * yij[r] * xi[c];
       if (DAMAGE) {
         const uint32_t bit = cur.bit();
-        if (!(cur.flags & bit) && stretch_breaks(test, xi, yij))
+        if (!cur.bond_broken() && stretch_breaks(test, xi, yij))
           fresh |= bit;
         if (((cur.k + 1) & 3) == 0 || cur.k + 1 >= cur.len) { // leaving the group
           if (fresh)
@@ -561,11 +567,11 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
   }
 }
 
-template <bool DAMAGE>
+template <bool DAMAGE, int W>
 __global__ void __launch_bounds__(256)
     k_defgrad_w(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
                 Planes fout, uint32_t *broken, double initiation, double failure,
-                int useCompression, Batch batch, int W) {
+                int useCompression, Batch batch) {
   extern __shared__ d2v wlds[];
   d2v *win = wlds + (threadIdx.x >> 6) * (4 * W);
   const int64_t blk = xcd_block(rd.blocksPerRound);
@@ -590,8 +596,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int c = 0; c < 9; ++c)
       num[c] = 0.0;
-    defgrad_window<DAMAGE>(rd.n, mine ? i : -1, xv, u, bonds, broken, test, W, win, x0, x1, x2,
-                           yi0, yi1, yi2, num);
+    defgrad_window<DAMAGE, W>(rd.n, mine ? i : -1, xv, u, bonds, broken, test, win, x0, x1,
+                              x2, yi0, yi1, yi2, num);
     if (!mine)
       continue;
     T3 nm, ki;
@@ -1024,8 +1030,8 @@ __device__ __forceinline__ void force_sum_direct(const ForceArgs &g, int64_t i, 
 // Bond pass 2 with windowed gathers (see BondCursor): X_j, V_j and A_j of W consecutive
 // particles per window as 7 component pairs in the wavefront's LDS. i < 0: a lane with no
 // particle, which still takes part in the wavefront's window steps.
-template <bool DAMAGE>
-__device__ __forceinline__ void force_sum_window(const ForceArgs &g, int64_t i, int W, d2v *win,
+template <bool DAMAGE, int W>
+__device__ __forceinline__ void force_sum_window(const ForceArgs &g, int64_t i, d2v *win,
                                                  double x0, double x1, double x2, const T3 &ai,
                                                  double acc[3]) {
   BondCursor cur;
@@ -1034,11 +1040,12 @@ __device__ __forceinline__ void force_sum_window(const ForceArgs &g, int64_t i, 
     const int w0 = wave_min(cur.next);
     if (w0 == INT_MAX)
       break;
-    stage_window<7>(ForcePlanes{g.xv, g.amap}, g.n, w0, W, win);
+    stage_window<7, W>(ForcePlanes{g.xv, g.amap}, g.n, w0, win);
     wave_lds_sync();
-    while ((int64_t)cur.next < (int64_t)w0 + W) {
+    const int end = w0 > INT_MAX - W ? INT_MAX : w0 + W; // (a finished lane's next is INT_MAX)
+    while (cur.next < end) {
       const int o = cur.next - w0;
-      if (!(DAMAGE && (cur.flags & cur.bit()))) {
+      if (!(DAMAGE && cur.bond_broken())) {
         const d2v a = win[o], b = win[W + o];
         T3 aj;
 #pragma unroll
@@ -1161,8 +1168,8 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
 
 // The force pass with windowed gathers (no diagnostics form). Every wavefront keeps all its
 // lanes in the window steps; the rounds loop exits on the wavefront's first particle.
-template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true>
-__global__ void __launch_bounds__(256) k_force_w(ForceArgs g, int W) {
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W>
+__global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
   extern __shared__ d2v wlds[];
   d2v *win = wlds + (threadIdx.x >> 6) * (7 * W);
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
@@ -1185,7 +1192,7 @@ __global__ void __launch_bounds__(256) k_force_w(ForceArgs g, int W) {
       ai = load_amap(g.amap, i);
     }
     double acc[3] = {0.0, 0.0, 0.0};
-    force_sum_window<DAMAGE>(g, mine ? i : -1, W, win, x0, x1, x2, ai, acc);
+    force_sum_window<DAMAGE, W>(g, mine ? i : -1, win, x0, x1, x2, ai, acc);
     if (mine)
       force_finish<FUSE_NEXT, INTEGRATE, false>(g, i, acc, nullptr);
   }
