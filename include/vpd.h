@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define VPD_ABI_VERSION 1
+#define VPD_ABI_VERSION 2
 
 enum vpd_status {
   VPD_OK = 0,
@@ -42,6 +42,12 @@ enum vpd_return_map {
   VPD_RETURN_MAP_BISECTION_TIGHT = 1, /* the same scheme to |r| <= rtol·q_trial */
   VPD_RETURN_MAP_NEWTON = 2,          /* safeguarded Newton seeded with state 5 */
   VPD_RETURN_MAP_LEARNED = 3          /* safeguarded Newton seeded by the warm-start network */
+};
+
+/* The VUMAT the material stage runs (the plugin of VumatMaterialConfig::pluginPath). */
+enum vpd_material_model {
+  VPD_MATERIAL_JOHNSON_COOK = 0, /* materials/examples/JohnsonCook.f90: 14 props, >= 6 states */
+  VPD_MATERIAL_ELASTIC = 1       /* materials/examples/Elastic.f: props E, nu; state untouched */
 };
 
 enum vpd_damage_mode {
@@ -98,7 +104,8 @@ typedef struct vpd_config {
   int damage_mode;
   double damage_initiation;
   double damage_failure;
-  int return_map; /* enum vpd_return_map */
+  int return_map; /* enum vpd_return_map (Johnson-Cook only) */
+  int material_model; /* enum vpd_material_model; 0 (zero-initialised) is Johnson-Cook */
 } vpd_config;
 
 typedef struct vpd_context vpd_context;
@@ -401,13 +408,15 @@ int64_t vpd_select_boundary_region(const double *x, int64_t n, const char *side,
                                    uint8_t *mask);
 
 /* ---- the JSON case runner ----------------------------------------------------------- */
-/* runDemoCaseFromJson (src/host/DemoCaseRunner.cpp:435-458) for "vumat_pd_boundary_case"
- * files (runBoundaryValueCase, src/host/BoundaryValueCaseRunner.cpp:309-440) on the GPU: the
- * same case and material JSON schema, the same timeseries.csv, field_snapshots.csv and
- * metadata.json in output.directory, the same stdout summary line. Materials must be the
- * Johnson-Cook VUMAT (cmname JOHNSONCOOK); damage "none" or "critical_stretch". Returns 0
- * when every material passes its acceptance checks, 2 when one fails, 1 on an error (the
- * message goes to stderr as the reference prints it). */
+/* runDemoCaseFromJson (src/host/DemoCaseRunner.cpp:435-458) on the GPU, with the same case
+ * and material JSON schema, output files and stdout lines as the reference:
+ *  - "vumat_pd_boundary_case" (runBoundaryValueCase, BoundaryValueCaseRunner.cpp:309-440):
+ *    timeseries.csv, field_snapshots.csv, metadata.json; Johnson-Cook materials;
+ *  - "vumat_pd_demo_case" (the affine host, DemoCaseRunner.cpp:295-431): timeseries.csv,
+ *    field_final.csv, metadata.json; Johnson-Cook or linear elastic (Elastic.f) materials.
+ * Damage "none" or "critical_stretch". "vumat_material_point_case" files are refused.
+ * Returns 0 when every material passes its acceptance checks, 2 when one fails, 1 on an
+ * error (the message goes to stderr as the reference prints it). */
 int vpd_run_case_json(const char *case_path, const char *plugin_directory);
 
 #ifdef __cplusplus

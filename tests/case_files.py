@@ -70,6 +70,48 @@ def write_case(directory, name="case", nx=21, ny=7, nz=3, spacing=2.0, horizon=6
     return path
 
 
+ELASTIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "affine",
+                            "elastic.json")
+
+
+def write_demo_case(directory, name="demo", nx=5, ny=3, nz=3, spacing=1.0, horizon=1.75,
+                    steps=20, dt=1e-6, mode="uniaxial_xx", max_strain=1e-4, material=None,
+                    damage=None, tolerances=None, reference=True, **material_entry):
+    """Writes an affine demo case (the schema of examples/affine_bar_elastic.json) and returns
+    its path; material None uses the elastic fixture, a dict is written as the material file."""
+    os.makedirs(directory, exist_ok=True)
+    if material is None:
+        mat_path = ELASTIC_FILE
+    else:
+        mat_path = os.path.join(directory, name + "_material.json")
+        with open(mat_path, "w") as f:
+            json.dump(material, f)
+    case = {
+        "schemaVersion": "1.0", "name": name, "type": "vumat_pd_demo_case",
+        "scope": "test affine case",
+        "geometry": {"type": "patch", "nx": nx, "ny": ny, "nz": nz, "spacing": spacing,
+                     "horizon": horizon},
+        "time_integration": {"steps": steps, "dt": dt, "totalTime": steps * dt},
+        "boundary_conditions": {"type": "affine_displacement", "mode": mode,
+                                "maxStrain": max_strain},
+        "tolerances": tolerances or {"deformationGradientAbsolute": 1e-12,
+                                     "referenceStressAbsolute": 0.002,
+                                     "netForceAbsolute": 1e-7},
+        "output": {"directory": os.path.join(directory, name + "_out")},
+        "materials": [dict({"name": "M", "materialConfigPath": mat_path,
+                            "kinematicsMode": "corotational", "reference": reference,
+                            "elasticE": 210000.0, "elasticNu": 0.3,
+                            "referenceTemperature": 293.0}, **material_entry)],
+        "checks": ["affine deformation-gradient recovery", "global force balance"],
+        "damage": damage or {"model": "none", "initiation": 1e30, "failure": 1e30,
+                             "deletionEnabled": False},
+    }
+    path = os.path.join(directory, name + ".json")
+    with open(path, "w") as f:
+        json.dump(case, f, indent=1)
+    return path
+
+
 def output_dir(case_path):
     with open(case_path) as f:
         return json.load(f)["output"]["directory"]

@@ -125,3 +125,46 @@ def test_select_boundary_region_errors():
         vpd.select_boundary_region(x, "front", 1.0)
     with pytest.raises(vpd.VpdError):
         vpd.select_boundary_region(x, "left", 0.0)
+
+
+DEMO_BAD = [
+    ("mode", lambda c: c["boundary_conditions"].update(mode="biaxial"),
+     "Unsupported boundary_conditions type or mode."),
+    ("bc type", lambda c: c["boundary_conditions"].update(type="regional_smooth_displacement"),
+     "Unsupported boundary_conditions type or mode."),
+    ("kinematics", lambda c: c["materials"][0].update(kinematicsMode="legacy_small_strain"),
+     "Unsupported kinematicsMode for material: M"),
+    ("no materials", lambda c: c.update(materials=[]),
+     "Demo case requires at least one material."),
+    ("zero strain", lambda c: c["boundary_conditions"].update(maxStrain=0.0),
+     "Affine displacement boundary condition requires maxStrain."),
+]
+
+
+@pytest.mark.parametrize("name,fn,message", DEMO_BAD, ids=[b[0] for b in DEMO_BAD])
+def test_rejected_demo_cases_match_reference(tmp_path, capfd, orc, name, fn, message):
+    """Affine demo cases (src/host/DemoCaseRunner.cpp) end in the reference's messages; a zero
+    maxStrain fails after the output files and their headers exist, as there."""
+    import vpd
+    path = cf.write_demo_case(str(tmp_path))
+    mutate(path, fn)
+    ref = orc if orc.ref_available() else None
+    rc, err = run_both(vpd, path, capfd, ref)
+    assert rc == 1
+    assert err == "VUMAT-PD demo case failed: " + message
+    if name == "zero strain":
+        out = cf.output_dir(path)
+        header = open(os.path.join(out, "timeseries.csv")).readline()
+        assert header.startswith("material,step,time,applied_strain,max_f_error")
+        assert open(os.path.join(out, "field_final.csv")).readline().startswith(
+            "material,particle,x0,y0,z0")
+
+
+def test_material_point_cases_are_refused(tmp_path, capfd):
+    import vpd
+    path = cf.write_demo_case(str(tmp_path))
+    mutate(path, lambda c: c.update(type="vumat_material_point_case"))
+    rc = vpd.run_case_json(path, "")
+    err = capfd.readouterr().err.strip()
+    assert rc == 1 and "vumat_material_point_case" in err
+

@@ -69,7 +69,9 @@ struct vpd_context {
   std::vector<double> props, initialState;
   int nstatev = 0, damageMode = 0, returnMap = 0;
   double initiation = 0, failure = 1;
+  int materialModel = VPD_MATERIAL_JOHNSON_COOK;
   JcParams jc{};
+  ElasticParams elastic{};
   // block
   int64_t n = 0, nnz = 0;
   double *x = nullptr, *vol = nullptr;
@@ -573,6 +575,12 @@ int ensure_list(vpd_context *c) {
 
 template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
   const int grid = grid8_of(c->n);
+  if (c->materialModel == VPD_MATERIAL_ELASTIC) {
+    if (DIAG)
+      return fail(c, VPD_ERR_UNSUPPORTED, "Run diagnostics need the Johnson-Cook material.");
+    return timed(c, K_MATERIAL,
+                 [&] { k_material<RM_ELASTIC, false, false><<<grid, 256, 0, c->stream>>>(g); });
+  }
   switch (c->returnMap) {
   case RM_BISECTION:
     return timed(c, K_MATERIAL,
@@ -648,6 +656,7 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
   g.fallbackOut = fields ? c->fallbacks : nullptr;
   g.seed = nullptr;
   g.jc = c->jc;
+  g.elastic = c->elastic;
   g.dt = dt;
   g.rtol = c->rtol;
   g.err = c->derr;
@@ -789,8 +798,14 @@ int vpd_create(const vpd_config *cfg, vpd_context **out) {
     return fail(nullptr, VPD_ERR_ARGUMENT, "VUMAT-PD material bridge requires charLength > 0.");
   if (material && (cfg->nprops <= 0 || !cfg->props))
     return fail(nullptr, VPD_ERR_ARGUMENT, "VUMAT props cannot be empty.");
-  if (material && (cfg->nprops < 14 || cfg->nstatev < 6))
+  if (cfg->material_model != VPD_MATERIAL_JOHNSON_COOK &&
+      cfg->material_model != VPD_MATERIAL_ELASTIC)
+    return fail(nullptr, VPD_ERR_UNSUPPORTED, "Unsupported material model.");
+  const bool elastic = cfg->material_model == VPD_MATERIAL_ELASTIC;
+  if (material && !elastic && (cfg->nprops < 14 || cfg->nstatev < 6))
     return fail(nullptr, VPD_ERR_ARGUMENT, "JohnsonCook VUMAT requires 14 props and 6 states");
+  if (material && elastic && (cfg->nprops < 2 || cfg->nstatev < 0))
+    return fail(nullptr, VPD_ERR_ARGUMENT, "The elastic VUMAT requires 2 props (E, nu).");
   if (cfg->damage_mode != VPD_DAMAGE_NONE && cfg->damage_mode != VPD_DAMAGE_CRITICAL_STRETCH)
     return fail(nullptr, VPD_ERR_UNSUPPORTED, "Unsupported damage mode.");
   if (cfg->damage_mode == VPD_DAMAGE_CRITICAL_STRETCH && !(cfg->damage_initiation > 0.0))
@@ -816,7 +831,10 @@ int vpd_create(const vpd_config *cfg, vpd_context **out) {
   c->initiation = cfg->damage_initiation;
   c->failure = cfg->damage_failure;
   c->returnMap = cfg->return_map;
-  if (material)
+  c->materialModel = cfg->material_model;
+  if (material && elastic)
+    c->elastic = make_elastic(c->props[0], c->props[1]);
+  else if (material)
     c->jc = make_jc(c->props);
   if (hipGetDevice(&c->device) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1756,6 +1774,8 @@ int vpd_enable_diagnostics(vpd_context *c, const vpd_diag_config *cfg) {
   if (c->batch.id)
     return fail(c, VPD_ERR_UNSUPPORTED,
                 "Run diagnostics cover a block that is one specimen, not batched specimens.");
+  if (c->materialModel != VPD_MATERIAL_JOHNSON_COOK)
+    return fail(c, VPD_ERR_UNSUPPORTED, "Run diagnostics need the Johnson-Cook material.");
   const int ax = cfg->load_axis, lat = cfg->lateral_axis;
   if (ax < 0 || ax > 2 || lat < 0 || lat > 2)
     return fail(c, VPD_ERR_ARGUMENT, "Diagnostics axes must lie in [0, 2].");

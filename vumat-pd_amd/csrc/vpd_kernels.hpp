@@ -630,6 +630,7 @@ struct MaterialArgs {
   int *iterOut, *fallbackOut;
   const double *seed;
   JcParams jc;
+  ElasticParams elastic;
   double dt, rtol;
   int *err;
   DiagArgs diag;
@@ -639,7 +640,7 @@ struct MaterialArgs {
 // The stages after the return map, shared by the fused and the split material kernels:
 // history commit, σ = R σ̄ Rᵀ, optional field outputs, the run diagnostics' material terms
 // (DIAG: into dv, from the old stress sOld and the strain increment dE), A = P K⁻¹.
-template <bool DIAG>
+template <bool DIAG, bool JC_STATE = true>
 __device__ __forceinline__ void material_finish(const MaterialArgs &g, int64_t i, const T3 &f1,
                                                 const T3 &r1, const double sNew[6],
                                                 const double st[6], double eint, double einel,
@@ -650,11 +651,14 @@ __device__ __forceinline__ void material_finish(const MaterialArgs &g, int64_t i
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     g.sig(c, i) = sNew[c];
-    g.statev(c, i) = st[c];
+    if (JC_STATE)
+      g.statev(c, i) = st[c];
   }
-  g.temp[i] = st[2];
-  g.eint[i] = eint;
-  g.einel[i] = einel;
+  if (JC_STATE) { // the elastic VUMAT leaves state, temperature and energies as they were
+    g.temp[i] = st[2];
+    g.eint[i] = eint;
+    g.einel[i] = einel;
+  }
   if (g.iterOut)
     g.iterOut[i] = stats.iterations;
   if (g.fallbackOut)
@@ -744,14 +748,25 @@ __device__ __forceinline__ void material_one(const MaterialArgs &g, int64_t i, d
   }
   double sOld[6], sNew[6], st[6];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
+  for (int c = 0; c < 6; ++c)
     sOld[c] = g.sig(c, i);
-    st[c] = g.statev(c, i);
+  if constexpr (MODE == RM_ELASTIC) {
+    // (no diagnostics terms: the host refuses run diagnostics with this material)
+    elastic_point(g.elastic, inc, sOld, sNew);
+    ReturnMapStats none{};
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      st[c] = 0.0;
+    material_finish<false, false>(g, i, f1, k.R1, sNew, st, 0.0, 0.0, none, sOld, dE, dv);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+      st[c] = g.statev(c, i);
+    double eint = g.eint[i], einel = g.einel[i];
+    ReturnMapStats stats;
+    jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
+    material_finish<DIAG>(g, i, f1, k.R1, sNew, st, eint, einel, stats, sOld, dE, dv);
   }
-  double eint = g.eint[i], einel = g.einel[i];
-  ReturnMapStats stats;
-  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
-  material_finish<DIAG>(g, i, f1, k.R1, sNew, st, eint, einel, stats, sOld, dE, dv);
 }
 
 // Material point update, fused: VumatPdCorrespondenceMaterial.cpp:110-173 for every point at

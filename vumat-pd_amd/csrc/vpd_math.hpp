@@ -304,6 +304,36 @@ __device__ __forceinline__ double jc_yield(const JcParams &p, double ep, double 
 //                 |r| <= rtol·q_trial.
 //   RM_LEARNED    RM_NEWTON seeded with an external Δε̄ᵖ estimate (the learned warm start).
 enum ReturnMap { RM_BISECTION = 0, RM_BISECTION_TIGHT = 1, RM_NEWTON = 2, RM_LEARNED = 3 };
+// The material stage's form for the linear elastic VUMAT (not a return-map solver).
+constexpr int RM_ELASTIC = 4;
+
+// The linear isotropic elastic VUMAT of materials/examples/Elastic.f: props E, ν;
+// λ = Eν/((1+ν)(1−2ν)), μ = E/(2(1+ν)); the 6×6 stiffness with λ + 2μ / λ on the normal
+// block and 2μ on the shear diagonal.
+struct ElasticParams {
+  double lam, mu;
+};
+
+inline ElasticParams make_elastic(double e, double nu) {
+  return ElasticParams{e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), e / (2.0 * (1.0 + nu))};
+}
+
+// σ_new(i) = σ_old(i) + Σ_j C(i, j) Δε(j), summed over j = 1..6 in order, zero entries
+// included (Elastic.f:41-47); state, temperature and energies are left unchanged.
+__device__ __forceinline__ void elastic_point(const ElasticParams &p, const double inc[6],
+                                              const double sOld[6], double sNew[6]) {
+  const double diag = p.lam + 2.0 * p.mu, shear = 2.0 * p.mu;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double s = sOld[i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double cij = (i < 3 && j < 3) ? (i == j ? diag : p.lam) : (i == j ? shear : 0.0);
+      s = s + cij * inc[j];
+    }
+    sNew[i] = s;
+  }
+}
 
 struct ReturnMapStats {
   int iterations = 0;

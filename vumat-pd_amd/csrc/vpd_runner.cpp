@@ -1,9 +1,13 @@
-// The JSON boundary-value case runner on the GPU: reads the reference's case and material
-// files (src/config/DemoConfig.cpp, src/config/MaterialConfigJson.cpp), runs every material
-// of the case through vpd_step with the fused run diagnostics, and writes the reference's
-// outputs (src/host/BoundaryValueCaseRunner.cpp). Only host code; every particle-sized
-// computation runs on the device, and per step only one 16-double diagnostics record
-// crosses to the host (in batches).
+// The JSON case runner on the GPU: reads the reference's case and material files
+// (src/config/DemoConfig.cpp, src/config/MaterialConfigJson.cpp) and writes the reference's
+// outputs.
+// - Boundary-value cases (src/host/BoundaryValueCaseRunner.cpp): every material runs through
+//   vpd_step with the fused run diagnostics; per step only one 16-double diagnostics record
+//   crosses to the host (in batches).
+// - Affine demo cases (src/host/DemoCaseRunner.cpp): the prescribed affine displacement, F,
+//   the material update (Johnson-Cook or the linear elastic VUMAT), damage and the force
+//   density run on the device each step; the small per-step summaries are formed on the host
+//   in the reference's expressions.
 #include "vpd.h"
 #include "vpd_json.hpp"
 
@@ -43,9 +47,10 @@ struct CaseMaterial { // DemoMaterialConfig (include/DemoConfig.hpp:83-96)
   int eqpsIndex = -1;
   std::vector<int> indicatorIndices;
   bool indicatorMin = false, indicatorOneMinus = false;
+  bool reference = false;
 };
 
-struct CaseConfig { // DemoCaseConfig (include/DemoConfig.hpp:98-113), boundary cases only
+struct CaseConfig { // DemoCaseConfig (include/DemoConfig.hpp:98-113)
   std::string name, type, scope, classification;
   std::string unitLength = "mm", unitTime = "ms", unitForce = "N", unitStress = "MPa",
               unitMassDensity = "N*ms^2/mm^4";
@@ -55,7 +60,8 @@ struct CaseConfig { // DemoCaseConfig (include/DemoConfig.hpp:98-113), boundary 
   int steps = 1;
   double dt = 1.0e-6, totalTime = 0.0;
   std::string bcType = "affine_displacement", supportRegion = "left", loadRegion = "right",
-              component = "x";
+              component = "x", bcMode = "uniaxial_xx";
+  double maxStrain = 0.0;
   double amplitude = 0.0, riseTime = 0.0;
   int regionLayers = 1;
   bool planeStrain = false, rigidBodySuppression = false;
@@ -67,6 +73,9 @@ struct CaseConfig { // DemoCaseConfig (include/DemoConfig.hpp:98-113), boundary 
   std::string outputDir;
   double axialMin = std::numeric_limits<double>::quiet_NaN();
   double axialMax = std::numeric_limits<double>::quiet_NaN();
+  // ValidationToleranceConfig (include/DemoConfig.hpp:72-76) and the checks list.
+  double tolDefgrad = 1.0e-12, tolRefStress = 1.0e-6, tolNetForce = 1.0e-7;
+  std::vector<std::string> checks;
   std::vector<CaseMaterial> materials;
 };
 
@@ -147,7 +156,7 @@ CaseMaterial load_case_material(const Value &e, const std::string &pluginDir) {
   if (m.material.stressFrame != "corotational")
     throw std::runtime_error(
         "JSON case hosts require corotational stress; use the dedicated spatial demo.");
-  (void)get_bool(e, "reference", false, false);
+  m.reference = get_bool(e, "reference", false, false);
   m.elasticE = get_double(e, "elasticE", false, 0.0);
   m.elasticNu = get_double(e, "elasticNu", false, 0.0);
   m.referenceTemperature = get_double(e, "referenceTemperature", false, m.referenceTemperature);
@@ -242,8 +251,8 @@ CaseConfig load_case(const std::string &path, const std::string &pluginDir) {
   c.totalTime = get_double(tsrc, "totalTime", false, c.totalTime);
   if (const Value *bc = find(root, "boundary_conditions"); bc && bc->kind == Value::Object) {
     c.bcType = get_string(*bc, "type", false, c.bcType);
-    (void)get_string(*bc, "mode", false, "");
-    (void)get_double(*bc, "maxStrain", false, 0.0);
+    c.bcMode = get_string(*bc, "mode", false, c.bcMode);
+    c.maxStrain = get_double(*bc, "maxStrain", false, c.maxStrain);
     c.amplitude = get_double(*bc, "amplitude", false, c.amplitude);
     c.riseTime = get_double(*bc, "riseTime", false, c.riseTime);
     (void)get_bool(*bc, "transverseConstraint", false, false);
@@ -284,6 +293,15 @@ CaseConfig load_case(const std::string &path, const std::string &pluginDir) {
     c.axialMin = get_double(*ob, "axialMin", false, c.axialMin);
     c.axialMax = get_double(*ob, "axialMax", false, c.axialMax);
   }
+  if (const Value *t = find(root, "tolerances"); t && t->kind == Value::Object) {
+    c.tolDefgrad = get_double(*t, "deformationGradientAbsolute", false, c.tolDefgrad);
+    c.tolRefStress = get_double(*t, "referenceStressAbsolute", false, c.tolRefStress);
+    c.tolNetForce = get_double(*t, "netForceAbsolute", false, c.tolNetForce);
+  }
+  if (const Value *ch = find(root, "checks"); ch && ch->kind == Value::Array)
+    for (const Value &item : ch->array)
+      if (item.kind == Value::String)
+        c.checks.push_back(item.string);
   const Value *mats = find(root, "materials");
   if (!mats || mats->kind != Value::Array)
     throw std::runtime_error("Demo case requires a materials array.");
@@ -297,9 +315,6 @@ CaseConfig load_case(const std::string &path, const std::string &pluginDir) {
   if (c.type != "vumat_pd_demo_case" && c.type != "vumat_pd_boundary_case" &&
       c.type != "vumat_material_point_case")
     throw std::runtime_error("Unsupported case type: " + c.type);
-  if (c.type != "vumat_pd_boundary_case")
-    throw std::runtime_error("The GPU case runner runs vumat_pd_boundary_case files; got: " +
-                             c.type);
   if (c.spacing <= 0.0 || c.horizon <= 0.0)
     throw std::runtime_error("PD geometry requires positive spacing and horizon.");
   if (c.geometryType != "patch")
@@ -313,6 +328,15 @@ CaseConfig load_case(const std::string &path, const std::string &pluginDir) {
     throw std::runtime_error("time_integration duration must be finite.");
   if (std::abs(c.totalTime - expected) > 1.0e-12 * std::max(1.0, expected))
     throw std::runtime_error("time_integration.totalTime must equal steps * dt.");
+  if (c.type != "vumat_pd_boundary_case") { // demo and material-point cases
+    if (c.bcType != "affine_displacement" ||
+        (c.bcMode != "uniaxial_xx" && c.bcMode != "simple_shear_xy" &&
+         c.bcMode != "plane_strain_x_uniaxial_stress"))
+      throw std::runtime_error("Unsupported boundary_conditions type or mode.");
+    if (c.materials.empty())
+      throw std::runtime_error("Demo case requires at least one material.");
+    return c;
+  }
   if (c.bcType != "regional_smooth_displacement" || c.amplitude == 0.0 || c.riseTime <= 0.0 ||
       c.regionLayers < 1 || !c.rigidBodySuppression)
     throw std::runtime_error(
@@ -764,6 +788,311 @@ int run_boundary_case(const CaseConfig &cfg) {
   return pass ? 0 : 2;
 }
 
+
+// ---- affine demo cases (src/host/DemoCaseRunner.cpp) -------------------------------------
+
+// The device VUMAT a material file names (its plugin file or cmname).
+int device_material(const CaseMaterial &mat) {
+  const std::string plugin = std::filesystem::path(mat.material.pluginPath).filename().string();
+  std::string cm = mat.material.cmname;
+  std::transform(cm.begin(), cm.end(), cm.begin(), [](unsigned char ch) { return std::toupper(ch); });
+  if (cm.rfind("JOHNSONCOOK", 0) == 0 || plugin.rfind("jc_vumat_plugin", 0) == 0)
+    return VPD_MATERIAL_JOHNSON_COOK;
+  if (cm.rfind("ELASTIC", 0) == 0 || plugin.rfind("elastic_vumat_plugin", 0) == 0)
+    return VPD_MATERIAL_ELASTIC;
+  throw std::runtime_error("Only the Johnson-Cook and elastic VUMATs run on the device; material " +
+                           mat.label + " uses " + plugin);
+}
+
+struct DemoSummary { // MaterialRunSummary (DemoCaseRunner.cpp:36-44)
+  std::string key;
+  bool reference = false;
+  double maxFError = 0.0, maxStressReferenceError = 0.0, maxNetForceNorm = 0.0, maxDamage = 0.0;
+  bool finite = true;
+};
+
+bool all_finite(const double *v, int count) {
+  for (int k = 0; k < count; ++k)
+    if (!std::isfinite(v[k]))
+      return false;
+  return true;
+}
+
+DemoSummary run_demo_material(const CaseConfig &cfg, const CaseMaterial &mat, std::ostream &ts,
+                              std::ostream &field) {
+  const int64_t n = static_cast<int64_t>(cfg.nx) * cfg.ny * cfg.nz;
+  std::vector<double> x(3 * n), vol(n);
+  if (vpd_make_patch(cfg.nx, cfg.ny, cfg.nz, cfg.spacing, x.data(), vol.data()) != VPD_OK)
+    throw std::runtime_error("Particle patch requires positive dimensions and spacing.");
+  const bool shear = cfg.bcMode == "simple_shear_xy";
+  vpd_config vc{};
+  vc.horizon = cfg.horizon;
+  vc.density = 0.0; // no explicit dynamics in the affine host
+  vc.nprops = static_cast<int>(mat.material.props.size());
+  vc.props = mat.material.props.data();
+  vc.nstatev = mat.material.nstatev;
+  vc.initial_state = mat.initialState.empty() ? nullptr : mat.initialState.data();
+  vc.reference_temperature = mat.referenceTemperature;
+  vc.char_length = cfg.spacing;
+  vc.material_density = mat.material.density;
+  vc.material_model = device_material(mat);
+  if (cfg.damageMode == "critical_stretch") {
+    if (cfg.damageInitiation <= 0.0)
+      throw std::runtime_error("critical_stretch requires initiation > 0.");
+    vc.damage_mode = VPD_DAMAGE_CRITICAL_STRETCH;
+  } else if (cfg.damageMode != "none") {
+    throw std::runtime_error("Unsupported damage model on the device: " + cfg.damageMode);
+  }
+  vc.damage_initiation = cfg.damageInitiation;
+  vc.damage_failure = cfg.damageFailure;
+  vpd_context *raw = nullptr;
+  if (vpd_create(&vc, &raw) != VPD_OK)
+    throw std::runtime_error(vpd_last_error(nullptr));
+  std::unique_ptr<vpd_context, ContextDeleter> owner(raw);
+  vpd_context *ctx = raw;
+  check(ctx, vpd_set_particles(ctx, x.data(), vol.data(), n));
+  int64_t nnz = 0;
+  check(ctx, vpd_build_neighbors(ctx, &nnz));
+  check(ctx, vpd_initialize_shape(ctx));
+  check(ctx, vpd_initialize_material(ctx)); // F at zero displacement (DemoCaseRunner.cpp:325)
+
+  // Elastic references (DemoCaseRunner.cpp:51-62) from the case's elasticE and elasticNu.
+  const double lam = mat.elasticE * mat.elasticNu / ((1.0 + mat.elasticNu) * (1.0 - 2.0 * mat.elasticNu));
+  const double mu = mat.elasticE / (2.0 * (1.0 + mat.elasticNu));
+  double minX = std::numeric_limits<double>::max(), maxX = -std::numeric_limits<double>::max();
+  for (int64_t i = 0; i < n; ++i) {
+    minX = std::min(minX, x[3 * i]);
+    maxX = std::max(maxX, x[3 * i]);
+  }
+  const int nst = mat.material.nstatev;
+  std::vector<double> u(3 * n), F(9 * n), sigma(9 * n), strain(9 * n), force(3 * n),
+      dmg(n, 0.0), statev(static_cast<size_t>(std::max(nst, 1)) * n);
+  DemoSummary summary;
+  summary.key = mat.label;
+  summary.reference = mat.reference;
+  const int steps = cfg.steps;
+  const double dt = cfg.dt;
+  for (int step = 1; step <= steps; ++step) {
+    // AffineDisplacementBoundaryCondition::apply / loadValue (BoundaryConditions.cpp:24-40)
+    const double load = cfg.maxStrain * static_cast<double>(step) / static_cast<double>(steps);
+    for (int64_t i = 0; i < n; ++i) {
+      u[3 * i] = load * x[3 * i + (shear ? 1 : 0)];
+      u[3 * i + 1] = 0.0;
+      u[3 * i + 2] = 0.0;
+    }
+    check(ctx, vpd_upload(ctx, VPD_FIELD_DISPLACEMENT, u.data()));
+    check(ctx, vpd_compute_deformation_gradient(ctx));
+    check(ctx, vpd_update_material(ctx, dt, static_cast<double>(step - 1) * dt,
+                                   static_cast<double>(step - 1) * dt));
+    if (vc.damage_mode == VPD_DAMAGE_CRITICAL_STRETCH)
+      check(ctx, vpd_update_damage(ctx));
+    check(ctx, vpd_download(ctx, VPD_FIELD_STRESS, sigma.data()));
+    // Both damage modes leave the stress undegraded: the force state uses the raw stress.
+    check(ctx, vpd_compute_force_density(ctx, sigma.data(), force.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_DEFGRAD, F.data()));
+    if (vc.damage_mode == VPD_DAMAGE_CRITICAL_STRETCH)
+      check(ctx, vpd_download(ctx, VPD_FIELD_DAMAGE_INDEX, dmg.data()));
+    if (nst > 0)
+      check(ctx, vpd_download(ctx, VPD_FIELD_STATEV, statev.data()));
+    const double refF[9] = {shear ? 1.0 : 1.0 + load, shear ? load : 0.0, 0.0, 0.0, 1.0, 0.0,
+                            0.0, 0.0, 1.0};
+    const double refXx = shear ? 0.0 : (lam + 2.0 * mu) * load;
+    const double refXy = shear ? mu * load : 0.0;
+    double maxFError = 0.0;
+    for (int64_t i = 0; i < n; ++i)
+      for (int c = 0; c < 9; ++c)
+        maxFError = std::max(maxFError, std::abs(F[9 * i + c] - refF[c]));
+    double rawSum = 0.0, rawXySum = 0.0, damageSum = 0.0, maxDamage = 0.0, maxRefError = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double *s = &sigma[9 * i];
+      summary.finite = summary.finite && all_finite(s, 9) && all_finite(&force[3 * i], 3) &&
+                       std::isfinite(dmg[i]);
+      rawSum += s[0];
+      const double sxy = 0.5 * (s[1] + s[3]);
+      rawXySum += sxy;
+      damageSum += dmg[i];
+      maxDamage = std::max(maxDamage, dmg[i]);
+      maxRefError = std::max(maxRefError, std::max(std::abs(s[0] - refXx), std::abs(sxy - refXy)));
+    }
+    const double count = static_cast<double>(n);
+    double net[3] = {0.0, 0.0, 0.0}, left = 0.0, right = 0.0; // summarizeForces
+    for (int64_t i = 0; i < n; ++i) {
+      for (int d = 0; d < 3; ++d)
+        net[d] += force[3 * i + d] * vol[i];
+      if (std::abs(x[3 * i] - minX) < 1.0e-12)
+        left += force[3 * i] * vol[i];
+      if (std::abs(x[3 * i] - maxX) < 1.0e-12)
+        right += force[3 * i] * vol[i];
+    }
+    const double netNorm = std::sqrt(net[0] * net[0] + net[1] * net[1] + net[2] * net[2]);
+    double indSum = 0.0, indMax = 0.0;
+    bool stateOk = true;
+    for (int64_t i = 0; i < n; ++i) {
+      const double *st = &statev[static_cast<size_t>(nst) * i];
+      stateOk = stateOk && all_finite(st, nst);
+      const double ind = mat.indicatorIndices.empty() ? 0.0 : std::max(0.0, internal_indicator(mat, st));
+      indSum += ind;
+      indMax = std::max(indMax, ind);
+    }
+    const double indAvg = n > 0 ? indSum / count : 0.0;
+    summary.maxFError = std::max(summary.maxFError, maxFError);
+    summary.maxStressReferenceError = std::max(summary.maxStressReferenceError, maxRefError);
+    summary.maxNetForceNorm = std::max(summary.maxNetForceNorm, netNorm);
+    summary.maxDamage = std::max(summary.maxDamage, maxDamage);
+    summary.finite = summary.finite && all_finite(net, 3) && stateOk && std::isfinite(indAvg) &&
+                     std::isfinite(indMax) && std::isfinite(left) && std::isfinite(right);
+    const double row[] = {step * dt, load, maxFError, rawSum / count, rawSum / count,
+                          rawXySum / count, rawXySum / count, refXx, refXy, maxRefError,
+                          maxDamage, damageSum / count, net[0], net[1], net[2], netNorm, left,
+                          right, indAvg, indMax};
+    ts << mat.label << ',' << step;
+    for (double v : row)
+      ts << ',' << num(v, 6);
+    ts << ',' << (summary.finite ? 1 : 0) << '\n';
+  }
+  check(ctx, vpd_download(ctx, VPD_FIELD_STRAIN, strain.data()));
+  for (int64_t i = 0; i < n; ++i) { // the field_final rows (DemoCaseRunner.cpp:410-426)
+    field << mat.label << ',' << i;
+    for (int d = 0; d < 3; ++d)
+      field << ',' << num(x[3 * i + d], 6);
+    for (int d = 0; d < 3; ++d)
+      field << ',' << num(u[3 * i + d], 6);
+    for (int d = 0; d < 3; ++d)
+      field << ',' << num(x[3 * i + d] + u[3 * i + d], 6);
+    field << ',';
+    for (int c = 0; c < 9; ++c)
+      field << num(F[9 * i + c], 6) << ',';
+    const double ind =
+        mat.indicatorIndices.empty() ? 0.0
+                                     : std::max(0.0, internal_indicator(mat, &statev[static_cast<size_t>(nst) * i]));
+    field << num(strain[9 * i], 6) << ',' << num(sigma[9 * i], 6) << ',' << num(sigma[9 * i], 6)
+          << ',' << num(dmg[i], 6) << ',' << num(ind, 6) << ',' << num(force[3 * i], 6) << ','
+          << num(force[3 * i + 1], 6) << ',' << num(force[3 * i + 2], 6) << '\n';
+  }
+  return summary;
+}
+
+std::string json_escape(const std::string &text) {
+  std::string out;
+  for (char ch : text) {
+    switch (ch) {
+    case '\\':
+      out += "\\\\";
+      break;
+    case '"':
+      out += "\\\"";
+      break;
+    case '\n':
+      out += "\\n";
+      break;
+    case '\r':
+      out += "\\r";
+      break;
+    case '\t':
+      out += "\\t";
+      break;
+    default:
+      out += ch;
+    }
+  }
+  return out;
+}
+
+int run_demo_case(const CaseConfig &cfg) {
+  std::filesystem::create_directories(cfg.outputDir);
+  std::ofstream ts(std::filesystem::path(cfg.outputDir) / "timeseries.csv");
+  ts << "material,step,time,applied_strain,max_f_error,avg_raw_stress_xx,"
+        "avg_effective_stress_xx,avg_raw_stress_xy,avg_effective_stress_xy,"
+        "elastic_reference_stress_xx,elastic_reference_stress_xy,"
+        "max_stress_reference_error,max_damage,avg_damage,net_force_x,"
+        "net_force_y,net_force_z,net_force_norm,left_reaction_x,"
+        "right_reaction_x,avg_internal_indicator,max_internal_indicator,finite\n";
+  std::ofstream field(std::filesystem::path(cfg.outputDir) / "field_final.csv");
+  field << "material,particle,x0,y0,z0,ux,uy,uz,x,y,z,"
+           "Fxx,Fxy,Fxz,Fyx,Fyy,Fyz,Fzx,Fzy,Fzz,"
+           "strain_xx,raw_stress_xx,effective_stress_xx,damage,"
+           "internal_indicator,force_x,force_y,force_z\n";
+  if (cfg.bcType != "affine_displacement")
+    throw std::runtime_error("Unsupported boundary condition type: " + cfg.bcType);
+  if (cfg.maxStrain == 0.0)
+    throw std::runtime_error("Affine displacement boundary condition requires maxStrain.");
+  std::vector<DemoSummary> summaries;
+  for (const auto &m : cfg.materials)
+    summaries.push_back(run_demo_material(cfg, m, ts, field));
+  ts.flush();
+  field.flush();
+
+  // writeMetadata (DemoCaseRunner.cpp:231-293); the stream keeps precision 16 once set.
+  double maxF = 0.0, maxRef = 0.0, maxNet = 0.0, maxDmg = 0.0;
+  bool finite = true;
+  for (const auto &s : summaries) {
+    maxF = std::max(maxF, s.maxFError);
+    if (s.reference)
+      maxRef = std::max(maxRef, s.maxStressReferenceError);
+    maxNet = std::max(maxNet, s.maxNetForceNorm);
+    maxDmg = std::max(maxDmg, s.maxDamage);
+    finite = finite && s.finite;
+  }
+  const bool metaPass = finite && maxF < cfg.tolDefgrad &&
+                        (summaries.empty() || maxRef < cfg.tolRefStress) && maxNet < cfg.tolNetForce;
+  std::ofstream md(std::filesystem::path(cfg.outputDir) / "metadata.json");
+  const auto p16 = [](double v) { return num(v, 16); };
+  md << "{\n";
+  md << "  \"name\": \"" << cfg.name << "\",\n";
+  md << "  \"status\": \"" << (metaPass ? "pass" : "fail") << "\",\n";
+  md << "  \"scope\": \"" << cfg.scope << "\",\n";
+  md << "  \"particles\": " << cfg.nx * cfg.ny * cfg.nz << ",\n";
+  md << "  \"steps\": " << cfg.steps << ",\n";
+  md << "  \"maxDeformationGradientError\": " << p16(maxF) << ",\n";
+  md << "  \"maxReferenceStressError\": " << p16(maxRef) << ",\n";
+  md << "  \"maxJcRawStressError\": " << p16(maxRef) << ",\n";
+  md << "  \"maxNetForceNorm\": " << p16(maxNet) << ",\n";
+  md << "  \"maxDamage\": " << p16(maxDmg) << ",\n";
+  md << "  \"allValuesFinite\": " << (finite ? "true" : "false") << ",\n";
+  md << "  \"tolerances\": {\"deformationGradientAbsolute\": " << p16(cfg.tolDefgrad)
+     << ", \"referenceStressAbsolute\": " << p16(cfg.tolRefStress)
+     << ", \"netForceAbsolute\": " << p16(cfg.tolNetForce) << "},\n";
+  std::string notes = cfg.scope;
+  if (!cfg.checks.empty()) {
+    notes += ". Checks: ";
+    for (size_t i = 0; i < cfg.checks.size(); ++i)
+      notes += (i ? "; " : "") + cfg.checks[i];
+  }
+  md << "  \"notes\": \"" << json_escape(notes) << "\",\n";
+  md << "  \"checks\": [\n";
+  for (size_t i = 0; i < cfg.checks.size(); ++i)
+    md << "    \"" << cfg.checks[i] << "\"" << (i + 1 == cfg.checks.size() ? "\n" : ",\n");
+  md << "  ],\n";
+  md << "  \"materials\": [\n";
+  for (size_t i = 0; i < summaries.size(); ++i) {
+    const auto &s = summaries[i];
+    md << "    {\"name\": \"" << s.key << "\", \"reference\": " << (s.reference ? "true" : "false")
+       << ", \"maxDeformationGradientError\": " << p16(s.maxFError)
+       << ", \"maxStressReferenceError\": " << p16(s.maxStressReferenceError)
+       << ", \"maxNetForceNorm\": " << p16(s.maxNetForceNorm)
+       << ", \"maxDamage\": " << p16(s.maxDamage)
+       << ", \"finite\": " << (s.finite ? "true" : "false") << "}"
+       << (i + 1 == summaries.size() ? "\n" : ",\n");
+  }
+  md << "  ]\n";
+  md << "}\n";
+  md.close();
+
+  bool pass = true;
+  for (const auto &s : summaries) {
+    pass = pass && s.finite && s.maxFError < cfg.tolDefgrad && s.maxNetForceNorm < cfg.tolNetForce;
+    if (s.reference)
+      pass = pass && s.maxStressReferenceError < cfg.tolRefStress;
+    std::cout << s.key << ": max_f_error=" << num(s.maxFError, 6)
+              << " max_stress_reference_error=" << num(s.maxStressReferenceError, 6)
+              << " max_damage=" << num(s.maxDamage, 6)
+              << " max_net_force_norm=" << num(s.maxNetForceNorm, 6)
+              << " finite=" << (s.finite ? "true" : "false") << "\n";
+  }
+  std::cout << "VUMAT-PD demo case written to " << cfg.outputDir << "\n";
+  return pass ? 0 : 2;
+}
+
 } // namespace
 
 extern "C" int vpd_run_case_json(const char *case_path, const char *plugin_directory) {
@@ -771,6 +1100,11 @@ extern "C" int vpd_run_case_json(const char *case_path, const char *plugin_direc
     if (!case_path)
       throw std::runtime_error("Failed to open JSON file: (null)");
     const CaseConfig cfg = load_case(case_path, plugin_directory ? plugin_directory : "");
+    if (cfg.type == "vumat_pd_demo_case")
+      return run_demo_case(cfg);
+    if (cfg.type == "vumat_material_point_case")
+      throw std::runtime_error("The GPU case runner runs vumat_pd_boundary_case and "
+                               "vumat_pd_demo_case files; got: vumat_material_point_case");
     return run_boundary_case(cfg);
   } catch (const std::exception &error) {
     std::cerr << "VUMAT-PD demo case failed: " << error.what() << "\n";

@@ -25,6 +25,7 @@ FIELD = dict(x=0, volume=1, u=2, v=3, a=4, fN=5, vN=6, rN=7, stressN=8, statev=9
              eint=11, einel=12, sigma=13, strain=14, force=15, damage=16, damage_index=17,
              kinv=18, F=19, iterations=20, strain_inc=21, fallbacks=22)
 DAMAGE_NONE, DAMAGE_CRITICAL_STRETCH = 0, 1
+MATERIAL_JOHNSON_COOK, MATERIAL_ELASTIC = 0, 1
 RETURN_MAP_BISECTION, RETURN_MAP_BISECTION_TIGHT, RETURN_MAP_NEWTON, RETURN_MAP_LEARNED = 0, 1, 2, 3
 MLP_PARAMS, MLP_NORM = 8961, 18
 
@@ -39,7 +40,8 @@ class Config(C.Structure):
                 ("initial_state", C.POINTER(C.c_double)), ("reference_temperature", C.c_double),
                 ("char_length", C.c_double), ("material_density", C.c_double),
                 ("damage_mode", C.c_int), ("damage_initiation", C.c_double),
-                ("damage_failure", C.c_double), ("return_map", C.c_int)]
+                ("damage_failure", C.c_double), ("return_map", C.c_int),
+                ("material_model", C.c_int)]
 
 
 class StepArgs(C.Structure):
@@ -237,7 +239,7 @@ class Context:
 
     def __init__(self, horizon, density, props, nstatev=6, initial_state=None,
                  reference_temperature=293.0, char_length=1.0, material_density=7850.0,
-                 damage=None, return_map=RETURN_MAP_BISECTION):
+                 damage=None, return_map=RETURN_MAP_BISECTION, material_model=0):
         self._props = _f64(props)
         self._init = None if initial_state is None else _f64(initial_state)
         cfg = Config()
@@ -255,6 +257,7 @@ class Context:
             cfg.damage_mode = DAMAGE_CRITICAL_STRETCH
             cfg.damage_initiation, cfg.damage_failure = damage
         cfg.return_map = return_map
+        cfg.material_model = material_model
         self.h = C.c_void_p()
         self.n = 0
         self.nnz = 0
