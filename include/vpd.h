@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define VPD_ABI_VERSION 2
+#define VPD_ABI_VERSION 3
 
 enum vpd_status {
   VPD_OK = 0,
@@ -262,9 +262,17 @@ int vpd_attach_warm_start(vpd_context *ctx, vpd_mlp *mlp);
  *  - gather window: 0 gathers neighbour data straight from memory; W in {64, 80, 96, 128,
  *    160} stages runs of W consecutive particles' data in each wavefront's LDS and serves
  *    the bonds that fall in them from there (the LDS reservation is then at least the
- *    windows').
+ *    windows');
+ *  - bonds per window (windowed gathers): the most bonds a particle takes from one window
+ *    before the wavefront moves on, 0 for no limit, at most 1024;
+ *  - window form (windowed gathers): 0 walks the bond lists during the pass; 1 replays a
+ *    window plan (built once per bond structure, window and limit: the windows of every
+ *    wavefront and, per window and particle, where its bonds' neighbours sit), 2 does that
+ *    and stages the next window while summing the current one. Forms 1 and 2 need a window
+ *    of 64, 80, 96 or 128 and 1 to 8 bonds per window; a step with another combination
+ *    fails with VPD_ERR_ARGUMENT.
  * The sum order, and so every result bit, is the same for every setting. vpd_autotune times
- * both passes over the grid of the three on the context's own block and keeps the fastest;
+ * both passes over the grid of these on the context's own block and keeps the fastest;
  * tuned[] (VPD_TUNE_COUNT ints) receives the values indexed by enum vpd_tuning. */
 enum vpd_tuning {
   VPD_TUNE_DEFGRAD_LDS = 0,
@@ -273,7 +281,11 @@ enum vpd_tuning {
   VPD_TUNE_FORCE_ROUNDS = 3,
   VPD_TUNE_DEFGRAD_WINDOW = 4,
   VPD_TUNE_FORCE_WINDOW = 5,
-  VPD_TUNE_COUNT = 6
+  VPD_TUNE_DEFGRAD_WINDOW_BONDS = 6,
+  VPD_TUNE_FORCE_WINDOW_BONDS = 7,
+  VPD_TUNE_DEFGRAD_WINDOW_PLAN = 8,
+  VPD_TUNE_FORCE_WINDOW_PLAN = 9,
+  VPD_TUNE_COUNT = 10
 };
 int vpd_set_tuning(vpd_context *ctx, int knob, int value);
 int vpd_get_tuning(vpd_context *ctx, int knob, int *value);

@@ -24,7 +24,7 @@ def test_library_exports_header(orc):
     assert len(names) >= 25
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.vpd_abi_version() == 2
+    assert lib.vpd_abi_version() == 3
 
 
 @pytest.mark.parametrize("dims", [(30, 8, 8, 1.0), (11, 5, 3, 0.2), (2, 2, 2, 1.0)])

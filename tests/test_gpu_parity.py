@@ -273,6 +273,14 @@ def test_fused_equals_unfused(vpd, orc):
     dict(defgrad_window=64, force_window=80),
     dict(defgrad_window=128, force_window=96, defgrad_rounds=2, force_rounds=3,
          force_lds=64000),
+    dict(defgrad_window=128, force_window=160, defgrad_window_bonds=8, force_window_bonds=7),
+    dict(defgrad_window=64, force_window=80, defgrad_window_bonds=1, force_window_bonds=3),
+    dict(defgrad_window=128, force_window=80, defgrad_window_bonds=6, force_window_bonds=6,
+         defgrad_window_plan=1, force_window_plan=1),
+    dict(defgrad_window=64, force_window=96, defgrad_window_bonds=8, force_window_bonds=5,
+         defgrad_window_plan=2, force_window_plan=2, defgrad_rounds=3, force_rounds=2),
+    dict(defgrad_window=80, force_window=128, defgrad_window_bonds=1, force_window_bonds=2,
+         defgrad_window_plan=2, force_window_plan=1, force_lds=64000),
 ])
 def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
     # Launch knobs change how the bond passes are scheduled, never what they sum or in which
@@ -291,17 +299,21 @@ def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
         assert np.array_equal(a.get(name), b.get(name)), name
 
 
-@pytest.mark.parametrize("voids", [0.0, 0.08])
-def test_windowed_gathers_on_irregular_rows(vpd, voids):
+@pytest.mark.parametrize("voids,take,plan", [(0.0, 0, 0), (0.08, 0, 0), (0.08, 5, 0), (0.0, 2, 0),
+                                            (0.08, 6, 1), (0.08, 7, 2), (0.0, 8, 2)])
+def test_windowed_gathers_on_irregular_rows(vpd, voids, take, plan):
     # Voids make every row irregular: the windows then cover fewer bonds per step, and the
-    # results must still equal the direct gathers bit for bit.
-    banner("windowed gathers, void fraction %g" % voids)
+    # results must still equal the direct gathers bit for bit, with or without a limit on the
+    # bonds a particle takes from one window, walked or replayed from a window plan.
+    banner("windowed gathers, void fraction %g, bonds per window %d, plan form %d"
+           % (voids, take, plan))
     points = vpd.voided_patch(24, 9, 7, 1.0, voids, 5)
     kw = dict(nx=24, ny=9, nz=7, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
               density=7.85e-3, velocity=300.0, damage=(0.002, 0.0), points=points)
     a = vpd.Bar(**kw)
     b = vpd.Bar(**kw)
-    b.ctx.set_tuning(defgrad_window=64, force_window=64)
+    b.ctx.set_tuning(defgrad_window=64, force_window=64, defgrad_window_bonds=take,
+                     force_window_bonds=take, defgrad_window_plan=plan, force_window_plan=plan)
     a.step(50, want_fields=True)
     b.step(50, want_fields=True)
     assert a.get("damage").sum() > 0
@@ -317,15 +329,39 @@ def test_window_knob_limits(vpd):
             b.ctx.set_tuning(force_window=bad)
     b.ctx.set_tuning(force_window=0, defgrad_window=160)
     assert b.ctx.get_tuning()["defgrad_window"] == 160
+    for bad in (-1, 1025):
+        with pytest.raises(vpd.VpdError, match="Bonds per lane and window"):
+            b.ctx.set_tuning(force_window_bonds=bad)
+    b.ctx.set_tuning(force_window_bonds=1024, defgrad_window_bonds=6)
+    t = b.ctx.get_tuning()
+    assert (t["force_window_bonds"], t["defgrad_window_bonds"]) == (1024, 6)
+    for bad in (-1, 3):
+        with pytest.raises(vpd.VpdError, match="Window form"):
+            b.ctx.set_tuning(force_window_plan=bad)
+    # A planned form needs a planned window size and 1..8 bonds per window: the step refuses
+    # other combinations, and runs once they are valid.
+    for knobs in (dict(force_window=160, force_window_bonds=8), dict(force_window=80,
+                                                                       force_window_bonds=0),
+                  dict(force_window=80, force_window_bonds=9)):
+        b.ctx.set_tuning(force_window_plan=2, **knobs)
+        with pytest.raises(vpd.VpdError, match="Planned windows need"):
+            b.step(1)
+    b.ctx.set_tuning(force_window=80, force_window_bonds=8, force_window_plan=2)
+    b.step(1)
+    assert b.ctx.get_tuning()["force_window_plan"] == 2
 
 
-def test_full_size_windowed_equals_direct(vpd):
-    banner("500k bar: windowed gathers == direct, 12 steps with damage")
+@pytest.mark.parametrize("take,plan", [(0, 0), (8, 0), (6, 1), (7, 2)])
+def test_full_size_windowed_equals_direct(vpd, take, plan):
+    banner("500k bar: windowed gathers (bonds per window %d, plan form %d) == direct, 12 steps "
+           "with damage" % (take, plan))
     kw = dict(nx=200, ny=50, nz=50, spacing=0.2, horizon=0.603, props=JC_PROPS, dt=1.7075e-5,
               density=7.85e-3, velocity=60.0, damage=(0.0005, 0.0))
     a = vpd.Bar(**kw)
     b = vpd.Bar(**kw)
-    b.ctx.set_tuning(defgrad_window=80, force_window=80, defgrad_rounds=4, force_rounds=4)
+    b.ctx.set_tuning(defgrad_window=80 if take == 0 else 128, force_window=80 if take == 0 else 128,
+                     defgrad_rounds=4, force_rounds=4, defgrad_window_bonds=take,
+                     force_window_bonds=take, defgrad_window_plan=plan, force_window_plan=plan)
     a.step(12)
     b.step(12)
     assert a.get("damage").sum() > 0

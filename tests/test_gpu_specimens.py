@@ -65,14 +65,19 @@ def assert_same(batch, s, bar, what):
         assert np.array_equal(got, want), (what, name, np.abs(got - want).max())
 
 
-@pytest.mark.parametrize("return_map,windows", [(0, 0), (2, 0), (0, 80)],
-                         ids=["bisection", "newton", "bisection-windowed"])
+@pytest.mark.parametrize("return_map,windows", [(0, 0), (2, 0), (0, 80), (2, 128), (0, 96)],
+                         ids=["bisection", "newton", "bisection-windowed",
+                              "newton-windowed-limited", "bisection-planned-prefetch"])
 def test_batch_equals_specimens_alone(vpd, return_map, windows):
     banner("batched specimens vs each alone, return map %d, windows %d" % (return_map, windows))
     damage = (0.003, 0.0)
     sps = specimens(vpd)
     batch = vpd.SpecimenBatch(sps, JC_PROPS, RHO, damage=damage, return_map=return_map)
-    batch.ctx.set_tuning(defgrad_window=windows, force_window=windows)
+    take = {128: 7, 96: 6}.get(windows, 0)
+    batch.ctx.set_tuning(defgrad_window=windows, force_window=windows,
+                         defgrad_window_bonds=take, force_window_bonds=take,
+                         defgrad_window_plan=2 if windows == 96 else 0,
+                         force_window_plan=2 if windows == 96 else 0)
     assert batch.n == sum(sp.n for sp in sps)
     batch.step()
     assert batch.step_index == max(sp.steps for sp in sps)
@@ -158,16 +163,22 @@ def test_batch_errors(vpd):
     assert np.array_equal(u_batch, ref.get("u"))
 
 
-@pytest.mark.parametrize("return_map", [0, 2, 3], ids=["bisection", "newton", "learned"])
-def test_graph_replay_equals_eager(vpd, return_map):
+@pytest.mark.parametrize("return_map,planned", [(0, False), (2, False), (3, False), (0, True)],
+                         ids=["bisection", "newton", "learned", "bisection-planned"])
+def test_graph_replay_equals_eager(vpd, return_map, planned):
     """Every replay of the captured step against eager steps, with eager steps and host
-    synchronisations between the replays."""
-    banner("batched specimens: graph-replayed steps vs eager, return map %d" % return_map)
+    synchronisations between the replays (the graph replays with planned windows in the
+    last case, the eager steps gather directly)."""
+    banner("batched specimens: graph-replayed steps vs eager, return map %d, planned %s"
+           % (return_map, planned))
     sps = specimens(vpd)[:4]
     net = vpd.load_warm_start() if return_map == vpd.RETURN_MAP_LEARNED else None
     kw = dict(damage=(0.003, 0.0), return_map=return_map, warm_start=net)
     eager = vpd.SpecimenBatch(sps, JC_PROPS, RHO, **kw)
     graph = vpd.SpecimenBatch(sps, JC_PROPS, RHO, **kw)
+    if planned:
+        graph.ctx.set_tuning(defgrad_window=80, force_window=64, defgrad_window_bonds=7,
+                             force_window_bonds=8, defgrad_window_plan=2, force_window_plan=1)
     # Segments: replayed by the graph (True) or stepped eagerly (False), each ending with an
     # unfused eager step on both sides so that the states can be compared.
     for replay, k in ((True, 9), (False, 4), (True, 1), (True, 12), (False, 2), (True, 30)):

@@ -103,8 +103,23 @@ struct vpd_context {
   // Sweeps per bond pass (see Rounds in vpd_kernels.hpp). Chosen by vpd_autotune.
   int roundsDefgrad = 1, roundsForce = 1;
   // Window of the windowed gathers per bond pass (particles staged per window; 0: direct
-  // gathers). Chosen by vpd_autotune.
+  // gathers) and the most bonds a lane takes from one window (0: no limit). Chosen by
+  // vpd_autotune.
   int winDefgrad = 0, winForce = 0;
+  int takeDefgrad = 0, takeForce = 0;
+  // Form of the windowed gathers per bond pass: 0 walks the bond lists, 1 replays a window
+  // plan, 2 replays it with the next window prefetched (WindowPlan in vpd_kernels.hpp).
+  // Chosen by vpd_autotune.
+  int modeDefgrad = 0, modeForce = 0;
+  // The window plan of each bond pass, built for the pass's window and bond limit when first
+  // needed after the bond lists or those knobs change.
+  struct PlanBuffers {
+    int *first = nullptr, *start = nullptr;
+    uint64_t *rec = nullptr;
+    int64_t capacity = 0; // windows `start` and `rec` hold
+    int window = 0, take = 0;
+    bool valid = false;
+  } plan[2];
   // Return map of the material stage; the learned mode adds the network, the strain
   // increments it reads and the seeds it writes.
   double rtol = 1.0e-12;
@@ -443,6 +458,10 @@ int ensure_outputs(vpd_context *c) {
 // longest row; bond flags are indexed by sliced slot.
 int build_sell(vpd_context *c) {
   const int64_t n = c->n;
+  for (auto &p : c->plan) { // plans follow the bond lists (and the particle count)
+    dfree(p.first);
+    p.valid = false;
+  }
   c->hostRowptr.resize(n + 1);
   HIPTRY(c, hipMemcpyAsync(c->hostRowptr.data(), c->rowptr, sizeof(int) * (n + 1),
                            hipMemcpyDeviceToHost, c->stream));
@@ -527,17 +546,156 @@ template <class F> void with_window(int window, F &&f) {
 int launch_lds(int cap, int pairs, int window) {
   return window > 0 ? std::max(cap, window_lds(pairs, window)) : cap;
 }
+// Dynamic LDS of a prefetching planned pass: its two window buffers are static, so only the
+// rest of the reservation (an occupancy cap above them) is asked for.
+int prefetch_lds(int cap, int pairs, int window) {
+  return std::max(0, cap - 2 * window_lds(pairs, window));
+}
+// The kernels' per-window bond limit for a tuning value (0: none).
+int take_arg(int take) { return take > 0 ? take : INT_MAX; }
+constexpr int kMaxTake = 1024;
+
+// Planned windows (WindowPlan): window sizes with planned kernels, and a compile-time window.
+constexpr int kPlannedWindows[] = {64, 80, 96, 128};
+bool planned_supported(int window, int take) {
+  if (take < 1 || take > kPlanTake)
+    return false;
+  for (int w : kPlannedWindows)
+    if (w == window)
+      return true;
+  return false;
+}
+template <class F> void with_planned_window(int window, F &&f) {
+  switch (window) {
+  case 64:
+    f(std::integral_constant<int, 64>{});
+    break;
+  case 80:
+    f(std::integral_constant<int, 80>{});
+    break;
+  case 96:
+    f(std::integral_constant<int, 96>{});
+    break;
+  default:
+    f(std::integral_constant<int, 128>{});
+    break;
+  }
+}
+
+// How a bond pass gathers its neighbour data: window 0 (direct) or W, the bond limit per
+// window, the form (0 cursor walk, 1 planned, 2 planned with prefetch) and the plan.
+struct Gather {
+  int window = 0, take = 0, mode = 0;
+  WindowPlan plan{};
+};
+
+// Builds the window plan of bond pass `pass` (0 F, 1 force) for (window, take), unless it is
+// already there: count the windows of every wavefront, lay them out, fill them.
+int build_plan(vpd_context *c, int pass, int window, int take) {
+  auto &p = c->plan[pass];
+  if (p.valid && p.first && p.window == window && p.take == take)
+    return VPD_OK;
+  if (!planned_supported(window, take))
+    return fail(c, VPD_ERR_ARGUMENT, "Planned windows need a window of 64, 80, 96 or 128 "
+                                     "particles and 1 to 8 bonds per window.");
+  p.valid = false;
+  const int64_t n = c->n, waves = ceil_div(n, 64);
+  int *count = nullptr;
+  if (dalloc(c, count, waves) || (!p.first && dalloc(c, p.first, waves + 1)))
+    return VPD_ERR_DEVICE;
+  with_planned_window(window, [&](auto w) {
+    k_window_plan<decltype(w)::value, false><<<grid_of(n), 256, 0, c->stream>>>(
+        n, sell_of(c), take, count, nullptr, nullptr, nullptr);
+  });
+  HIPTRY(c, hipGetLastError());
+  std::vector<int> hostCount(waves), first(waves + 1, 0);
+  HIPTRY(c, hipMemcpyAsync(hostCount.data(), count, sizeof(int) * waves, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  dfree(count);
+  int64_t total = 0;
+  for (int64_t w = 0; w < waves; ++w) {
+    first[w] = (int)total;
+    total += hostCount[w];
+    if (total >= INT_MAX)
+      return fail(c, VPD_ERR_ARGUMENT, "Too many windows for a window plan.");
+  }
+  first[waves] = (int)total;
+  if (total > p.capacity) {
+    if (dalloc(c, p.start, total) || dalloc(c, p.rec, total * 64))
+      return VPD_ERR_DEVICE;
+    p.capacity = total;
+  }
+  HIPTRY(c, hipMemcpyAsync(p.first, first.data(), sizeof(int) * (waves + 1),
+                           hipMemcpyHostToDevice, c->stream));
+  with_planned_window(window, [&](auto w) {
+    k_window_plan<decltype(w)::value, true><<<grid_of(n), 256, 0, c->stream>>>(
+        n, sell_of(c), take, nullptr, p.first, p.start, p.rec);
+  });
+  HIPTRY(c, hipGetLastError());
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  p.window = window;
+  p.take = take;
+  p.valid = true;
+  return VPD_OK;
+}
+
+WindowPlan plan_of(const vpd_context *c, int pass) {
+  const auto &p = c->plan[pass];
+  return WindowPlan{p.first, p.start, p.rec};
+}
+
+// The gather of bond pass `pass` as the knobs set it.
+Gather gather_of(const vpd_context *c, int pass) {
+  Gather gx;
+  gx.window = pass == 0 ? c->winDefgrad : c->winForce;
+  gx.take = pass == 0 ? c->takeDefgrad : c->takeForce;
+  gx.mode = gx.window > 0 ? (pass == 0 ? c->modeDefgrad : c->modeForce) : 0;
+  if (gx.mode > 0)
+    gx.plan = plan_of(c, pass);
+  return gx;
+}
+
+// Builds the plans the knobs ask for (outside any graph capture: this allocates and waits).
+int prepare_plans(vpd_context *c) {
+  for (int pass = 0; pass < 2; ++pass) {
+    const Gather gx = gather_of(c, pass);
+    if (gx.mode > 0)
+      if (int s = build_plan(c, pass, gx.window, gx.take))
+        return s;
+  }
+  return VPD_OK;
+}
 
 template <bool DAMAGE>
 void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression,
-                      const Batch &batch, int window) {
+                      const Batch &batch, const Gather &gx) {
+  const int window = gx.window;
+  if (window > 0 && gx.mode > 0) {
+    with_planned_window(window, [&](auto w) {
+      constexpr int W = decltype(w)::value;
+      if (gx.mode == 2)
+        k_defgrad_p<DAMAGE, W, true><<<grid_rounds(rd), 256, prefetch_lds(lds, 4, window),
+                                        c->stream>>>(
+            rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
+            planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation,
+            c->failure, useCompression, batch, gx.plan);
+      else
+        k_defgrad_p<DAMAGE, W, false><<<grid_rounds(rd), 256, launch_lds(lds, 4, window),
+                                         c->stream>>>(
+            rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
+            planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation,
+            c->failure, useCompression, batch, gx.plan);
+    });
+    return;
+  }
   if (window > 0) {
     with_window(window, [&](auto w) {
       k_defgrad_w<DAMAGE, decltype(w)::value>
           <<<grid_rounds(rd), 256, launch_lds(lds, 4, window), c->stream>>>(
               rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
               planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation,
-              c->failure, useCompression, batch);
+              c->failure, useCompression, batch, take_arg(gx.take));
     });
     return;
   }
@@ -548,20 +706,22 @@ void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompress
 }
 
 void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, const Batch &batch,
-                         int window) {
+                         const Gather &gx) {
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
   const Rounds rd = rounds_of(c->n, rounds);
   if (dmg)
-    launch_defgrad_t<true>(c, lds, rd, useCompression, batch, window);
+    launch_defgrad_t<true>(c, lds, rd, useCompression, batch, gx);
   else
-    launch_defgrad_t<false>(c, lds, rd, useCompression, batch, window);
+    launch_defgrad_t<false>(c, lds, rd, useCompression, batch, gx);
 }
 
 // F of the step being taken: with batched specimens, of those that have not finished.
 int launch_defgrad(vpd_context *c) {
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
+  if (int s = prepare_plans(c))
+    return s;
   return timed(c, K_DEFGRAD, [&] {
-    launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad, c->batch, c->winDefgrad);
+    launch_defgrad_with(c, dmg, c->ldsDefgrad, c->roundsDefgrad, c->batch, gather_of(c, 0));
   });
 }
 
@@ -668,27 +828,40 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
 }
 
 template <bool DAMAGE, bool FUSE, bool INTEGRATE, bool DIAG = false>
-void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, int window = 0) {
-  if (!DIAG && window > 0)
+void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, const Gather &gx) {
+  const int window = gx.window;
+  ForceArgs h = g;
+  h.take = take_arg(gx.take);
+  if (!DIAG && window > 0 && gx.mode > 0)
+    with_planned_window(window, [&](auto w) {
+      constexpr int W = decltype(w)::value;
+      if (gx.mode == 2)
+        k_force_p<DAMAGE, FUSE, INTEGRATE, W, true>
+            <<<grid_rounds(h.rd), 256, prefetch_lds(lds, 7, window), c->stream>>>(h, gx.plan);
+      else
+        k_force_p<DAMAGE, FUSE, INTEGRATE, W, false>
+            <<<grid_rounds(h.rd), 256, launch_lds(lds, 7, window), c->stream>>>(h, gx.plan);
+    });
+  else if (!DIAG && window > 0)
     with_window(window, [&](auto w) {
       k_force_w<DAMAGE, FUSE, INTEGRATE, decltype(w)::value>
-          <<<grid_rounds(g.rd), 256, launch_lds(lds, 7, window), c->stream>>>(g);
+          <<<grid_rounds(h.rd), 256, launch_lds(lds, 7, window), c->stream>>>(h);
     });
   else
-    k_force<DAMAGE, FUSE, INTEGRATE, DIAG><<<grid_rounds(g.rd), 256, lds, c->stream>>>(g);
+    k_force<DAMAGE, FUSE, INTEGRATE, DIAG><<<grid_rounds(h.rd), 256, lds, c->stream>>>(h);
 }
 
 template <bool DIAG>
 void launch_force_integrating(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, int lds,
-                              int window) {
+                              const Gather &gx) {
   if (dmg && fuse)
-    launch_force_t<true, true, true, DIAG>(c, g, lds, window);
+    launch_force_t<true, true, true, DIAG>(c, g, lds, gx);
   else if (dmg)
-    launch_force_t<true, false, true, DIAG>(c, g, lds, window);
+    launch_force_t<true, false, true, DIAG>(c, g, lds, gx);
   else if (fuse)
-    launch_force_t<false, true, true, DIAG>(c, g, lds, window);
+    launch_force_t<false, true, true, DIAG>(c, g, lds, gx);
   else
-    launch_force_t<false, false, true, DIAG>(c, g, lds, window);
+    launch_force_t<false, false, true, DIAG>(c, g, lds, gx);
 }
 
 // The force pass in one of its forms: with or without damage flags, the next step's predictor
@@ -696,16 +869,16 @@ void launch_force_integrating(vpd_context *c, const ForceArgs &g, bool dmg, bool
 // diagnostics' force terms (integrating forms only, direct gathers), direct or windowed
 // gathers (window > 0).
 void launch_force_with(vpd_context *c, const ForceArgs &g, bool dmg, bool fuse, bool integrate,
-                       int lds, bool diag = false, int window = 0) {
+                       int lds, bool diag = false, const Gather &gx = Gather{}) {
   if (!integrate) {
     if (dmg)
-      launch_force_t<true, false, false>(c, g, lds, window);
+      launch_force_t<true, false, false>(c, g, lds, gx);
     else
-      launch_force_t<false, false, false>(c, g, lds, window);
+      launch_force_t<false, false, false>(c, g, lds, gx);
   } else if (diag) {
-    launch_force_integrating<true>(c, g, dmg, fuse, lds, 0);
+    launch_force_integrating<true>(c, g, dmg, fuse, lds, Gather{});
   } else {
-    launch_force_integrating<false>(c, g, dmg, fuse, lds, window);
+    launch_force_integrating<false>(c, g, dmg, fuse, lds, gx);
   }
 }
 
@@ -733,8 +906,10 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.rd = rounds_of(c->n, c->roundsForce);
   g.diag = c->diag;
   g.batch = c->batch;
+  if (int s = prepare_plans(c))
+    return s;
   return timed(c, K_FORCE, [&] {
-    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag, c->winForce);
+    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag, gather_of(c, 1));
   });
 }
 
@@ -892,6 +1067,11 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->broken);
   dfree(c->colS);
   dfree(c->sliceBase);
+  for (auto &p : c->plan) {
+    dfree(p.first);
+    dfree(p.start);
+    dfree(p.rec);
+  }
   dfree(c->diagPartials);
   dfree(c->diagRecords);
   dfree(c->diagW);
@@ -1144,8 +1324,11 @@ int vpd_initialize_material(vpd_context *c) {
     return fail(c, VPD_ERR_STATE, "PD correspondence state is not initialized.");
   // F₀ at the current displacement into fN (VumatPdCorrespondenceMaterial.cpp:54).
   // Every particle, whatever step a batched specimen has reached.
+  if (int s = prepare_plans(c))
+    return s;
   if (int s = timed(c, K_DEFGRAD, [&] {
-        launch_defgrad_with(c, false, c->ldsDefgrad, c->roundsDefgrad, Batch{}, c->winDefgrad);
+        launch_defgrad_with(c, false, c->ldsDefgrad, c->roundsDefgrad, Batch{},
+                            gather_of(c, 0));
       }))
     return s;
   HIPTRY(c, hipMemcpyAsync(c->fN, c->fnew, sizeof(double) * 9 * c->n, hipMemcpyDeviceToDevice,
@@ -1405,6 +1588,8 @@ int vpd_step(vpd_context *c, const vpd_step_args *g) {
   if (g->want_fields)
     if (int s = ensure_outputs(c))
       return s;
+  if (int s = prepare_plans(c)) // before the step changes any state
+    return s;
   if (!c->predicted)
     if (int s = launch_predict(c, g->dt, g->ramp))
       return s;
@@ -1506,6 +1691,8 @@ int vpd_step_specimens(vpd_context *c, int32_t step, int fuseNext, int wantField
   if (wantFields)
     if (int s = ensure_outputs(c))
       return s;
+  if (int s = prepare_plans(c)) // before the step changes any state
+    return s;
   c->batch.step = step;
   // Scalar arguments are unused: every particle takes its specimen's dt and velocity band.
   const double none[3] = {0.0, 0.0, 0.0};
@@ -1541,6 +1728,8 @@ int vpd_run_specimens(vpd_context *c, int32_t firstStep, int32_t count) {
     if (int s = ensure_list(c)) // no allocation may happen while the step is captured
       return s;
   }
+  if (int s = prepare_plans(c)) // nor a plan build
+    return s;
   c->batch.step = firstStep;
   const double none[3] = {0.0, 0.0, 0.0};
   if (!c->predicted)
@@ -1868,6 +2057,16 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
       return fail(c, VPD_ERR_ARGUMENT,
                   "Gather window must be 0 (direct) or one of 64, 80, 96, 128, 160 particles.");
     (knob == VPD_TUNE_DEFGRAD_WINDOW ? c->winDefgrad : c->winForce) = value;
+  } else if (knob == VPD_TUNE_DEFGRAD_WINDOW_BONDS || knob == VPD_TUNE_FORCE_WINDOW_BONDS) {
+    if (value < 0 || value > kMaxTake)
+      return fail(c, VPD_ERR_ARGUMENT,
+                  "Bonds per lane and window must lie in [0, 1024] (0: no limit).");
+    (knob == VPD_TUNE_DEFGRAD_WINDOW_BONDS ? c->takeDefgrad : c->takeForce) = value;
+  } else if (knob == VPD_TUNE_DEFGRAD_WINDOW_PLAN || knob == VPD_TUNE_FORCE_WINDOW_PLAN) {
+    if (value < 0 || value > 2)
+      return fail(c, VPD_ERR_ARGUMENT, "Window form must be 0 (walk), 1 (planned) or 2 "
+                                       "(planned with prefetch).");
+    (knob == VPD_TUNE_DEFGRAD_WINDOW_PLAN ? c->modeDefgrad : c->modeForce) = value;
   } else {
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
   }
@@ -1896,15 +2095,28 @@ int vpd_get_tuning(vpd_context *c, int knob, int *value) {
   case VPD_TUNE_FORCE_WINDOW:
     *value = c->winForce;
     return VPD_OK;
+  case VPD_TUNE_DEFGRAD_WINDOW_BONDS:
+    *value = c->takeDefgrad;
+    return VPD_OK;
+  case VPD_TUNE_FORCE_WINDOW_BONDS:
+    *value = c->takeForce;
+    return VPD_OK;
+  case VPD_TUNE_DEFGRAD_WINDOW_PLAN:
+    *value = c->modeDefgrad;
+    return VPD_OK;
+  case VPD_TUNE_FORCE_WINDOW_PLAN:
+    *value = c->modeForce;
+    return VPD_OK;
   default:
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
   }
 }
 
 // Times both bond passes over a grid of (sweeps per pass) × (workgroups per CU) × (gather
-// window) and keeps the fastest for each. The timing runs leave the state as it was: F goes
-// to its scratch plane (with the stretch test when damage is on; the bond flags are restored
-// afterwards), and the force gather runs without the integrator.
+// window), then the windows with per-window bond limits, and keeps the fastest for each. The
+// timing runs leave the state as it was: F goes to its scratch plane (with the stretch test
+// when damage is on; the bond flags are restored afterwards), and the force gather runs
+// without the integrator.
 int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   if (int s = require_ready(c))
     return s;
@@ -1943,10 +2155,47 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   // Gather windows (0: direct gathers); a windowed pass reserves at least the LDS of its
   // windows, so smaller caps repeat the uncapped setting and are skipped.
   const int windowChoices[] = {0, 64, 80, 96, 128, 160};
+  const int takeChoices[] = {5, 6, 7, 8, 10, 12};
   const int pairsOf[2] = {4, 7};
   float best[2] = {1e30f, 1e30f};
-  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1}, pickWin[2] = {0, 0};
-  for (int pass = 0; pass < 2; ++pass)
+  int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1};
+  Gather pick[2];
+  // Mean time of `reps` runs after a warming one; keeps the setting if it is the fastest yet.
+  auto trial = [&](int pass, int lds, int rounds, const Gather &gx) -> int {
+    g.rd = rounds_of(c->n, rounds);
+    float total = 0.f;
+    for (int r = 0; r <= reps; ++r) {
+      HIPTRY(c, hipEventRecord(a, c->stream));
+      if (pass == 0)
+        launch_defgrad_with(c, dmg, lds, rounds, c->batch, gx);
+      else
+        launch_force_with(c, g, dmg, false, false, lds, false, gx);
+      HIPTRY(c, hipGetLastError());
+      HIPTRY(c, hipEventRecord(b, c->stream));
+      HIPTRY(c, hipEventSynchronize(b));
+      float ms = 0.f;
+      HIPTRY(c, hipEventElapsedTime(&ms, a, b));
+      if (r > 0) // the first run warms caches and code
+        total += ms;
+    }
+    if (total < best[pass]) {
+      best[pass] = total;
+      pickLds[pass] = lds;
+      pickRounds[pass] = rounds;
+      pick[pass] = gx;
+    }
+    return VPD_OK;
+  };
+  auto walk = [](int win, int take) {
+    Gather gx;
+    gx.window = win;
+    gx.take = take;
+    return gx;
+  };
+  // Stage 1: sweeps × LDS cap × window (walked, no bond limit per window); stage 2: every
+  // window with each bond limit, and every planned form, at the sweeps and LDS cap stage 1
+  // chose.
+  for (int pass = 0; pass < 2; ++pass) {
     for (int win : windowChoices)
       for (int lds : ldsChoices) {
         if (win > 0 && lds != 0 && lds <= window_lds(pairsOf[pass], win))
@@ -1954,30 +2203,29 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
         for (int rounds : roundChoices) {
           if (rounds > 1 && rounds > ceil_div(c->n, 256))
             continue;
-          g.rd = rounds_of(c->n, rounds);
-          float total = 0.f;
-          for (int r = 0; r <= reps; ++r) {
-            HIPTRY(c, hipEventRecord(a, c->stream));
-            if (pass == 0)
-              launch_defgrad_with(c, dmg, lds, rounds, c->batch, win);
-            else
-              launch_force_with(c, g, dmg, false, false, lds, false, win);
-            HIPTRY(c, hipGetLastError());
-            HIPTRY(c, hipEventRecord(b, c->stream));
-            HIPTRY(c, hipEventSynchronize(b));
-            float ms = 0.f;
-            HIPTRY(c, hipEventElapsedTime(&ms, a, b));
-            if (r > 0) // the first run warms caches and code
-              total += ms;
-          }
-          if (total < best[pass]) {
-            best[pass] = total;
-            pickLds[pass] = lds;
-            pickRounds[pass] = rounds;
-            pickWin[pass] = win;
-          }
+          if (int s = trial(pass, lds, rounds, walk(win, 0)))
+            return s;
         }
       }
+    const int lds = pickLds[pass], rounds = pickRounds[pass];
+    for (int win : windowChoices)
+      for (int take : takeChoices)
+        if (win > 0)
+          if (int s = trial(pass, lds, rounds, walk(win, take)))
+            return s;
+    for (int win : kPlannedWindows)
+      for (int take = 5; take <= kPlanTake; ++take) {
+        if (int s = build_plan(c, pass, win, take))
+          return s;
+        for (int mode = 1; mode <= 2; ++mode) {
+          Gather gx = walk(win, take);
+          gx.mode = mode;
+          gx.plan = plan_of(c, pass);
+          if (int s = trial(pass, lds, rounds, gx))
+            return s;
+        }
+      }
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (savedFlags) {
@@ -1990,9 +2238,15 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   c->roundsDefgrad = pickRounds[0];
   c->ldsForce = pickLds[1];
   c->roundsForce = pickRounds[1];
-  c->winDefgrad = pickWin[0];
-  c->winForce = pickWin[1];
+  c->winDefgrad = pick[0].window;
+  c->winForce = pick[1].window;
+  c->takeDefgrad = pick[0].take;
+  c->takeForce = pick[1].take;
+  c->modeDefgrad = pick[0].mode;
+  c->modeForce = pick[1].mode;
   c->haveFnew = false; // F scratch now holds the tuning runs' values
+  if (int s = prepare_plans(c)) // the plans of the picks (the trials replaced them)
+    return s;
   if (tuned) {
     tuned[VPD_TUNE_DEFGRAD_LDS] = c->ldsDefgrad;
     tuned[VPD_TUNE_FORCE_LDS] = c->ldsForce;
@@ -2000,6 +2254,10 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
     tuned[VPD_TUNE_FORCE_ROUNDS] = c->roundsForce;
     tuned[VPD_TUNE_DEFGRAD_WINDOW] = c->winDefgrad;
     tuned[VPD_TUNE_FORCE_WINDOW] = c->winForce;
+    tuned[VPD_TUNE_DEFGRAD_WINDOW_BONDS] = c->takeDefgrad;
+    tuned[VPD_TUNE_FORCE_WINDOW_BONDS] = c->takeForce;
+    tuned[VPD_TUNE_DEFGRAD_WINDOW_PLAN] = c->modeDefgrad;
+    tuned[VPD_TUNE_FORCE_WINDOW_PLAN] = c->modeForce;
   }
   return check_device(c);
 }

@@ -100,6 +100,16 @@ struct Rounds {
 // where the neighbour data comes from changes. The window never moves back (every lane's
 // next index is at least its end), and the lane holding the smallest index always advances,
 // so the loop ends.
+//
+// A lane may also stop after `take` bonds of a window and finish the rest in a later one
+// (take ≥ 1; the lane holding the window's first index still advances). With rows made of
+// runs of consecutive indices (a lattice numbered line by line) a window can hold one run of
+// some lanes and one and a half of others, and the lanes with fewer bonds idle while the
+// others finish; a cap near one run evens the lanes out at the price of more windows.
+__device__ __forceinline__ int window_stop(int k, int take) {
+  return k > INT_MAX - take ? INT_MAX : k + take;
+}
+
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
@@ -151,13 +161,15 @@ struct ForcePlanes {
 
 // Cursor over one lane's bonds in CSR order. The 16-byte column group (and flag word) of the
 // next 4 bonds is loaded one group ahead, so that its latency is spent on the current group.
-// `next` is the neighbour index of bond k, or INT_MAX when done.
+// Inside a group the columns and flag bits shift down one place per bond, so bond k's column
+// is always `cols.x` and its flag bit 0 of `broken4`. `next` is bond k's column, or INT_MAX
+// when the row is done.
 struct BondCursor {
   int64_t g0;
   int k, len, next;
   i4v cols, colsAhead;
-  uint32_t flags, flagsAhead;
-  uint32_t broken4; // the current group's 4 flag bits, bond k & 3 at bit k & 3
+  uint32_t flagsAhead;
+  uint32_t broken4; // flag bits of bond k and the rest of its group, bond k at bit 0
   __device__ __forceinline__ int64_t group_slot() const { return g0 + 64 * (int64_t)(k & ~3); }
   __device__ __forceinline__ void fetch_ahead(const Sell &b, const uint32_t *broken, bool damage,
                                               int groupStart) {
@@ -173,37 +185,33 @@ struct BondCursor {
     len = i >= 0 ? b.rowptr[i + 1] - b.rowptr[i] : 0;
     g0 = i >= 0 ? sell_pos(b, i, 0) : 0;
     next = INT_MAX;
-    flags = 0u;
     broken4 = 0u;
     if (len > 0) {
       cols = *(const i4v *)(b.col + g0);
-      flags = damage ? broken[g0 >> 5] : 0u;
+      const uint32_t flags = damage ? broken[g0 >> 5] : 0u;
       broken4 = (flags >> (g0 & 31)) & 0xFu;
       next = cols.x;
       fetch_ahead(b, broken, damage, 4);
     }
   }
-  __device__ __forceinline__ bool bond_broken() const { return (broken4 >> (k & 3)) & 1u; }
-  __device__ __forceinline__ uint32_t bit() const { return 1u << ((group_slot() + (k & 3)) & 31); }
-  // cols[q] by selects: a runtime index into the vector would go through scratch memory.
-  __device__ __forceinline__ int column(int q) const {
-    return q == 0 ? cols.x : q == 1 ? cols.y : q == 2 ? cols.z : cols.w;
-  }
+  __device__ __forceinline__ bool bond_broken() const { return broken4 & 1u; }
+  // Bond k's bit in its flag word (a group's 4 slots are consecutive within one word).
+  __device__ __forceinline__ uint32_t bit() const { return 1u << ((g0 + (k & 3)) & 31); }
   // Moves to bond k + 1; entering a new group takes the group loaded ahead and fetches the
   // one after it.
   __device__ __forceinline__ void advance(const Sell &b, const uint32_t *broken, bool damage) {
     ++k;
-    if (k >= len) {
-      next = INT_MAX;
-      return;
-    }
     if ((k & 3) == 0) {
       cols = colsAhead;
-      flags = flagsAhead;
-      broken4 = (flags >> (group_slot() & 31)) & 0xFu;
+      broken4 = (flagsAhead >> (g0 & 31)) & 0xFu;
       fetch_ahead(b, broken, damage, k + 4);
+    } else {
+      cols.x = cols.y;
+      cols.y = cols.z;
+      cols.z = cols.w;
+      broken4 >>= 1;
     }
-    next = column(k & 3);
+    next = k < len ? cols.x : INT_MAX;
   }
 };
 
@@ -523,12 +531,13 @@ __device__ __forceinline__ void store_amap(PairPlanes amap, int64_t i, const T3 
 }
 
 // Bond pass 1 with windowed neighbour gathers (see BondCursor): X_j, V_j and u_j of W
-// consecutive particles per window, as 4 component pairs in the wavefront's LDS. New breaks
-// of a group are collected and set with one atomic OR when the lane leaves the group.
+// consecutive particles per window, as 4 component pairs in the wavefront's LDS. A lane takes
+// at most `take` bonds from one window (see window_stop). New breaks of a group are collected
+// and set with one atomic OR when the lane leaves the group.
 template <bool DAMAGE, int W>
 __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes xv, UPairs u,
                                                Sell bonds, uint32_t *broken,
-                                               const StretchTest &test, d2v *win,
+                                               const StretchTest &test, d2v *win, int take,
                                                double x0, double x1, double x2, double yi0,
                                                double yi1, double yi2, double num[9]) {
   BondCursor cur;
@@ -541,7 +550,8 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
     stage_window<4, W>(DefgradPlanes{xv, u}, n, w0, win);
     wave_lds_sync();
     const int end = w0 > INT_MAX - W ? INT_MAX : w0 + W; // (a finished lane's next is INT_MAX)
-    while (cur.next < end) {
+    const int stop = window_stop(cur.k, take);
+    while (cur.next < end && cur.k < stop) {
       const int o = cur.next - w0;
       const d2v a = win[o], b = win[W + o], ua = win[2 * W + o], ub = win[3 * W + o];
       const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
@@ -571,7 +581,7 @@ template <bool DAMAGE, int W>
 __global__ void __launch_bounds__(256)
     k_defgrad_w(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
                 Planes fout, uint32_t *broken, double initiation, double failure,
-                int useCompression, Batch batch) {
+                int useCompression, Batch batch, int take) {
   extern __shared__ d2v wlds[];
   d2v *win = wlds + (threadIdx.x >> 6) * (4 * W);
   const int64_t blk = xcd_block(rd.blocksPerRound);
@@ -596,8 +606,263 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int c = 0; c < 9; ++c)
       num[c] = 0.0;
-    defgrad_window<DAMAGE, W>(rd.n, mine ? i : -1, xv, u, bonds, broken, test, win, x0, x1,
-                              x2, yi0, yi1, yi2, num);
+    defgrad_window<DAMAGE, W>(rd.n, mine ? i : -1, xv, u, bonds, broken, test, win, take, x0,
+                              x1, x2, yi0, yi1, yi2, num);
+    if (!mine)
+      continue;
+    T3 nm, ki;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      nm.m[c] = num[c];
+      ki.m[c] = kinv(c, i);
+    }
+    const T3 f = t3_mul(nm, ki);
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      fout(c, i) = f.m[c];
+  }
+}
+
+// ---- Planned windows --------------------------------------------------------------------
+// The window walk of the windowed kernels depends only on the bond lists, the window W and the
+// bond limit, so it can be done once (k_window_plan) and replayed every step: per wavefront the
+// first particle of each of its windows, and per window and lane one 8-byte record holding the
+// LDS offsets (j − window start) of the bonds the lane takes from that window, in CSR order,
+// lowest byte first, 0xFF after the last. The planned kernels (`_p`) then read no column
+// indices, take no wave-wide minimum and keep no cursor: a lane's bonds in a window are the
+// next ones of its row and the record says where their neighbours sit. With PREFETCH the next
+// window is staged into a second LDS buffer while the current one is summed. Every bond is
+// summed in the same order with the same arithmetic as before, so the results equal the
+// direct gathers bit for bit.
+struct WindowPlan {
+  const int *first;     // per wavefront: index of its first window; first[wavefronts] = total
+  const int *start;     // per window: the first particle it stages
+  const uint64_t *rec;  // per window and lane: the bond offsets, see above
+};
+constexpr int kPlanTake = 8; // the most bonds per lane and window a record holds
+
+// Wavefront w plans particles 64w..64w+63 (the wavefronts of every bond-pass launch). FILL
+// false counts each wavefront's windows into `count`; FILL true writes them from first[w] on.
+// take ≤ kPlanTake.
+template <int W, bool FILL>
+__global__ void __launch_bounds__(256)
+    k_window_plan(int64_t n, Sell bonds, int take, int *count, const int *first, int *start,
+                  uint64_t *rec) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = i >> 6;
+  if (wave * 64 >= n) // the same for the whole wavefront
+    return;
+  BondCursor cur;
+  cur.start(bonds, nullptr, false, i < n ? i : -1);
+  int64_t t = FILL ? first[wave] : 0;
+  while (true) {
+    const int w0 = wave_min(cur.next);
+    if (w0 == INT_MAX)
+      break;
+    const int end = w0 > INT_MAX - W ? INT_MAX : w0 + W;
+    const int stop = window_stop(cur.k, take);
+    uint64_t r = ~0ull;
+    int q = 0;
+    while (cur.next < end && cur.k < stop) {
+      r &= ~(0xFFull << (8 * q));
+      r |= (uint64_t)(cur.next - w0) << (8 * q);
+      ++q;
+      cur.advance(bonds, nullptr, false);
+    }
+    if (FILL) {
+      rec[t * 64 + lane] = r;
+      if (lane == 0)
+        start[t] = w0;
+    }
+    ++t;
+  }
+  if (!FILL && lane == 0)
+    count[wave] = (int)t;
+}
+
+// Bonds in a record (its offsets form a prefix; an offset is below W ≤ 255).
+__device__ __forceinline__ int record_bonds(uint64_t r) {
+  const uint64_t v = ~r;
+  return v ? (71 - __builtin_clzll(v)) >> 3 : 0;
+}
+
+// Flag bits of bonds k, k+1, ... of a lane's row (bond k at bit 0, up to 12 bonds reaching
+// into the next two groups): the flag words of the three groups from bond k & ~3 on that the
+// row has (plan_words), then their 4-bit fields combined (plan_bits). g0 is the slot of the
+// lane's bond 0; a group's 4 slots sit in one word, groups 8 words apart.
+struct FlagWords {
+  uint32_t w[3];
+};
+__device__ __forceinline__ FlagWords plan_words(const uint32_t *broken, int64_t g0, int k,
+                                                int len) {
+  const int64_t w = (g0 >> 5) + 8 * (int64_t)(k >> 2);
+  FlagWords f;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    f.w[q] = (k & ~3) + 4 * q < len ? broken[w + 8 * q] : 0u;
+  return f;
+}
+__device__ __forceinline__ uint32_t plan_bits(const FlagWords &f, int64_t g0, int k) {
+  const int sh = (int)(g0 & 31);
+  uint32_t m = 0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    m |= ((f.w[q] >> sh) & 0xFu) << (4 * q);
+  return m >> (k & 3);
+}
+
+// Sets new breaks of bonds k, k+1, ... (bond k at bit 0 of `fresh`) in the lane's flag words.
+__device__ __forceinline__ void plan_flush(uint32_t *broken, int64_t g0, int k, uint32_t fresh) {
+  const uint32_t m = fresh << (k & 3);
+  const int64_t w = (g0 >> 5) + 8 * (int64_t)(k >> 2);
+  const int sh = (int)(g0 & 31);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const uint32_t bits = (m >> (4 * q)) & 0xFu;
+    if (bits)
+      atomicOr(&broken[w + 8 * q], bits << sh);
+  }
+}
+
+// Uniform loads of the plan through the constant address space: scalar loads.
+typedef __attribute__((address_space(4))) const int plan_int;
+__device__ __forceinline__ int plan_load(const int *p, int64_t k) { return ((const plan_int *)p)[k]; }
+
+// The window loop of a planned bond pass for the lane's particle (mine false: a lane with no
+// particle, which still stages its share). body(buf, r, brk, k, g0) sums the lane's bonds of
+// one window: `buf` holds the window, `r` the record, `brk` the flag bits from the lane's
+// bond k on, g0 the slot of its bond 0. Without PREFETCH the windows go through the
+// wavefront's part of the dynamic LDS, `win`; with it, through two static buffers, the next
+// window being staged into one while the current one is summed from the other. (The buffers
+// are separate LDS objects and the loop alternates them explicitly, so the compiler can tell
+// the window being staged from the one being read and does not wait for the staging.)
+template <int PAIRS, int W, bool PREFETCH, bool DAMAGE, class Planeset, class Body>
+__device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t n,
+                                                const WindowPlan &plan, const Sell &bonds,
+                                                const uint32_t *broken, int64_t i, bool mine,
+                                                d2v *win, Body &&body) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)((i - lane) >> 6));
+  int t = plan_load(plan.first, wave);
+  const int t1 = plan_load(plan.first, wave + 1);
+  const int len = mine ? bonds.rowptr[i + 1] - bonds.rowptr[i] : 0;
+  const int64_t g0 = mine ? sell_pos(bonds, i, 0) : 0;
+  int k = 0;
+  int wNext = t < t1 ? plan_load(plan.start, t) : 0; // first particle of window t
+  if constexpr (!PREFETCH) {
+    for (; t < t1; ++t) {
+      const int w0 = wNext;
+      if (t + 1 < t1)
+        wNext = plan_load(plan.start, t + 1);
+      stage_window<PAIRS, W>(planes, n, w0, win);
+      const uint64_t r = mine ? plan.rec[(int64_t)t * 64 + lane] : ~0ull;
+      const uint32_t brk = DAMAGE && mine ? plan_bits(plan_words(broken, g0, k, len), g0, k) : 0u;
+      wave_lds_sync();
+      body(win, r, brk, k, g0);
+      k += record_bonds(r);
+      wave_lds_sync();
+    }
+  } else {
+    __shared__ d2v bufA[4 * PAIRS * W], bufB[4 * PAIRS * W];
+    d2v *a = bufA + (threadIdx.x >> 6) * (PAIRS * W);
+    d2v *b = bufB + (threadIdx.x >> 6) * (PAIRS * W);
+    // The record and flag words of the window to be summed next are loaded with its data
+    // and used only after the wait for it, so that no wait falls inside a window's sum.
+    uint64_t r = ~0ull;
+    FlagWords fw{};
+    if (t < t1) {
+      stage_window<PAIRS, W>(planes, n, wNext, a);
+      r = mine ? plan.rec[(int64_t)t * 64 + lane] : ~0ull;
+      if (DAMAGE && mine)
+        fw = plan_words(broken, g0, k, len);
+      if (t + 1 < t1)
+        wNext = plan_load(plan.start, t + 1);
+    }
+    // Window t from `cur`; window t + 1 staged into `next` meanwhile.
+    auto one = [&](const d2v *cur, d2v *next) {
+      wave_lds_sync(); // window t is staged; the reads of window t − 1 (in `next`) are done
+      const uint32_t brk = DAMAGE ? plan_bits(fw, g0, k) : 0u;
+      const uint64_t rc = r;
+      const int kNext = k + record_bonds(rc);
+      r = ~0ull;
+      if (t + 1 < t1) {
+        stage_window<PAIRS, W>(planes, n, wNext, next);
+        r = mine ? plan.rec[(int64_t)(t + 1) * 64 + lane] : ~0ull;
+        if (DAMAGE && mine)
+          fw = plan_words(broken, g0, kNext, len);
+        if (t + 2 < t1)
+          wNext = plan_load(plan.start, t + 2);
+      }
+      body(cur, rc, brk, k, g0);
+      k = kNext;
+      ++t;
+    };
+    while (t < t1) {
+      one(a, b);
+      if (t >= t1)
+        break;
+      one(b, a);
+    }
+    wave_lds_sync(); // the last window's reads are done before the buffers are reused
+  }
+}
+
+// Bond pass 1 with planned windows (4 component pairs: X_j, V_j, u_j).
+template <bool DAMAGE, int W, bool PREFETCH>
+__global__ void __launch_bounds__(256)
+    k_defgrad_p(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
+                Planes fout, uint32_t *broken, double initiation, double failure,
+                int useCompression, Batch batch, WindowPlan plan) {
+  extern __shared__ d2v wlds[];
+  d2v *win = wlds + (threadIdx.x >> 6) * (4 * W); // (unused with PREFETCH)
+  const int64_t blk = xcd_block(rd.blocksPerRound);
+  if (blk < 0)
+    return;
+  const StretchTest test = make_stretch_test(initiation, failure, useCompression);
+  for (int r = 0; r < rd.rounds; ++r) {
+    const int64_t i = rd.particle(r, blk);
+    if (i - (threadIdx.x & 63) >= rd.n) // the wavefront's first particle: a uniform exit
+      break;
+    const bool mine = i < rd.n && point_active(batch, i);
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0, yi0 = 0.0, yi1 = 0.0, yi2 = 0.0;
+    if (mine) {
+      x0 = x(0, i);
+      x1 = x(1, i);
+      x2 = x(2, i);
+      yi0 = x0 + u(0, i);
+      yi1 = x1 + u(1, i);
+      yi2 = x2 + u(2, i);
+    }
+    double num[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      num[c] = 0.0;
+    planned_windows<4, W, PREFETCH, DAMAGE>(
+        DefgradPlanes{xv, u}, rd.n, plan, bonds, broken, i, mine, win,
+        [&](const d2v *buf, uint64_t rec, uint32_t brk, int k, int64_t g0) {
+          uint32_t fresh = 0;
+          for (int q = 0; q < kPlanTake; ++q) {
+            const int o = (int)(rec & 0xFFu);
+            if (o == 0xFF)
+              break;
+            const d2v a = buf[o], b = buf[W + o], ua = buf[2 * W + o], ub = buf[3 * W + o];
+            const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
+            const double yij[3] = {(a.x + ua.x) - yi0, (a.y + ua.y) - yi1,
+                                   (b.x + ub.x) - yi2};
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                num[3 * rr + c] += b.y * yij[rr] * xi[c];
+            if (DAMAGE && !(brk & 1u) && stretch_breaks(test, xi, yij))
+              fresh |= 1u << q;
+            rec >>= 8;
+            brk >>= 1;
+          }
+          if (DAMAGE && fresh)
+            plan_flush(broken, g0, k, fresh);
+        });
     if (!mine)
       continue;
     T3 nm, ki;
@@ -977,6 +1242,7 @@ struct ForceArgs {
   Ramp rampNow, rampNext;
   DiagArgs diag;
   Batch batch;
+  int take; // windowed gathers: bonds a lane takes from one window at most (see window_stop)
 };
 
 // A of particle j from its 5 component pairs.
@@ -1043,8 +1309,9 @@ __device__ __forceinline__ void force_sum_direct(const ForceArgs &g, int64_t i, 
 }
 
 // Bond pass 2 with windowed gathers (see BondCursor): X_j, V_j and A_j of W consecutive
-// particles per window as 7 component pairs in the wavefront's LDS. i < 0: a lane with no
-// particle, which still takes part in the wavefront's window steps.
+// particles per window as 7 component pairs in the wavefront's LDS, at most g.take bonds per
+// lane and window. i < 0: a lane with no particle, which still takes part in the wavefront's
+// window steps.
 template <bool DAMAGE, int W>
 __device__ __forceinline__ void force_sum_window(const ForceArgs &g, int64_t i, d2v *win,
                                                  double x0, double x1, double x2, const T3 &ai,
@@ -1058,7 +1325,8 @@ __device__ __forceinline__ void force_sum_window(const ForceArgs &g, int64_t i, 
     stage_window<7, W>(ForcePlanes{g.xv, g.amap}, g.n, w0, win);
     wave_lds_sync();
     const int end = w0 > INT_MAX - W ? INT_MAX : w0 + W; // (a finished lane's next is INT_MAX)
-    while (cur.next < end) {
+    const int stop = window_stop(cur.k, g.take);
+    while (cur.next < end && cur.k < stop) {
       const int o = cur.next - w0;
       if (!(DAMAGE && cur.bond_broken())) {
         const d2v a = win[o], b = win[W + o];
@@ -1208,6 +1476,64 @@ __global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
     }
     double acc[3] = {0.0, 0.0, 0.0};
     force_sum_window<DAMAGE, W>(g, mine ? i : -1, win, x0, x1, x2, ai, acc);
+    if (mine)
+      force_finish<FUSE_NEXT, INTEGRATE, false>(g, i, acc, nullptr);
+  }
+}
+
+// The force pass with planned windows (7 component pairs: X_j, V_j, A_j; see WindowPlan).
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W, bool PREFETCH>
+__global__ void __launch_bounds__(256) k_force_p(ForceArgs g, WindowPlan plan) {
+  extern __shared__ d2v wlds[];
+  d2v *win = wlds + (threadIdx.x >> 6) * (7 * W); // (unused with PREFETCH)
+  const int64_t blk = xcd_block(g.rd.blocksPerRound);
+  if (blk < 0)
+    return;
+  for (int r = 0; r < g.rd.rounds; ++r) {
+    const int64_t i = g.rd.particle(r, blk);
+    if (i - (threadIdx.x & 63) >= g.rd.n)
+      break;
+    const bool mine = i < g.rd.n && point_active(g.batch, i);
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    T3 ai;
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      ai.m[c] = 0.0;
+    if (mine) {
+      x0 = g.x(0, i);
+      x1 = g.x(1, i);
+      x2 = g.x(2, i);
+      ai = load_amap(g.amap, i);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    planned_windows<7, W, PREFETCH, DAMAGE>(
+        ForcePlanes{g.xv, g.amap}, g.n, plan, g.bonds, g.broken, i, mine, win,
+        [&](const d2v *buf, uint64_t rec, uint32_t brk, int, int64_t) {
+          for (int q = 0; q < kPlanTake; ++q) {
+            const int o = (int)(rec & 0xFFu);
+            if (o == 0xFF)
+              break;
+            if (!(DAMAGE && (brk & 1u))) {
+              const d2v a = buf[o], b = buf[W + o];
+              T3 aj;
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                const d2v v = buf[(2 + c) * W + o];
+                aj.m[2 * c] = v.x;
+                aj.m[2 * c + 1] = v.y;
+              }
+              aj.m[8] = buf[6 * W + o].x;
+              const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
+              double term[3];
+              force_bond_term(ai, aj, xi, b.y, term);
+#pragma unroll
+              for (int d = 0; d < 3; ++d)
+                acc[d] += term[d];
+            }
+            rec >>= 8;
+            brk >>= 1;
+          }
+        });
     if (mine)
       force_finish<FUSE_NEXT, INTEGRATE, false>(g, i, acc, nullptr);
   }

@@ -392,7 +392,8 @@ class Context:
             self._check(lib().vpd_attach_warm_start(self.h, warm_start.h))
 
     TUNE_KEYS = ("defgrad_lds", "force_lds", "defgrad_rounds", "force_rounds", "defgrad_window",
-                 "force_window")
+                 "force_window", "defgrad_window_bonds", "force_window_bonds",
+                 "defgrad_window_plan", "force_window_plan")
 
     def autotune(self, reps=3):
         out = (C.c_int * len(self.TUNE_KEYS))()
