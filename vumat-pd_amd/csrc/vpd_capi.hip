@@ -2192,9 +2192,9 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
     gx.take = take;
     return gx;
   };
-  // Stage 1: sweeps × LDS cap × window (walked, no bond limit per window); stage 2: every
-  // window with each bond limit, and every planned form, at the sweeps and LDS cap stage 1
-  // chose.
+  // Stage 1: sweeps × LDS cap × window (walked, no bond limit per window); stage 2, at the
+  // sweeps stage 1 chose: every window with each bond limit (walked, at stage 1's LDS cap),
+  // and every planned form, window and bond limit with each LDS cap.
   for (int pass = 0; pass < 2; ++pass) {
     for (int win : windowChoices)
       for (int lds : ldsChoices) {
@@ -2217,13 +2217,18 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
       for (int take = 5; take <= kPlanTake; ++take) {
         if (int s = build_plan(c, pass, win, take))
           return s;
-        for (int mode = 1; mode <= 2; ++mode) {
-          Gather gx = walk(win, take);
-          gx.mode = mode;
-          gx.plan = plan_of(c, pass);
-          if (int s = trial(pass, lds, rounds, gx))
-            return s;
-        }
+        for (int mode = 1; mode <= 2; ++mode)
+          for (int cap : ldsChoices) {
+            // A cap at or below the pass's own window LDS repeats the uncapped launch.
+            const int own = window_lds(pairsOf[pass], win) * (mode == 2 ? 2 : 1);
+            if (cap != 0 && cap <= own)
+              continue;
+            Gather gx = walk(win, take);
+            gx.mode = mode;
+            gx.plan = plan_of(c, pass);
+            if (int s = trial(pass, cap, rounds, gx))
+              return s;
+          }
       }
   }
   (void)hipEventDestroy(a);

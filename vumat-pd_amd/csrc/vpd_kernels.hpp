@@ -114,7 +114,7 @@ __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1)
     v = min(v, __shfl_xor(v, off));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v); // the same in every lane: a scalar
 }
 
 // The wavefront's window stores (LDS-direct loads, vmcnt) and LDS accesses (lgkmcnt) are
@@ -127,21 +127,38 @@ __device__ __forceinline__ void wave_lds_sync() {
 typedef __attribute__((address_space(3))) void lds_void;
 
 // Window staging: each lane loads the 16-byte pair c of particle w0 + q0 + lane straight into
-// LDS (no registers), at window slot q0 + lane of plane c. `planes` gives the pair planes.
+// LDS (no registers), at window slot q0 + lane of plane c. `planes` gives the pair planes. w0
+// and `win` are the same for the whole wavefront, so the plane addresses are a scalar base
+// plus the lane's offset and the LDS destinations are scalar.
+// A pointer the whole wavefront holds, as a scalar (so that a load from it plus a lane
+// offset takes the scalar-base form and no per-plane vector address is kept).
+__device__ __forceinline__ const d2v *scalar_ptr(const d2v *p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<const d2v *>(((uint64_t)hi << 32) | lo);
+}
+
 template <int PAIRS, int W, class Planeset>
 __device__ __forceinline__ void stage_window(const Planeset &planes, int64_t n, int w0, d2v *win) {
   const int lane = threadIdx.x & 63;
+  const int64_t left = n - (int64_t)w0; // particles from w0 to the end (≥ 1)
+  const int count = left < W ? (int)left : W;
 #pragma unroll
   for (int q0 = 0; q0 < W; q0 += 64) {
-    const int q = q0 + lane;
-    const int64_t p = (int64_t)w0 + q;
-    if (q < W && p < n) {
+    if (q0 + lane < count) {
 #pragma unroll
       for (int c = 0; c < PAIRS; ++c)
-        __builtin_amdgcn_global_load_lds((const void *)planes.ptr(c, p),
-                                         (lds_void *)(win + c * W + q0), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(scalar_ptr(planes.ptr(c, (int64_t)w0 + q0)) + lane),
+            (lds_void *)(win + c * W + q0), 16, 0, 0);
     }
   }
+}
+
+// The wavefront's slot among the 4 of a 256-thread workgroup, as a scalar.
+__device__ __forceinline__ int wave_slot() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
 // The pair planes of the two windowed passes, in window order.
@@ -583,7 +600,7 @@ __global__ void __launch_bounds__(256)
                 Planes fout, uint32_t *broken, double initiation, double failure,
                 int useCompression, Batch batch, int take) {
   extern __shared__ d2v wlds[];
-  d2v *win = wlds + (threadIdx.x >> 6) * (4 * W);
+  d2v *win = wlds + wave_slot() * (4 * W);
   const int64_t blk = xcd_block(rd.blocksPerRound);
   if (blk < 0)
     return;
@@ -765,8 +782,8 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
     }
   } else {
     __shared__ d2v bufA[4 * PAIRS * W], bufB[4 * PAIRS * W];
-    d2v *a = bufA + (threadIdx.x >> 6) * (PAIRS * W);
-    d2v *b = bufB + (threadIdx.x >> 6) * (PAIRS * W);
+    d2v *a = bufA + wave_slot() * (PAIRS * W);
+    d2v *b = bufB + wave_slot() * (PAIRS * W);
     // The record and flag words of the window to be summed next are loaded with its data
     // and used only after the wait for it, so that no wait falls inside a window's sum.
     uint64_t r = ~0ull;
@@ -809,13 +826,15 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
 }
 
 // Bond pass 1 with planned windows (4 component pairs: X_j, V_j, u_j).
+// Without PREFETCH the pass fits 5 wavefronts per SIMD in registers (95 VGPRs) when asked to;
+// with it the two window buffers already limit the workgroups per CU.
 template <bool DAMAGE, int W, bool PREFETCH>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFETCH ? 1 : 5)))
     k_defgrad_p(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
                 Planes fout, uint32_t *broken, double initiation, double failure,
                 int useCompression, Batch batch, WindowPlan plan) {
   extern __shared__ d2v wlds[];
-  d2v *win = wlds + (threadIdx.x >> 6) * (4 * W); // (unused with PREFETCH)
+  d2v *win = wlds + wave_slot() * (4 * W); // (unused with PREFETCH)
   const int64_t blk = xcd_block(rd.blocksPerRound);
   if (blk < 0)
     return;
@@ -1454,7 +1473,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W>
 __global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
   extern __shared__ d2v wlds[];
-  d2v *win = wlds + (threadIdx.x >> 6) * (7 * W);
+  d2v *win = wlds + wave_slot() * (7 * W);
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
   if (blk < 0)
     return;
@@ -1485,7 +1504,7 @@ __global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W, bool PREFETCH>
 __global__ void __launch_bounds__(256) k_force_p(ForceArgs g, WindowPlan plan) {
   extern __shared__ d2v wlds[];
-  d2v *win = wlds + (threadIdx.x >> 6) * (7 * W); // (unused with PREFETCH)
+  d2v *win = wlds + wave_slot() * (7 * W); // (unused with PREFETCH)
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
   if (blk < 0)
     return;
