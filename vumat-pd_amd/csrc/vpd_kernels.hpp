@@ -749,12 +749,12 @@ __device__ __forceinline__ int plan_load(const int *p, int64_t k) { return ((con
 // The window loop of a planned bond pass for the lane's particle (mine false: a lane with no
 // particle, which still stages its share). body(buf, r, brk, k, g0) sums the lane's bonds of
 // one window: `buf` holds the window, `r` the record, `brk` the flag bits from the lane's
-// bond k on, g0 the slot of its bond 0. Without PREFETCH the windows go through the
+// bond k on (with FLAGS; 0 otherwise), g0 the slot of its bond 0. Without PREFETCH the windows go through the
 // wavefront's part of the dynamic LDS, `win`; with it, through two static buffers, the next
 // window being staged into one while the current one is summed from the other. (The buffers
 // are separate LDS objects and the loop alternates them explicitly, so the compiler can tell
 // the window being staged from the one being read and does not wait for the staging.)
-template <int PAIRS, int W, bool PREFETCH, bool DAMAGE, class Planeset, class Body>
+template <int PAIRS, int W, bool PREFETCH, bool FLAGS, class Planeset, class Body>
 __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t n,
                                                 const WindowPlan &plan, const Sell &bonds,
                                                 const uint32_t *broken, int64_t i, bool mine,
@@ -774,7 +774,7 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
         wNext = plan_load(plan.start, t + 1);
       stage_window<PAIRS, W>(planes, n, w0, win);
       const uint64_t r = mine ? plan.rec[(int64_t)t * 64 + lane] : ~0ull;
-      const uint32_t brk = DAMAGE && mine ? plan_bits(plan_words(broken, g0, k, len), g0, k) : 0u;
+      const uint32_t brk = FLAGS && mine ? plan_bits(plan_words(broken, g0, k, len), g0, k) : 0u;
       wave_lds_sync();
       body(win, r, brk, k, g0);
       k += record_bonds(r);
@@ -791,7 +791,7 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
     if (t < t1) {
       stage_window<PAIRS, W>(planes, n, wNext, a);
       r = mine ? plan.rec[(int64_t)t * 64 + lane] : ~0ull;
-      if (DAMAGE && mine)
+      if (FLAGS && mine)
         fw = plan_words(broken, g0, k, len);
       if (t + 1 < t1)
         wNext = plan_load(plan.start, t + 1);
@@ -799,14 +799,14 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
     // Window t from `cur`; window t + 1 staged into `next` meanwhile.
     auto one = [&](const d2v *cur, d2v *next) {
       wave_lds_sync(); // window t is staged; the reads of window t − 1 (in `next`) are done
-      const uint32_t brk = DAMAGE ? plan_bits(fw, g0, k) : 0u;
+      const uint32_t brk = FLAGS ? plan_bits(fw, g0, k) : 0u;
       const uint64_t rc = r;
       const int kNext = k + record_bonds(rc);
       r = ~0ull;
       if (t + 1 < t1) {
         stage_window<PAIRS, W>(planes, n, wNext, next);
         r = mine ? plan.rec[(int64_t)(t + 1) * 64 + lane] : ~0ull;
-        if (DAMAGE && mine)
+        if (FLAGS && mine)
           fw = plan_words(broken, g0, kNext, len);
         if (t + 2 < t1)
           wNext = plan_load(plan.start, t + 2);
@@ -857,9 +857,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
 #pragma unroll
     for (int c = 0; c < 9; ++c)
       num[c] = 0.0;
-    planned_windows<4, W, PREFETCH, DAMAGE>(
+    // The stretch test runs on every bond, broken or not: setting a bit that is set changes
+    // nothing, so the flag words are read only in the windows where a bond breaks, to keep
+    // the atomics to the new breaks.
+    planned_windows<4, W, PREFETCH, false>(
         DefgradPlanes{xv, u}, rd.n, plan, bonds, broken, i, mine, win,
-        [&](const d2v *buf, uint64_t rec, uint32_t brk, int k, int64_t g0) {
+        [&](const d2v *buf, uint64_t rec, uint32_t, int k, int64_t g0) {
           uint32_t fresh = 0;
           for (int q = 0; q < kPlanTake; ++q) {
             const int o = (int)(rec & 0xFFu);
@@ -874,13 +877,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
 #pragma unroll
               for (int c = 0; c < 3; ++c)
                 num[3 * rr + c] += b.y * yij[rr] * xi[c];
-            if (DAMAGE && !(brk & 1u) && stretch_breaks(test, xi, yij))
+            if (DAMAGE && stretch_breaks(test, xi, yij))
               fresh |= 1u << q;
             rec >>= 8;
-            brk >>= 1;
           }
-          if (DAMAGE && fresh)
-            plan_flush(broken, g0, k, fresh);
+          if (DAMAGE && fresh) { // (only a lane with a particle has bonds)
+            const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
+            fresh &= ~plan_bits(plan_words(broken, g0, k, len), g0, k);
+            if (fresh)
+              plan_flush(broken, g0, k, fresh);
+          }
         });
     if (!mine)
       continue;
