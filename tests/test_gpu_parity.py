@@ -321,6 +321,61 @@ def test_windowed_gathers_on_irregular_rows(vpd, voids, take, plan):
         assert np.array_equal(a.get(name), b.get(name)), name
 
 
+FORMS = {
+    "walked": dict(defgrad_window=64, force_window=64),
+    "planned-1": dict(defgrad_window=64, force_window=64, defgrad_window_bonds=1,
+                      force_window_bonds=1, defgrad_window_plan=1, force_window_plan=1),
+    "planned-prefetch": dict(defgrad_window=80, force_window=96, defgrad_window_bonds=3,
+                             force_window_bonds=8, defgrad_window_plan=2, force_window_plan=2),
+}
+
+
+@pytest.mark.parametrize("shape", [(5, 3, 3), (11, 3, 2), (13, 4, 3)],
+                         ids=["45-particles", "66-particles", "156-particles"])
+def test_gather_forms_on_small_blocks(vpd, shape):
+    # Blocks smaller than a workgroup, and ones whose last wavefront is nearly empty: every
+    # gather form equals the direct gathers bit for bit, damage included.
+    banner("gather forms on a %dx%dx%d block" % shape)
+    kw = dict(spacing=1.0, horizon=2.015, props=JC_PROPS, dt=8.5377e-5, density=7.85e-3,
+              velocity=300.0, damage=(0.002, 0.0))
+    ref = vpd.Bar(*shape, **kw)
+    ref.step(40, want_fields=True)
+    assert ref.get("damage").sum() > 0
+    for name, knobs in FORMS.items():
+        b = vpd.Bar(*shape, **kw)
+        b.ctx.set_tuning(**knobs)
+        b.step(40, want_fields=True)
+        for field in ["u", "v", "stressN", "statev", "damage", "force", "F"]:
+            assert np.array_equal(b.get(field), ref.get(field)), (name, field)
+
+
+def test_plan_follows_knob_and_neighbor_changes(vpd):
+    # The window plan is rebuilt when the knobs change between steps and when the bond lists
+    # are replaced: the run stays equal to one with direct gathers.
+    banner("window plan rebuilt after knob and bond-list changes")
+    kw = dict(nx=16, ny=6, nz=5, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
+              density=7.85e-3, velocity=300.0, damage=(0.002, 0.0))
+    ref, b = vpd.Bar(**kw), vpd.Bar(**kw)
+    b.ctx.set_tuning(**FORMS["planned-1"])
+    for knobs in (dict(), dict(force_window_bonds=5, defgrad_window=128, defgrad_window_bonds=8),
+                  dict(force_window_plan=2, defgrad_window_plan=2)):
+        b.ctx.set_tuning(**knobs)
+        ref.step(8)
+        b.step(8)
+        for field in ["u", "v", "stressN", "damage"]:
+            assert np.array_equal(b.get(field), ref.get(field)), (knobs, field)
+    # New bond lists (a smaller horizon's) on the planned context, then a fresh start.
+    small = vpd.Bar(**dict(kw, horizon=2.015))
+    rowptr, col = small.ctx.get_neighbors()
+    b.ctx.set_neighbors(rowptr, col)
+    b.ctx.initialize_shape()
+    b.reset()
+    small.step(20)
+    b.step(20)
+    for field in ["u", "v", "stressN", "statev", "damage"]:
+        assert np.array_equal(b.get(field), small.get(field)), field
+
+
 def test_window_knob_limits(vpd):
     b = vpd.Bar(nx=6, ny=4, nz=4, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
                 density=7.85e-3, velocity=1.0)
