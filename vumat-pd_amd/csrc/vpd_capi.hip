@@ -2194,7 +2194,8 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   };
   // Stage 1: sweeps × LDS cap × window (walked, no bond limit per window); stage 2, at the
   // sweeps stage 1 chose: every window with each bond limit (walked, at stage 1's LDS cap),
-  // and every planned form, window and bond limit with each LDS cap.
+  // and every planned form, window and bond limit with each LDS cap; stage 3: sweeps × LDS
+  // cap again for the form, window and limit chosen.
   for (int pass = 0; pass < 2; ++pass) {
     for (int win : windowChoices)
       for (int lds : ldsChoices) {
@@ -2229,6 +2230,22 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
             if (int s = trial(pass, cap, rounds, gx))
               return s;
           }
+      }
+    Gather chosen = pick[pass];
+    if (chosen.mode > 0) {
+      if (int s = build_plan(c, pass, chosen.window, chosen.take))
+        return s;
+      chosen.plan = plan_of(c, pass);
+    }
+    const int own = chosen.window > 0
+                        ? window_lds(pairsOf[pass], chosen.window) * (chosen.mode == 2 ? 2 : 1)
+                        : 0;
+    for (int cap : ldsChoices)
+      for (int r : roundChoices) {
+        if ((cap != 0 && cap <= own) || (r > 1 && r > ceil_div(c->n, 256)))
+          continue;
+        if (int s = trial(pass, cap, r, chosen))
+          return s;
       }
   }
   (void)hipEventDestroy(a);
