@@ -335,6 +335,10 @@ int vpd_enable_diagnostics(vpd_context *ctx, const vpd_diag_config *cfg);
 int vpd_fetch_diagnostics(vpd_context *ctx, double *records, int capacity, int *count);
 
 /* ---- per-kernel timing ------------------------------------------------------------ */
+/* enable = 0: off. enable = k >= 1: HIP events bracket the kernels of every k-th vpd_step /
+ * vpd_step_specimens call (counting from the next one) and every launch of the other entries;
+ * k > 1 keeps the events' own cost (a few microseconds per kernel) out of most steps. No
+ * reference counterpart. */
 int vpd_set_timing(vpd_context *ctx, int enable);
 /* Synchronises, sums the recorded launches per kernel and clears them. Writes up to
  * `capacity` entries and returns their count in *count. */

@@ -376,6 +376,24 @@ def test_plan_follows_knob_and_neighbor_changes(vpd):
         assert np.array_equal(b.get(field), small.get(field)), field
 
 
+def test_sampled_kernel_timing(vpd):
+    # Timing stride k: the kernels of every k-th step carry events, counted from the first
+    # step after the call.
+    b = vpd.Bar(nx=8, ny=4, nz=4, spacing=1.0, horizon=2.015, props=JC_PROPS, dt=8.5377e-5,
+                density=7.85e-3, velocity=1.0)
+    with pytest.raises(vpd.VpdError, match="timing stride"):
+        b.ctx.set_timing(True, -2)
+    for stride, timed in ((1, 9), (3, 3), (4, 3)):
+        b.ctx.set_timing(True, stride)
+        b.step(9)
+        kt = b.ctx.collect_timing()
+        for name in ("defgrad_bond_pass", "material_jc", "force_bond_pass"):
+            assert kt[name][1] == timed, (stride, name, kt[name])
+    b.ctx.set_timing(False)
+    b.step(2)
+    assert b.ctx.collect_timing() == {}
+
+
 def test_window_knob_limits(vpd):
     b = vpd.Bar(nx=6, ny=4, nz=4, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
                 density=7.85e-3, velocity=1.0)

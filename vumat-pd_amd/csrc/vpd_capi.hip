@@ -96,6 +96,11 @@ struct vpd_context {
   bool haveNeighbors = false, haveShape = false, haveMaterial = false, haveDynamics = false;
   bool haveFnew = false, haveFields = false, haveForce = false, predicted = false;
   bool timing = false;
+  // With timing on, the kernels of every timingStride-th step are timed (timingTick counts
+  // steps; timeThis says whether the step being taken is one of them).
+  int timingStride = 1;
+  int64_t timingTick = 0;
+  bool timeThis = true;
   // LDS bytes reserved per 256-thread workgroup of each bond pass: a cap on how many
   // workgroups share a CU, and with it on the span of particles whose neighbour data an
   // XCD's L2 must hold at once. Chosen by vpd_autotune.
@@ -200,19 +205,29 @@ hipEvent_t take_event(vpd_context *c) {
 // Runs `launch` on the context stream, bracketed by events when timing is on.
 template <class F> int timed(vpd_context *c, int kernel, F &&launch) {
   TimingRec r{kernel, nullptr, nullptr};
-  if (c->timing) {
+  const bool on = c->timing && c->timeThis;
+  if (on) {
     r.start = take_event(c);
     r.stop = take_event(c);
     HIPTRY(c, hipEventRecord(r.start, c->stream));
   }
   launch();
   HIPTRY(c, hipGetLastError());
-  if (c->timing) {
+  if (on) {
     HIPTRY(c, hipEventRecord(r.stop, c->stream));
     c->recs.push_back(r);
   }
   return VPD_OK;
 }
+
+// For the duration of one step: whether its kernels are timed (every timingStride-th step).
+struct StepTiming {
+  vpd_context *c;
+  explicit StepTiming(vpd_context *ctx) : c(ctx) {
+    c->timeThis = c->timingTick++ % c->timingStride == 0;
+  }
+  ~StepTiming() { c->timeThis = true; }
+};
 
 JcParams make_jc(const std::vector<double> &p) {
   // JohnsonCook.f90:55-72
@@ -1590,6 +1605,7 @@ int vpd_step(vpd_context *c, const vpd_step_args *g) {
       return s;
   if (int s = prepare_plans(c)) // before the step changes any state
     return s;
+  const StepTiming stepTiming(c);
   if (!c->predicted)
     if (int s = launch_predict(c, g->dt, g->ramp))
       return s;
@@ -1693,6 +1709,7 @@ int vpd_step_specimens(vpd_context *c, int32_t step, int fuseNext, int wantField
       return s;
   if (int s = prepare_plans(c)) // before the step changes any state
     return s;
+  const StepTiming stepTiming(c);
   c->batch.step = step;
   // Scalar arguments are unused: every particle takes its specimen's dt and velocity band.
   const double none[3] = {0.0, 0.0, 0.0};
@@ -2313,7 +2330,11 @@ int vpd_synchronize(vpd_context *c) {
 int vpd_set_timing(vpd_context *c, int enable) {
   if (!c)
     return fail(c, VPD_ERR_ARGUMENT, "Null vpd_context.");
+  if (enable < 0)
+    return fail(c, VPD_ERR_ARGUMENT, "The timing stride must be 0 (off) or positive.");
   c->timing = enable != 0;
+  c->timingStride = std::max(enable, 1);
+  c->timingTick = 0;
   return VPD_OK;
 }
 

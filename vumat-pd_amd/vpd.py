@@ -412,8 +412,9 @@ class Context:
             out[k] = v.value
         return out
 
-    def set_timing(self, on=True):
-        self._check(lib().vpd_set_timing(self.h, int(on)))
+    def set_timing(self, on=True, stride=1):
+        """Per-kernel HIP-event timing of every `stride`-th step (and every other launch)."""
+        self._check(lib().vpd_set_timing(self.h, int(stride) if on else 0))
 
     def enable_diagnostics(self, load_axis=0, lateral_axis=1, observation_weight=None,
                            lateral_sets=None, eqps_index=1, indicator=(), indicator_min=False,
