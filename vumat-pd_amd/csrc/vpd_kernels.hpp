@@ -123,6 +123,11 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_s_waitcnt(0x0070); // vmcnt(0), lgkmcnt(0); expcnt unconstrained
   __builtin_amdgcn_wave_barrier();
 }
+// The wavefront's LDS accesses are complete (its memory loads may still be in flight).
+__device__ __forceinline__ void wave_lds_wait() {
+  __builtin_amdgcn_s_waitcnt(0xC07F); // lgkmcnt(0); vmcnt and expcnt unconstrained
+  __builtin_amdgcn_wave_barrier();
+}
 
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -768,17 +773,31 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
   int k = 0;
   int wNext = t < t1 ? plan_load(plan.start, t) : 0; // first particle of window t
   if constexpr (!PREFETCH) {
+    // The record and flag words of window t + 1 are loaded once window t is staged, so that
+    // their memory latency passes while window t is summed (the records are read once per
+    // pass and come from HBM); the wait at the end of a window is for its LDS reads only.
+    uint64_t r = mine && t < t1 ? plan.rec[(int64_t)t * 64 + lane] : ~0ull;
+    FlagWords fw{};
+    if (FLAGS && mine && t < t1)
+      fw = plan_words(broken, g0, k, len);
     for (; t < t1; ++t) {
       const int w0 = wNext;
       if (t + 1 < t1)
         wNext = plan_load(plan.start, t + 1);
       stage_window<PAIRS, W>(planes, n, w0, win);
-      const uint64_t r = mine ? plan.rec[(int64_t)t * 64 + lane] : ~0ull;
-      const uint32_t brk = FLAGS && mine ? plan_bits(plan_words(broken, g0, k, len), g0, k) : 0u;
-      wave_lds_sync();
-      body(win, r, brk, k, g0);
-      k += record_bonds(r);
-      wave_lds_sync();
+      wave_lds_sync(); // window t is staged (and its record and flag words are in)
+      const uint64_t rc = r;
+      const uint32_t brk = FLAGS ? plan_bits(fw, g0, k) : 0u;
+      const int kNext = k + record_bonds(rc);
+      r = ~0ull;
+      if (t + 1 < t1 && mine) {
+        if (FLAGS)
+          fw = plan_words(broken, g0, kNext, len);
+        r = plan.rec[(int64_t)(t + 1) * 64 + lane];
+      }
+      body(win, rc, brk, k, g0);
+      k = kNext;
+      wave_lds_wait(); // window t's LDS reads are done before it is restaged
     }
   } else {
     __shared__ d2v bufA[4 * PAIRS * W], bufB[4 * PAIRS * W];
