@@ -364,6 +364,16 @@ def test_plan_follows_knob_and_neighbor_changes(vpd):
         b.step(8)
         for field in ["u", "v", "stressN", "damage"]:
             assert np.array_equal(b.get(field), ref.get(field)), (knobs, field)
+    # Bonds broken from the host (the flags and the per-particle marks both come from the
+    # upload): the planned force pass must skip them as the direct one does.
+    d = np.zeros(ref.ctx.nnz)
+    d[::7] = 1.0
+    for bar in (ref, b):
+        bar.reset()
+        bar.ctx.upload("damage", d)
+        bar.step(6)
+    for field in ["u", "v", "stressN", "damage"]:
+        assert np.array_equal(b.get(field), ref.get(field)), ("uploaded damage", field)
     # New bond lists (a smaller horizon's) on the planned context, then a fresh start.
     small = vpd.Bar(**dict(kw, horizon=2.015))
     rowptr, col = small.ctx.get_neighbors()

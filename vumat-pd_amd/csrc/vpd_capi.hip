@@ -86,6 +86,9 @@ struct vpd_context {
   double *sigOut = nullptr, *strainOut = nullptr, *force = nullptr, *dmgIndex = nullptr;
   int *iters = nullptr;
   uint32_t *broken = nullptr; // one bit per sliced bond slot
+  // One byte per particle, 1 when any of its bonds may be broken: set by every kernel that
+  // breaks a bond, rebuilt when the flags are uploaded, cleared with them.
+  uint8_t *anyBroken = nullptr;
   int64_t brokenWords = 0;
   int *colS = nullptr;         // sliced bond layout (see Sell in vpd_kernels.hpp)
   int64_t *sliceBase = nullptr;
@@ -503,6 +506,9 @@ int build_sell(vpd_context *c) {
     return VPD_ERR_DEVICE;
   HIPTRY(c, hipMemsetAsync(c->broken, 0, sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords),
                            c->stream));
+  if (dalloc(c, c->anyBroken, n))
+    return VPD_ERR_DEVICE;
+  HIPTRY(c, hipMemsetAsync(c->anyBroken, 0, std::max<int64_t>(1, n), c->stream));
   HIPTRY(c, hipStreamSynchronize(c->stream));
   return VPD_OK;
 }
@@ -693,14 +699,14 @@ void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompress
         k_defgrad_p<DAMAGE, W, true><<<grid_rounds(rd), 256, prefetch_lds(lds, 4, window),
                                         c->stream>>>(
             rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
-            planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation,
-            c->failure, useCompression, batch, gx.plan);
+            planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->anyBroken,
+            c->initiation, c->failure, useCompression, batch, gx.plan);
       else
         k_defgrad_p<DAMAGE, W, false><<<grid_rounds(rd), 256, launch_lds(lds, 4, window),
                                          c->stream>>>(
             rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
-            planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation,
-            c->failure, useCompression, batch, gx.plan);
+            planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->anyBroken,
+            c->initiation, c->failure, useCompression, batch, gx.plan);
     });
     return;
   }
@@ -709,15 +715,15 @@ void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompress
       k_defgrad_w<DAMAGE, decltype(w)::value>
           <<<grid_rounds(rd), 256, launch_lds(lds, 4, window), c->stream>>>(
               rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
-              planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation,
-              c->failure, useCompression, batch, take_arg(gx.take));
+              planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->anyBroken,
+              c->initiation, c->failure, useCompression, batch, take_arg(gx.take));
     });
     return;
   }
   k_defgrad<DAMAGE><<<grid_rounds(rd), 256, lds, c->stream>>>(
       rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
-      planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->initiation, c->failure,
-      useCompression, batch);
+      planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->anyBroken, c->initiation,
+      c->failure, useCompression, batch);
 }
 
 void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, const Batch &batch,
@@ -912,6 +918,7 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.vol = c->vol;
   g.bonds = sell_of(c);
   g.broken = c->broken;
+  g.anyBroken = c->anyBroken;
   g.codes = c->codes;
   g.dt = dt;
   g.density = c->density;
@@ -1080,6 +1087,7 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->list);
   dfree(c->incList);
   dfree(c->broken);
+  dfree(c->anyBroken);
   dfree(c->colS);
   dfree(c->sliceBase);
   for (auto &p : c->plan) {
@@ -1374,6 +1382,8 @@ int vpd_initialize_dynamics(vpd_context *c) {
   if (c->broken)
     HIPTRY(c, hipMemsetAsync(c->broken, 0,
                              sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords), c->stream));
+  if (c->anyBroken)
+    HIPTRY(c, hipMemsetAsync(c->anyBroken, 0, std::max<int64_t>(1, c->n), c->stream));
   HIPTRY(c, hipStreamSynchronize(c->stream));
   c->haveDynamics = true;
   c->predicted = false;
@@ -1390,16 +1400,21 @@ int vpd_upload(vpd_context *c, int field, const void *host) {
       return fail(c, VPD_ERR_STATE, "Neighbour lists have not been built.");
     const double *d = static_cast<const double *>(host);
     std::vector<uint32_t> bits(std::max<int64_t>(1, c->brokenWords), 0u);
+    std::vector<uint8_t> any(std::max<int64_t>(1, c->n), 0);
     for (int64_t i = 0; i < c->n; ++i)
       for (int64_t p = c->hostRowptr[i]; p < c->hostRowptr[i + 1]; ++p) {
         const int64_t q = sell_slot(c, i, p);
-        if (std::min(std::max(d[p], 0.0), 1.0) >= 1.0)
+        if (std::min(std::max(d[p], 0.0), 1.0) >= 1.0) {
           bits[q >> 5] |= 1u << (q & 31);
-        else if (d[p] > 0.0)
+          any[i] = 1;
+        } else if (d[p] > 0.0) {
           return fail(c, VPD_ERR_UNSUPPORTED, "Partial bond damage is not representable.");
+        }
       }
     HIPTRY(c, hipMemcpyAsync(c->broken, bits.data(), sizeof(uint32_t) * bits.size(),
                              hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->anyBroken, any.data(), any.size(), hipMemcpyHostToDevice,
+                             c->stream));
     HIPTRY(c, hipStreamSynchronize(c->stream));
     return VPD_OK;
   }
@@ -1824,6 +1839,7 @@ int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *forc
   g.vol = c->vol;
   g.bonds = sell_of(c);
   g.broken = c->broken;
+  g.anyBroken = c->anyBroken;
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   g.rd = rounds_of(c->n, c->roundsForce);
   if (int s = timed(c, K_FORCE, [&] {
@@ -1963,7 +1979,8 @@ int vpd_update_damage(vpd_context *c) {
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
   k_bond_damage<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->x, c->n),
                                                       upairs(c), sell_of(c), c->broken,
-                                                      c->initiation, c->failure, useCompression);
+                                                      c->anyBroken, c->initiation, c->failure,
+                                                      useCompression);
   HIPTRY(c, hipGetLastError());
   HIPTRY(c, hipStreamSynchronize(c->stream));
   return VPD_OK;
@@ -2157,16 +2174,22 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   g.vol = c->vol;
   g.bonds = sell_of(c);
   g.broken = c->broken;
+  g.anyBroken = c->anyBroken;
   g.batch = c->batch;
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   // The F pass is timed in the form the steps use (with the stretch test when damage is on);
   // the flags it may set are restored afterwards.
   uint32_t *savedFlags = nullptr;
+  uint8_t *savedAny = nullptr;
   const int64_t flagBytes = sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords);
+  const int64_t anyBytes = std::max<int64_t>(1, c->n);
   if (dmg) {
-    if (dalloc(c, savedFlags, std::max<int64_t>(1, c->brokenWords)))
+    if (dalloc(c, savedFlags, std::max<int64_t>(1, c->brokenWords)) ||
+        dalloc(c, savedAny, anyBytes))
       return VPD_ERR_DEVICE;
     HIPTRY(c, hipMemcpyAsync(savedFlags, c->broken, flagBytes, hipMemcpyDeviceToDevice,
+                             c->stream));
+    HIPTRY(c, hipMemcpyAsync(savedAny, c->anyBroken, anyBytes, hipMemcpyDeviceToDevice,
                              c->stream));
   }
   // Gather windows (0: direct gathers); a windowed pass reserves at least the LDS of its
@@ -2177,11 +2200,26 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   float best[2] = {1e30f, 1e30f};
   int pickLds[2] = {0, 0}, pickRounds[2] = {1, 1};
   Gather pick[2];
-  // Mean time of `reps` runs after a warming one; keeps the setting if it is the fastest yet.
-  auto trial = [&](int pass, int lds, int rounds, const Gather &gx) -> int {
+  struct Candidate {
+    int lds, rounds;
+    Gather gx;
+    float ms;
+  };
+  std::vector<Candidate> seen[2];
+  // Before every timed launch the caches are flushed by writing a buffer larger than the L2s
+  // and the 256 MB MALL together: in a step the other kernels evict a pass's data between
+  // its launches, and a pass timed on warm caches can rank the settings differently.
+  const int64_t flushBytes = int64_t(512) << 20;
+  uint8_t *flush = nullptr;
+  if (dalloc(c, flush, flushBytes))
+    return VPD_ERR_DEVICE;
+  // Mean time of `runs` launches after a warming one.
+  auto time_setting = [&](int pass, int lds, int rounds, const Gather &gx, int runs,
+                          float *mean) -> int {
     g.rd = rounds_of(c->n, rounds);
     float total = 0.f;
-    for (int r = 0; r <= reps; ++r) {
+    for (int r = 0; r <= runs; ++r) {
+      HIPTRY(c, hipMemsetAsync(flush, r & 0xFF, flushBytes, c->stream));
       HIPTRY(c, hipEventRecord(a, c->stream));
       if (pass == 0)
         launch_defgrad_with(c, dmg, lds, rounds, c->batch, gx);
@@ -2195,12 +2233,24 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
       if (r > 0) // the first run warms caches and code
         total += ms;
     }
-    if (total < best[pass]) {
-      best[pass] = total;
+    *mean = total / runs;
+    return VPD_OK;
+  };
+  auto keep_if_best = [&](int pass, int lds, int rounds, const Gather &gx, float ms) {
+    if (ms < best[pass]) {
+      best[pass] = ms;
       pickLds[pass] = lds;
       pickRounds[pass] = rounds;
       pick[pass] = gx;
     }
+  };
+  // One setting with `reps` runs; keeps it if it is the fastest yet.
+  auto trial = [&](int pass, int lds, int rounds, const Gather &gx) -> int {
+    float ms = 0.f;
+    if (int s = time_setting(pass, lds, rounds, gx, reps, &ms))
+      return s;
+    seen[pass].push_back(Candidate{lds, rounds, gx, ms});
+    keep_if_best(pass, lds, rounds, gx, ms);
     return VPD_OK;
   };
   auto walk = [](int win, int take) {
@@ -2212,7 +2262,8 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   // Stage 1: sweeps × LDS cap × window (walked, no bond limit per window); stage 2, at the
   // sweeps stage 1 chose: every window with each bond limit (walked, at stage 1's LDS cap),
   // and every planned form, window and bond limit with each LDS cap; stage 3: sweeps × LDS
-  // cap again for the form, window and limit chosen.
+  // cap again for the form, window and limit chosen; stage 4: the fastest few settings timed
+  // again with four times the runs, against timing noise between close ones.
   for (int pass = 0; pass < 2; ++pass) {
     for (int win : windowChoices)
       for (int lds : ldsChoices) {
@@ -2264,14 +2315,44 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
         if (int s = trial(pass, cap, r, chosen))
           return s;
       }
+    auto &cand = seen[pass];
+    std::stable_sort(cand.begin(), cand.end(),
+                     [](const Candidate &x, const Candidate &y) { return x.ms < y.ms; });
+    auto same = [](const Candidate &x, const Candidate &y) {
+      return x.lds == y.lds && x.rounds == y.rounds && x.gx.window == y.gx.window &&
+             x.gx.take == y.gx.take && x.gx.mode == y.gx.mode;
+    };
+    std::vector<Candidate> top;
+    for (const Candidate &x : cand) {
+      if (top.size() == 4)
+        break;
+      if (std::none_of(top.begin(), top.end(), [&](const Candidate &y) { return same(x, y); }))
+        top.push_back(x);
+    }
+    best[pass] = 1e30f;
+    for (Candidate x : top) {
+      if (x.gx.mode > 0) {
+        if (int s = build_plan(c, pass, x.gx.window, x.gx.take))
+          return s;
+        x.gx.plan = plan_of(c, pass);
+      }
+      float ms = 0.f;
+      if (int s = time_setting(pass, x.lds, x.rounds, x.gx, 4 * reps, &ms))
+        return s;
+      keep_if_best(pass, x.lds, x.rounds, x.gx, ms);
+    }
   }
+  dfree(flush);
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (savedFlags) {
     HIPTRY(c, hipMemcpyAsync(c->broken, savedFlags, flagBytes, hipMemcpyDeviceToDevice,
                              c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->anyBroken, savedAny, anyBytes, hipMemcpyDeviceToDevice,
+                             c->stream));
     HIPTRY(c, hipStreamSynchronize(c->stream));
     dfree(savedFlags);
+    dfree(savedAny);
   }
   c->ldsDefgrad = pickLds[0];
   c->roundsDefgrad = pickRounds[0];

@@ -464,11 +464,12 @@ __global__ void __launch_bounds__(256) k_shape(int64_t n, Planes x, const double
 // compression threshold armed, falls below −failure. Damage does not enter F (:181-196).
 // Bonds run in groups of 4 (one 16-byte column load); each bond's X_j and V_j come as two
 // component pairs, u_j as two more. A group's new breaks are set with one atomic OR on
-// the flag word the lane shares with 7 others.
+// the flag word the lane shares with 7 others, and the particle is marked in anyBroken.
 template <bool DAMAGE>
 __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, UPairs u,
                                             Sell bonds, Planes kinv, Planes fout,
-                                            uint32_t *broken, const StretchTest &test) {
+                                            uint32_t *broken, uint8_t *anyBroken,
+                                            const StretchTest &test) {
   const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
   const double yi0 = x0 + u(0, i), yi1 = x1 + u(1, i), yi2 = x2 + u(2, i);
   double num[9];
@@ -503,8 +504,10 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, 
         }
       }
     }
-    if (DAMAGE && fresh)
+    if (DAMAGE && fresh) {
       atomicOr(&broken[pg >> 5], fresh);
+      anyBroken[i] = 1;
+    }
   }
   T3 nm, ki;
 #pragma unroll
@@ -521,8 +524,8 @@ __device__ __forceinline__ void defgrad_one(int64_t i, Planes x, PairPlanes xv, 
 template <bool DAMAGE>
 __global__ void __launch_bounds__(256)
     k_defgrad(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv, Planes fout,
-              uint32_t *broken, double initiation, double failure, int useCompression,
-              Batch batch) {
+              uint32_t *broken, uint8_t *anyBroken, double initiation, double failure,
+              int useCompression, Batch batch) {
   const int64_t blk = xcd_block(rd.blocksPerRound);
   if (blk < 0)
     return;
@@ -533,7 +536,7 @@ __global__ void __launch_bounds__(256)
       break;
     if (!point_active(batch, i))
       continue; // a finished specimen keeps its state
-    defgrad_one<DAMAGE>(i, x, xv, u, bonds, kinv, fout, broken, test);
+    defgrad_one<DAMAGE>(i, x, xv, u, bonds, kinv, fout, broken, anyBroken, test);
   }
 }
 
@@ -558,7 +561,7 @@ __device__ __forceinline__ void store_amap(PairPlanes amap, int64_t i, const T3 
 // and set with one atomic OR when the lane leaves the group.
 template <bool DAMAGE, int W>
 __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes xv, UPairs u,
-                                               Sell bonds, uint32_t *broken,
+                                               Sell bonds, uint32_t *broken, uint8_t *anyBroken,
                                                const StretchTest &test, d2v *win, int take,
                                                double x0, double x1, double x2, double yi0,
                                                double yi1, double yi2, double num[9]) {
@@ -588,8 +591,10 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
         if (!cur.bond_broken() && stretch_breaks(test, xi, yij))
           fresh |= bit;
         if (((cur.k + 1) & 3) == 0 || cur.k + 1 >= cur.len) { // leaving the group
-          if (fresh)
+          if (fresh) {
             atomicOr(&broken[cur.group_slot() >> 5], fresh);
+            anyBroken[i] = 1;
+          }
           fresh = 0;
         }
       }
@@ -602,8 +607,8 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
 template <bool DAMAGE, int W>
 __global__ void __launch_bounds__(256)
     k_defgrad_w(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
-                Planes fout, uint32_t *broken, double initiation, double failure,
-                int useCompression, Batch batch, int take) {
+                Planes fout, uint32_t *broken, uint8_t *anyBroken, double initiation,
+                double failure, int useCompression, Batch batch, int take) {
   extern __shared__ d2v wlds[];
   d2v *win = wlds + wave_slot() * (4 * W);
   const int64_t blk = xcd_block(rd.blocksPerRound);
@@ -628,8 +633,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
     for (int c = 0; c < 9; ++c)
       num[c] = 0.0;
-    defgrad_window<DAMAGE, W>(rd.n, mine ? i : -1, xv, u, bonds, broken, test, win, take, x0,
-                              x1, x2, yi0, yi1, yi2, num);
+    defgrad_window<DAMAGE, W>(rd.n, mine ? i : -1, xv, u, bonds, broken, anyBroken, test, win,
+                              take, x0, x1, x2, yi0, yi1, yi2, num);
     if (!mine)
       continue;
     T3 nm, ki;
@@ -754,7 +759,8 @@ __device__ __forceinline__ int plan_load(const int *p, int64_t k) { return ((con
 // The window loop of a planned bond pass for the lane's particle (mine false: a lane with no
 // particle, which still stages its share). body(buf, r, brk, k, g0) sums the lane's bonds of
 // one window: `buf` holds the window, `r` the record, `brk` the flag bits from the lane's
-// bond k on (with FLAGS; 0 otherwise), g0 the slot of its bond 0. Without PREFETCH the windows go through the
+// bond k on (with FLAGS, for a lane whose particle is `flagged`; 0 otherwise), g0 the slot of
+// its bond 0. Without PREFETCH the windows go through the
 // wavefront's part of the dynamic LDS, `win`; with it, through two static buffers, the next
 // window being staged into one while the current one is summed from the other. (The buffers
 // are separate LDS objects and the loop alternates them explicitly, so the compiler can tell
@@ -763,7 +769,7 @@ template <int PAIRS, int W, bool PREFETCH, bool FLAGS, class Planeset, class Bod
 __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t n,
                                                 const WindowPlan &plan, const Sell &bonds,
                                                 const uint32_t *broken, int64_t i, bool mine,
-                                                d2v *win, Body &&body) {
+                                                bool flagged, d2v *win, Body &&body) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)((i - lane) >> 6));
   int t = plan_load(plan.first, wave);
@@ -778,7 +784,7 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
     // pass and come from HBM); the wait at the end of a window is for its LDS reads only.
     uint64_t r = mine && t < t1 ? plan.rec[(int64_t)t * 64 + lane] : ~0ull;
     FlagWords fw{};
-    if (FLAGS && mine && t < t1)
+    if (FLAGS && flagged && t < t1)
       fw = plan_words(broken, g0, k, len);
     for (; t < t1; ++t) {
       const int w0 = wNext;
@@ -791,7 +797,7 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
       const int kNext = k + record_bonds(rc);
       r = ~0ull;
       if (t + 1 < t1 && mine) {
-        if (FLAGS)
+        if (FLAGS && flagged)
           fw = plan_words(broken, g0, kNext, len);
         r = plan.rec[(int64_t)(t + 1) * 64 + lane];
       }
@@ -810,7 +816,7 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
     if (t < t1) {
       stage_window<PAIRS, W>(planes, n, wNext, a);
       r = mine ? plan.rec[(int64_t)t * 64 + lane] : ~0ull;
-      if (FLAGS && mine)
+      if (FLAGS && flagged)
         fw = plan_words(broken, g0, k, len);
       if (t + 1 < t1)
         wNext = plan_load(plan.start, t + 1);
@@ -825,7 +831,7 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
       if (t + 1 < t1) {
         stage_window<PAIRS, W>(planes, n, wNext, next);
         r = mine ? plan.rec[(int64_t)(t + 1) * 64 + lane] : ~0ull;
-        if (FLAGS && mine)
+        if (FLAGS && flagged)
           fw = plan_words(broken, g0, kNext, len);
         if (t + 2 < t1)
           wNext = plan_load(plan.start, t + 2);
@@ -850,8 +856,8 @@ __device__ __forceinline__ void planned_windows(const Planeset &planes, int64_t 
 template <bool DAMAGE, int W, bool PREFETCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFETCH ? 1 : 5)))
     k_defgrad_p(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
-                Planes fout, uint32_t *broken, double initiation, double failure,
-                int useCompression, Batch batch, WindowPlan plan) {
+                Planes fout, uint32_t *broken, uint8_t *anyBroken, double initiation,
+                double failure, int useCompression, Batch batch, WindowPlan plan) {
   extern __shared__ d2v wlds[];
   d2v *win = wlds + wave_slot() * (4 * W); // (unused with PREFETCH)
   const int64_t blk = xcd_block(rd.blocksPerRound);
@@ -880,7 +886,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
     // nothing, so the flag words are read only in the windows where a bond breaks, to keep
     // the atomics to the new breaks.
     planned_windows<4, W, PREFETCH, false>(
-        DefgradPlanes{xv, u}, rd.n, plan, bonds, broken, i, mine, win,
+        DefgradPlanes{xv, u}, rd.n, plan, bonds, broken, i, mine, false, win,
         [&](const d2v *buf, uint64_t rec, uint32_t, int k, int64_t g0) {
           uint32_t fresh = 0;
           for (int q = 0; q < kPlanTake; ++q) {
@@ -903,8 +909,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
           if (DAMAGE && fresh) { // (only a lane with a particle has bonds)
             const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
             fresh &= ~plan_bits(plan_words(broken, g0, k, len), g0, k);
-            if (fresh)
+            if (fresh) {
               plan_flush(broken, g0, k, fresh);
+              anyBroken[i] = 1;
+            }
           }
         });
     if (!mine)
@@ -1281,6 +1289,9 @@ struct ForceArgs {
   const double *vol;
   Sell bonds;
   const uint32_t *broken;
+  // Per particle: 1 when any of its bonds may be broken (set with every new break, cleared
+  // with the flags; a particle marked with no broken bond only costs the flag reads).
+  const uint8_t *anyBroken;
   const uint8_t *codes;
   double dt, density;
   Ramp rampNow, rampNext;
@@ -1550,8 +1561,10 @@ __global__ void __launch_bounds__(256) k_force_p(ForceArgs g, WindowPlan plan) {
       ai = load_amap(g.amap, i);
     }
     double acc[3] = {0.0, 0.0, 0.0};
+    // Only a particle marked in anyBroken can have broken bonds: the others skip the flags.
     planned_windows<7, W, PREFETCH, DAMAGE>(
-        ForcePlanes{g.xv, g.amap}, g.n, plan, g.bonds, g.broken, i, mine, win,
+        ForcePlanes{g.xv, g.amap}, g.n, plan, g.bonds, g.broken, i, mine,
+        DAMAGE && mine && g.anyBroken[i] != 0, win,
         [&](const d2v *buf, uint64_t rec, uint32_t brk, int, int64_t) {
           for (int q = 0; q < kPlanTake; ++q) {
             const int o = (int)(rec & 0xFFu);
@@ -1693,8 +1706,9 @@ __global__ void __launch_bounds__(256) k_correct(int64_t n, Planes v, Planes a, 
 
 // The bond loop of PdSimpleDamage.cpp:63-84 on its own (critical stretch).
 __global__ void __launch_bounds__(256) k_bond_damage(int64_t n, Planes x, UPairs u, Sell bonds,
-                                                     uint32_t *broken, double initiation,
-                                                     double failure, int useCompression) {
+                                                     uint32_t *broken, uint8_t *anyBroken,
+                                                     double initiation, double failure,
+                                                     int useCompression) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n)
     return;
@@ -1711,8 +1725,10 @@ __global__ void __launch_bounds__(256) k_bond_damage(int64_t n, Planes x, UPairs
     const double xj0 = x(0, j), xj1 = x(1, j), xj2 = x(2, j);
     const double xi[3] = {xj0 - x0, xj1 - x1, xj2 - x2};
     const double yij[3] = {(xj0 + u(0, j)) - yi0, (xj1 + u(1, j)) - yi1, (xj2 + u(2, j)) - yi2};
-    if (stretch_breaks(test, xi, yij))
+    if (stretch_breaks(test, xi, yij)) {
       atomicOr(&broken[p >> 5], bit);
+      anyBroken[i] = 1;
+    }
   }
 }
 
