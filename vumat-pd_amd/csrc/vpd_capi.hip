@@ -2147,10 +2147,11 @@ int vpd_get_tuning(vpd_context *c, int knob, int *value) {
 }
 
 // Times both bond passes over a grid of (sweeps per pass) × (workgroups per CU) × (gather
-// window), then the windows with per-window bond limits, and keeps the fastest for each. The
-// timing runs leave the state as it was: F goes to its scratch plane (with the stretch test
-// when damage is on; the bond flags are restored afterwards), and the force gather runs
-// without the integrator.
+// window), then the windows with per-window bond limits and the planned forms, and keeps the
+// fastest for each. Each pass is timed in the form the steps use, and the timing runs leave
+// the state as it was: F goes to its scratch plane (with the stretch test when damage is on;
+// the bond flags and marks are restored afterwards), and the force pass integrates copies of
+// u, v and a.
 int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   if (int s = require_ready(c))
     return s;
@@ -2164,17 +2165,37 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   hipEvent_t a, b;
   HIPTRY(c, hipEventCreate(&a));
   HIPTRY(c, hipEventCreate(&b));
+  // The force pass is timed in the form the steps use (integrating, with the next prediction
+  // fused), on copies of u, v and a so that the state is left as it was.
+  double *su = nullptr, *sv = nullptr, *sa = nullptr;
+  if (dalloc(c, su, 4 * c->n) || dalloc(c, sv, 3 * c->n) || dalloc(c, sa, 3 * c->n))
+    return VPD_ERR_DEVICE;
+  HIPTRY(c, hipMemcpyAsync(su, c->u, sizeof(double) * 4 * c->n, hipMemcpyDeviceToDevice,
+                           c->stream));
+  HIPTRY(c, hipMemcpyAsync(sv, c->v, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice,
+                           c->stream));
+  HIPTRY(c, hipMemcpyAsync(sa, c->a, sizeof(double) * 3 * c->n, hipMemcpyDeviceToDevice,
+                           c->stream));
   ForceArgs g{};
   g.n = c->n;
   g.nblocks = ceil_div(c->n, 256);
   g.x = planes(c->x, c->n);
+  g.u = UPairs{su, c->n};
+  g.v = planes(sv, c->n);
+  g.a = planes(sa, c->n);
   g.amap = pairs(c->amap, c->n);
   g.xv = pairs(c->xv, c->n);
-  g.forceOut = planes(c->force, c->n);
+  g.forceOut = planes(nullptr, c->n);
   g.vol = c->vol;
   g.bonds = sell_of(c);
   g.broken = c->broken;
   g.anyBroken = c->anyBroken;
+  g.codes = c->codes;
+  g.dt = 1.0e-9;
+  g.density = c->density;
+  g.rampNow = Ramp{0, 0, 0};
+  g.rampNext = Ramp{0, 0, 0};
+  g.diag = c->diag;
   g.batch = c->batch;
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   // The F pass is timed in the form the steps use (with the stretch test when damage is on);
@@ -2207,8 +2228,11 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   };
   std::vector<Candidate> seen[2];
   // Before every timed launch the caches are flushed by writing a buffer larger than the L2s
-  // and the 256 MB MALL together: in a step the other kernels evict a pass's data between
-  // its launches, and a pass timed on warm caches can rank the settings differently.
+  // and the 256 MB MALL together, and then the plane that the kernel before the pass in a
+  // step writes is read once (u before the F pass, written by the force pass's prediction;
+  // A before the force pass, written by the material update): in a step the other kernels
+  // evict the rest of a pass's data between its launches, and a pass timed on warm caches
+  // ranks the settings differently.
   const int64_t flushBytes = int64_t(512) << 20;
   uint8_t *flush = nullptr;
   if (dalloc(c, flush, flushBytes))
@@ -2220,11 +2244,15 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
     float total = 0.f;
     for (int r = 0; r <= runs; ++r) {
       HIPTRY(c, hipMemsetAsync(flush, r & 0xFF, flushBytes, c->stream));
+      const int64_t warmBytes = (int64_t)sizeof(double) * (pass == 0 ? 4 : 10) * c->n;
+      HIPTRY(c, hipMemcpyAsync(flush, pass == 0 ? c->u : c->amap,
+                               std::min(warmBytes, flushBytes), hipMemcpyDeviceToDevice,
+                               c->stream));
       HIPTRY(c, hipEventRecord(a, c->stream));
       if (pass == 0)
         launch_defgrad_with(c, dmg, lds, rounds, c->batch, gx);
       else
-        launch_force_with(c, g, dmg, false, false, lds, false, gx);
+        launch_force_with(c, g, dmg, true, true, lds, false, gx);
       HIPTRY(c, hipGetLastError());
       HIPTRY(c, hipEventRecord(b, c->stream));
       HIPTRY(c, hipEventSynchronize(b));
@@ -2343,6 +2371,10 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
     }
   }
   dfree(flush);
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  dfree(su);
+  dfree(sv);
+  dfree(sa);
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (savedFlags) {

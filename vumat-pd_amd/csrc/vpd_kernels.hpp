@@ -752,6 +752,11 @@ __device__ __forceinline__ void plan_flush(uint32_t *broken, int64_t g0, int k, 
   }
 }
 
+// The first double of a staged pair (an 8-byte LDS read where the second is padding).
+__device__ __forceinline__ double lds_first(const d2v *p) {
+  return *reinterpret_cast<const double *>(p);
+}
+
 // Uniform loads of the plan through the constant address space: scalar loads.
 typedef __attribute__((address_space(4))) const int plan_int;
 __device__ __forceinline__ int plan_load(const int *p, int64_t k) { return ((const plan_int *)p)[k]; }
@@ -893,10 +898,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
             const int o = (int)(rec & 0xFFu);
             if (o == 0xFF)
               break;
-            const d2v a = buf[o], b = buf[W + o], ua = buf[2 * W + o], ub = buf[3 * W + o];
+            const d2v a = buf[o], b = buf[W + o], ua = buf[2 * W + o];
+            const double uz = lds_first(buf + 3 * W + o); // (u_z, 0): 8 bytes suffice
             const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
-            const double yij[3] = {(a.x + ua.x) - yi0, (a.y + ua.y) - yi1,
-                                   (b.x + ub.x) - yi2};
+            const double yij[3] = {(a.x + ua.x) - yi0, (a.y + ua.y) - yi1, (b.x + uz) - yi2};
 #pragma unroll
             for (int rr = 0; rr < 3; ++rr)
 #pragma unroll
@@ -1538,7 +1543,8 @@ __global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
 
 // The force pass with planned windows (7 component pairs: X_j, V_j, A_j; see WindowPlan).
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W, bool PREFETCH>
-__global__ void __launch_bounds__(256) k_force_p(ForceArgs g, WindowPlan plan) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFETCH ? 1 : 5)))
+    k_force_p(ForceArgs g, WindowPlan plan) {
   extern __shared__ d2v wlds[];
   d2v *win = wlds + wave_slot() * (7 * W); // (unused with PREFETCH)
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
@@ -1579,7 +1585,7 @@ __global__ void __launch_bounds__(256) k_force_p(ForceArgs g, WindowPlan plan) {
                 aj.m[2 * c] = v.x;
                 aj.m[2 * c + 1] = v.y;
               }
-              aj.m[8] = buf[6 * W + o].x;
+              aj.m[8] = lds_first(buf + 6 * W + o); // (A_8, 0): 8 bytes suffice
               const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
               double term[3];
               force_bond_term(ai, aj, xi, b.y, term);
