@@ -281,16 +281,24 @@ struct JcParams {
   double mu, lambda, tolerance;
 };
 
-// JohnsonCook.f90:160-170
-__device__ __forceinline__ double jc_yield(const JcParams &p, double ep, double rate,
-                                           double temp) {
+// JohnsonCook.f90:160-170, split in the thermal factor 1 − θᵐ (which a return-map solve,
+// at fixed temperature, evaluates once) and the rest, multiplied in the reference's order:
+// (A + B ε̄ⁿ)·(1 − θᵐ)·(1 + C ln(ε̇/ε̇₀)).
+__device__ __forceinline__ double jc_softening(const JcParams &p, double temp) {
   double theta = (temp - p.tref) / (p.tmelt - p.tref);
   theta = fmin(1.0, fmax(0.0, theta));
+  return 1.0 - pow(theta, p.thermalM);
+}
+__device__ __forceinline__ double jc_yield_soft(const JcParams &p, double ep, double rate,
+                                                double soft) {
   double er = rate;
   if (er == 0.0)
     er = p.rate0;
-  return (p.a0 + p.b0 * pow(ep, p.hardN)) * (1.0 - pow(theta, p.thermalM)) *
-         (1.0 + p.rateC * log(er / p.rate0));
+  return (p.a0 + p.b0 * pow(ep, p.hardN)) * soft * (1.0 + p.rateC * log(er / p.rate0));
+}
+__device__ __forceinline__ double jc_yield(const JcParams &p, double ep, double rate,
+                                           double temp) {
+  return jc_yield_soft(p, ep, rate, jc_softening(p, temp));
 }
 
 // Return-map solvers of the Johnson-Cook stage.
@@ -340,12 +348,9 @@ struct ReturnMapStats {
   int fallbacks = 0; // Newton steps replaced by bisection
 };
 
-// Y and dY/dΔ at Δ > 0 (JohnsonCook.f90:160-170 differentiated in Δ).
+// Y and dY/dΔ at Δ > 0 (JohnsonCook.f90:160-170 differentiated in Δ); soft = 1 − θᵐ.
 __device__ __forceinline__ void jc_yield_slope(const JcParams &p, double epOld, double dep,
-                                               double dt, double temp, double &Y, double &dY) {
-  double theta = (temp - p.tref) / (p.tmelt - p.tref);
-  theta = fmin(1.0, fmax(0.0, theta));
-  const double soft = 1.0 - pow(theta, p.thermalM);
+                                               double dt, double soft, double &Y, double &dY) {
   const double ep = epOld + dep;
   const double epn = pow(ep, p.hardN);
   const double hard = p.a0 + p.b0 * epn;
@@ -437,11 +442,12 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
   if (MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT) {
     const double tol = MODE == RM_BISECTION ? p.tolerance : rtol * qTrial;
     const int maxIt = MODE == RM_BISECTION ? 50 : 200;
+    const double soft = jc_softening(p, temperature); // the same at every iteration
     double epLower = 0.0, epUpper = qTrial / (2.0 * p.mu);
     while (iteration < maxIt) { // JohnsonCook.f90:119-136
       iteration = iteration + 1;
       const double epRate = epInc / dt;
-      yield = jc_yield(p, epOld + epInc, epRate, temperature);
+      yield = jc_yield_soft(p, epOld + epInc, epRate, soft);
       yield = fmax(yield, oldYield);
       const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
       const double residual = qNew - yield;
@@ -460,7 +466,8 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
   } else {
     // Iteration 1 tests Δ = 0 exactly as the reference does (elastic exit).
     iteration = 1;
-    yield = fmax(jc_yield(p, epOld, 0.0, temperature), oldYield);
+    const double soft = jc_softening(p, temperature);
+    yield = fmax(jc_yield_soft(p, epOld, 0.0, soft), oldYield);
     const double r0 = fmax(0.0, qTrial) - yield;
     if (r0 > 0.0 && fabs(r0) >= rtol * qTrial) {
       double lo = 0.0, hi = qTrial / (3.0 * p.mu);
@@ -471,7 +478,7 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
       for (int it = 0; it < 60; ++it) {
         ++iteration;
         double Y, dY;
-        jc_yield_slope(p, epOld, d, dt, temperature, Y, dY);
+        jc_yield_slope(p, epOld, d, dt, soft, Y, dY);
         const bool capped = Y < oldYield;
         const double Yc = capped ? oldYield : Y;
         const double r = (qTrial - 3.0 * p.mu * d) - Yc;
