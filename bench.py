@@ -143,9 +143,9 @@ def return_map_study(points, seed=4321):
     z = np.zeros(m)
     dt = ws.cfl_dt(1.0)
     props = ws.JC_PROPS
-    judge = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, props, dt, inc, stress, state, z, z,
+    exact = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, props, dt, inc, stress, state, z, z,
                            rtol=1e-14)
-    scale = np.abs(judge["stress"]).max()
+    scale = np.abs(exact["stress"]).max()
     net = vpd.load_warm_start()
     out = {"points": m, "source": "plastic points of 30x8x8 JC bar plain runs, seed %d" % seed}
     for name, mode in (("bisection_parity", vpd.RETURN_MAP_BISECTION),
@@ -164,13 +164,13 @@ def return_map_study(points, seed=4321):
                 best = (total, r)
         total, r = best
         row = {"ms": total, "mean_iterations": float(r["iterations"].mean()),
-               "max_stress_err_rel": float(np.abs(r["stress"] - judge["stress"]).max() / scale),
-               "max_eqps_inc_err": float(np.abs(r["state"][4] - judge["state"][4]).max()),
+               "max_stress_err_rel": float(np.abs(r["stress"] - exact["stress"]).max() / scale),
+               "max_eqps_inc_err": float(np.abs(r["state"][4] - exact["state"][4]).max()),
                "newton_fallbacks_per_point": float(r["fallbacks"].mean())}
         if mode == vpd.RETURN_MAP_LEARNED:
             row["seed_ms"] = r["seed_ms"]
             row["seed_median_rel_err"] = float(np.median(
-                np.abs(r["seed"] - judge["state"][4]) / np.maximum(judge["state"][4], 1e-30)))
+                np.abs(r["seed"] - exact["state"][4]) / np.maximum(exact["state"][4], 1e-30)))
         out[name] = row
     learned = out["newton_learned_seed"]["ms"]
     out["speedup_learned_vs_tight_bisection"] = out["bisection_tight"]["ms"] / learned
@@ -282,8 +282,8 @@ def main():
                     help="measure the 48-bar batched sweep instead (JSON on stdout)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this path is not sharded across GPUs "
-              "(see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: this path is not sharded across GPUs. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if args.sweep:
         out = sweep_measure(min(args.steps, 100), min(args.warmup, 5))

@@ -38,7 +38,7 @@ extern "C" {
 
 // JohnsonCook.f90:12-156 with the bisection's stopping rule as a parameter: rtol <= 0 keeps
 // the reference's |r| < E·1e-9 and 50 iterations; rtol > 0 stops at |r| < rtol·q_trial or
-// maxIt iterations (the tightly converged judge of the fast return maps). Returns 0, or 1
+// maxIt iterations (the tightly converged solution the fast return maps are checked on). Returns 0, or 1
 // for the dimension errors on which the Fortran routine executes STOP (JohnsonCook.f90:52-53).
 int orc_vumat_jc_tol(int nblock, int ndir, int nshr, int nstatev, int nfieldv, int nprops,
                      const double *dtArray, const double *props, const double *strainInc,

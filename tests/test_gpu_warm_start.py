@@ -207,9 +207,9 @@ def test_trained_warm_start_gain(vpd):
     state = np.ascontiguousarray(rec["state"].T)
     z = np.zeros(m)
     dt = ws.cfl_dt(1.0)
-    judge = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, ws.JC_PROPS, dt, inc, stress, state,
+    exact = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, ws.JC_PROPS, dt, inc, stress, state,
                            z, z, rtol=1e-14)
-    scale = np.abs(judge["stress"]).max()
+    scale = np.abs(exact["stress"]).max()
     runs = {
         "learned": vpd.return_map_learned(net, ws.JC_PROPS, dt, inc, stress, state, z, z),
         "no_seed": vpd.return_map(vpd.RETURN_MAP_LEARNED, ws.JC_PROPS, dt, inc, stress, state,
@@ -220,11 +220,11 @@ def test_trained_warm_start_gain(vpd):
                                           stress, state, z, z),
     }
     it = {k: float(r["iterations"].mean()) for k, r in runs.items()}
-    seed_err = np.median(np.abs(runs["learned"]["seed"] - judge["state"][4]) /
-                         judge["state"][4])
+    seed_err = np.median(np.abs(runs["learned"]["seed"] - exact["state"][4]) /
+                         exact["state"][4])
     print("mean iterations:", it, "median seed error %.3g" % seed_err)
     for k, r in runs.items():
-        assert np.abs(r["stress"] - judge["stress"]).max() <= 1e-10 * scale, k
+        assert np.abs(r["stress"] - exact["stress"]).max() <= 1e-10 * scale, k
     assert seed_err < 0.05
     assert it["learned"] < 0.8 * it["no_seed"]
     assert it["learned"] < 0.25 * it["bisection_tight"]

@@ -94,7 +94,7 @@ def log_pairs(target_pairs, seed=1234, steps=2000, nx=30, ny=8, nz=8, spacing=1.
 
 def evaluate(net, records, dt=None, limit=200000):
     """Return-map solvers on logged plastic points: mean iterations, kernel time and the
-    largest stress deviation from the tight bisection (the accuracy judge), per mode."""
+    largest stress deviation from the tight bisection (the accuracy reference), per mode."""
     dt = cfl_dt(1.0) if dt is None else dt
     m = min(limit, len(records["y"]))
     inc = np.ascontiguousarray(records["inc"][:m].T)
@@ -107,16 +107,16 @@ def evaluate(net, records, dt=None, limit=200000):
     xn = (records["x"][:m] - norm[:8]) / norm[8:16]  # FP64, as the seed kernels do
     seed = np.maximum(net.predict(xn).astype(np.float64) * norm[17] + norm[16], 0.0)
     out = {}
-    judge = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, JC_PROPS, dt, inc, stress, state, z,
+    exact = vpd.return_map(vpd.RETURN_MAP_BISECTION_TIGHT, JC_PROPS, dt, inc, stress, state, z,
                            z, rtol=1e-14)
-    scale = np.abs(judge["stress"]).max()
+    scale = np.abs(exact["stress"]).max()
     for name, mode, sd in (("bisection", vpd.RETURN_MAP_BISECTION, None),
                            ("bisection_tight", vpd.RETURN_MAP_BISECTION_TIGHT, None),
                            ("newton", vpd.RETURN_MAP_NEWTON, None),
                            ("learned", vpd.RETURN_MAP_LEARNED, seed)):
         r = vpd.return_map(mode, JC_PROPS, dt, inc, stress, state, z, z, seed=sd)
         out[name] = dict(mean_iterations=float(r["iterations"].mean()), ms=r["ms"],
-                         max_stress_err_rel=float(np.abs(r["stress"] - judge["stress"]).max()
+                         max_stress_err_rel=float(np.abs(r["stress"] - exact["stress"]).max()
                                                   / scale),
                          fallbacks=float(r["fallbacks"].mean()))
     out["seed_rel_err_median"] = float(np.median(np.abs(seed - records["y"][:m]) /
