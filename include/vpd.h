@@ -254,7 +254,7 @@ typedef struct vpd_mlp vpd_mlp;
 int vpd_attach_warm_start(vpd_context *ctx, vpd_mlp *mlp);
 
 /* ---- launch tuning ---------------------------------------------------------------- */
-/* Three knobs per bond pass (no reference counterpart):
+/* Knobs per bond pass (no reference counterpart):
  *  - sweeps: the pass walks the particle range in this many consecutive sweeps, so the
  *    neighbour data an XCD touches at once (its stretch of the sweep plus a horizon either
  *    side) can fit its L2;
