@@ -11,6 +11,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <string>
 #include <type_traits>
@@ -2162,12 +2163,31 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   // LDS reservations giving at most 8 (no limit), 4, 3 and 2 workgroups per CU.
   const int ldsChoices[] = {0, 36408, 50000, 64000};
   const int roundChoices[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32};
-  hipEvent_t a, b;
+  hipEvent_t a = nullptr, b = nullptr;
+  double *su = nullptr, *sv = nullptr, *sa = nullptr;
+  uint8_t *flush = nullptr, *savedAny = nullptr;
+  uint32_t *savedFlags = nullptr;
+  // Whatever way the function leaves, its events and scratch buffers go with it.
+  struct Release {
+    std::function<void()> f;
+    ~Release() { f(); }
+  } release{[&] {
+    if (a)
+      (void)hipEventDestroy(a);
+    if (b)
+      (void)hipEventDestroy(b);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(su);
+    dfree(sv);
+    dfree(sa);
+    dfree(flush);
+    dfree(savedFlags);
+    dfree(savedAny);
+  }};
   HIPTRY(c, hipEventCreate(&a));
   HIPTRY(c, hipEventCreate(&b));
   // The force pass is timed in the form the steps use (integrating, with the next prediction
   // fused), on copies of u, v and a so that the state is left as it was.
-  double *su = nullptr, *sv = nullptr, *sa = nullptr;
   if (dalloc(c, su, 4 * c->n) || dalloc(c, sv, 3 * c->n) || dalloc(c, sa, 3 * c->n))
     return VPD_ERR_DEVICE;
   HIPTRY(c, hipMemcpyAsync(su, c->u, sizeof(double) * 4 * c->n, hipMemcpyDeviceToDevice,
@@ -2200,8 +2220,6 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   // The F pass is timed in the form the steps use (with the stretch test when damage is on);
   // the flags it may set are restored afterwards.
-  uint32_t *savedFlags = nullptr;
-  uint8_t *savedAny = nullptr;
   const int64_t flagBytes = sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords);
   const int64_t anyBytes = std::max<int64_t>(1, c->n);
   if (dmg) {
@@ -2234,7 +2252,6 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   // other kernels evict the rest of a pass's data between its launches, and a pass timed on
   // warm caches ranks the settings differently.
   const int64_t flushBytes = int64_t(512) << 20;
-  uint8_t *flush = nullptr;
   if (dalloc(c, flush, flushBytes))
     return VPD_ERR_DEVICE;
   // Mean time of `runs` launches after a warming one.
@@ -2390,22 +2407,13 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
         pick[pass] = x.gx;
       }
   }
-  dfree(flush);
-  HIPTRY(c, hipStreamSynchronize(c->stream));
-  dfree(su);
-  dfree(sv);
-  dfree(sa);
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   if (savedFlags) {
     HIPTRY(c, hipMemcpyAsync(c->broken, savedFlags, flagBytes, hipMemcpyDeviceToDevice,
                              c->stream));
     HIPTRY(c, hipMemcpyAsync(c->anyBroken, savedAny, anyBytes, hipMemcpyDeviceToDevice,
                              c->stream));
-    HIPTRY(c, hipStreamSynchronize(c->stream));
-    dfree(savedFlags);
-    dfree(savedAny);
   }
+  HIPTRY(c, hipStreamSynchronize(c->stream));
   c->ldsDefgrad = pickLds[0];
   c->roundsDefgrad = pickRounds[0];
   c->ldsForce = pickLds[1];
