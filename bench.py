@@ -276,6 +276,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="launch knobs as JSON (the launch_tuning of an earlier run) instead of "
+                         "autotuning: a profiled run then holds the step's launches only")
     ap.add_argument("--no-warm-start", action="store_true")
     ap.add_argument("--return-map-points", type=int, default=1000000)
     ap.add_argument("--sweep", action="store_true",
@@ -301,7 +304,11 @@ def main():
                   velocity=w["velocity"], damage=w["damage"])
     n, nnz = bar.n, bar.ctx.nnz
     bbar = nnz / n
-    tuned = bar.ctx.get_tuning() if args.no_autotune else bar.ctx.autotune(3)
+    if args.tuning:
+        bar.ctx.set_tuning(**json.loads(args.tuning))
+        tuned = bar.ctx.get_tuning()
+    else:
+        tuned = bar.ctx.get_tuning() if args.no_autotune else bar.ctx.autotune(3)
     print("[bench] launch tuning %s" % tuned, file=sys.stderr, flush=True)
     print("[bench] timing %d plain steps" % args.steps, file=sys.stderr, flush=True)
     elapsed, kt = timed_run(bar, args.steps, args.warmup)
