@@ -281,6 +281,10 @@ def test_fused_equals_unfused(vpd, orc):
          defgrad_window_plan=2, force_window_plan=2, defgrad_rounds=3, force_rounds=2),
     dict(defgrad_window=80, force_window=128, defgrad_window_bonds=1, force_window_bonds=2,
          defgrad_window_plan=2, force_window_plan=1, force_lds=64000),
+    dict(defgrad_window=512, force_window=384, defgrad_window_plan=3, force_window_plan=3,
+         defgrad_rounds=2, force_rounds=3),
+    dict(defgrad_window=256, force_window=256, defgrad_window_plan=3, force_window_plan=3,
+         defgrad_lds=64000, force_lds=50000),
 ])
 def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
     # Launch knobs change how the bond passes are scheduled, never what they sum or in which
@@ -300,7 +304,8 @@ def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
 
 
 @pytest.mark.parametrize("voids,take,plan", [(0.0, 0, 0), (0.08, 0, 0), (0.08, 5, 0), (0.0, 2, 0),
-                                            (0.08, 6, 1), (0.08, 7, 2), (0.0, 8, 2)])
+                                            (0.08, 6, 1), (0.08, 7, 2), (0.0, 8, 2), (0.08, 0, 3),
+                                            (0.0, 0, 3)])
 def test_windowed_gathers_on_irregular_rows(vpd, voids, take, plan):
     # Voids make every row irregular: the windows then cover fewer bonds per step, and the
     # results must still equal the direct gathers bit for bit, with or without a limit on the
@@ -312,7 +317,8 @@ def test_windowed_gathers_on_irregular_rows(vpd, voids, take, plan):
               density=7.85e-3, velocity=300.0, damage=(0.002, 0.0), points=points)
     a = vpd.Bar(**kw)
     b = vpd.Bar(**kw)
-    b.ctx.set_tuning(defgrad_window=64, force_window=64, defgrad_window_bonds=take,
+    window = 256 if plan == 3 else 64
+    b.ctx.set_tuning(defgrad_window=window, force_window=window, defgrad_window_bonds=take,
                      force_window_bonds=take, defgrad_window_plan=plan, force_window_plan=plan)
     a.step(50, want_fields=True)
     b.step(50, want_fields=True)
@@ -327,6 +333,8 @@ FORMS = {
                       force_window_bonds=1, defgrad_window_plan=1, force_window_plan=1),
     "planned-prefetch": dict(defgrad_window=80, force_window=96, defgrad_window_bonds=3,
                              force_window_bonds=8, defgrad_window_plan=2, force_window_plan=2),
+    "group": dict(defgrad_window=384, force_window=256, defgrad_window_plan=3,
+                  force_window_plan=3),
 }
 
 
@@ -349,6 +357,36 @@ def test_gather_forms_on_small_blocks(vpd, shape):
             assert np.array_equal(b.get(field), ref.get(field)), (name, field)
 
 
+def test_gather_forms_on_unsorted_rows(vpd):
+    # Bond lists supplied with every row shuffled: each lane still sums its row in the order
+    # given, so every gather form equals the direct gathers bit for bit (a window takes a
+    # bond only while its neighbour lies inside the window, also when it lies below the
+    # window's start).
+    banner("gather forms on shuffled bond lists")
+    kw = dict(nx=14, ny=6, nz=5, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
+              density=7.85e-3, velocity=300.0, damage=(0.002, 0.0))
+    rowptr, col = vpd.Bar(**kw).ctx.get_neighbors()
+    col = col.copy()
+    rng = np.random.default_rng(11)
+    for i in range(len(rowptr) - 1):
+        rng.shuffle(col[rowptr[i]:rowptr[i + 1]])
+    assert any(np.any(np.diff(col[rowptr[i]:rowptr[i + 1]]) < 0) for i in range(len(rowptr) - 1))
+    runs = {}
+    for name, knobs in [("direct", {})] + list(FORMS.items()):
+        b = vpd.Bar(**kw)
+        b.ctx.set_neighbors(rowptr, col)
+        b.ctx.initialize_shape()
+        b.reset()
+        b.ctx.set_tuning(**knobs)
+        b.step(30, want_fields=True)
+        runs[name] = b
+    ref = runs.pop("direct")
+    assert ref.get("damage").sum() > 0
+    for name, b in runs.items():
+        for field in ["u", "v", "stressN", "statev", "damage", "force", "F"]:
+            assert np.array_equal(b.get(field), ref.get(field)), (name, field)
+
+
 def test_plan_follows_knob_and_neighbor_changes(vpd):
     # The window plan is rebuilt when the knobs change between steps and when the bond lists
     # are replaced: the run stays equal to one with direct gathers.
@@ -358,7 +396,8 @@ def test_plan_follows_knob_and_neighbor_changes(vpd):
     ref, b = vpd.Bar(**kw), vpd.Bar(**kw)
     b.ctx.set_tuning(**FORMS["planned-1"])
     for knobs in (dict(), dict(force_window_bonds=5, defgrad_window=128, defgrad_window_bonds=8),
-                  dict(force_window_plan=2, defgrad_window_plan=2)):
+                  dict(force_window_plan=2, defgrad_window_plan=2),
+                  dict(force_window=512, force_window_bonds=0, force_window_plan=3)):
         b.ctx.set_tuning(**knobs)
         ref.step(8)
         b.step(8)
@@ -418,7 +457,7 @@ def test_window_knob_limits(vpd):
     b.ctx.set_tuning(force_window_bonds=1024, defgrad_window_bonds=6)
     t = b.ctx.get_tuning()
     assert (t["force_window_bonds"], t["defgrad_window_bonds"]) == (1024, 6)
-    for bad in (-1, 3):
+    for bad in (-1, 4):
         with pytest.raises(vpd.VpdError, match="Window form"):
             b.ctx.set_tuning(force_window_plan=bad)
     # A planned form needs a planned window size and 1..8 bonds per window: the step refuses
@@ -432,9 +471,25 @@ def test_window_knob_limits(vpd):
     b.ctx.set_tuning(force_window=80, force_window_bonds=8, force_window_plan=2)
     b.step(1)
     assert b.ctx.get_tuning()["force_window_plan"] == 2
+    # Group windows need a group window size and no bond limit; walked windows a walked size.
+    for knobs, msg in ((dict(force_window=256, force_window_bonds=8), "Group windows need"),
+                       (dict(force_window=128, force_window_bonds=0), "Group windows need")):
+        b.ctx.set_tuning(force_window_plan=3, **knobs)
+        with pytest.raises(vpd.VpdError, match=msg):
+            b.step(1)
+    b.ctx.set_tuning(force_window=384, force_window_bonds=0, force_window_plan=3)
+    b.step(1)
+    b.ctx.set_tuning(force_window_plan=0)
+    with pytest.raises(vpd.VpdError, match="Walked windows need"):
+        b.step(1)
+    b.ctx.set_tuning(force_window_plan=1, force_window_bonds=4)
+    with pytest.raises(vpd.VpdError, match="Planned windows need"):
+        b.step(1)
+    b.ctx.set_tuning(force_window=0)
+    b.step(1)
 
 
-@pytest.mark.parametrize("take,plan", [(0, 0), (8, 0), (6, 1), (7, 2)])
+@pytest.mark.parametrize("take,plan", [(0, 0), (8, 0), (6, 1), (7, 2), (0, 3)])
 def test_full_size_windowed_equals_direct(vpd, take, plan):
     banner("500k bar: windowed gathers (bonds per window %d, plan form %d) == direct, 12 steps "
            "with damage" % (take, plan))
@@ -442,7 +497,8 @@ def test_full_size_windowed_equals_direct(vpd, take, plan):
               density=7.85e-3, velocity=60.0, damage=(0.0005, 0.0))
     a = vpd.Bar(**kw)
     b = vpd.Bar(**kw)
-    b.ctx.set_tuning(defgrad_window=80 if take == 0 else 128, force_window=80 if take == 0 else 128,
+    window = 512 if plan == 3 else 80 if take == 0 else 128
+    b.ctx.set_tuning(defgrad_window=window, force_window=384 if plan == 3 else window,
                      defgrad_rounds=4, force_rounds=4, defgrad_window_bonds=take,
                      force_window_bonds=take, defgrad_window_plan=plan, force_window_plan=plan)
     a.step(12)

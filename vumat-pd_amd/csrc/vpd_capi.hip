@@ -117,8 +117,8 @@ struct vpd_context {
   int winDefgrad = 0, winForce = 0;
   int takeDefgrad = 0, takeForce = 0;
   // Form of the windowed gathers per bond pass: 0 walks the bond lists, 1 replays a window
-  // plan, 2 replays it with the next window prefetched (WindowPlan in vpd_kernels.hpp).
-  // Chosen by vpd_autotune.
+  // plan, 2 replays it with the next window prefetched (WindowPlan in vpd_kernels.hpp), 3
+  // replays a plan of windows shared by each workgroup (GroupPlan). Chosen by vpd_autotune.
   int modeDefgrad = 0, modeForce = 0;
   // The window plan of each bond pass, built for the pass's window and bond limit when first
   // needed after the bond lists or those knobs change.
@@ -129,6 +129,14 @@ struct vpd_context {
     int window = 0, take = 0;
     bool valid = false;
   } plan[2];
+  // The group-window plan of each bond pass, kept like `plan`.
+  struct GroupBuffers {
+    int *first = nullptr, *meta = nullptr;
+    uint64_t *off = nullptr;
+    int64_t windows = 0, blocks = 0; // what `meta` (windows) and `off` (entry blocks) hold
+    int window = 0;
+    bool valid = false;
+  } gplan[2];
   // Return map of the material stage; the learned mode adds the network, the strain
   // increments it reads and the seeds it writes.
   double rtol = 1.0e-12;
@@ -481,6 +489,10 @@ int build_sell(vpd_context *c) {
     dfree(p.first);
     p.valid = false;
   }
+  for (auto &p : c->gplan) {
+    dfree(p.first);
+    p.valid = false;
+  }
   c->hostRowptr.resize(n + 1);
   HIPTRY(c, hipMemcpyAsync(c->hostRowptr.data(), c->rowptr, sizeof(int) * (n + 1),
                            hipMemcpyDeviceToHost, c->stream));
@@ -604,11 +616,37 @@ template <class F> void with_planned_window(int window, F &&f) {
   }
 }
 
+// Group windows (GroupPlan): window sizes with group-window kernels, and a compile-time
+// window. The workgroup holds one window: `pairs` planes of W pairs.
+constexpr int kGroupWindows[] = {256, 384, 512};
+bool group_window_supported(int window) {
+  for (int w : kGroupWindows)
+    if (w == window)
+      return true;
+  return false;
+}
+int group_lds(int pairs, int window) { return pairs * window * 16; }
+template <class F> void with_group_window(int window, F &&f) {
+  switch (window) {
+  case 256:
+    f(std::integral_constant<int, 256>{});
+    break;
+  case 384:
+    f(std::integral_constant<int, 384>{});
+    break;
+  default:
+    f(std::integral_constant<int, 512>{});
+    break;
+  }
+}
+
 // How a bond pass gathers its neighbour data: window 0 (direct) or W, the bond limit per
-// window, the form (0 cursor walk, 1 planned, 2 planned with prefetch) and the plan.
+// window, the form (0 cursor walk, 1 planned, 2 planned with prefetch, 3 group windows) and
+// the plan of the form.
 struct Gather {
   int window = 0, take = 0, mode = 0;
   WindowPlan plan{};
+  GroupPlan gplan{};
 };
 
 // Builds the window plan of bond pass `pass` (0 F, 1 force) for (window, take), unless it is
@@ -667,25 +705,111 @@ WindowPlan plan_of(const vpd_context *c, int pass) {
   return WindowPlan{p.first, p.start, p.rec};
 }
 
+// Builds the group-window plan of bond pass `pass` for `window`, unless it is already there:
+// count the windows and entry blocks of every workgroup, lay them out, fill them.
+int build_group_plan(vpd_context *c, int pass, int window) {
+  auto &p = c->gplan[pass];
+  if (p.valid && p.first && p.window == window)
+    return VPD_OK;
+  if (!group_window_supported(window))
+    return fail(c, VPD_ERR_ARGUMENT, "Group windows need a window of 256, 384 or 512 "
+                                     "particles and no bond limit per window (0).");
+  p.valid = false;
+  const int64_t n = c->n, groups = ceil_div(n, 256);
+  int *count = nullptr, *firstBlock = nullptr;
+  if (dalloc(c, count, 2 * groups) || (!p.first && dalloc(c, p.first, groups + 1)))
+    return VPD_ERR_DEVICE;
+  with_group_window(window, [&](auto w) {
+    k_group_plan<decltype(w)::value, false><<<groups, 256, 0, c->stream>>>(
+        n, sell_of(c), count, nullptr, nullptr, nullptr, nullptr);
+  });
+  HIPTRY(c, hipGetLastError());
+  std::vector<int> hostCount(2 * groups), first(groups + 1, 0), firstBlk(groups, 0);
+  HIPTRY(c, hipMemcpyAsync(hostCount.data(), count, sizeof(int) * 2 * groups,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  dfree(count);
+  int64_t windows = 0, blocks = 0;
+  for (int64_t g = 0; g < groups; ++g) {
+    first[g] = (int)windows;
+    firstBlk[g] = (int)blocks;
+    windows += hostCount[2 * g];
+    blocks += hostCount[2 * g + 1];
+    if (windows >= INT_MAX / kGroupMeta || blocks >= INT_MAX)
+      return fail(c, VPD_ERR_ARGUMENT, "Too many windows for a group-window plan.");
+  }
+  first[groups] = (int)windows;
+  if (windows > p.windows) {
+    if (dalloc(c, p.meta, kGroupMeta * windows))
+      return VPD_ERR_DEVICE;
+    p.windows = windows;
+  }
+  if (blocks > p.blocks) {
+    if (dalloc(c, p.off, 256 * blocks))
+      return VPD_ERR_DEVICE;
+    p.blocks = blocks;
+  }
+  if (dalloc(c, firstBlock, groups))
+    return VPD_ERR_DEVICE;
+  HIPTRY(c, hipMemcpyAsync(p.first, first.data(), sizeof(int) * (groups + 1),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPTRY(c, hipMemcpyAsync(firstBlock, firstBlk.data(), sizeof(int) * groups,
+                           hipMemcpyHostToDevice, c->stream));
+  with_group_window(window, [&](auto w) {
+    k_group_plan<decltype(w)::value, true><<<groups, 256, 0, c->stream>>>(
+        n, sell_of(c), nullptr, p.first, firstBlock, p.meta, p.off);
+  });
+  HIPTRY(c, hipGetLastError());
+  HIPTRY(c, hipStreamSynchronize(c->stream));
+  dfree(firstBlock);
+  p.window = window;
+  p.valid = true;
+  return VPD_OK;
+}
+
+GroupPlan group_plan_of(const vpd_context *c, int pass) {
+  const auto &p = c->gplan[pass];
+  return GroupPlan{p.first, p.meta, p.off};
+}
+
+// Builds the plan a gather needs (none for direct and walked gathers; a walked gather is
+// checked for a supported window).
+int ensure_plan(vpd_context *c, int pass, const Gather &gx) {
+  if (gx.window > 0 && gx.mode == 0 && !window_supported(gx.window))
+    return fail(c, VPD_ERR_ARGUMENT, "Walked windows need a window of 64, 80, 96, 128 or 160 "
+                                     "particles.");
+  if (gx.mode == 3) {
+    if (gx.take != 0)
+      return fail(c, VPD_ERR_ARGUMENT, "Group windows need a window of 256, 384 or 512 "
+                                       "particles and no bond limit per window (0).");
+    return build_group_plan(c, pass, gx.window);
+  }
+  return gx.mode > 0 ? build_plan(c, pass, gx.window, gx.take) : VPD_OK;
+}
+
+// The plan pointers of a gather whose plan has been built.
+Gather with_plan(const vpd_context *c, int pass, Gather gx) {
+  if (gx.mode == 3)
+    gx.gplan = group_plan_of(c, pass);
+  else if (gx.mode > 0)
+    gx.plan = plan_of(c, pass);
+  return gx;
+}
+
 // The gather of bond pass `pass` as the knobs set it.
 Gather gather_of(const vpd_context *c, int pass) {
   Gather gx;
   gx.window = pass == 0 ? c->winDefgrad : c->winForce;
   gx.take = pass == 0 ? c->takeDefgrad : c->takeForce;
   gx.mode = gx.window > 0 ? (pass == 0 ? c->modeDefgrad : c->modeForce) : 0;
-  if (gx.mode > 0)
-    gx.plan = plan_of(c, pass);
-  return gx;
+  return with_plan(c, pass, gx);
 }
 
 // Builds the plans the knobs ask for (outside any graph capture: this allocates and waits).
 int prepare_plans(vpd_context *c) {
-  for (int pass = 0; pass < 2; ++pass) {
-    const Gather gx = gather_of(c, pass);
-    if (gx.mode > 0)
-      if (int s = build_plan(c, pass, gx.window, gx.take))
-        return s;
-  }
+  for (int pass = 0; pass < 2; ++pass)
+    if (int s = ensure_plan(c, pass, gather_of(c, pass)))
+      return s;
   return VPD_OK;
 }
 
@@ -693,6 +817,16 @@ template <bool DAMAGE>
 void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression,
                       const Batch &batch, const Gather &gx) {
   const int window = gx.window;
+  if (window > 0 && gx.mode == 3) {
+    with_group_window(window, [&](auto w) {
+      k_defgrad_g<DAMAGE, decltype(w)::value>
+          <<<grid_rounds(rd), 256, std::max(lds, group_lds(4, window)), c->stream>>>(
+              rd, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
+              planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->anyBroken,
+              c->initiation, c->failure, useCompression, batch, gx.gplan);
+    });
+    return;
+  }
   if (window > 0 && gx.mode > 0) {
     with_planned_window(window, [&](auto w) {
       constexpr int W = decltype(w)::value;
@@ -854,7 +988,13 @@ void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, const Gather &g
   const int window = gx.window;
   ForceArgs h = g;
   h.take = take_arg(gx.take);
-  if (!DIAG && window > 0 && gx.mode > 0)
+  if (!DIAG && window > 0 && gx.mode == 3)
+    with_group_window(window, [&](auto w) {
+      k_force_g<DAMAGE, FUSE, INTEGRATE, decltype(w)::value>
+          <<<grid_rounds(h.rd), 256, std::max(lds, group_lds(7, window)), c->stream>>>(
+              h, gx.gplan);
+    });
+  else if (!DIAG && window > 0 && gx.mode > 0)
     with_planned_window(window, [&](auto w) {
       constexpr int W = decltype(w)::value;
       if (gx.mode == 2)
@@ -1095,6 +1235,11 @@ void vpd_destroy(vpd_context *c) {
     dfree(p.first);
     dfree(p.start);
     dfree(p.rec);
+  }
+  for (auto &p : c->gplan) {
+    dfree(p.first);
+    dfree(p.meta);
+    dfree(p.off);
   }
   dfree(c->diagPartials);
   dfree(c->diagRecords);
@@ -2088,9 +2233,10 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
       return fail(c, VPD_ERR_ARGUMENT, "Sweeps per bond pass must lie in [1, 1024].");
     (knob == VPD_TUNE_DEFGRAD_ROUNDS ? c->roundsDefgrad : c->roundsForce) = value;
   } else if (knob == VPD_TUNE_DEFGRAD_WINDOW || knob == VPD_TUNE_FORCE_WINDOW) {
-    if (value != 0 && !window_supported(value))
+    if (value != 0 && !window_supported(value) && !group_window_supported(value))
       return fail(c, VPD_ERR_ARGUMENT,
-                  "Gather window must be 0 (direct) or one of 64, 80, 96, 128, 160 particles.");
+                  "Gather window must be 0 (direct) or one of 64, 80, 96, 128, 160 particles "
+                  "(256, 384, 512 with group windows).");
     (knob == VPD_TUNE_DEFGRAD_WINDOW ? c->winDefgrad : c->winForce) = value;
   } else if (knob == VPD_TUNE_DEFGRAD_WINDOW_BONDS || knob == VPD_TUNE_FORCE_WINDOW_BONDS) {
     if (value < 0 || value > kMaxTake)
@@ -2098,9 +2244,9 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
                   "Bonds per lane and window must lie in [0, 1024] (0: no limit).");
     (knob == VPD_TUNE_DEFGRAD_WINDOW_BONDS ? c->takeDefgrad : c->takeForce) = value;
   } else if (knob == VPD_TUNE_DEFGRAD_WINDOW_PLAN || knob == VPD_TUNE_FORCE_WINDOW_PLAN) {
-    if (value < 0 || value > 2)
-      return fail(c, VPD_ERR_ARGUMENT, "Window form must be 0 (walk), 1 (planned) or 2 "
-                                       "(planned with prefetch).");
+    if (value < 0 || value > 3)
+      return fail(c, VPD_ERR_ARGUMENT, "Window form must be 0 (walk), 1 (planned), 2 (planned "
+                                       "with prefetch) or 3 (group windows).");
     (knob == VPD_TUNE_DEFGRAD_WINDOW_PLAN ? c->modeDefgrad : c->modeForce) = value;
   } else {
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
@@ -2148,8 +2294,8 @@ int vpd_get_tuning(vpd_context *c, int knob, int *value) {
 }
 
 // Times both bond passes over a grid of (sweeps per pass) × (workgroups per CU) × (gather
-// window), then the windows with per-window bond limits and the planned forms, and keeps the
-// fastest for each. Each pass is timed in the form the steps use, and the timing runs leave
+// window), then the windows with per-window bond limits, the planned forms and the group
+// windows, and keeps the fastest for each. Each pass is timed in the form the steps use, and the timing runs leave
 // the state as it was: F goes to its scratch plane (with the stretch test when damage is on;
 // the bond flags and marks are restored afterwards), and the force pass integrates copies of
 // u, v and a.
@@ -2313,7 +2459,8 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   };
   // Stage 1: sweeps × LDS cap × window (walked, no bond limit per window); stage 2, at the
   // sweeps stage 1 chose: every window with each bond limit (walked, at stage 1's LDS cap),
-  // and every planned form, window and bond limit with each LDS cap; stage 3: sweeps × LDS
+  // every planned form, window and bond limit with each LDS cap, and the group windows with
+  // each LDS cap; stage 3: sweeps × LDS
   // cap again for the form, window and limit chosen; stage 4: the fastest few settings timed
   // again with four times the runs, against timing noise between close ones.
   for (int pass = 0; pass < 2; ++pass) {
@@ -2351,15 +2498,27 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
               return s;
           }
       }
-    Gather chosen = pick[pass];
-    if (chosen.mode > 0) {
-      if (int s = build_plan(c, pass, chosen.window, chosen.take))
+    for (int win : kGroupWindows) {
+      Gather gx = walk(win, 0);
+      gx.mode = 3;
+      if (int s = ensure_plan(c, pass, gx))
         return s;
-      chosen.plan = plan_of(c, pass);
+      gx = with_plan(c, pass, gx);
+      for (int cap : ldsChoices) {
+        if (cap != 0 && cap <= group_lds(pairsOf[pass], win))
+          continue;
+        if (int s = trial(pass, cap, rounds, gx))
+          return s;
+      }
     }
-    const int own = chosen.window > 0
-                        ? window_lds(pairsOf[pass], chosen.window) * (chosen.mode == 2 ? 2 : 1)
-                        : 0;
+    Gather chosen = pick[pass];
+    if (int s = ensure_plan(c, pass, chosen))
+      return s;
+    chosen = with_plan(c, pass, chosen);
+    const int own = chosen.window == 0   ? 0
+                    : chosen.mode == 3   ? group_lds(pairsOf[pass], chosen.window)
+                                         : window_lds(pairsOf[pass], chosen.window) *
+                                             (chosen.mode == 2 ? 2 : 1);
     for (int cap : ldsChoices)
       for (int r : roundChoices) {
         if ((cap != 0 && cap <= own) || (r > 1 && r > ceil_div(c->n, 256)))
@@ -2386,11 +2545,9 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
     // run slower in steps).
     float fastest = 1e30f;
     for (Candidate &x : top) {
-      if (x.gx.mode > 0) {
-        if (int s = build_plan(c, pass, x.gx.window, x.gx.take))
-          return s;
-        x.gx.plan = plan_of(c, pass);
-      }
+      if (int s = ensure_plan(c, pass, x.gx))
+        return s;
+      x.gx = with_plan(c, pass, x.gx);
       if (int s = time_setting(pass, x.lds, x.rounds, x.gx, 4 * reps, &x.ms))
         return s;
       fastest = std::min(fastest, x.ms);

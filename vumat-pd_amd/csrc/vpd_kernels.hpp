@@ -97,9 +97,10 @@ struct Rounds {
 // coalesced load per lane and component pair, let every lane consume its next bonds whose
 // neighbour lies in the run, and move on. Each lane still walks its own row in CSR order
 // with the arithmetic of the direct kernels, so results are identical bit for bit; only
-// where the neighbour data comes from changes. The window never moves back (every lane's
-// next index is at least its end), and the lane holding the smallest index always advances,
-// so the loop ends.
+// where the neighbour data comes from changes. A lane takes its next bonds only while their
+// neighbour lies inside the window, from its start to its end: rows need not be sorted (a
+// bond whose neighbour lies below the window start waits for a later window). The lane
+// holding the smallest index always advances, so the loop ends.
 //
 // A lane may also stop after `take` bonds of a window and finish the rest in a later one
 // (take ≥ 1; the lane holding the window's first index still advances). With rows made of
@@ -318,9 +319,10 @@ __global__ void k_advance_step(int *stepDev) {
 // The reference decides stretch = (‖y‖ − ‖ξ‖)/‖ξ‖ > s_c (or < −f), with two square roots and a
 // division per bond. With a = ‖ξ‖², b = ‖y‖² (the same sums the reference takes the roots
 // of), that test lands on the same side as b ⋛ a(1+s_c)² whenever b/a is more than 1e-9
-// away from the threshold ratio: the rounded stretch differs from the exact one by a few
-// ulps (≤ 1e-15 relative), far inside that margin. Only bonds within the margin, and the
-// degenerate a = 0, take the reference's exact expression.
+// away from the threshold ratio: the rounded stretch differs from the exact one by about
+// 1e-15, far inside that margin (≥ 5e-10 in stretch). The filter may therefore round a and b
+// its own way (fused multiply-adds, a few ulps from the exact sums); only bonds within the
+// margin, and the degenerate a = 0, take the reference's exact expression.
 struct StretchTest {
   double tLo, tHi; // (1+s_c)² (1 ∓ 1e-9)
   double cLo, cHi; // (1−f)² (1 ∓ 1e-9), used when the compression test is armed and f < 1
@@ -345,20 +347,24 @@ __host__ __device__ inline StretchTest make_stretch_test(double initiation, doub
 
 __device__ __forceinline__ bool stretch_breaks(const StretchTest &t, const double xi[3],
                                                const double yij[3]) {
-  const double a = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
-  const double b = yij[0] * yij[0] + yij[1] * yij[1] + yij[2] * yij[2];
+  const double a = fma(xi[0], xi[0], fma(xi[1], xi[1], xi[2] * xi[2]));
+  const double b = fma(yij[0], yij[0], fma(yij[1], yij[1], yij[2] * yij[2]));
   if (a > 0.0) {
-    const bool tensionDecided = b <= a * t.tLo || b >= a * t.tHi;
-    const bool compressionDecided =
-        t.compression != 1 || b >= a * t.cHi || b <= a * t.cLo;
-    if (tensionDecided && compressionDecided) {
+    if (t.compression != 1) { // the tension test alone
+      if (b <= a * t.tLo)
+        return false;
       if (b >= a * t.tHi)
         return true;
-      return t.compression == 1 && b <= a * t.cLo;
+    } else {
+      const bool tensionDecided = b <= a * t.tLo || b >= a * t.tHi;
+      const bool compressionDecided = b >= a * t.cHi || b <= a * t.cLo;
+      if (tensionDecided && compressionDecided)
+        return b >= a * t.tHi || b <= a * t.cLo;
     }
   }
-  const double initial = sqrt(a);
-  const double stretch = (sqrt(b) - initial) / initial;
+  const double initial = sqrt(xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2]);
+  const double stretch = (sqrt(yij[0] * yij[0] + yij[1] * yij[1] + yij[2] * yij[2]) - initial) /
+                         initial;
   return stretch > t.initiation || (t.compression != 0 && stretch < -t.failure);
 }
 
@@ -576,7 +582,7 @@ __device__ __forceinline__ void defgrad_window(int64_t n, int64_t i, PairPlanes 
     wave_lds_sync();
     const int end = w0 > INT_MAX - W ? INT_MAX : w0 + W; // (a finished lane's next is INT_MAX)
     const int stop = window_stop(cur.k, take);
-    while (cur.next < end && cur.k < stop) {
+    while (cur.next >= w0 && cur.next < end && cur.k < stop) {
       const int o = cur.next - w0;
       const d2v a = win[o], b = win[W + o], ua = win[2 * W + o], ub = win[3 * W + o];
       const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
@@ -691,7 +697,7 @@ __global__ void __launch_bounds__(256)
     const int stop = window_stop(cur.k, take);
     uint64_t r = ~0ull;
     int q = 0;
-    while (cur.next < end && cur.k < stop) {
+    while (cur.next >= w0 && cur.next < end && cur.k < stop) {
       r &= ~(0xFFull << (8 * q));
       r |= (uint64_t)(cur.next - w0) << (8 * q);
       ++q;
@@ -919,6 +925,243 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
               anyBroken[i] = 1;
             }
           }
+        });
+    if (!mine)
+      continue;
+    T3 nm, ki;
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+      nm.m[c] = num[c];
+      ki.m[c] = kinv(c, i);
+    }
+    const T3 f = t3_mul(nm, ki);
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      fout(c, i) = f.m[c];
+  }
+}
+
+// ---- Group windows ----------------------------------------------------------------------
+// The windows above serve one wavefront each. With the lattice numbered line by line a
+// wavefront's 64 particles cover about one line, and every wavefront stages the neighbour
+// lines around it again. The 256 consecutive particles of a workgroup share most of their
+// neighbours, so here the whole workgroup stages one window of W consecutive particles (from
+// the smallest neighbour index any of its lanes still needs) and every lane takes all its
+// next bonds whose neighbour lies in it. On the 200×50×50 lattice (δ = 3.015Δx, W = 512) a
+// workgroup's ~8 windows stage about 11 neighbour particles per particle, against 28–40 for
+// the wavefront windows, and its lanes take near-equal numbers of bonds per window (one
+// neighbour plane each). The plan (k_group_plan) holds per window its first particle, the
+// particles it stages, its first entry block and the entries each wavefront reads; per entry
+// block and thread one 8-byte entry with the LDS offsets (j − window start) of four of the
+// lane's bonds in CSR order, 16 bits each, lowest first, 0xFFFF after the last. Each lane
+// sums its bonds in CSR order with the arithmetic of the direct kernels: results equal them
+// bit for bit.
+struct GroupPlan {
+  const int *first;     // per workgroup: index of its first window; first[groups] = total
+  const int *meta;      // per window, 8 ints: first particle, particles staged, first entry
+                        // block, 0, then the entries wavefronts 0..3 read
+  const uint64_t *off;  // per entry block and thread (block·256 + thread): see above
+};
+constexpr int kGroupMeta = 8;
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    v = max(v, __shfl_xor(v, off));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+// Minimum / maximum over the 256 threads of the workgroup (every thread calls; `red` holds
+// 4 ints of LDS).
+template <bool MIN> __device__ __forceinline__ int group_reduce(int v, int *red) {
+  v = MIN ? wave_min(v) : wave_max(v);
+  __syncthreads(); // earlier readers of `red` are done
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return MIN ? min(min(red[0], red[1]), min(red[2], red[3]))
+             : max(max(red[0], red[1]), max(red[2], red[3]));
+}
+
+// Workgroup g plans particles 256g..256g+255 (the workgroups of every bond-pass launch: the
+// grid has one workgroup per 256 particles). FILL false counts each workgroup's windows and
+// entry blocks into count[2g], count[2g + 1]; FILL true writes them from firstWin[g] and
+// firstBlock[g] on.
+template <int W, bool FILL>
+__global__ void __launch_bounds__(256)
+    k_group_plan(int64_t n, Sell bonds, int *count, const int *firstWin, const int *firstBlock,
+                 int *meta, uint64_t *off) {
+  __shared__ int red[4];
+  const int64_t g = blockIdx.x;
+  const int64_t i = g * 256 + threadIdx.x;
+  BondCursor cur;
+  cur.start(bonds, nullptr, false, i < n ? i : -1);
+  int64_t t = FILL ? firstWin[g] : 0;
+  int64_t blk = FILL ? firstBlock[g] : 0;
+  while (true) {
+    const int w0 = group_reduce<true>(cur.next, red);
+    if (w0 == INT_MAX)
+      break;
+    const int end = w0 > INT_MAX - W ? INT_MAX : w0 + W;
+    int q = 0, reach = 0; // bonds taken, and one past the largest offset among them
+    uint64_t e = ~0ull;
+    while (cur.next >= w0 && cur.next < end) {
+      const int o = cur.next - w0;
+      reach = max(reach, o + 1);
+      const int sh = 16 * (q & 3);
+      e = (e & ~(0xFFFFull << sh)) | ((uint64_t)o << sh);
+      ++q;
+      if ((q & 3) == 0) {
+        if (FILL)
+          off[(blk + (q >> 2) - 1) * 256 + threadIdx.x] = e;
+        e = ~0ull;
+      }
+      cur.advance(bonds, nullptr, false);
+    }
+    const int entries = (q + 3) >> 2;
+    if (FILL && (q & 3))
+      off[(blk + entries - 1) * 256 + threadIdx.x] = e;
+    const int ext = group_reduce<false>(reach, red);
+    const int waveEntries = wave_max(entries);
+    const int blocks = group_reduce<false>(waveEntries, red);
+    if (FILL) {
+      for (int x = entries; x < blocks; ++x)
+        off[(blk + x) * 256 + threadIdx.x] = ~0ull;
+      int *m = meta + kGroupMeta * t;
+      if (threadIdx.x == 0) {
+        m[0] = w0;
+        m[1] = ext;
+        m[2] = (int)blk;
+        m[3] = 0;
+      }
+      if ((threadIdx.x & 63) == 0)
+        m[4 + (threadIdx.x >> 6)] = waveEntries;
+    }
+    ++t;
+    blk += blocks;
+  }
+  if (!FILL && threadIdx.x == 0) {
+    count[2 * g] = (int)t;
+    count[2 * g + 1] = blk < INT_MAX ? (int)blk : INT_MAX;
+  }
+}
+
+// The window loop of a group-window bond pass for workgroup `group`; every thread of the
+// workgroup runs it (one without a particle, mine false, stages its share and sums nothing).
+// Wavefront q stages window slots q·64 + lane, q·64 + 256 + lane, ... into `win` (PAIRS
+// planes of W pairs); after a workgroup barrier every lane reads its entries and calls
+// body(win, entry, k) with k its index of the entry's first bond; body returns the bonds it
+// summed. A second barrier keeps the window until every wavefront has read it.
+template <int PAIRS, int W, class Planeset, class Body>
+__device__ __forceinline__ void group_windows(const Planeset &planes, const GroupPlan &plan,
+                                              int64_t group, bool mine, d2v *win, Body &&body) {
+  const int lane = threadIdx.x & 63;
+  const int wave = wave_slot();
+  const int t1 = plan_load(plan.first, group + 1);
+  int k = 0;
+  for (int t = plan_load(plan.first, group); t < t1; ++t) {
+    const int64_t m = kGroupMeta * (int64_t)t;
+    const int w0 = plan_load(plan.meta, m);
+    const int ext = plan_load(plan.meta, m + 1);
+    const int64_t blk = plan_load(plan.meta, m + 2);
+    const int entries = plan_load(plan.meta, m + 4 + wave);
+    // The first two entries are loaded with the window's data (the wait below covers them);
+    // later ones two entries ahead of their use.
+    const uint64_t *offs = plan.off + blk * 256 + threadIdx.x;
+    uint64_t e = mine && entries > 0 ? offs[0] : ~0ull;
+    uint64_t eNext = mine && entries > 1 ? offs[256] : ~0ull;
+#pragma unroll
+    for (int q0 = 0; q0 < W; q0 += 256) {
+      const int s = q0 + 64 * wave;
+      if (s < ext && s + lane < ext) {
+#pragma unroll
+        for (int c = 0; c < PAIRS; ++c)
+          __builtin_amdgcn_global_load_lds(
+              (const void *)(scalar_ptr(planes.ptr(c, (int64_t)w0 + s)) + lane),
+              (lds_void *)(win + c * W + s), 16, 0, 0);
+      }
+    }
+    wave_lds_sync(); // this wavefront's staging is complete
+    __syncthreads(); // and every other's
+    for (int x = 0; x < entries; ++x) {
+      const uint64_t cur = e;
+      e = eNext;
+      if (mine && x + 2 < entries)
+        eNext = offs[256 * (x + 2)];
+      k += body(win, cur, k);
+    }
+    wave_lds_wait();
+    __syncthreads(); // every wavefront has read the window
+  }
+}
+
+// Bond pass 1 with group windows (4 component pairs: X_j, V_j, u_j).
+template <int PAIRS, int W> constexpr int group_waves() {
+  return 163840 / (PAIRS * W * 16) < 5 ? 163840 / (PAIRS * W * 16) : 5;
+}
+template <bool DAMAGE, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(group_waves<4, W>())))
+    k_defgrad_g(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
+                Planes fout, uint32_t *broken, uint8_t *anyBroken, double initiation,
+                double failure, int useCompression, Batch batch, GroupPlan plan) {
+  extern __shared__ d2v wlds[];
+  const int64_t blk = xcd_block(rd.blocksPerRound);
+  if (blk < 0)
+    return;
+  const StretchTest test = make_stretch_test(initiation, failure, useCompression);
+  for (int r = 0; r < rd.rounds; ++r) {
+    const int64_t i = rd.particle(r, blk);
+    const int64_t group = i >> 8;
+    if (group * 256 >= rd.n) // the same for the whole workgroup
+      break;
+    const bool mine = i < rd.n && point_active(batch, i);
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0, yi0 = 0.0, yi1 = 0.0, yi2 = 0.0;
+    int64_t g0 = 0;
+    if (mine) {
+      x0 = x(0, i);
+      x1 = x(1, i);
+      x2 = x(2, i);
+      yi0 = x0 + u(0, i);
+      yi1 = x1 + u(1, i);
+      yi2 = x2 + u(2, i);
+      g0 = sell_pos(bonds, i, 0);
+    }
+    double num[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      num[c] = 0.0;
+    // As in k_defgrad_p, the stretch test runs on every bond and the flag words are read
+    // only where a bond breaks.
+    group_windows<4, W>(
+        DefgradPlanes{xv, u}, plan, group, mine, wlds,
+        [&](const d2v *buf, uint64_t e, int k) -> int {
+          uint32_t fresh = 0;
+          int q = 0;
+          for (; q < 4; ++q) {
+            const int o = (int)(e & 0xFFFFu);
+            if (o == 0xFFFF)
+              break;
+            const d2v a = buf[o], b = buf[W + o], ua = buf[2 * W + o];
+            const double uz = lds_first(buf + 3 * W + o); // (u_z, 0): 8 bytes suffice
+            const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
+            const double yij[3] = {(a.x + ua.x) - yi0, (a.y + ua.y) - yi1, (b.x + uz) - yi2};
+#pragma unroll
+            for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                num[3 * rr + c] += b.y * yij[rr] * xi[c];
+            if (DAMAGE && stretch_breaks(test, xi, yij))
+              fresh |= 1u << q;
+            e >>= 16;
+          }
+          if (DAMAGE && fresh) { // (only a lane with a particle has bonds)
+            const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
+            fresh &= ~plan_bits(plan_words(broken, g0, k, len), g0, k);
+            if (fresh) {
+              plan_flush(broken, g0, k, fresh);
+              anyBroken[i] = 1;
+            }
+          }
+          return q;
         });
     if (!mine)
       continue;
@@ -1386,7 +1629,7 @@ __device__ __forceinline__ void force_sum_window(const ForceArgs &g, int64_t i, 
     wave_lds_sync();
     const int end = w0 > INT_MAX - W ? INT_MAX : w0 + W; // (a finished lane's next is INT_MAX)
     const int stop = window_stop(cur.k, g.take);
-    while (cur.next < end && cur.k < stop) {
+    while (cur.next >= w0 && cur.next < end && cur.k < stop) {
       const int o = cur.next - w0;
       if (!(DAMAGE && cur.bond_broken())) {
         const d2v a = win[o], b = win[W + o];
@@ -1596,6 +1839,77 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
             rec >>= 8;
             brk >>= 1;
           }
+        });
+    if (mine)
+      force_finish<FUSE_NEXT, INTEGRATE, false>(g, i, acc, nullptr);
+  }
+}
+
+// The force pass with group windows (7 component pairs: X_j, V_j, A_j; see GroupPlan).
+template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(group_waves<7, W>())))
+    k_force_g(ForceArgs g, GroupPlan plan) {
+  extern __shared__ d2v wlds[];
+  const int64_t blk = xcd_block(g.rd.blocksPerRound);
+  if (blk < 0)
+    return;
+  for (int r = 0; r < g.rd.rounds; ++r) {
+    const int64_t i = g.rd.particle(r, blk);
+    const int64_t group = i >> 8;
+    if (group * 256 >= g.rd.n) // the same for the whole workgroup
+      break;
+    const bool mine = i < g.rd.n && point_active(g.batch, i);
+    double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+    T3 ai;
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+      ai.m[c] = 0.0;
+    int64_t g0 = 0;
+    int len = 0;
+    bool flagged = false;
+    if (mine) {
+      x0 = g.x(0, i);
+      x1 = g.x(1, i);
+      x2 = g.x(2, i);
+      ai = load_amap(g.amap, i);
+      g0 = sell_pos(g.bonds, i, 0);
+      len = g.bonds.rowptr[i + 1] - g.bonds.rowptr[i];
+      // Only a particle marked in anyBroken can have broken bonds: the others skip the flags.
+      flagged = DAMAGE && g.anyBroken[i] != 0;
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    group_windows<7, W>(
+        ForcePlanes{g.xv, g.amap}, plan, group, mine, wlds,
+        [&](const d2v *buf, uint64_t e, int k) -> int {
+          uint32_t brk = 0;
+          if (DAMAGE && flagged)
+            brk = plan_bits(plan_words(g.broken, g0, k, len), g0, k);
+          int q = 0;
+          for (; q < 4; ++q) {
+            const int o = (int)(e & 0xFFFFu);
+            if (o == 0xFFFF)
+              break;
+            if (!(DAMAGE && (brk & 1u))) {
+              const d2v a = buf[o], b = buf[W + o];
+              T3 aj;
+#pragma unroll
+              for (int c = 0; c < 4; ++c) {
+                const d2v v = buf[(2 + c) * W + o];
+                aj.m[2 * c] = v.x;
+                aj.m[2 * c + 1] = v.y;
+              }
+              aj.m[8] = lds_first(buf + 6 * W + o); // (A_8, 0): 8 bytes suffice
+              const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
+              double term[3];
+              force_bond_term(ai, aj, xi, b.y, term);
+#pragma unroll
+              for (int d = 0; d < 3; ++d)
+                acc[d] += term[d];
+            }
+            e >>= 16;
+            brk >>= 1;
+          }
+          return q;
         });
     if (mine)
       force_finish<FUSE_NEXT, INTEGRATE, false>(g, i, acc, nullptr);
