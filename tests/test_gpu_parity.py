@@ -281,7 +281,7 @@ def test_fused_equals_unfused(vpd, orc):
          defgrad_window_plan=2, force_window_plan=2, defgrad_rounds=3, force_rounds=2),
     dict(defgrad_window=80, force_window=128, defgrad_window_bonds=1, force_window_bonds=2,
          defgrad_window_plan=2, force_window_plan=1, force_lds=64000),
-    dict(defgrad_window=512, force_window=384, defgrad_window_plan=3, force_window_plan=3,
+    dict(defgrad_window=512, force_window=480, defgrad_window_plan=3, force_window_plan=3,
          defgrad_rounds=2, force_rounds=3),
     dict(defgrad_window=256, force_window=256, defgrad_window_plan=3, force_window_plan=3,
          defgrad_lds=64000, force_lds=50000),
@@ -498,7 +498,7 @@ def test_full_size_windowed_equals_direct(vpd, take, plan):
     a = vpd.Bar(**kw)
     b = vpd.Bar(**kw)
     window = 512 if plan == 3 else 80 if take == 0 else 128
-    b.ctx.set_tuning(defgrad_window=window, force_window=384 if plan == 3 else window,
+    b.ctx.set_tuning(defgrad_window=window, force_window=480 if plan == 3 else window,
                      defgrad_rounds=4, force_rounds=4, defgrad_window_bonds=take,
                      force_window_bonds=take, defgrad_window_plan=plan, force_window_plan=plan)
     a.step(12)
