@@ -64,15 +64,33 @@ def step_bytes_per_particle(bbar, damage):
     return 816 + 136 + 144 + 24 + 8 + 8 * bbar + (bbar / 4.0 if damage else 0.0)
 
 
-def measured_traffic(kernel, workload):
+def bond_pass_device_kernel(kernel, tuning):
+    """The device kernel (as rocprofv3 names it) that runs bond pass `kernel` in the step's
+    form (damage on; the force pass integrating, with the next prediction fused) under the
+    launch tuning `tuning`."""
+    p = "defgrad" if kernel == "defgrad_bond_pass" else "force"
+    window, form = tuning[p + "_window"], tuning[p + "_window_plan"]
+    head, args = ("k_defgrad", "true") if p == "defgrad" else ("k_force", "true, true, true")
+    if window == 0:
+        return "k_defgrad<true>" if p == "defgrad" else "k_force<true, true, true, false>"
+    if form == 0:
+        return "%s_w<%s, %d>" % (head, args, window)
+    if form == 3:
+        return "%s_g<%s, %d>" % (head, args, window)
+    return "%s_p<%s, %d, %s>" % (head, args, window, "true" if form == 2 else "false")
+
+
+def measured_traffic(kernel, workload, device_kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of this workload (calibrated against a device copy), or None."""
+    summary of this workload (calibrated against a device copy), or None when there is none
+    for the device kernel that runs it now."""
     path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (kernel, workload))
     if not os.path.exists(path):
         return None
     with open(path) as f:
         data = json.load(f)
-    if data.get("kernel") != kernel or data.get("workload") != workload:
+    if (data.get("kernel") != kernel or data.get("workload") != workload
+            or data.get("device_kernel") != device_kernel):
         return None
     return data["hbm_bytes_per_launch"]
 
@@ -343,7 +361,9 @@ def main():
                    "return_map": "bisection (parity)", "steps_from": 1, "launch_tuning": tuned},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9 * 1e9,
                      "unit": "B/s", "frac": achieved / HBM_PEAK,
-                     "traffic": measured_traffic(dom, w["name"]),
+                     "traffic": measured_traffic(
+                         dom, w["name"],
+                         bond_pass_device_kernel(dom, tuned) if dom.endswith("bond_pass") else None),
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_s * 1e3},
         "step_model": {"bytes_per_particle_step": step_bps,
                        "hbm_fraction": step_bps * value / HBM_PEAK},
