@@ -1094,10 +1094,13 @@ __device__ __forceinline__ void group_windows(const Planeset &planes, const Grou
   }
 }
 
-// Bond pass 1 with group windows (4 component pairs: X_j, V_j, u_j).
+// Wavefronts per SIMD a group-window pass can hold: each workgroup (one wavefront per SIMD)
+// takes PAIRS × W × 16 bytes of the CU's 160 KB of LDS; at most 5 (96 VGPRs each).
 template <int PAIRS, int W> constexpr int group_waves() {
   return 163840 / (PAIRS * W * 16) < 5 ? 163840 / (PAIRS * W * 16) : 5;
 }
+
+// Bond pass 1 with group windows (4 component pairs: X_j, V_j, u_j).
 template <bool DAMAGE, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(group_waves<4, W>())))
     k_defgrad_g(Rounds rd, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
