@@ -285,6 +285,7 @@ def test_fused_equals_unfused(vpd, orc):
          defgrad_rounds=2, force_rounds=3),
     dict(defgrad_window=256, force_window=256, defgrad_window_plan=3, force_window_plan=3,
          defgrad_lds=64000, force_lds=50000),
+    dict(defgrad_window=640, force_window=512, defgrad_window_plan=3, force_window_plan=3),
 ])
 def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
     # Launch knobs change how the bond passes are scheduled, never what they sum or in which
@@ -477,8 +478,13 @@ def test_window_knob_limits(vpd):
         b.ctx.set_tuning(force_window_plan=3, **knobs)
         with pytest.raises(vpd.VpdError, match=msg):
             b.step(1)
-    b.ctx.set_tuning(force_window=384, force_window_bonds=0, force_window_plan=3)
+    b.ctx.set_tuning(force_window=640, force_window_bonds=0, force_window_plan=3)
+    with pytest.raises(vpd.VpdError, match="force pass need a window of at most 512"):
+        b.step(1)
+    b.ctx.set_tuning(force_window=384, force_window_bonds=0, force_window_plan=3,
+                     defgrad_window=640, defgrad_window_bonds=0, defgrad_window_plan=3)
     b.step(1)
+    b.ctx.set_tuning(defgrad_window=160, defgrad_window_plan=0)
     b.ctx.set_tuning(force_window_plan=0)
     with pytest.raises(vpd.VpdError, match="Walked windows need"):
         b.step(1)

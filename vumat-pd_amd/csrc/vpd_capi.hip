@@ -617,8 +617,10 @@ template <class F> void with_planned_window(int window, F &&f) {
 }
 
 // Group windows (GroupPlan): window sizes with group-window kernels, and a compile-time
-// window. The workgroup holds one window: `pairs` planes of W pairs.
-constexpr int kGroupWindows[] = {256, 384, 480, 512};
+// window. The workgroup holds one window: `pairs` planes of W pairs, at most 64 KB (the
+// force pass's 7 planes allow W ≤ 512).
+constexpr int kGroupWindows[] = {256, 384, 480, 512, 640};
+constexpr int kMaxGroupLds = 65536;
 bool group_window_supported(int window) {
   for (int w : kGroupWindows)
     if (w == window)
@@ -636,6 +638,9 @@ template <class F> void with_group_window(int window, F &&f) {
     break;
   case 480:
     f(std::integral_constant<int, 480>{});
+    break;
+  case 640:
+    f(std::integral_constant<int, 640>{});
     break;
   default:
     f(std::integral_constant<int, 512>{});
@@ -715,8 +720,9 @@ int build_group_plan(vpd_context *c, int pass, int window) {
   if (p.valid && p.first && p.window == window)
     return VPD_OK;
   if (!group_window_supported(window))
-    return fail(c, VPD_ERR_ARGUMENT, "Group windows need a window of 256, 384, 480 or 512 "
-                                     "particles and no bond limit per window (0).");
+    return fail(c, VPD_ERR_ARGUMENT,
+                "Group windows need a window of 256, 384, 480, 512 or 640 particles and no "
+                "bond limit per window (0).");
   p.valid = false;
   const int64_t n = c->n, groups = ceil_div(n, 256);
   int *count = nullptr, *firstBlock = nullptr;
@@ -783,8 +789,12 @@ int ensure_plan(vpd_context *c, int pass, const Gather &gx) {
                                      "particles.");
   if (gx.mode == 3) {
     if (gx.take != 0)
-      return fail(c, VPD_ERR_ARGUMENT, "Group windows need a window of 256, 384, 480 or 512 "
-                                       "particles and no bond limit per window (0).");
+      return fail(c, VPD_ERR_ARGUMENT,
+                  "Group windows need a window of 256, 384, 480, 512 or 640 particles and no "
+                  "bond limit per window (0).");
+    if (group_lds(pass == 0 ? 4 : 7, gx.window) > kMaxGroupLds)
+      return fail(c, VPD_ERR_ARGUMENT,
+                  "Group windows of the force pass need a window of at most 512 particles.");
     return build_group_plan(c, pass, gx.window);
   }
   return gx.mode > 0 ? build_plan(c, pass, gx.window, gx.take) : VPD_OK;
@@ -2239,7 +2249,7 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
     if (value != 0 && !window_supported(value) && !group_window_supported(value))
       return fail(c, VPD_ERR_ARGUMENT,
                   "Gather window must be 0 (direct) or one of 64, 80, 96, 128, 160 particles "
-                  "(256, 384, 480, 512 with group windows).");
+                  "(256, 384, 480, 512, 640 with group windows).");
     (knob == VPD_TUNE_DEFGRAD_WINDOW ? c->winDefgrad : c->winForce) = value;
   } else if (knob == VPD_TUNE_DEFGRAD_WINDOW_BONDS || knob == VPD_TUNE_FORCE_WINDOW_BONDS) {
     if (value < 0 || value > kMaxTake)
@@ -2502,6 +2512,8 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
           }
       }
     for (int win : kGroupWindows) {
+      if (group_lds(pairsOf[pass], win) > kMaxGroupLds)
+        continue;
       Gather gx = walk(win, 0);
       gx.mode = 3;
       if (int s = ensure_plan(c, pass, gx))
