@@ -4,10 +4,15 @@ Johnson-Cook VUMAT on one MI355X, on the 500k-particle JC tensile bar (200×50×
 1 mm/ms, dt = 0.5·Δx/c_p), with the dominant kernel's HBM roofline and the reference's CPU
 path timed on this host beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--from-step S] [--no-cpu-baseline]
 
 Prints one JSON line. A step is one fused increment (F + damage bond pass, material point
 update, force bond pass + velocity-Verlet) over all 500k particles, inputs resident in HBM.
+The timed window is steps S .. S+K−1 (default 4001-5000, the plastic tail of the bar's
+5000-step run: yielding starts near step 3500, and by step 4000 some 75k points take a
+plastic increment per step); the bar is advanced to step S−1−W untimed and runs W warmup
+steps first. The same window is timed with the learned return map from the same state, and
+a second window from step 1 (elastic) is reported beside it.
 """
 from __future__ import annotations
 
@@ -17,6 +22,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "vumat-pd_amd"))
@@ -95,32 +102,45 @@ def measured_traffic(kernel, workload, device_kernel):
     return data["hbm_bytes_per_launch"]
 
 
-def cpu_baseline(sample_steps, w):
-    """The reference's own OpenMP increment (oracle/_ref) on this host, or the restated
-    oracle when the reference build is absent, on the same bar for `sample_steps` steps."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+def cpu_baseline(state, window, small_steps, w):
+    """The reference's own OpenMP increment (oracle/_ref; the restated oracle when the
+    reference build is absent) on this host's cores, run by oracle/cpu_sample.py in a child
+    process with OMP_PROC_BIND=close and OMP_DYNAMIC=FALSE set before its OpenMP runtime
+    starts: the 500k bar over `window` steps from `state` (the GPU run's state just before its
+    timed window, so the CPU sample covers the same plastic steps), and the 30×8×8 bar for its
+    whole `small_steps`-step run."""
+    import subprocess
+    import tempfile
 
-    kw = dict(nx=w["nx"], ny=w["ny"], nz=w["nz"], spacing=w["spacing"], horizon=w["horizon"],
-              props=JC_PROPS, dt=cfl_dt(w["spacing"]), density=RHO, amplitude=1.0,
-              rise_time=1.0, damage=w["damage"])
-    if O.ref_available():
-        kind = "reference"
-        # The reference runner only offers the half-cosine ramp; the velocity-driven band
-        # changes which constraint values are written, not the cost of a step.
-        bar = O.RefBar(**kw)
-    else:
-        kind = "port"
-        O.build()
-        bar = O.OracleBar(**dict(kw, velocity=w["velocity"]))
-    n = w["nx"] * w["ny"] * w["nz"]
-    t0 = time.perf_counter()
-    bar.step(sample_steps)
-    dt = time.perf_counter() - t0
-    threads = O.lib().orc_num_threads()
-    return dict(value=n * sample_steps / dt, unit="particle-steps/s", cores=threads, kind=kind,
-                sample="steps 1-%d of the %d-particle bar (setup untimed), %d OpenMP threads"
-                % (sample_steps, n, threads), seconds=dt)
+    import numpy as np
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_DYNAMIC="FALSE")
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "state.npz")
+        np.savez(path, **state)
+        cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_sample.py"), "--state", path,
+               "--window", str(window), "--small-steps", str(small_steps),
+               "--velocity", str(w["velocity"])]
+        run = subprocess.run(cmd, env=env, capture_output=True, text=True)
+    if run.returncode != 0:
+        raise SystemExit("CPU baseline failed:\n" + run.stderr[-2000:])
+    res = json.loads(run.stdout.strip().splitlines()[-1])
+    large, small = res["large"], res.get("small")
+    cpu = res["cpu"]
+    sample = ("steps %s of the %d-particle bar (the GPU run's timed window, plastic phase: %d "
+              "points plastic in the first step) from the GPU state, setup untimed; %d OpenMP "
+              "threads, OMP_PROC_BIND=close, on %s (%s socket(s), %s physical cores, %d CPUs "
+              "allowed)" % (large["steps"], large["particles"], large["plastic_points_at_start"],
+                            res["threads"], cpu.get("model"), cpu.get("sockets"),
+                            cpu.get("physical_cores"), cpu.get("cpus_allowed")))
+    out = dict(value=large["value"], unit="particle-steps/s", cores=res["threads"],
+               kind=res["kind"], sample=sample, seconds=large["seconds"], cpu=cpu,
+               omp=res["omp"], window=large)
+    if small:
+        out["small_bar"] = small
+    return out
 
 
 # Kernel means come from HIP events on every TIMING_STRIDE-th step of the timed region: events
@@ -128,20 +148,37 @@ def cpu_baseline(sample_steps, w):
 TIMING_STRIDE = 10
 
 
-def timed_run(bar, steps, warmup):
-    """Warmup steps, reset to the state before step 1, then `steps` timed steps. Returns
-    (seconds, per-kernel HIP-event times of every TIMING_STRIDE-th step)."""
-    if warmup:
-        bar.step(warmup)
-        bar.ctx.synchronize()
-        bar.reset()
+def timed_window(bar, steps):
+    """`steps` timed steps from the bar's current state. Returns (seconds, per-kernel HIP-event
+    times of every TIMING_STRIDE-th step)."""
     bar.ctx.set_timing(True, TIMING_STRIDE)
     bar.ctx.synchronize()
     t0 = time.perf_counter()
     bar.step(steps)
     bar.ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    return elapsed, bar.ctx.collect_timing()
+    kt = bar.ctx.collect_timing()
+    bar.ctx.set_timing(False)
+    return elapsed, kt
+
+
+def timed_run(bar, steps, warmup):
+    """Warmup steps, reset to the state before step 1, then `steps` timed steps from step 1.
+    Returns (seconds, per-kernel HIP-event times of every TIMING_STRIDE-th step)."""
+    if warmup:
+        bar.step(warmup)
+        bar.ctx.synchronize()
+    bar.reset()
+    return timed_window(bar, steps)
+
+
+def window_result(bar, first, steps, elapsed, kt, n):
+    st = bar.get("statev")
+    return {"first_step": first, "last_step": first + steps - 1, "value": n * steps / elapsed,
+            "ms_per_step": elapsed / steps * 1e3, "max_eqps": float(st[:, 1].max()),
+            "plastic_points_last_step": int((st[:, 4] > 0.0).sum()),
+            "broken_bonds": int(bar.get("damage").sum()),
+            "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()}}
 
 
 def return_map_study(points, seed=4321):
@@ -289,10 +326,19 @@ def small_bar_graph(steps=2000):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--from-step", type=int, default=4001,
+                    help="first timed step: the bar is advanced untimed to step from-step − 1 − "
+                         "warmup, then runs `warmup` steps and times `steps` steps (4001 times "
+                         "the plastic phase of the 5000-step run; 1 the start)")
+    ap.add_argument("--no-step1-window", action="store_true",
+                    help="skip the second timed window from step 1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-window", type=int, default=10,
+                    help="500k-bar steps the CPU baseline times (from the first timed step)")
+    ap.add_argument("--cpu-small-steps", type=int, default=2000,
+                    help="steps of the 30x8x8 bar the CPU baseline also times (0: none)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", default=None,
                     help="launch knobs as JSON (the launch_tuning of an earlier run) instead of "
@@ -311,8 +357,9 @@ def main():
         out["small_bar"] = small_bar_graph()
         print(json.dumps(out))
         return
+    if args.from_step < 1:
+        raise SystemExit("--from-step must be at least 1")
 
-    import numpy as np
     import vpd
 
     w = WORKLOAD
@@ -328,13 +375,27 @@ def main():
     else:
         tuned = bar.ctx.get_tuning() if args.no_autotune else bar.ctx.autotune(3)
     print("[bench] launch tuning %s" % tuned, file=sys.stderr, flush=True)
-    print("[bench] timing %d plain steps" % args.steps, file=sys.stderr, flush=True)
-    elapsed, kt = timed_run(bar, args.steps, args.warmup)
+
+    # The timed window: steps from_step .. from_step + steps − 1, after an untimed advance and
+    # `warmup` untimed steps. Both return-map modes start from the same snapshot.
+    first = args.from_step
+    warm = min(args.warmup, first - 1)
+    bar.reset()
+    advance = first - 1 - warm
+    if advance:
+        print("[bench] advancing to step %d untimed" % advance, file=sys.stderr, flush=True)
+        bar.step(advance)
+    snap = bar.snapshot() if not args.no_warm_start else None
+    if warm:
+        bar.step(warm)
+    cpu_state = bar.snapshot() if not args.no_cpu_baseline else None
+    print("[bench] timing %d plain steps from step %d" % (args.steps, first), file=sys.stderr,
+          flush=True)
+    elapsed, kt = timed_window(bar, args.steps)
     u = bar.get("u")
-    eqps = bar.get("statev")[:, 1]
-    broken = bar.get("damage").sum()
     if not np.isfinite(u).all():
         raise SystemExit("non-finite displacement after the timed run")
+    win = window_result(bar, first, args.steps, elapsed, kt, n)
 
     value = n * args.steps / elapsed
     dom = max((k for k in kt if k != "neighbor_build"), key=lambda k: kt[k][0])
@@ -342,6 +403,8 @@ def main():
     per_launch = bytes_per_particle(dom, bbar, w["damage"] is not None) * n
     achieved = per_launch / avg_s
     step_bps = step_bytes_per_particle(bbar, w["damage"] is not None)
+    dev_kernel = bond_pass_device_kernel(dom, tuned) if dom.endswith("bond_pass") else None
+    traffic = measured_traffic(dom, w["name"], dev_kernel)
     out = {
         "metric": "particle-steps/s (500k JC bar, \u03b4=3.015\u0394x) GPU plain vs warm-start; %HBM peak",
         "value": value,
@@ -355,46 +418,55 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (deterministic lattice, JC material of the benchmark config)",
-        "config": {"workload": w["name"], "particles": n, "bonds": nnz, "grid": [w["nx"], w["ny"], w["nz"]],
-                   "spacing_mm": w["spacing"], "horizon_over_spacing": 3.015, "dt_ms": dt,
+        "config": {"workload": w["name"], "particles": n, "bonds": nnz,
+                   "grid": [w["nx"], w["ny"], w["nz"]], "spacing_mm": w["spacing"],
+                   "horizon_over_spacing": 3.015, "dt_ms": dt,
                    "end_velocity_mm_per_ms": w["velocity"], "damage": "critical_stretch s_c=0.02",
-                   "return_map": "bisection (parity)", "steps_from": 1, "launch_tuning": tuned},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK / 1e9 * 1e9,
-                     "unit": "B/s", "frac": achieved / HBM_PEAK,
-                     "traffic": measured_traffic(
-                         dom, w["name"],
-                         bond_pass_device_kernel(dom, tuned) if dom.endswith("bond_pass") else None),
+                   "return_map": "bisection (parity)", "steps_from": first,
+                   "launch_tuning": tuned},
+        "roofline": {"bound": "hbm", "kernel": dom, "device_kernel": dev_kernel,
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": None if traffic is None else
+                     "committed rocprofv3 FETCH_SIZE+WRITE_SIZE of this device kernel "
+                     "(profiles/traffic_%s_%s.json), not measured in this run" % (dom, w["name"]),
                      "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": avg_s * 1e3},
         "step_model": {"bytes_per_particle_step": step_bps,
                        "hbm_fraction": step_bps * value / HBM_PEAK},
-        "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in kt.items()},
-        "result": {"max_eqps": float(eqps.max()), "broken_bonds": int(broken)},
+        "kernels_ms_per_launch": win["kernels_ms_per_launch"],
+        "window": {k: v for k, v in win.items() if k != "kernels_ms_per_launch"},
+        "result": {"max_eqps": win["max_eqps"], "broken_bonds": win["broken_bonds"]},
     }
-    out["roofline"]["achieved"] /= 1e9
-    out["roofline"]["peak"] = HBM_PEAK / 1e9
-    out["roofline"]["unit"] = "GB/s"
     if not args.no_warm_start:
-        # The same bar and launch tuning with the learned return map (compacted: network and
-        # Newton on the plastic points only).
-        print("[bench] timing %d warm-start steps" % args.steps, file=sys.stderr, flush=True)
+        # The same bar, snapshot and launch tuning with the learned return map (compacted:
+        # network and Newton on the plastic points only).
+        print("[bench] timing %d warm-start steps from step %d" % (args.steps, first),
+              file=sys.stderr, flush=True)
         net = vpd.load_warm_start()
+        bar.restore(snap)
         bar.ctx.set_return_map(vpd.RETURN_MAP_LEARNED, 1e-12, net)
-        ws_elapsed, ws_kt = timed_run(bar, args.steps, args.warmup)
-        ws_eqps = bar.get("statev")[:, 1]
-        out["warm_start"] = {
-            "value": n * args.steps / ws_elapsed, "unit": "particle-steps/s",
-            "ms_per_step": ws_elapsed / args.steps * 1e3,
-            "kernels_ms_per_launch": {k: v[0] / v[1] for k, v in ws_kt.items()},
-            "max_eqps": float(ws_eqps.max())}
+        if warm:
+            bar.step(warm)
+        ws_elapsed, ws_kt = timed_window(bar, args.steps)
+        ws = window_result(bar, first, args.steps, ws_elapsed, ws_kt, n)
+        ws["unit"] = "particle-steps/s"
+        ws["over_plain"] = ws["value"] / value
+        out["warm_start"] = ws
         bar.ctx.set_return_map(vpd.RETURN_MAP_BISECTION, 1e-12)
-        if args.return_map_points:
-            print("[bench] return-map study on %d plastic points" % args.return_map_points,
-                  file=sys.stderr, flush=True)
-            out["return_map"] = return_map_study(args.return_map_points)
+        del snap
+    if not args.no_step1_window and first > 1:
+        print("[bench] timing %d plain steps from step 1" % args.steps, file=sys.stderr,
+              flush=True)
+        s1_elapsed, s1_kt = timed_run(bar, args.steps, args.warmup)
+        out["window_step1"] = window_result(bar, 1, args.steps, s1_elapsed, s1_kt, n)
+    if not args.no_warm_start and args.return_map_points:
+        print("[bench] return-map study on %d plastic points" % args.return_map_points,
+              file=sys.stderr, flush=True)
+        out["return_map"] = return_map_study(args.return_map_points)
     if not args.no_cpu_baseline:
         del bar
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
-        cb = cpu_baseline(args.cpu_steps, w)
+        cb = cpu_baseline(cpu_state, args.cpu_window, args.cpu_small_steps, w)
         cb["gpu_over_cpu"] = value / cb["value"]
         out["cpu_baseline"] = cb
     else:

@@ -257,6 +257,7 @@ class OracleBar:
         if rowptr is None:
             rowptr, col = build_neighbors(self.x, horizon)
         self.rowptr, self.col = rowptr, col
+        self.nnz = len(col)
         self.kinv = shape_inverse(self.x, self.vol, rowptr, col)
         self.codes, self.anchor, self.pin = constraint_codes(self.x, component, region_layers,
                                                              spacing, plane_strain)
@@ -280,6 +281,29 @@ class OracleBar:
         if self.velocity is not None:
             return f64([self.velocity * t, self.velocity, 0.0])
         return ramp(self.amplitude, self.rise_time, t)
+
+    def get(self, name):
+        """State field `name` (RefBar's names)."""
+        if name in ("u", "v", "a"):
+            return getattr(self, name).copy()
+        if name == "damage":
+            return self.damage.copy()
+        return self.mat[name].copy()
+
+    def set(self, name, arr):
+        """Overwrites state field `name`; F_n also resets E_n = greenLagrangeStrain(F_n)."""
+        a = np.array(arr, dtype=np.float64)
+        if name in ("u", "v", "a"):
+            getattr(self, name)[...] = a.reshape(self.n, 3)
+        elif name == "damage":
+            self.damage[...] = a
+        else:
+            self.mat[name][...] = a.reshape(self.mat[name].shape)
+            if name == "fN":
+                self.mat["strainN"][...] = green_lagrange(self.mat["fN"])
+
+    def set_step(self, step):
+        self.step_index = int(step)
 
     def step(self, k=1):
         for _ in range(k):
@@ -326,6 +350,9 @@ def ref():
         R.ref_case_size.restype = _L
         R.ref_case_step.argtypes = [C.c_void_p, C.c_int]
         R.ref_case_get.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        R.ref_case_set.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        R.ref_case_set_step.argtypes = [C.c_void_p, C.c_int]
+        R.ref_case_set_step.restype = None
         R.ref_run_case_json.argtypes = [C.c_char_p, C.c_char_p]
         _ref = R
     return _ref
@@ -436,6 +463,23 @@ class RefBar:
         if ref().ref_case_get(self.h, fid, out.ctypes.data_as(C.c_void_p)) != 0:
             raise RuntimeError("field %s not available yet" % name)
         return out
+
+    SETTABLE = ("u", "v", "a", "fN", "vN", "rN", "stressN", "statev", "temp", "eint", "einel",
+                "damage")
+
+    def set(self, name, arr):
+        """Overwrites state field `name` (a field a step reads; shapes as `get` returns)."""
+        if name not in self.SETTABLE:
+            raise ValueError("field %s is not settable" % name)
+        fid, _ = REF_FIELDS[name]
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        if a.size != self.get(name).size:
+            raise ValueError("field %s: wrong size %d" % (name, a.size))
+        _check(ref().ref_case_set(self.h, fid, a.ctypes.data_as(C.c_void_p)), _rerr)
+
+    def set_step(self, step):
+        """The number of completed steps: the next `step` call runs step + 1."""
+        ref().ref_case_set_step(self.h, int(step))
 
     def neighbors(self):
         rowptr = np.zeros(self.n + 1, np.int32)

@@ -438,6 +438,72 @@ int ref_case_get(void *h, int id, void *out) {
   }
 }
 
+// Field import, the inverse of ref_case_get for the state a step reads: 2 u, 3 v, 4 a,
+// 5 F_n (strainN follows as greenLagrangeStrain(F_n), CorotationalKinematics.cpp:141-149, as
+// VumatPdCorrespondenceMaterial::update leaves it), 6 V_n, 7 R_n, 8 σ̄_n, 9 statev,
+// 10 temperature, 11 internal energy, 12 inelastic energy, 16 bond damage. Lets a sample of
+// the reference start from a state reached elsewhere (e.g. a plastic phase).
+int ref_case_set(void *h, int id, const void *in) {
+  auto *c = static_cast<RefCase *>(h);
+  const long n = (long)c->block.referencePosition.size();
+  const double *d = static_cast<const double *>(in);
+  switch (id) {
+  case 2:
+    c->block.displacement = vecs(d, n);
+    return 0;
+  case 3:
+    c->dyn.velocity = vecs(d, n);
+    return 0;
+  case 4:
+    c->dyn.acceleration = vecs(d, n);
+    return 0;
+  case 5:
+    c->mat.deformationGradientN = tens(d, n);
+    for (long i = 0; i < n; ++i)
+      c->mat.strainN[i] = vumat::pd::greenLagrangeStrain(c->mat.deformationGradientN[i]);
+    return 0;
+  case 6:
+    c->mat.leftStretchN = tens(d, n);
+    return 0;
+  case 7:
+    c->mat.rotationN = tens(d, n);
+    return 0;
+  case 8:
+    c->mat.rawStressN = tens(d, n);
+    return 0;
+  case 9: {
+    size_t k = 0;
+    for (auto &r : c->mat.vumatStateN)
+      for (double &v : r)
+        v = d[k++];
+    return 0;
+  }
+  case 10:
+    std::memcpy(c->mat.temperatureN.data(), d, sizeof(double) * n);
+    return 0;
+  case 11:
+    std::memcpy(c->mat.internalEnergyN.data(), d, sizeof(double) * n);
+    return 0;
+  case 12:
+    std::memcpy(c->mat.inelasticEnergyN.data(), d, sizeof(double) * n);
+    return 0;
+  case 16: {
+    size_t k = 0;
+    for (auto &r : c->dstate.bondDamageN)
+      for (double &v : r)
+        v = d[k++];
+    return 0;
+  }
+  default:
+    return 1;
+  }
+}
+
+// Sets the number of completed steps (the next ref_case_step runs step + 1).
+void ref_case_set_step(void *h, int step) {
+  static_cast<RefCase *>(h)->step = step;
+}
+
 // The reference CLI path: runDemoCaseFromJson (DemoCaseRunner.cpp:435-492).
 int ref_run_case_json(const char *casePath, const char *pluginDirectory) {
   try {

@@ -262,6 +262,7 @@ class Context:
         self.n = 0
         self.nnz = 0
         self.nstatev = nstatev
+        self.damage = damage is not None
         if lib().vpd_create(C.byref(cfg), C.byref(self.h)):
             raise VpdError(lib().vpd_last_error(None).decode())
 
@@ -697,6 +698,31 @@ class Bar:
 
     def get(self, name):
         return self.ctx.download(name)
+
+    STATE_FIELDS = ("u", "v", "a", "fN", "vN", "rN", "stressN", "statev", "temp", "eint",
+                    "einel")
+
+    def snapshot(self):
+        """The end-of-step state a following step reads (host copies), with the step index and
+        the indices of the broken bonds in CSR order."""
+        st = {k: self.get(k) for k in self.STATE_FIELDS}
+        st["step"] = np.array(self.step_index)
+        if self.ctx.damage:
+            st["broken"] = np.flatnonzero(self.get("damage"))
+        else:
+            st["broken"] = np.zeros(0, np.int64)
+        return st
+
+    def restore(self, st):
+        """Back to a snapshot: the next step is st['step'] + 1, bit for bit as if the run had
+        gone on from there."""
+        for k in self.STATE_FIELDS:
+            self.ctx.upload(k, st[k])
+        if self.ctx.damage:
+            d = np.zeros(self.ctx.nnz)
+            d[st["broken"]] = 1.0
+            self.ctx.upload("damage", d)
+        self.step_index = int(st["step"])
 
 
 # ------------------------------------------------------------------ batched specimens
