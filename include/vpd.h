@@ -372,6 +372,27 @@ int vpd_device_malloc(uint64_t bytes, void **out);
 int vpd_device_free(void *ptr);
 int vpd_memcpy(void *dst, const void *src, uint64_t bytes, int to_device);
 
+/* The neighbour build of vpd_build_neighbors (buildNeighborList,
+ * src/pd/PdCorrespondenceSkeleton.cpp:95-129: the same rows, bit for bit) on caller device
+ * arrays, allocating nothing. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (serving loops and harnesses that forbid hipMalloc). Its work memory
+ * is one static device array of about 44 MiB that the library holds on each GPU it runs on
+ * until the process ends; it limits a call to n <= 2097152 particles and a cell grid of at
+ * most 1048576 cells (cells of width >= horizon over the bounding box), and larger inputs are
+ * refused (VPD_ERR_ARGUMENT / VPD_ERR_UNSUPPORTED). Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other host threads, other streams) race and must be
+ * serialised by the caller.
+ *   x        device, three planes x[d*n + i] (d = 0, 1, 2)
+ *   rowptr   device int32[n + 1], written
+ *   col      device int32[col_capacity], written (may be NULL when col_capacity is 0)
+ *   *nnz     host: the neighbour count, also when col_capacity is too small (then the call
+ *            returns VPD_ERR_ARGUMENT with rowptr written and col untouched: size col and call
+ *            again)
+ *   stream   a hipStream_t, or NULL for the default stream. The call synchronises it (it reads
+ *            the coordinate bounds and the counts back), so it cannot be graph-captured. */
+int vpd_build_neighbors_noalloc(const double *x, int64_t n, double horizon, int32_t *rowptr,
+                                int32_t *col, int64_t col_capacity, int64_t *nnz, void *stream);
+
 /* The Johnson-Cook return map alone on device arrays (column-major [k + nblock*c]), for
  * comparing solvers on identical inputs. No reference counterpart beyond the solver of
  * materials/examples/JohnsonCook.f90:114-136. mode: 0 the reference bisection (parity),

@@ -88,3 +88,20 @@ def test_config_rejections_precede_device():
         vpd.Context(1.0, 1.0, [1.0] * 13)
     with pytest.raises(vpd.VpdError, match="initiation > 0"):
         vpd.Context(1.0, 1.0, [1.0] * 14, damage=(0.0, 0.0))
+
+
+def test_no_alloc_neighbors_refuse_before_any_device_work():
+    # Refusals of the allocation-free neighbour build are decided on the host, before the
+    # pointers are used.
+    import ctypes
+    import vpd
+    L = vpd.lib()
+    dummy = ctypes.c_void_p(1)
+    nnz = ctypes.c_int64()
+    assert L.vpd_build_neighbors_noalloc(dummy, (1 << 21) + 1, 1.0, dummy, dummy, 10,
+                                         ctypes.byref(nnz), None) == 1
+    assert b"at most 2097152" in L.vpd_last_error(None)
+    assert L.vpd_build_neighbors_noalloc(dummy, 10, 0.0, dummy, dummy, 10,
+                                         ctypes.byref(nnz), None) == 1
+    assert L.vpd_build_neighbors_noalloc(None, 10, 1.0, dummy, dummy, 10,
+                                         ctypes.byref(nnz), None) == 1

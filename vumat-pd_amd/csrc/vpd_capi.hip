@@ -13,6 +13,7 @@
 #include <cstring>
 #include <functional>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -445,25 +446,37 @@ __global__ void k_material_init(int64_t n, Planes vN, Planes rN, Planes sig, Pla
 }
 
 // Exclusive scan of `count` ints into int32 `out`; *total receives the sum.
+// The scan itself in caller-supplied work buffers: part (count int64) and bt
+// (ceil(count / SCAN_ITEMS) + 1 int64). Synchronises the stream; *total receives the sum, and
+// `out` is written only when the sum fits int32 (otherwise *total > INT_MAX tells the caller).
+hipError_t scan_ints_in(hipStream_t st, const int *in, int *out, int64_t count, int64_t *part,
+                        int64_t *bt, int64_t *total) {
+  const int64_t nb = ceil_div(count, SCAN_ITEMS);
+  k_scan_block<<<(int)nb, 256, 0, st>>>(count, in, part, bt);
+  k_scan_totals<<<1, 256, 0, st>>>(nb, bt, bt + nb);
+  hipError_t e = hipMemcpyAsync(total, bt + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(st);
+  if (e != hipSuccess || *total > INT_MAX)
+    return e;
+  k_scan_add<<<grid_of(count), 256, 0, st>>>(count, part, bt, out);
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(st);
+  return e;
+}
+
 int scan_ints(vpd_context *c, const int *in, int *out, int64_t count, int64_t *total) {
   const int64_t nb = ceil_div(count, SCAN_ITEMS);
   int64_t *part = nullptr, *bt = nullptr;
   if (dalloc(c, part, count) || dalloc(c, bt, nb + 1))
     return VPD_ERR_DEVICE;
-  k_scan_block<<<(int)nb, 256, 0, c->stream>>>(count, in, part, bt);
-  k_scan_totals<<<1, 256, 0, c->stream>>>(nb, bt, bt + nb);
-  HIPTRY(c, hipMemcpyAsync(total, bt + nb, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
-  HIPTRY(c, hipStreamSynchronize(c->stream));
-  if (*total > INT_MAX) {
-    dfree(part);
-    dfree(bt);
-    return fail(c, VPD_ERR_UNSUPPORTED, "Neighbour count exceeds the int32 CSR range.");
-  }
-  k_scan_add<<<grid_of(count), 256, 0, c->stream>>>(count, part, bt, out);
-  HIPTRY(c, hipGetLastError());
-  HIPTRY(c, hipStreamSynchronize(c->stream));
+  const hipError_t e = scan_ints_in(c->stream, in, out, count, part, bt, total);
   dfree(part);
   dfree(bt);
+  HIPTRY(c, e);
+  if (*total > INT_MAX)
+    return fail(c, VPD_ERR_UNSUPPORTED, "Neighbour count exceeds the int32 CSR range.");
   return VPD_OK;
 }
 
@@ -2702,6 +2715,181 @@ int vpd_memcpy(void *dst, const void *src, uint64_t bytes, int to_device) {
   return e == hipSuccess ? VPD_OK
                          : fail(nullptr, VPD_ERR_DEVICE,
                                 std::string("HIP error: ") + hipGetErrorString(e));
+}
+
+} // extern "C"
+
+// ---- allocation-free neighbour build ---------------------------------------------------
+// The work memory of vpd_build_neighbors_noalloc is one file-scope device array, `scratch`;
+// every GPU the library runs on holds its own copy, at its own address, until the process
+// ends. In host code the array's name is not a device address: each device's address comes
+// from hipGetSymbolAddress (scratch_base) and only that address is handed to kernels.
+// Layout, for at most kNoAllocMaxParticles particles and kNoAllocMaxCells cells:
+//   Bounds                                       (256 B)
+//   part      int64[max(n + 1, cells)]            scan partial sums
+//   totals    int64[ceil(max(n + 1, cells) / SCAN_ITEMS) + 1]   scan block totals
+//   cellOf    int32[n + 1]   cell of each particle; reused for the row counts
+//   items     int32[n]       particle ids by cell
+//   cellCount, cellStart, cursor   int32[cells] each
+namespace {
+
+constexpr int64_t kNoAllocMaxParticles = int64_t(1) << 21; // 2,097,152
+constexpr int64_t kNoAllocMaxCells = int64_t(1) << 20;     // 1,048,576
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kScanItems = (size_t)kNoAllocMaxParticles + 1; // >= kNoAllocMaxCells
+constexpr size_t kOffPart = 256;
+constexpr size_t kOffTotals = kOffPart + align256(8 * kScanItems);
+constexpr size_t kOffCellOf =
+    kOffTotals + align256(8 * ((kScanItems + SCAN_ITEMS - 1) / SCAN_ITEMS + 1));
+constexpr size_t kOffItems = kOffCellOf + align256(4 * ((size_t)kNoAllocMaxParticles + 1));
+constexpr size_t kOffCells = kOffItems + align256(4 * (size_t)kNoAllocMaxParticles);
+constexpr size_t kScratchBytes = kOffCells + 3 * align256(4 * (size_t)kNoAllocMaxCells);
+static_assert(sizeof(Bounds) <= 256, "Bounds must fit its slot");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void *g_scratchBase[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device with that device
+// current (hipGetSymbolAddress; the array's name itself is not a device address here).
+int scratch_base(unsigned char **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return fail(nullptr, VPD_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(nullptr, VPD_ERR_UNSUPPORTED, "Device index out of range.");
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchBase[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(nullptr, VPD_ERR_DEVICE,
+                  std::string("HIP error: ") + hipGetErrorString(e) + " at hipGetSymbolAddress");
+    g_scratchBase[dev] = p;
+  }
+  *out = static_cast<unsigned char *>(g_scratchBase[dev]);
+  return VPD_OK;
+}
+
+__global__ void __launch_bounds__(256) k_fill_int(int *p, int64_t count, int value) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count)
+    p[i] = value;
+}
+
+} // namespace
+
+extern "C" {
+
+int vpd_build_neighbors_noalloc(const double *x, int64_t n, double horizon, int32_t *rowptr,
+                                int32_t *col, int64_t colCapacity, int64_t *nnzOut,
+                                void *stream) {
+  if (!x || !rowptr || !nnzOut || n < 0 || colCapacity < 0 || (colCapacity > 0 && !col))
+    return fail(nullptr, VPD_ERR_ARGUMENT, "Invalid neighbour-build arguments.");
+  *nnzOut = 0;
+  if (n > kNoAllocMaxParticles)
+    return fail(nullptr, VPD_ERR_ARGUMENT,
+                "The allocation-free neighbour build takes at most 2097152 particles.");
+  if (!(horizon > 0.0) || !std::isfinite(horizon))
+    return fail(nullptr, VPD_ERR_ARGUMENT, "PD horizon must be positive and finite.");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto hipfail = [](hipError_t e) {
+    return fail(nullptr, VPD_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e));
+  };
+#define NATRY(expr)                                                                            \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return hipfail(e_);                                                                      \
+  } while (0)
+  if (n == 0) {
+    const int32_t zero = 0;
+    NATRY(hipMemcpyAsync(rowptr, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st));
+    NATRY(hipStreamSynchronize(st));
+    return VPD_OK;
+  }
+  unsigned char *base = nullptr;
+  if (int s = scratch_base(&base))
+    return s;
+  Bounds *db = reinterpret_cast<Bounds *>(base);
+  int64_t *part = reinterpret_cast<int64_t *>(base + kOffPart);
+  int64_t *totals = reinterpret_cast<int64_t *>(base + kOffTotals);
+  int *cellOf = reinterpret_cast<int *>(base + kOffCellOf);
+  int *items = reinterpret_cast<int *>(base + kOffItems);
+  int *cellCount = reinterpret_cast<int *>(base + kOffCells);
+  int *cellStart = reinterpret_cast<int *>(base + kOffCells + align256(4 * (size_t)kNoAllocMaxCells));
+  int *cursor = reinterpret_cast<int *>(base + kOffCells + 2 * align256(4 * (size_t)kNoAllocMaxCells));
+
+  // Search radius and cell grid as in vpd_build_neighbors (PdCorrespondenceSkeleton.cpp:30-39,
+  // 101-102).
+  Bounds hb;
+  hb.maxAbsBits = 0;
+  for (int d = 0; d < 3; ++d) {
+    hb.cmin[d] = LLONG_MAX;
+    hb.cmax[d] = LLONG_MIN;
+  }
+  hb.nonfinite = 0;
+  NATRY(hipMemcpyAsync(db, &hb, sizeof(Bounds), hipMemcpyHostToDevice, st));
+  k_abs_max<<<grid_of(n), 256, 0, st>>>(n, x, db);
+  NATRY(hipGetLastError());
+  NATRY(hipMemcpyAsync(&hb, db, sizeof(Bounds), hipMemcpyDeviceToHost, st));
+  NATRY(hipStreamSynchronize(st));
+  if (hb.nonfinite)
+    return fail(nullptr, VPD_ERR_NUMERIC, "Nonfinite PD coordinate.");
+  double maxAbs;
+  std::memcpy(&maxAbs, &hb.maxAbsBits, sizeof(double));
+  const double scale = std::max(horizon, maxAbs);
+  const double tol = 64.0 * std::numeric_limits<double>::epsilon() * scale;
+  Grid g;
+  g.radius = horizon + tol;
+  g.cw = horizon + 2.0 * tol;
+  k_cell_bounds<<<grid_of(n), 256, 0, st>>>(n, x, g.cw, db);
+  NATRY(hipGetLastError());
+  NATRY(hipMemcpyAsync(&hb, db, sizeof(Bounds), hipMemcpyDeviceToHost, st));
+  NATRY(hipStreamSynchronize(st));
+  double cells = 1.0;
+  for (int d = 0; d < 3; ++d) {
+    g.c0[d] = hb.cmin[d];
+    g.dim[d] = hb.cmax[d] - hb.cmin[d] + 1;
+    cells *= (double)g.dim[d];
+  }
+  if (cells > (double)kNoAllocMaxCells)
+    return fail(nullptr, VPD_ERR_UNSUPPORTED,
+                "Cell grid too large for the allocation-free neighbour build (more than "
+                "1048576 cells).");
+  const int64_t nc = (int64_t)cells;
+
+  // Cells, then per-particle counts -> rowptr, then rows; every counter starts at zero.
+  k_fill_int<<<grid_of(nc), 256, 0, st>>>(cellCount, nc, 0);
+  k_fill_int<<<grid_of(nc), 256, 0, st>>>(cursor, nc, 0);
+  k_cell_count<<<grid_of(n), 256, 0, st>>>(n, x, g, cellOf, cellCount);
+  NATRY(hipGetLastError());
+  int64_t placed = 0;
+  NATRY(scan_ints_in(st, cellCount, cellStart, nc, part, totals, &placed));
+  k_cell_fill<<<grid_of(n), 256, 0, st>>>(n, cellOf, cellStart, cursor, items);
+  int *rowCount = cellOf; // cellOf is not read again
+  k_fill_int<<<grid_of(n + 1), 256, 0, st>>>(rowCount, n + 1, 0);
+  k_row_scan<false><<<grid_of(n), 256, 0, st>>>(n, x, g, cellStart, cellCount, items, rowCount,
+                                                nullptr, nullptr);
+  NATRY(hipGetLastError());
+  int64_t nnz = 0;
+  NATRY(scan_ints_in(st, rowCount, rowptr, n + 1, part, totals, &nnz));
+  if (nnz > INT_MAX)
+    return fail(nullptr, VPD_ERR_UNSUPPORTED, "Neighbour count exceeds the int32 CSR range.");
+  *nnzOut = nnz;
+  if (nnz > colCapacity)
+    return fail(nullptr, VPD_ERR_ARGUMENT,
+                "Neighbour list capacity too small: " + std::to_string(nnz) + " entries needed.");
+  k_row_scan<true><<<grid_of(n), 256, 0, st>>>(n, x, g, cellStart, cellCount, items, nullptr,
+                                               rowptr, col);
+  k_row_sort<<<(int)n, 64, 0, st>>>(n, rowptr, col);
+  NATRY(hipGetLastError());
+  NATRY(hipStreamSynchronize(st));
+#undef NATRY
+  return VPD_OK;
 }
 
 int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *props, int nprops,

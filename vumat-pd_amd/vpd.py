@@ -146,6 +146,9 @@ def lib():
         L.vpd_return_map_device.argtypes = ([C.c_int, C.c_int64, C.c_int, _D, C.c_int, C.c_double,
                                              C.c_double] + [C.c_void_p] * 12 +
                                             [C.POINTER(C.c_double)])
+        L.vpd_build_neighbors_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_void_p,
+                                                  C.c_void_p, C.c_int64,
+                                                  C.POINTER(C.c_int64), C.c_void_p]
         L.vpd_device_malloc.argtypes = [C.c_uint64, C.POINTER(C.c_void_p)]
         L.vpd_device_free.argtypes = [C.c_void_p]
         L.vpd_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
@@ -461,8 +464,8 @@ class Context:
 class DeviceBuffer:
     """A device allocation made through libvpd's own HIP runtime."""
 
-    def __init__(self, host):
-        self.host = np.ascontiguousarray(host, dtype=np.float64)
+    def __init__(self, host, dtype=np.float64):
+        self.host = np.ascontiguousarray(host, dtype=dtype)
         self.ptr = C.c_void_p()
         if lib().vpd_device_malloc(self.host.nbytes, C.byref(self.ptr)):
             raise VpdError(lib().vpd_last_error(None).decode())
