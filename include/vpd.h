@@ -41,7 +41,11 @@ enum vpd_return_map {
   VPD_RETURN_MAP_BISECTION = 0,       /* exact bracketed bisection of JohnsonCook.f90:114-136 */
   VPD_RETURN_MAP_BISECTION_TIGHT = 1, /* the same scheme to |r| <= rtol·q_trial */
   VPD_RETURN_MAP_NEWTON = 2,          /* safeguarded Newton seeded with state 5 */
-  VPD_RETURN_MAP_LEARNED = 3          /* safeguarded Newton seeded by the warm-start network */
+  VPD_RETURN_MAP_LEARNED = 3,         /* safeguarded Newton seeded by the warm-start network */
+  /* vpd_return_map_device only: mode 0 with the FP64 yield evaluated at every iteration (modes
+   * 0 and 1 evaluate it only where an FP32 estimate with an error bound cannot decide the
+   * iteration's branch, with the same iterates and results bit for bit); for checking. */
+  VPD_RETURN_MAP_BISECTION_UNSCREENED = 4
 };
 
 /* The VUMAT the material stage runs (the plugin of VumatMaterialConfig::pluginPath). */
@@ -397,7 +401,8 @@ int vpd_build_neighbors_noalloc(const double *x, int64_t n, double horizon, int3
  * comparing solvers on identical inputs. No reference counterpart beyond the solver of
  * materials/examples/JohnsonCook.f90:114-136. mode: 0 the reference bisection (parity),
  * 1 the same bisection run to |r| <= rtol·q, 2 safeguarded Newton seeded with state 5,
- * 3 safeguarded Newton seeded with `seed` (the learned warm start). iterations/fallbacks
+ * 3 safeguarded Newton seeded with `seed` (the learned warm start), 4 mode 0 evaluating the
+ * FP64 yield at every iteration. iterations/fallbacks
  * (per point, may be NULL) report the solver work; *elapsed_ms the kernel time. */
 int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *props, int nprops,
                           double dt, double rtol, const double *strain_inc,

@@ -77,6 +77,46 @@ def test_return_map_modes(vpd, orc):
     assert it[3] <= 3.0   # an exact seed needs only the acceptance test
 
 
+@pytest.mark.parametrize("case", ["plastic", "first_yield", "heated", "high_rate", "tiny",
+                                  "reference_props"])
+def test_screened_bisection_equals_unscreened(vpd, case):
+    """The parity bisection decides most iterations from an FP32 estimate with an error bound
+    and evaluates the FP64 yield only where that cannot decide: every output (stress, state
+    including the iteration count in state 6, energies, iterations) must equal the bisection
+    that evaluates every iteration, bit for bit."""
+    from conftest import JC_PROPS_REFERENCE
+    banner("screened bisection, %s" % case)
+    n = 200000
+    rng = np.random.default_rng(sum(map(ord, case)))
+    inc, stress, state = plastic_block(n, 11)
+    props, dt = JC_PROPS, 8.5377e-5
+    if case == "first_yield":
+        state[1] = 0.0
+        state[3] = 0.0
+        state[4] = 0.0
+    elif case == "heated":
+        state[2] = 293.0 + rng.random(n) * 1499.0
+    elif case == "high_rate":
+        inc *= 20.0
+        dt = 1e-6
+    elif case == "tiny":
+        inc *= 1e-3
+        state[3] = 0.0
+    elif case == "reference_props":
+        props = JC_PROPS_REFERENCE
+        state[1] = rng.random(n) * 2.0
+    eint = rng.random(n)
+    einel = rng.random(n)
+    a = vpd.return_map(0, props, dt, inc, stress, state, eint, einel)
+    b = vpd.return_map(4, props, dt, inc, stress, state, eint, einel)
+    plastic = (b["state"][4] > 0).mean()
+    print("%s: plastic %.3f, mean iterations %.2f, screened %.3f ms, every-iteration %.3f ms"
+          % (case, plastic, b["iterations"].mean(), a["ms"], b["ms"]))
+    for k in ("stress", "state", "eint", "einel", "iterations"):
+        assert np.array_equal(a[k], b[k]), k
+    assert case == "tiny" or plastic > 0.1
+
+
 def test_return_map_elastic_points_unchanged(vpd, orc):
     banner("return-map elastic points")
     n = 5000

@@ -2899,7 +2899,7 @@ int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *p
                           const double *seed, double *stressNew, double *stateNew,
                           double *enerInternNew, double *enerInelasNew, int32_t *iterations,
                           int32_t *fallbacks, double *elapsedMs) {
-  if (mode < RM_BISECTION || mode > RM_LEARNED)
+  if (mode < RM_BISECTION || mode > VPD_RETURN_MAP_BISECTION_UNSCREENED)
     return fail(nullptr, VPD_ERR_ARGUMENT, "Unknown return-map mode.");
   if (!props || nprops < 14 || nstatev < 6)
     return fail(nullptr, VPD_ERR_ARGUMENT, "JohnsonCook VUMAT requires 14 props and 6 states");
@@ -2941,6 +2941,9 @@ int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *p
     break;
   case RM_NEWTON:
     k_vumat_jc<RM_NEWTON><<<grid_of(nblock), 256>>>(g);
+    break;
+  case VPD_RETURN_MAP_BISECTION_UNSCREENED:
+    k_vumat_jc<RM_BISECTION_UNSCREENED><<<grid_of(nblock), 256>>>(g);
     break;
   default:
     k_vumat_jc<RM_LEARNED><<<grid_of(nblock), 256>>>(g);

@@ -301,6 +301,54 @@ __device__ __forceinline__ double jc_yield(const JcParams &p, double ep, double 
   return jc_yield_soft(p, ep, rate, jc_softening(p, temp));
 }
 
+// Screen of the bisection's yield evaluation. The bisection's iterates depend only on the
+// sign of each residual and on whether it is below the tolerance, and it stores only the last
+// evaluated yield; so an iteration whose residual is far from 0 and from ±tol can take its
+// decision from an estimate. The estimate forms (A + B ε̄ⁿ)·soft·(1 + C ln ρ), ρ = Δ/(dt ε̇₀),
+// with log2 x = e + log2(m) for x = m·2^e (m in [0.5, 1), exponent exact, the FP32 hardware
+// log2 of m) and 2^t = 2^⌊t⌋·2^(t − ⌊t⌋) (the FP32 hardware exp2 of the fraction); the
+// rest is FP64. With the hardware log2/exp2 within 2 ulp (about 2.4e-7), |log2 x| errs by at
+// most 3e-7, ε̄ⁿ by a relative (0.21·|n| + 0.28)·1e-6 and ln ρ by 2.1e-7; `margin` bounds the
+// estimate's distance from the exact FP64 value with at least 5x of slack on those terms:
+//   1e-5·|soft|·[(|A| + (1 + |n|)·|B ε̄ⁿ|)·|1 + C ln ρ| + (|A| + |B ε̄ⁿ|)·|C|·(|ln ρ| + 1)].
+// Returns false (no estimate) for arguments outside the range the FP32 steps cover.
+__device__ __forceinline__ double screen_log2(double x) { // x > 0, normal
+  const float m = (float)__builtin_amdgcn_frexp_mant(x);
+  return (double)__builtin_amdgcn_frexp_exp(x) + (double)__builtin_amdgcn_logf(m);
+}
+__device__ __forceinline__ bool jc_yield_screen(const JcParams &p, double ep, double dep,
+                                                double rateScale, double soft, double &y,
+                                                double &margin) {
+  double pw = 0.0; // ε̄ⁿ
+  if (ep != 0.0) {
+    if (!(ep > 1e-300 && ep < 1e300))
+      return false;
+    const double t = p.hardN * screen_log2(ep);
+    if (!(fabs(t) < 1000.0))
+      return false;
+    const double k = floor(t);
+    pw = __builtin_amdgcn_ldexp((double)__builtin_amdgcn_exp2f((float)(t - k)), (int)k);
+  } else if (!(p.hardN > 0.0)) {
+    return false; // pow(0, n) for n <= 0 is left to the exact evaluation
+  }
+  double lr = 0.0; // ln ρ (0 for Δ = 0: the rate is then ε̇₀)
+  if (dep != 0.0) {
+    const double rho = dep * rateScale;
+    if (!(rho > 1e-300 && rho < 1e300))
+      return false;
+    lr = 0.6931471805599453 * screen_log2(rho);
+  }
+  const double bpw = p.b0 * pw;
+  const double hard = p.a0 + bpw;
+  const double rfac = 1.0 + p.rateC * lr;
+  y = hard * soft * rfac;
+  const double absA = fabs(p.a0), absB = fabs(bpw);
+  margin = 1e-5 * fabs(soft) *
+           ((absA + (1.0 + fabs(p.hardN)) * absB) * fabs(rfac) +
+            (absA + absB) * fabs(p.rateC) * (fabs(lr) + 1.0));
+  return true;
+}
+
 // Return-map solvers of the Johnson-Cook stage.
 //   RM_BISECTION  the exact bracketed bisection of JohnsonCook.f90:114-136 (parity mode):
 //                 tolerance E·1e-9, 50 iterations, iteration-1 seed = state 5.
@@ -311,9 +359,15 @@ __device__ __forceinline__ double jc_yield(const JcParams &p, double ep, double 
 //                 falling back to bisection whenever a step leaves the bracket; stops at
 //                 |r| <= rtol·q_trial.
 //   RM_LEARNED    RM_NEWTON seeded with an external Δε̄ᵖ estimate (the learned warm start).
+// Both bisections screen their iterations (jc_yield_screen): an iteration evaluates the FP64
+// yield only when the estimate cannot decide, or when the loop ends there; iterates, the
+// stored yield and every result are those of evaluating every iteration exactly, bit for bit.
+//   RM_BISECTION_UNSCREENED  RM_BISECTION with the FP64 yield at every iteration (the literal
+//                 loop, kept to check the screened one against).
 enum ReturnMap { RM_BISECTION = 0, RM_BISECTION_TIGHT = 1, RM_NEWTON = 2, RM_LEARNED = 3 };
 // The material stage's form for the linear elastic VUMAT (not a return-map solver).
 constexpr int RM_ELASTIC = 4;
+constexpr int RM_BISECTION_UNSCREENED = 5;
 
 // The linear isotropic elastic VUMAT of materials/examples/Elastic.f: props E, ν;
 // λ = Eν/((1+ν)(1−2ν)), μ = E/(2(1+ν)); the 6×6 stiffness with λ + 2μ / λ on the normal
@@ -439,29 +493,69 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
 
   double epInc = 0.0, yield = oldYield;
   int iteration = 0;
-  if (MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT) {
-    const double tol = MODE == RM_BISECTION ? p.tolerance : rtol * qTrial;
-    const int maxIt = MODE == RM_BISECTION ? 50 : 200;
+  if (MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT || MODE == RM_BISECTION_UNSCREENED) {
+    constexpr bool SCREEN = MODE != RM_BISECTION_UNSCREENED;
+    const double tol = MODE == RM_BISECTION_TIGHT ? rtol * qTrial : p.tolerance;
+    const int maxIt = MODE == RM_BISECTION_TIGHT ? 200 : 50;
     const double soft = jc_softening(p, temperature); // the same at every iteration
+    const double rateScale = SCREEN ? 1.0 / (dt * p.rate0) : 0.0;
     double epLower = 0.0, epUpper = qTrial / (2.0 * p.mu);
-    while (iteration < maxIt) { // JohnsonCook.f90:119-136
-      iteration = iteration + 1;
+    // JohnsonCook.f90:119-136: iterations 1..maxIt; iteration k evaluates the yield at the
+    // current Δ, ends the loop when |r| < tol (or at Δ = 0 with r < 0), else halves the
+    // bracket (iteration 1 then moves to the state-5 seed). Here each round first runs a
+    // lane's screened iterations until one needs the FP64 yield (or is the last allowed),
+    // then evaluates that one exactly: the FP64 evaluations of a wavefront's lanes fall in
+    // the same round instead of one per iteration. The iterates and the stored yield are
+    // the literal loop's.
+    bool done = false, exact = true; // iteration 1 is evaluated exactly
+    iteration = 1;
+    while (!done) {
+      if (SCREEN) {
+        while (!exact) {
+          const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
+          double ya, margin, ra = 0.0;
+          bool decided = false;
+          if (jc_yield_screen(p, epOld + epInc, epInc, rateScale, soft, ya, margin)) {
+            ra = qNew - fmax(ya, oldYield);
+            const double slack = margin + 1e-14 * (qNew + fabs(ya));
+            // Same sign as the exact residual, which is not below tol and ends nothing.
+            decided = fabs(ra) >= tol + slack && !(epInc == 0.0 && ra < 0.0);
+          }
+          if (!decided) {
+            exact = true;
+            break;
+          }
+          if (ra >= 0.0)
+            epLower = fmax(epLower, epInc);
+          else
+            epUpper = fmin(epUpper, epInc);
+          epInc = 0.5 * (epLower + epUpper);
+          iteration = iteration + 1;
+          exact = iteration == maxIt;
+        }
+      }
       const double epRate = epInc / dt;
       yield = jc_yield_soft(p, epOld + epInc, epRate, soft);
       yield = fmax(yield, oldYield);
       const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
       const double residual = qNew - yield;
-      if (fabs(residual) < tol)
-        break;
-      if (epInc == 0.0 && residual < 0.0)
-        break;
-      if (residual >= 0.0)
-        epLower = fmax(epLower, epInc);
-      else
-        epUpper = fmin(epUpper, epInc);
-      epInc = 0.5 * (epLower + epUpper);
-      if (iteration == 1)
-        epInc = fmin(fmax(state[4], epLower), epUpper);
+      if (fabs(residual) < tol || (epInc == 0.0 && residual < 0.0)) {
+        done = true;
+      } else {
+        if (residual >= 0.0)
+          epLower = fmax(epLower, epInc);
+        else
+          epUpper = fmin(epUpper, epInc);
+        epInc = 0.5 * (epLower + epUpper);
+        if (iteration == 1)
+          epInc = fmin(fmax(state[4], epLower), epUpper);
+        if (iteration == maxIt) {
+          done = true;
+        } else {
+          iteration = iteration + 1;
+          exact = !SCREEN || iteration == maxIt;
+        }
+      }
     }
   } else {
     // Iteration 1 tests Δ = 0 exactly as the reference does (elastic exit).

@@ -27,6 +27,7 @@ FIELD = dict(x=0, volume=1, u=2, v=3, a=4, fN=5, vN=6, rN=7, stressN=8, statev=9
 DAMAGE_NONE, DAMAGE_CRITICAL_STRETCH = 0, 1
 MATERIAL_JOHNSON_COOK, MATERIAL_ELASTIC = 0, 1
 RETURN_MAP_BISECTION, RETURN_MAP_BISECTION_TIGHT, RETURN_MAP_NEWTON, RETURN_MAP_LEARNED = 0, 1, 2, 3
+RETURN_MAP_BISECTION_UNSCREENED = 4  # vpd_return_map_device only
 MLP_PARAMS, MLP_NORM = 8961, 18
 
 
