@@ -105,3 +105,57 @@ def test_no_alloc_neighbors_refuse_before_any_device_work():
                                          ctypes.byref(nnz), None) == 1
     assert L.vpd_build_neighbors_noalloc(None, 10, 1.0, dummy, dummy, 10,
                                          ctypes.byref(nnz), None) == 1
+
+
+def test_device_vumat_rejections_precede_device():
+    # vpd_vumat_adapter_call_device refuses what the Fortran routine STOPs on
+    # (JohnsonCook.f90:52-53) and what VumatMaterialAdapter::updateBlock refuses
+    # (VumatMaterialAdapter.cpp:150-151), before any device pointer is used.
+    import ctypes as C
+    import math
+    import vpd
+    L = vpd.lib()
+
+    def call(nb=4, ndir=3, nshr=3, nstatev=6, nfieldv=1, nprops=14, dt=1e-4, strain=True):
+        ints = [C.c_int(v) for v in (nb, ndir, nshr, nstatev, nfieldv, nprops, 0)]
+        dbl = [C.c_double(v) for v in (0.0, 0.0, dt)]
+        p = C.c_void_p(16)
+        arrays = [p] * 22
+        if not strain:
+            arrays[4] = None  # strainInc
+        return L.vpd_vumat_adapter_call_device(*[C.byref(v) for v in ints],
+                                               *[C.byref(d) for d in dbl],
+                                               b"JOHNSONCOOK".ljust(80), *arrays)
+
+    def err():
+        return L.vpd_last_error(None).decode()
+
+    assert call(ndir=2) == 1 and err() == "JohnsonCook VUMAT requires ndir=3,nshr=3"
+    assert call(nshr=2) == 1 and err() == "JohnsonCook VUMAT requires ndir=3,nshr=3"
+    assert call(nstatev=5) == 1 and err() == "JohnsonCook VUMAT requires 14 props and 6 states"
+    assert call(nprops=13) == 1 and err() == "JohnsonCook VUMAT requires 14 props and 6 states"
+    for dt in (math.nan, 0.0, -1e-4, math.inf):
+        assert call(dt=dt) == 1 and err() == "VUMAT update requires dt > 0."
+    assert call(strain=False) == 1 and err() == "Null VUMAT array."
+    assert call(nb=0) == 0  # an empty block is a no-op
+
+
+def test_velocity_band_closed_form(orc):
+    """The velocity-driven end band (u = v·t, v, a = 0 at t = step·dt) against its closed form
+    after k steps of the oracle bar: the band's axial displacement, velocity and acceleration,
+    the support band held at zero, and the interior actually moving."""
+    from conftest import JC_PROPS
+    v, dt, k = 1.5, 8.5377e-5, 30
+    bar = orc.OracleBar(nx=16, ny=4, nz=4, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=dt,
+                        density=7.85e-3, amplitude=0.0, rise_time=1.0, velocity=v)
+    bar.step(k)
+    codes = bar.codes[:, 0]
+    band, support = codes == 2, codes == 1
+    assert band.sum() == support.sum() == 3 * 4 * 4
+    t = k * dt
+    assert np.array_equal(bar.u[band, 0], np.full(band.sum(), v * t))
+    assert np.array_equal(bar.v[band, 0], np.full(band.sum(), v))
+    assert np.array_equal(bar.a[band, 0], np.zeros(band.sum()))
+    assert not bar.u[support, 0].any() and not bar.v[support, 0].any()
+    inner = ~(band | support)
+    assert 0.0 < np.abs(bar.u[inner, 0]).max() < v * t

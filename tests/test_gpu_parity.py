@@ -479,8 +479,7 @@ def test_window_knob_limits(vpd):
         with pytest.raises(vpd.VpdError, match=msg):
             b.step(1)
     b.ctx.set_tuning(force_window=640, force_window_bonds=0, force_window_plan=3)
-    with pytest.raises(vpd.VpdError, match="force pass need a window of at most 512"):
-        b.step(1)
+    b.step(1)  # 7 planes of 640 pairs: 70 KB of the workgroup's LDS
     b.ctx.set_tuning(force_window=384, force_window_bonds=0, force_window_plan=3,
                      defgrad_window=640, defgrad_window_bonds=0, defgrad_window_plan=3)
     b.step(1)
@@ -495,16 +494,18 @@ def test_window_knob_limits(vpd):
     b.step(1)
 
 
-@pytest.mark.parametrize("take,plan", [(0, 0), (8, 0), (6, 1), (7, 2), (0, 3)])
-def test_full_size_windowed_equals_direct(vpd, take, plan):
-    banner("500k bar: windowed gathers (bonds per window %d, plan form %d) == direct, 12 steps "
-           "with damage" % (take, plan))
+@pytest.mark.parametrize("take,plan,gwin", [(0, 0, 0), (8, 0, 0), (6, 1, 0), (7, 2, 0),
+                                            (0, 3, (512, 480)), (0, 3, (480, 480)),
+                                            (0, 3, (640, 640))])
+def test_full_size_windowed_equals_direct(vpd, take, plan, gwin):
+    banner("500k bar: windowed gathers (bonds per window %d, plan form %d, group windows %s) "
+           "== direct, 12 steps with damage" % (take, plan, gwin))
     kw = dict(nx=200, ny=50, nz=50, spacing=0.2, horizon=0.603, props=JC_PROPS, dt=1.7075e-5,
               density=7.85e-3, velocity=60.0, damage=(0.0005, 0.0))
     a = vpd.Bar(**kw)
     b = vpd.Bar(**kw)
-    window = 512 if plan == 3 else 80 if take == 0 else 128
-    b.ctx.set_tuning(defgrad_window=window, force_window=480 if plan == 3 else window,
+    window = gwin[0] if plan == 3 else 80 if take == 0 else 128
+    b.ctx.set_tuning(defgrad_window=window, force_window=gwin[1] if plan == 3 else window,
                      defgrad_rounds=4, force_rounds=4, defgrad_window_bonds=take,
                      force_window_bonds=take, defgrad_window_plan=plan, force_window_plan=plan)
     a.step(12)

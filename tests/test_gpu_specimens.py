@@ -203,3 +203,29 @@ def test_graph_replay_equals_eager(vpd, return_map, planned):
     assert graph.step_index == eager.step_index == max(sp.steps for sp in sps)
     for name in ("u", "v", "a", "stressN", "statev", "damage", "fN", "rN"):
         assert np.array_equal(graph.get(name), eager.get(name)), name
+
+
+def test_velocity_band_closed_form(vpd):
+    """The velocity-driven band on the GPU against its closed form after k fused steps: the
+    band's axial u = v·(k·dt), v and a = 0 exactly, the support band at zero; and, for a batch
+    of specimens, each specimen's band at its own dt, velocity and last step."""
+    banner("velocity band closed form")
+    v, dt, k = 1.5, 8.5377e-5, 30
+    bar = vpd.Bar(16, 4, 4, 1.0, 3.015, JC_PROPS, dt, RHO, velocity=v)
+    bar.step(k)
+    codes = vpd.unpack_codes(bar.codes)[:, 0]
+    band, support = codes == 2, codes == 1
+    u, vel, acc = bar.get("u"), bar.get("v"), bar.get("a")
+    assert np.array_equal(u[band, 0], np.full(band.sum(), v * (k * dt)))
+    assert np.array_equal(vel[band, 0], np.full(band.sum(), v))
+    assert not acc[band, 0].any()
+    assert not u[support, 0].any() and not vel[support, 0].any()
+    sps = specimens(vpd)
+    batch = vpd.SpecimenBatch(sps, JC_PROPS, RHO)
+    batch.step()
+    for s, sp in enumerate(sps):
+        ax = sp.component
+        c = vpd.unpack_codes(batch.codes)[batch.first[s]:batch.first[s + 1], ax]
+        us, vs = batch.get("u", s)[:, ax], batch.get("v", s)[:, ax]
+        assert np.array_equal(us[c == 2], np.full((c == 2).sum(), sp.velocity * (sp.steps * sp.dt)))
+        assert np.array_equal(vs[c == 2], np.full((c == 2).sum(), sp.velocity))

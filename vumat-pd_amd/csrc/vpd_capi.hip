@@ -630,10 +630,10 @@ template <class F> void with_planned_window(int window, F &&f) {
 }
 
 // Group windows (GroupPlan): window sizes with group-window kernels, and a compile-time
-// window. The workgroup holds one window: `pairs` planes of W pairs, at most 64 KB (the
-// force pass's 7 planes allow W ≤ 512).
+// window. The workgroup holds one window: `pairs` planes of W pairs (the force pass's 7
+// planes of 640 pairs take 70 KB, two workgroups per CU), within the CU's 160 KB.
 constexpr int kGroupWindows[] = {256, 384, 480, 512, 640};
-constexpr int kMaxGroupLds = 65536;
+constexpr int kMaxGroupLds = kMaxLds;
 bool group_window_supported(int window) {
   for (int w : kGroupWindows)
     if (w == window)
@@ -806,8 +806,7 @@ int ensure_plan(vpd_context *c, int pass, const Gather &gx) {
                   "Group windows need a window of 256, 384, 480, 512 or 640 particles and no "
                   "bond limit per window (0).");
     if (group_lds(pass == 0 ? 4 : 7, gx.window) > kMaxGroupLds)
-      return fail(c, VPD_ERR_ARGUMENT,
-                  "Group windows of the force pass need a window of at most 512 particles.");
+      return fail(c, VPD_ERR_ARGUMENT, "Group window exceeds the LDS of a workgroup.");
     return build_group_plan(c, pass, gx.window);
   }
   return gx.mode > 0 ? build_plan(c, pass, gx.window, gx.take) : VPD_OK;
@@ -2339,11 +2338,24 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   double *su = nullptr, *sv = nullptr, *sa = nullptr;
   uint8_t *flush = nullptr, *savedAny = nullptr;
   uint32_t *savedFlags = nullptr;
-  // Whatever way the function leaves, its events and scratch buffers go with it.
+  const int64_t flagBytes = sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords);
+  const int64_t anyBytes = std::max<int64_t>(1, c->n);
+  bool trialsStarted = false;
+  // Whatever way the function leaves (an error included), the bond flags and marks the F-pass
+  // trials may have set are restored, the F scratch is marked stale once a trial has run, and
+  // the events and scratch buffers go.
   struct Release {
     std::function<void()> f;
     ~Release() { f(); }
   } release{[&] {
+    if (savedFlags && trialsStarted) {
+      (void)hipMemcpyAsync(c->broken, savedFlags, flagBytes, hipMemcpyDeviceToDevice,
+                           c->stream);
+      (void)hipMemcpyAsync(c->anyBroken, savedAny, anyBytes, hipMemcpyDeviceToDevice,
+                           c->stream);
+    }
+    if (trialsStarted)
+      c->haveFnew = false; // F scratch holds the tuning runs' values
     if (a)
       (void)hipEventDestroy(a);
     if (b)
@@ -2392,8 +2404,6 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   // The F pass is timed in the form the steps use (with the stretch test when damage is on);
   // the flags it may set are restored afterwards.
-  const int64_t flagBytes = sizeof(uint32_t) * std::max<int64_t>(1, c->brokenWords);
-  const int64_t anyBytes = std::max<int64_t>(1, c->n);
   if (dmg) {
     if (dalloc(c, savedFlags, std::max<int64_t>(1, c->brokenWords)) ||
         dalloc(c, savedAny, anyBytes))
@@ -2445,6 +2455,7 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
                                  hipMemcpyDeviceToDevice, c->stream));
       }
       HIPTRY(c, hipEventRecord(a, c->stream));
+      trialsStarted = true;
       if (pass == 0)
         launch_defgrad_with(c, dmg, lds, rounds, c->batch, gx);
       else
@@ -2599,6 +2610,7 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
                              c->stream));
   }
   HIPTRY(c, hipStreamSynchronize(c->stream));
+  trialsStarted = false; // restored here; nothing left for the release
   c->ldsDefgrad = pickLds[0];
   c->roundsDefgrad = pickRounds[0];
   c->ldsForce = pickLds[1];
@@ -2988,8 +3000,19 @@ int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstate
     return fail(nullptr, VPD_ERR_ARGUMENT, "JohnsonCook VUMAT requires ndir=3,nshr=3");
   if (*nprops < 14 || *nstatev < 6)
     return fail(nullptr, VPD_ERR_ARGUMENT, "JohnsonCook VUMAT requires 14 props and 6 states");
+  // VumatMaterialAdapter::updateBlock (src/material/VumatMaterialAdapter.cpp:150-151).
+  if (!std::isfinite(*dt) || *dt <= 0.0)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "VUMAT update requires dt > 0.");
+  if (*nfieldv < 0)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "nfieldv must be non-negative.");
   if (*nblock <= 0)
     return VPD_OK;
+  // Arrays the routine reads or writes (JohnsonCook.f90:74-155; the *New copies of stretch,
+  // deformation gradient and field variables read their *Old arrays).
+  if (!strainInc || !stressOld || !stateOld || !enerInternOld || !enerInelasOld || !stressNew ||
+      !stateNew || !enerInternNew || !enerInelasNew || (stretchNew && !stretchOld) ||
+      (defgradNew && !defgradOld) || (fieldNew && *nfieldv > 0 && !fieldOld))
+    return fail(nullptr, VPD_ERR_ARGUMENT, "Null VUMAT array.");
   std::vector<double> hp(*nprops);
   if (hipMemcpy(hp.data(), props, sizeof(double) * hp.size(), hipMemcpyDeviceToHost) !=
       hipSuccess)
