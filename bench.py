@@ -202,15 +202,21 @@ def return_map_study(points, seed=4321):
                            rtol=1e-14)
     scale = np.abs(exact["stress"]).max()
     net = vpd.load_warm_start()
-    out = {"points": m, "source": "plastic points of 30x8x8 JC bar plain runs, seed %d" % seed}
+    first = state[4] == 0.0  # first yield (or reloading): state 5 carries no seed
+    no5 = state.copy()
+    no5[4] = 0.0
+    out = {"points": m, "source": "plastic points of 30x8x8 JC bar plain runs, seed %d" % seed,
+           "first_yield_points": int(first.sum())}
     for name, mode in (("bisection_parity", vpd.RETURN_MAP_BISECTION),
                        ("bisection_tight", vpd.RETURN_MAP_BISECTION_TIGHT),
                        ("newton_state5_seed", vpd.RETURN_MAP_NEWTON),
-                       ("newton_learned_seed", vpd.RETURN_MAP_LEARNED)):
+                       ("newton_learned_seed", vpd.RETURN_MAP_LEARNED),
+                       ("newton_network_everywhere", -1)):
         best = None
         for _ in range(3):  # the fastest of three launches
-            if mode == vpd.RETURN_MAP_LEARNED:
-                r = vpd.return_map_learned(net, props, dt, inc, stress, state, z, z)
+            if mode in (vpd.RETURN_MAP_LEARNED, -1):
+                r = vpd.return_map_learned(net, props, dt, inc, stress,
+                                           state if mode >= 0 else no5, z, z)
                 total = r["ms"] + r["seed_ms"]
             else:
                 r = vpd.return_map(mode, props, dt, inc, stress, state, z, z)
@@ -219,10 +225,12 @@ def return_map_study(points, seed=4321):
                 best = (total, r)
         total, r = best
         row = {"ms": total, "mean_iterations": float(r["iterations"].mean()),
+               "mean_iterations_first_yield": float(r["iterations"][first].mean())
+               if first.any() else None,
                "max_stress_err_rel": float(np.abs(r["stress"] - exact["stress"]).max() / scale),
                "max_eqps_inc_err": float(np.abs(r["state"][4] - exact["state"][4]).max()),
                "newton_fallbacks_per_point": float(r["fallbacks"].mean())}
-        if mode == vpd.RETURN_MAP_LEARNED:
+        if mode in (vpd.RETURN_MAP_LEARNED, -1):
             row["seed_ms"] = r["seed_ms"]
             row["seed_median_rel_err"] = float(np.median(
                 np.abs(r["seed"] - exact["state"][4]) / np.maximum(exact["state"][4], 1e-30)))

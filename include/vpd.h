@@ -41,7 +41,8 @@ enum vpd_return_map {
   VPD_RETURN_MAP_BISECTION = 0,       /* exact bracketed bisection of JohnsonCook.f90:114-136 */
   VPD_RETURN_MAP_BISECTION_TIGHT = 1, /* the same scheme to |r| <= rtol·q_trial */
   VPD_RETURN_MAP_NEWTON = 2,          /* safeguarded Newton seeded with state 5 */
-  VPD_RETURN_MAP_LEARNED = 3,         /* safeguarded Newton seeded by the warm-start network */
+  VPD_RETURN_MAP_LEARNED = 3,         /* safeguarded Newton seeded by state 5 where it is
+                                         positive and by the warm-start network elsewhere */
   /* vpd_return_map_device only: mode 0 with the FP64 yield evaluated at every iteration (modes
    * 0 and 1 evaluate it only where an FP32 estimate with an error bound cannot decide the
    * iteration's branch, with the same iterates and results bit for bit); for checking. */
@@ -414,7 +415,7 @@ int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *p
                           int32_t *fallbacks, double *elapsed_ms);
 
 /* ---- the return-map warm start (no reference counterpart) -------------------------- */
-/* An 8 → 64 → 64 → 64 → 1 tanh network mapping (6 Voigt strain increments, ε̄ᵖ, T) to a
+/* An 8 → 64 → 64 → 64 → 1 ReLU network mapping (6 Voigt strain increments, ε̄ᵖ, T) to a
  * Δε̄ᵖ estimate that seeds the Newton return map (vpd_return_map_device mode 3). Trained on
  * the GPU in FP32 (forward, backward, Adam); inference runs as bf16 MFMA with FP32
  * accumulation. theta: 8961 FP32 values, W1 (64×8), b1, W2 (64×64), b2, W3 (64×64), b3,
@@ -432,8 +433,11 @@ int vpd_mlp_set_dataset(vpd_mlp *mlp, const float *x, const float *t, int64_t n,
 int vpd_mlp_train(vpd_mlp *mlp, int steps, int batch, float lr, float *last_loss);
 /* Raw network outputs (normalised units) for normalised inputs, through the MFMA kernel. */
 int vpd_mlp_predict(vpd_mlp *mlp, const float *x, int64_t n, float *y);
-/* Seeds Δε̄ᵖ for n points from device planes: inc[c*inc_stride + i] (c < 6) and the JC
- * state planes state[c*state_stride + i] (ε̄ᵖ = c 1, T = c 2). */
+/* Return-map seeds Δε̄ᵖ for n points from device planes: inc[c*inc_stride + i] (c < 6) and
+ * the JC state planes state[c*state_stride + i] (ε̄ᵖ = c 1, T = c 2, state 5 = c 4). A point
+ * with a positive state 5 (its previous plastic increment, the reference's own seed,
+ * JohnsonCook.f90:135) is seeded with it; the others (first yield) get the network's
+ * estimate, computed for those points only. *elapsed_ms: both passes. */
 int vpd_mlp_seed_device(vpd_mlp *mlp, const double *inc, int64_t inc_stride, const double *state,
                         int64_t state_stride, int64_t n, double *seed, double *elapsed_ms);
 

@@ -946,7 +946,7 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
           }))
         return s;
       if (int s = timed(c, K_MLP_SEED, [&] {
-            k_mlp_seed<true><<<mlp_grid(c->n), 256, 0, c->stream>>>(
+            k_mlp_seed<SEED_SLOTS><<<mlp_grid(c->n), 256, 0, c->stream>>>(
                 0, c->listCount, c->list, c->incList, c->n, c->statev, c->n, c->mlp->theta,
                 c->mlp->norm, c->seed);
           }))
@@ -960,7 +960,7 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
                       [&] { k_material<RM_LEARNED, true, DIAG><<<grid, 256, 0, c->stream>>>(g); }))
       return s;
     if (int s = timed(c, K_MLP_SEED, [&] {
-          k_mlp_seed<false><<<mlp_grid(c->n), 256, 0, c->stream>>>(
+          k_mlp_seed<SEED_ALL><<<mlp_grid(c->n), 256, 0, c->stream>>>(
               c->n, nullptr, nullptr, c->inc, c->n, c->statev, c->n, c->mlp->theta,
               c->mlp->norm, c->seed);
         }))
@@ -3312,9 +3312,8 @@ int vpd_mlp_seed_device(vpd_mlp *m, const double *inc, int64_t incStride, const 
   MLPTRY(m, hipEventCreate(&a));
   MLPTRY(m, hipEventCreate(&b));
   MLPTRY(m, hipEventRecord(a, m->stream));
-  k_mlp_seed<false><<<mlp_grid(n), 256, 0, m->stream>>>(n, nullptr, nullptr, inc, incStride,
-                                                         state, stateStride, m->theta, m->norm,
-                                                         seed);
+  k_mlp_seed<SEED_ALL><<<mlp_grid(n), 256, 0, m->stream>>>(
+      n, nullptr, nullptr, inc, incStride, state, stateStride, m->theta, m->norm, seed);
   MLPTRY(m, hipGetLastError());
   MLPTRY(m, hipEventRecord(b, m->stream));
   MLPTRY(m, hipEventSynchronize(b));

@@ -1427,76 +1427,95 @@ __global__ void k_zero_count(int *count) { *count = 0; }
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, CompactArgs cl) {
+  __shared__ int waveBase[4];
+  __shared__ int blockBase;
   const int64_t blk = xcd_block(g.nblocks);
   if (blk < 0)
     return;
   const int64_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= g.n)
-    return;
-  const Ramp none{0.0, 0.0, 0.0};
-  const PointStep ps = point_step(g.batch, i, g.dt, none, none);
-  if (!ps.active)
-    return;
+  // Every thread reaches the workgroup's list append below (one atomic per workgroup: one
+  // per wavefront, serialised on the counter, costs more than the screen's own work).
+  bool live = i < g.n;
+  PointStep ps{};
+  if (live) {
+    const Ramp none{0.0, 0.0, 0.0};
+    ps = point_step(g.batch, i, g.dt, none, none);
+    live = ps.active;
+  }
   const double dt = ps.dt;
-  T3 f1, f0, v0, r0;
+  T3 f1, r1;
+  double inc[6], sOld[6], st[6];
+  bool plastic = false;
+  if (live) {
+    T3 f0, v0, r0;
 #pragma unroll
-  for (int c = 0; c < 9; ++c) {
-    f1.m[c] = g.fnew(c, i);
-    f0.m[c] = g.fN(c, i);
-    v0.m[c] = g.vN(c, i);
-    r0.m[c] = g.rN(c, i);
-  }
-  Corot k;
-  if (!corotational(f0, f1, v0, r0, dt, k)) {
-    raise_error(g.err, ERR_SINGULAR_COROT);
-    return;
-  }
-  double inc[6];
-  to_voigt(k.Dbar, inc);
+    for (int c = 0; c < 9; ++c) {
+      f1.m[c] = g.fnew(c, i);
+      f0.m[c] = g.fN(c, i);
+      v0.m[c] = g.vN(c, i);
+      r0.m[c] = g.rN(c, i);
+    }
+    Corot k;
+    if (!corotational(f0, f1, v0, r0, dt, k)) {
+      raise_error(g.err, ERR_SINGULAR_COROT);
+      live = false;
+    } else {
+      to_voigt(k.Dbar, inc);
 #pragma unroll
-  for (int c = 0; c < 6; ++c)
-    inc[c] *= dt;
-  if (g.incOut.p) {
+      for (int c = 0; c < 6; ++c)
+        inc[c] *= dt;
+      if (g.incOut.p) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
-      g.incOut(c, i) = inc[c];
-  }
+        for (int c = 0; c < 6; ++c)
+          g.incOut(c, i) = inc[c];
+      }
 #pragma unroll
-  for (int c = 0; c < 9; ++c) {
-    g.fN(c, i) = f1.m[c];
-    g.vN(c, i) = k.V1.m[c];
-    g.rN(c, i) = k.R1.m[c];
-  }
-  double sOld[6], st[6];
+      for (int c = 0; c < 9; ++c) {
+        g.fN(c, i) = f1.m[c];
+        g.vN(c, i) = k.V1.m[c];
+        g.rN(c, i) = k.R1.m[c];
+      }
+      r1 = k.R1;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    sOld[c] = g.sig(c, i);
-    st[c] = g.statev(c, i);
+      for (int c = 0; c < 6; ++c) {
+        sOld[c] = g.sig(c, i);
+        st[c] = g.statev(c, i);
+      }
+      plastic = !jc_elastic(g.jc, inc, sOld, st, g.rtol);
+    }
   }
-  const bool plastic = !jc_elastic(g.jc, inc, sOld, st, g.rtol);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t mask = __ballot(plastic);
+  if (lane == 0)
+    waveBase[wv] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = 0;
+    for (int w = 0; w < 4; ++w) {
+      const int c = waveBase[w];
+      waveBase[w] = total;
+      total += c;
+    }
+    blockBase = total > 0 ? atomicAdd(cl.count, total) : 0;
+  }
+  __syncthreads();
   if (plastic) {
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    int base = 0;
-    if (lane == leader)
-      base = atomicAdd(cl.count, __popcll(mask));
-    base = __shfl(base, leader);
-    const int slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+    const int slot = blockBase + waveBase[wv] + __popcll(mask & ((1ull << lane) - 1ull));
     cl.list[slot] = (int)i;
 #pragma unroll
     for (int c = 0; c < 6; ++c)
       cl.inc(c, slot) = inc[c];
     return;
   }
+  if (!live)
+    return;
   double sNew[6];
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
   jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
-  material_finish<false>(g, i, f1, k.R1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
+  material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
 }
 
-// The Newton solve and the rest of the update for the listed points, seeded per slot.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_material_return_list(MaterialArgs g, CompactArgs cl) {
   const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -278,13 +278,20 @@ __device__ __forceinline__ bf16x4 normalised_bf16(const double x[4], const doubl
   return bf16_of(v);
 }
 
-// Network seeds for n points (LIST: for the `*count` points of a compacted list; slot s holds
-// particle list[s] and its strain increments at inc[c·incStride + s]). Inputs per point: the 6
-// strain increments, ε̄ᵖ and T (components 1 and 2 of the JC state planes, stride
-// stateStride). Writes seed[s] = max(0, y·std + mean). Each wavefront loads the weights once
-// and walks 16-point tiles with a grid stride; wavefronts with no tile exit at once, and the
-// tile loop is uniform per wavefront, so every MFMA runs with all lanes active.
-template <bool LIST>
+// Return-map seeds for n points (SEED_SLOTS: for the `*count` points of a compacted list;
+// slot s holds particle list[s] and its strain increments at inc[c·incStride + s]). A point
+// whose previous plastic increment (state 5, component 4 of the JC state planes, stride
+// stateStride) is positive keeps it as its seed: on a continuing plastic path it is closer to
+// the solution than the network's estimate (about 2% median error). A point with state 5 = 0
+// (first yield, or reloading after an elastic increment), where that seed says nothing, takes
+// the network's estimate from the 6 strain increments, ε̄ᵖ and T (components 1 and 2):
+// seed[s] = max(0, y·std + mean).
+// Each wavefront walks 16-point tiles with a grid stride, reading state 5 first; a tile whose
+// points all carry a state-5 seed skips the network and its inputs, and a wavefront loads the
+// weights at its first tile that needs them. Wavefronts with no tile exit at once, and the
+// tile loop (and the skip) is uniform per wavefront, so every MFMA runs with all lanes active.
+enum SeedForm { SEED_ALL = 0, SEED_SLOTS = 1 };
+template <int FORM>
 __global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__restrict__ count,
                                                   const int *__restrict__ list,
                                                   const double *__restrict__ inc,
@@ -294,62 +301,63 @@ __global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__
                                                   const float *__restrict__ theta,
                                                   const float *__restrict__ norm,
                                                   double *__restrict__ seed) {
+  constexpr bool LISTED = FORM == SEED_SLOTS;
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
-  const int64_t n = LIST ? (int64_t)*count : nPoints;
+  const int64_t n = LISTED ? (int64_t)*count : nPoints;
   const int64_t tiles = (n + 15) / 16;
   const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int64_t waves = (int64_t)gridDim.x * 4;
   if (wave >= tiles)
     return;
   MlpWeights w;
-  mlp_load_weights(theta, lane, w);
-  const double mean = norm[2 * MLP_IN], sdev = norm[2 * MLP_IN + 1];
-  // This lane's 4 raw inputs of one tile (lanes 0-15: increments 0-3; lanes 16-31:
-  // increments 4-5, ε̄ᵖ, T); the next tile's are loaded while the current one is computed.
-  auto fetch = [&](int64_t tile, double v[4]) {
-    const int64_t s = tile * 16 + r;
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      v[c] = norm[4 * q + c]; // normalises to 0
-    if (q < 2 && s < n) {
-      const int64_t i = LIST ? (int64_t)list[s] : s;
-      if (q == 0) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-          v[c] = inc[c * incStride + s];
-      } else {
-        v[0] = inc[4 * incStride + s];
-        v[1] = inc[5 * incStride + s];
-        v[2] = state[1 * stateStride + i];
-        v[3] = state[2 * stateStride + i];
-      }
-    }
-  };
-  double mu[4], inv[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int f = q < 2 ? 4 * q + c : c;
-    mu[c] = norm[f];
-    inv[c] = 1.0 / (double)norm[MLP_IN + f];
-  }
-  double cur[4];
-  fetch(wave, cur);
+  bool loaded = false;
+  double mean = 0.0, sdev = 0.0, mu[4], inv[4];
   for (int64_t tile = wave; tile < tiles; tile += waves) {
-    double nxt[4] = {0.0, 0.0, 0.0, 0.0};
-    if (tile + waves < tiles)
-      fetch(tile + waves, nxt);
-    bf16x4 x = {0, 0, 0, 0};
-    if (q < 2)
-      x = normalised_bf16(cur, mu, inv);
-    const float y = mlp_tile(w, x);
     const int64_t s = tile * 16 + r;
-    if (q == 0 && s < n) {
-      const double v = (double)y * sdev + mean;
-      seed[s] = v > 0.0 ? v : 0.0;
-    }
+    const int64_t i = s < n ? (LISTED ? (int64_t)list[s] : s) : 0;
+    const double prev = q == 0 && s < n ? state[4 * stateStride + i] : 1.0; // state 5
+    const bool needNet = q == 0 && s < n && !(prev > 0.0);
+    float y = 0.f;
+    if (__ballot(needNet) != 0) { // uniform: the tile has a point without a state-5 seed
+      if (!loaded) {
+        mlp_load_weights(theta, lane, w);
+        mean = norm[2 * MLP_IN];
+        sdev = norm[2 * MLP_IN + 1];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      cur[c] = nxt[c];
+        for (int c = 0; c < 4; ++c) {
+          const int f = q < 2 ? 4 * q + c : c;
+          mu[c] = norm[f];
+          inv[c] = 1.0 / (double)norm[MLP_IN + f];
+        }
+        loaded = true;
+      }
+      // This lane's 4 raw inputs (lanes 0-15: increments 0-3; lanes 16-31: increments 4-5,
+      // ε̄ᵖ, T); missing points normalise to 0.
+      double v[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        v[c] = mu[c];
+      if (q < 2 && s < n) {
+        if (q == 0) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+            v[c] = inc[c * incStride + s];
+        } else {
+          v[0] = inc[4 * incStride + s];
+          v[1] = inc[5 * incStride + s];
+          v[2] = state[1 * stateStride + i];
+          v[3] = state[2 * stateStride + i];
+        }
+      }
+      bf16x4 x = {0, 0, 0, 0};
+      if (q < 2)
+        x = normalised_bf16(v, mu, inv);
+      y = mlp_tile(w, x);
+    }
+    if (q == 0 && s < n) {
+      const double e = (double)y * sdev + mean;
+      seed[s] = needNet ? (e > 0.0 ? e : 0.0) : prev;
+    }
   }
 }
 
