@@ -230,7 +230,8 @@ int vpd_synchronize(vpd_context *ctx);
  * applies every specimen's constraints at t = 0. count 0 returns to one specimen. While a
  * table is set, the scalar-time entries (vpd_step, vpd_predict, vpd_apply_constraints,
  * vpd_compute_deformation_gradient, vpd_update_material, vpd_compute_force_and_correct)
- * and diagnostics are refused; vpd_initialize_material covers every particle.
+ * are refused; vpd_initialize_material covers every particle. Set the table before enabling
+ * run diagnostics: they then keep one record per specimen (see vpd_enable_diagnostics).
  * Each specimen's results equal those of the same specimen run alone through vpd_step with
  * ramp (v·t, v, 0), bit for bit. */
 int vpd_set_specimens(vpd_context *ctx, int count, const int64_t *first, const double *dt,
@@ -238,6 +239,17 @@ int vpd_set_specimens(vpd_context *ctx, int count, const int64_t *first, const d
 /* Step `step` (1-based, consecutive while fuse_next chains) of every specimen that has not
  * finished. With fuse_next the next step's predictor runs inside the force pass; a
  * specimen's final step is never followed by a prediction. */
+/* Prescribed band kinematics per specimen, e.g. the half-cosine ramp of
+ * RegionalConstraintSystem::ramp (src/host/BoundaryConditions.cpp:70-79) that the reference's
+ * boundary cases use. Specimen s takes rows offsets[s] .. offsets[s+1]−1 of `table` (3 doubles
+ * per row: the loaded band's u, v, a), one row per step 0..last_step[s], in place of its
+ * velocity band; a specimen with no rows keeps the velocity band. Call after
+ * vpd_set_specimens and before the first step; it applies every specimen's constraints at
+ * t = 0 again. offsets NULL removes the tables. The rows are the caller's own values (e.g.
+ * vpd_smooth_ramp at step·dt[s]), so each specimen's results equal its run alone through
+ * vpd_step with the same ramp values, bit for bit. No counterpart in the reference (its cases
+ * run one per process, benchmarks/tensile3d/run_cases.py:95-96). */
+int vpd_set_specimen_bands(vpd_context *ctx, const int64_t *offsets, const double *table);
 int vpd_step_specimens(vpd_context *ctx, int32_t step, int fuse_next, int want_fields);
 /* Steps first_step .. first_step+count−1 of every unfinished specimen as `count` replays of
  * one captured HIP graph of a fused step (F + damage, material, force + correct + next
@@ -307,7 +319,12 @@ int vpd_autotune(vpd_context *ctx, int reps, int *tuned);
  * device buffer of `capacity` records; vpd_fetch_diagnostics copies the pending records out
  * in step order and empties the buffer (vpd_step fails while it is full). Sums run in a
  * fixed order: identical inputs and particle count give identical records, whatever the
- * launch tuning; they differ from the CPU's serial sums by rounding only. */
+ * launch tuning; they differ from the CPU's serial sums by rounding only. With batched
+ * specimens (vpd_set_specimens before this call) vpd_step_specimens appends, per step, one
+ * record per specimen (specimen-minor: `capacity` steps hold capacity × count records); each
+ * specimen's chunks are laid out from its first particle, so its records equal those of its
+ * run alone bit for bit. The configuration (axes, weights, state indices) is shared: weights
+ * and face sets are per particle of the whole block. */
 enum vpd_diag_slot {
   VPD_DIAG_INTERNAL_ENERGY = 0,  /* Σ contraction(½(σ̄_n + σ_{n+1}), E_{n+1} − E_n)·V (:231-237) */
   VPD_DIAG_OBSERVED_STRESS = 1,  /* Σ σ[axis][axis]·w over the observation weights (:250-253) */
@@ -468,6 +485,15 @@ int64_t vpd_select_boundary_region(const double *x, int64_t n, const char *side,
  * Returns 0 when every material passes its acceptance checks, 2 when one fails, 1 on an
  * error (the message goes to stderr as the reference prints it). */
 int vpd_run_case_json(const char *case_path, const char *plugin_directory);
+/* Several "vumat_pd_boundary_case" files run as one batch: every material of every case is a
+ * specimen of one particle block (vpd_set_specimens, vpd_set_specimen_bands with each case's
+ * half-cosine ramp, per-specimen run diagnostics), stepped together; each case's output files
+ * are those vpd_run_case_json writes for it (runtimeSeconds is the batch's wall time), its
+ * rows bit for bit. Replaces the process pool of benchmarks/tensile3d/run_cases.py:95-96
+ * (and the serial runs of time_step_study.py:132-148) for cases that share the material,
+ * solver density, damage settings, load axis and state indices; other combinations are
+ * refused. Returns 0 when every material passes, 2 when one fails, 1 on an error. */
+int vpd_run_cases_json(const char *const *case_paths, int count, const char *plugin_directory);
 
 #ifdef __cplusplus
 }

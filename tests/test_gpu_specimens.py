@@ -151,8 +151,14 @@ def test_batch_errors(vpd):
     bar.ctx.set_specimens([0, n], [sp.dt], [sp.velocity], [5])
     with pytest.raises(vpd.VpdError, match="vpd_step_specimens"):
         bar.step(1)
-    with pytest.raises(vpd.VpdError, match="one specimen"):
-        bar.ctx.enable_diagnostics(0, 1)
+    # Diagnostics are laid out for the specimen table in place when they are enabled: the
+    # table cannot change under them.
+    bar.ctx.enable_diagnostics(0, 1)
+    with pytest.raises(vpd.VpdError, match="before enabling run diagnostics"):
+        bar.ctx.set_specimens([0, n], [sp.dt], [sp.velocity], [5])
+    with pytest.raises(vpd.VpdError, match="Disable run diagnostics"):
+        bar.ctx.set_specimens([], [], [], [])
+    bar.ctx.enable_diagnostics(None)
     bar.ctx.step_specimens(1, fuse_next=True)
     with pytest.raises(vpd.VpdError, match="prediction was made for step 2"):
         bar.ctx.step_specimens(3)

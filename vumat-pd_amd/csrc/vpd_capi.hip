@@ -158,6 +158,11 @@ struct vpd_context {
   double *diagDstrain = nullptr;
   uint8_t *diagLat = nullptr;
   int diagCapacity = 0, diagCount = 0;
+  // Batched specimens: the chunks the diagnostics reduce (each within one specimen, laid out
+  // from its first particle) and each specimen's first chunk; records per step = specimens.
+  int64_t *diagChunkFirst = nullptr, *diagChunkEnd = nullptr, *diagSpecChunk = nullptr;
+  int64_t diagChunks = 0;
+  int diagPerStep = 1;
   // Batched specimens (vpd_set_specimens): specimen of each particle, each specimen's time
   // step, end velocity and last step, and the step being taken. batch.id stays null for a
   // block that is one specimen.
@@ -166,8 +171,11 @@ struct vpd_context {
   double *specDt = nullptr, *specVel = nullptr;
   int *specLast = nullptr;
   int *stepDev = nullptr; // the step counter read by graph-replayed steps
+  double *specBand = nullptr; // vpd_set_specimen_bands: prescribed band rows
+  int64_t *specBandOff = nullptr;
   int specCount = 0;
   std::vector<int64_t> specFirst;
+  std::vector<int32_t> specLastHost;
 };
 
 namespace {
@@ -915,7 +923,7 @@ int ensure_list(vpd_context *c) {
 }
 
 template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
-  const int grid = grid8_of(c->n);
+  const int grid = (int)(ceil_div(g.nblocks, 8) * 8); // grid8_of(n), or the diagnostics' chunks
   if (c->materialModel == VPD_MATERIAL_ELASTIC) {
     if (DIAG)
       return fail(c, VPD_ERR_UNSUPPORTED, "Run diagnostics need the Johnson-Cook material.");
@@ -1005,6 +1013,8 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
   g.batch = c->batch;
   if (diag && !c->diagOn)
     return fail(c, VPD_ERR_STATE, "Run diagnostics are not enabled.");
+  if (diag && c->diag.chunkFirst) // batched specimens: one workgroup per specimen chunk
+    g.nblocks = c->diagChunks;
   return diag ? launch_material_t<true>(c, g) : launch_material_t<false>(c, g);
 }
 
@@ -1092,6 +1102,10 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.rampNext = fuse ? ramp_of(rampNext) : Ramp{0, 0, 0};
   const bool dmg = c->damageMode == VPD_DAMAGE_CRITICAL_STRETCH;
   g.rd = rounds_of(c->n, c->roundsForce);
+  if (diag && c->diag.chunkFirst) { // batched specimens: one workgroup per specimen chunk
+    g.rd.rounds = 1;
+    g.rd.blocksPerRound = c->diagChunks;
+  }
   g.diag = c->diag;
   g.batch = c->batch;
   if (int s = prepare_plans(c))
@@ -1268,6 +1282,11 @@ void vpd_destroy(vpd_context *c) {
   }
   dfree(c->diagPartials);
   dfree(c->diagRecords);
+  dfree(c->diagChunkFirst);
+  dfree(c->diagChunkEnd);
+  dfree(c->diagSpecChunk);
+  dfree(c->specBand);
+  dfree(c->specBandOff);
   dfree(c->diagW);
   dfree(c->diagDstrain);
   dfree(c->diagLat);
@@ -1823,14 +1842,17 @@ int vpd_set_specimens(vpd_context *c, int count, const int64_t *first, const dou
   if (int s = require_ready(c))
     return s;
   if (count == 0) { // back to one specimen
+    if (c->diagOn)
+      return fail(c, VPD_ERR_STATE,
+                  "Disable run diagnostics before leaving the specimen table.");
     c->batch = Batch{};
     c->specCount = 0;
     c->predicted = false;
     return VPD_OK;
   }
   if (c->diagOn)
-    return fail(c, VPD_ERR_UNSUPPORTED,
-                "Run diagnostics cover a block that is one specimen, not batched specimens.");
+    return fail(c, VPD_ERR_STATE,
+                "Set the specimen table before enabling run diagnostics (or disable them).");
   if (count < 0 || count > 65536 || !first || !dt || !velocity || !lastStep)
     return fail(c, VPD_ERR_ARGUMENT, "Invalid specimen table.");
   if (first[0] != 0 || first[count] != c->n)
@@ -1868,9 +1890,57 @@ int vpd_set_specimens(vpd_context *c, int count, const int64_t *first, const dou
   HIPTRY(c, hipMemcpy(c->specLast, lastStep, sizeof(int32_t) * count, hipMemcpyHostToDevice));
   c->specCount = count;
   c->specFirst.assign(first, first + count + 1);
-  c->batch = Batch{c->specId, c->specDt, c->specVel, c->specLast, 0};
+  c->specLastHost.assign(lastStep, lastStep + count);
+  c->batch = Batch{c->specId, c->specDt, c->specVel, c->specLast, 0, nullptr, nullptr, nullptr};
   c->predicted = false;
   // Each specimen's loaded band at t = 0 (the single-specimen host's initial constraints).
+  if (int s = timed(c, K_CONSTRAIN, [&] {
+        k_constrain<<<grid_of(c->n), 256, 0, c->stream>>>(
+            c->n, upairs(c), planes(c->v, c->n), planes(c->a, c->n), c->codes,
+            Ramp{0, 0, 0}, c->batch);
+      }))
+    return s;
+  return check_device(c);
+}
+
+int vpd_set_specimen_bands(vpd_context *c, const int64_t *offsets, const double *table) {
+  if (int s = require_ready(c))
+    return s;
+  if (!c->batch.id)
+    return fail(c, VPD_ERR_STATE, "No specimen table; call vpd_set_specimens first.");
+  if (c->batch.step != 0 || c->predicted)
+    return fail(c, VPD_ERR_STATE, "Set the specimen bands before the first step.");
+  if (offsets) {
+    const int count = c->specCount;
+    if (offsets[0] != 0)
+      return fail(c, VPD_ERR_ARGUMENT, "Band table offsets must start at 0.");
+    for (int sp = 0; sp < count; ++sp) {
+      const int64_t rows = offsets[sp + 1] - offsets[sp];
+      if (rows != 0 && rows != (int64_t)c->specLastHost[sp] + 1)
+        return fail(c, VPD_ERR_ARGUMENT,
+                    "A specimen's band table holds one row per step 0.." +
+                        std::to_string(c->specLastHost[sp]) + " (or none).");
+    }
+    const int64_t rows = offsets[count];
+    if (rows > 0 && !table)
+      return fail(c, VPD_ERR_ARGUMENT, "Null band table.");
+    for (int64_t k = 0; k < 3 * rows; ++k)
+      if (!std::isfinite(table[k]))
+        return fail(c, VPD_ERR_ARGUMENT, "Band table values must be finite.");
+    if (dalloc(c, c->specBand, std::max<int64_t>(1, 3 * rows)) ||
+        dalloc(c, c->specBandOff, count + 1))
+      return VPD_ERR_DEVICE;
+    if (rows > 0)
+      HIPTRY(c, hipMemcpy(c->specBand, table, sizeof(double) * 3 * rows, hipMemcpyHostToDevice));
+    HIPTRY(c, hipMemcpy(c->specBandOff, offsets, sizeof(int64_t) * (count + 1),
+                        hipMemcpyHostToDevice));
+    c->batch.band = c->specBand;
+    c->batch.bandOff = c->specBandOff;
+  } else {
+    c->batch.band = nullptr;
+    c->batch.bandOff = nullptr;
+  }
+  // Every specimen's band at t = 0 again (the single-specimen host's initial constraints).
   if (int s = timed(c, K_CONSTRAIN, [&] {
         k_constrain<<<grid_of(c->n), 256, 0, c->stream>>>(
             c->n, upairs(c), planes(c->v, c->n), planes(c->a, c->n), c->codes,
@@ -1902,12 +1972,22 @@ int vpd_step_specimens(vpd_context *c, int32_t step, int fuseNext, int wantField
   if (!c->predicted)
     if (int s = launch_predict(c, 1.0, none))
       return s;
+  if (c->diagOn && c->diagCount >= c->diagCapacity)
+    return fail(c, VPD_ERR_STATE,
+                "The diagnostics record buffer is full; fetch the records first.");
   if (int s = launch_defgrad(c))
     return s;
-  if (int s = launch_material(c, 1.0, wantFields != 0))
+  if (int s = launch_material(c, 1.0, wantFields != 0, c->diagOn))
     return s;
-  if (int s = launch_force(c, 1.0, none, fuseNext != 0, none, wantFields != 0))
+  if (int s = launch_force(c, 1.0, none, fuseNext != 0, none, wantFields != 0, c->diagOn))
     return s;
+  if (c->diagOn) { // one record per specimen
+    k_diag_reduce_specimens<<<c->specCount, 256, 0, c->stream>>>(
+        c->diagPartials, c->diagSpecChunk,
+        c->diagRecords + (int64_t)c->diagCount * c->diagPerStep * DIAG_STRIDE);
+    HIPTRY(c, hipGetLastError());
+    ++c->diagCount;
+  }
   c->predicted = fuseNext != 0;
   c->haveFnew = true;
   c->haveFields = wantFields != 0;
@@ -1922,6 +2002,9 @@ int vpd_run_specimens(vpd_context *c, int32_t firstStep, int32_t count) {
     return fail(c, VPD_ERR_STATE, "No specimen table; call vpd_set_specimens first.");
   if (firstStep < 1 || count < 1)
     return fail(c, VPD_ERR_ARGUMENT, "Specimen steps count from 1; count must be positive.");
+  if (c->diagOn)
+    return fail(c, VPD_ERR_UNSUPPORTED,
+                "Graph-replayed steps take no run diagnostics; use vpd_step_specimens.");
   if (c->predicted && firstStep != c->batch.step + 1)
     return fail(c, VPD_ERR_STATE, "The fused prediction was made for step " +
                                       std::to_string(c->batch.step + 1) + ".");
@@ -2165,9 +2248,6 @@ int vpd_enable_diagnostics(vpd_context *c, const vpd_diag_config *cfg) {
     c->diagCount = 0;
     return VPD_OK;
   }
-  if (c->batch.id)
-    return fail(c, VPD_ERR_UNSUPPORTED,
-                "Run diagnostics cover a block that is one specimen, not batched specimens.");
   if (c->materialModel != VPD_MATERIAL_JOHNSON_COOK)
     return fail(c, VPD_ERR_UNSUPPORTED, "Run diagnostics need the Johnson-Cook material.");
   const int ax = cfg->load_axis, lat = cfg->lateral_axis;
@@ -2187,13 +2267,42 @@ int vpd_enable_diagnostics(vpd_context *c, const vpd_diag_config *cfg) {
     if (cfg->indicator_indices[k] < 0 || cfg->indicator_indices[k] >= held)
       return fail(c, VPD_ERR_ARGUMENT,
                   "internalIndicator indices must name the first 6 state variables.");
-  const int64_t chunks = ceil_div(c->n, 256);
+  // With batched specimens every chunk lies in one specimen and starts where the specimen
+  // alone would start it, so each specimen's record equals its single run's bit for bit.
+  const int perStep = c->batch.id ? c->specCount : 1;
+  std::vector<int64_t> chunkFirst, chunkEnd, specChunk;
+  if (c->batch.id) {
+    for (int sp = 0; sp < c->specCount; ++sp) {
+      specChunk.push_back((int64_t)chunkFirst.size());
+      for (int64_t i = c->specFirst[sp]; i < c->specFirst[sp + 1]; i += 256) {
+        chunkFirst.push_back(i);
+        chunkEnd.push_back(c->specFirst[sp + 1]);
+      }
+    }
+    specChunk.push_back((int64_t)chunkFirst.size());
+  }
+  const int64_t chunks = c->batch.id ? (int64_t)chunkFirst.size() : ceil_div(c->n, 256);
   if (dalloc(c, c->diagPartials, chunks * DIAG_STRIDE) ||
-      dalloc(c, c->diagRecords, (int64_t)cfg->capacity * DIAG_STRIDE) ||
+      dalloc(c, c->diagRecords, (int64_t)cfg->capacity * perStep * DIAG_STRIDE) ||
       dalloc(c, c->diagDstrain, 6 * c->n))
     return VPD_ERR_DEVICE;
   DiagArgs d{};
   d.partials = c->diagPartials;
+  if (c->batch.id) {
+    if (dalloc(c, c->diagChunkFirst, chunks) || dalloc(c, c->diagChunkEnd, chunks) ||
+        dalloc(c, c->diagSpecChunk, c->specCount + 1))
+      return VPD_ERR_DEVICE;
+    HIPTRY(c, hipMemcpyAsync(c->diagChunkFirst, chunkFirst.data(), sizeof(int64_t) * chunks,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->diagChunkEnd, chunkEnd.data(), sizeof(int64_t) * chunks,
+                             hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemcpyAsync(c->diagSpecChunk, specChunk.data(),
+                             sizeof(int64_t) * specChunk.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    d.chunkFirst = c->diagChunkFirst;
+    d.chunkEnd = c->diagChunkEnd;
+  }
+  d.chunks = chunks;
   d.dstrain = planes(c->diagDstrain, c->n);
   d.vol = c->vol;
   if (cfg->observation_weight) {
@@ -2221,6 +2330,8 @@ int vpd_enable_diagnostics(vpd_context *c, const vpd_diag_config *cfg) {
     d.ind[k] = k < cfg->indicator_count ? cfg->indicator_indices[k] : 0;
   HIPTRY(c, hipStreamSynchronize(c->stream));
   c->diag = d;
+  c->diagChunks = chunks;
+  c->diagPerStep = perStep;
   c->diagCapacity = cfg->capacity;
   c->diagCount = 0;
   c->diagOn = true;
@@ -2238,7 +2349,8 @@ int vpd_fetch_diagnostics(vpd_context *c, double *records, int capacity, int *co
     return s;
   if (c->diagCount > 0)
     HIPTRY(c, hipMemcpyAsync(records, c->diagRecords,
-                             sizeof(double) * DIAG_STRIDE * (size_t)c->diagCount,
+                             sizeof(double) * DIAG_STRIDE * (size_t)c->diagPerStep *
+                                 (size_t)c->diagCount,
                              hipMemcpyDeviceToHost, c->stream));
   HIPTRY(c, hipStreamSynchronize(c->stream));
   *count = c->diagCount;

@@ -44,6 +44,11 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a < b) ? b 
 
 struct DiagArgs {
   double *partials;     // [chunks][DIAG_STRIDE]; null: diagnostics off
+  // Batched specimens: chunk c covers particles [chunkFirst[c], chunkEnd[c]), all of one
+  // specimen and laid out from its first particle as that specimen alone lays out its chunks
+  // (null: chunk c covers particles 256c .. 256c+255 of the block).
+  const int64_t *chunkFirst, *chunkEnd;
+  int64_t chunks;
   Planes dstrain;       // E_{n+1} − E_n (6 planes) between the two halves of a split update
   const double *vol;    // particle volumes
   const double *obsW;   // observation weights (0 outside the window)
@@ -52,6 +57,19 @@ struct DiagArgs {
   int indCount, indMin, indOneMinus;
   int ind[8];
 };
+
+// The particle this thread reduces for chunk `chunk` (valid: it exists).
+__device__ __forceinline__ int64_t diag_particle(const DiagArgs &d, int64_t chunk, int64_t n,
+                                                 bool &valid) {
+  if (d.chunkFirst) {
+    const int64_t i = d.chunkFirst[chunk] + threadIdx.x;
+    valid = i < d.chunkEnd[chunk];
+    return i;
+  }
+  const int64_t i = chunk * 256 + threadIdx.x;
+  valid = i < n;
+  return i;
+}
 
 // Green-Lagrange strain ½(FᵀF − I) in the operation order of CorotationalKinematics.cpp:141-149.
 __device__ __forceinline__ T3 gl_strain(const T3 &f) {
@@ -143,8 +161,25 @@ __device__ __forceinline__ void diag_block_store(const double (&v)[NS], double *
 
 // Sums the chunk rows in chunk order (each thread a strided run, then a fixed tree) into one
 // step record. One block of 256 threads.
+__device__ __forceinline__ void diag_reduce_rows(const double *__restrict__ partials,
+                                                 int64_t chunks, double *record);
 __global__ void __launch_bounds__(256) k_diag_reduce(const double *__restrict__ partials,
                                                      int64_t chunks, double *record) {
+  diag_reduce_rows(partials, chunks, record);
+}
+
+// One record per specimen: block s sums its chunks [specChunk[s], specChunk[s+1]) as
+// k_diag_reduce sums a single specimen's, into records[s].
+__global__ void __launch_bounds__(256) k_diag_reduce_specimens(
+    const double *__restrict__ partials, const int64_t *__restrict__ specChunk,
+    double *records) {
+  const int64_t s = blockIdx.x, c0 = specChunk[s];
+  diag_reduce_rows(partials + c0 * DIAG_STRIDE, specChunk[s + 1] - c0,
+                   records + s * DIAG_STRIDE);
+}
+
+__device__ __forceinline__ void diag_reduce_rows(const double *__restrict__ partials,
+                                                 int64_t chunks, double *record) {
   __shared__ double red[D_COUNT][256];
   const int t = threadIdx.x;
   double acc[D_COUNT];

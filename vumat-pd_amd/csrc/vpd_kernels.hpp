@@ -275,6 +275,11 @@ struct Batch {
   // When set, the step is read from this device counter instead, so that one captured HIP
   // graph of a step serves every step (k_advance_step moves it on).
   const int *stepDev;
+  // Prescribed band kinematics (vpd_set_specimen_bands): specimen s with
+  // bandOff[s] < bandOff[s+1] takes (u, v, a) of step k from band[3·(bandOff[s] + k)], the
+  // values its host computed (e.g. the half-cosine ramp), instead of the velocity band.
+  const double *band;
+  const int64_t *bandOff;
   __device__ __forceinline__ int current() const { return stepDev ? *stepDev : step; }
 };
 
@@ -297,10 +302,18 @@ __device__ __forceinline__ PointStep point_step(const Batch &b, int64_t i, doubl
     const int step = b.current();
     p.active = step <= b.lastStep[s];
     p.last = step >= b.lastStep[s];
-    const double v = b.velocity[s];
-    const double t = (double)step * p.dt, t1 = (double)(step + 1) * p.dt;
-    p.now = Ramp{v * t, v, 0.0};
-    p.next = Ramp{v * t1, v, 0.0};
+    const int64_t o = b.band ? b.bandOff[s] : 0;
+    if (b.band && b.bandOff[s + 1] > o) {
+      const int64_t k = o + (p.active ? step : b.lastStep[s]);
+      const int64_t k1 = p.last ? k : k + 1;
+      p.now = Ramp{b.band[3 * k], b.band[3 * k + 1], b.band[3 * k + 2]};
+      p.next = Ramp{b.band[3 * k1], b.band[3 * k1 + 1], b.band[3 * k1 + 2]};
+    } else {
+      const double v = b.velocity[s];
+      const double t = (double)step * p.dt, t1 = (double)(step + 1) * p.dt;
+      p.now = Ramp{v * t, v, 0.0};
+      p.next = Ramp{v * t1, v, 0.0};
+    }
   }
   return p;
 }
@@ -1352,14 +1365,23 @@ __global__ void __launch_bounds__(256) k_material(MaterialArgs g) {
   if (blk < 0)
     return;
   const int64_t i = blk * blockDim.x + threadIdx.x;
-  if (!DIAG || FT_ONLY) {
+  if (!DIAG) {
     if (i < g.n)
       material_one<MODE, FT_ONLY, DIAG>(g, i, nullptr);
     return;
   }
+  if (FT_ONLY) { // (the diagnostics launch walks the chunks)
+    bool valid;
+    const int64_t ip = diag_particle(g.diag, blk, g.n, valid);
+    if (valid)
+      material_one<MODE, FT_ONLY, DIAG>(g, ip, nullptr);
+    return;
+  }
   double dv[D_NET_X] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i < g.n)
-    material_one<MODE, false, true>(g, i, dv);
+  bool valid;
+  const int64_t ip = diag_particle(g.diag, blk, g.n, valid);
+  if (valid)
+    material_one<MODE, false, true>(g, ip, dv);
   diag_block_store<D_NET_X, 0>(dv, g.diag.partials + blk * DIAG_STRIDE);
 }
 
@@ -1405,8 +1427,10 @@ __global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
     return;
   }
   double dv[D_NET_X] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  if (i < g.n)
-    material_return_one<MODE, true>(g, i, dv);
+  bool valid;
+  const int64_t ip = diag_particle(g.diag, blk, g.n, valid);
+  if (valid)
+    material_return_one<MODE, true>(g, ip, dv);
   diag_block_store<D_NET_X, 0>(dv, g.diag.partials + blk * DIAG_STRIDE);
 }
 
@@ -1761,15 +1785,17 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
       continue;
     }
     const int64_t chunk = (int64_t)r * g.rd.blocksPerRound + blk;
-    if (chunk * 256 >= g.rd.n) // the same for every thread of the block
+    if (g.diag.chunkFirst ? chunk >= g.diag.chunks : chunk * 256 >= g.rd.n) // block-uniform
       break;
     constexpr int NS = D_COUNT - D_NET_X;
     double dv[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s)
       dv[s] = 0.0;
-    if (i < g.rd.n)
-      force_one<DAMAGE, FUSE_NEXT, INTEGRATE, true>(g, i, dv);
+    bool valid;
+    const int64_t ip = diag_particle(g.diag, chunk, g.rd.n, valid);
+    if (valid)
+      force_one<DAMAGE, FUSE_NEXT, INTEGRATE, true>(g, ip, dv);
     diag_block_store<NS, D_NET_X>(dv, g.diag.partials + chunk * DIAG_STRIDE + D_NET_X);
   }
 }

@@ -422,151 +422,166 @@ double mean_of(const std::vector<double> &field, const std::vector<char> &mask, 
   return sum / static_cast<double>(count);
 }
 
-// runMaterial (BoundaryValueCaseRunner.cpp:139-307) with the particle work on the GPU.
-RunSummary run_material(const CaseConfig &cfg, const CaseMaterial &mat, std::ostream &ts,
-                        std::ostream &snap) {
-  const auto started = std::chrono::steady_clock::now();
-  const int64_t n = static_cast<int64_t>(cfg.nx) * cfg.ny * cfg.nz;
-  std::vector<double> x(3 * n), vol(n);
-  if (vpd_make_patch(cfg.nx, cfg.ny, cfg.nz, cfg.spacing, x.data(), vol.data()) != VPD_OK)
-    throw std::runtime_error("Particle patch requires positive dimensions and spacing.");
-  const int axis = cfg.component == "y" ? 1 : 0;
-  const int lateral = axis == 0 ? 1 : 0;
-  const double width = static_cast<double>(cfg.regionLayers - 1) * cfg.spacing + 1.0e-9 * cfg.spacing;
-  const auto support = region(x, n, cfg.supportRegion.c_str(), width);
-  const auto loaded = region(x, n, cfg.loadRegion.c_str(), width);
-  std::vector<uint8_t> codes(n);
-  int64_t anchor = 0, pin = 0;
-  if (vpd_regional_constraint_codes(x.data(), n, axis, cfg.regionLayers, cfg.spacing,
-                                    cfg.planeStrain ? 1 : 0, codes.data(), &anchor,
-                                    &pin) != VPD_OK)
-    throw std::runtime_error("Boundary-region selection returned no particles.");
-  const double loadingSign = cfg.amplitude > 0.0 ? 1.0 : -1.0;
-  const double gaugeLength = std::abs(mean_of(x, loaded, axis) - mean_of(x, support, axis));
-  const auto latNeg = region(x, n, lateral == 0 ? "left" : "bottom", 1.0e-9 * cfg.spacing);
-  const auto latPos = region(x, n, lateral == 0 ? "right" : "top", 1.0e-9 * cfg.spacing);
-  const double lateralGauge = std::abs(mean_of(x, latPos, lateral) - mean_of(x, latNeg, lateral));
+// One boundary-value run: one material of one case (runMaterial, BoundaryValueCaseRunner.cpp:
+// 139-307), with the particle work on the GPU. setup() is runMaterial's preamble (patch, bands,
+// observation window and their checks); consume() is the host half of the per-step
+// diagnostics (:240-305) for one step record; snapshot() writes writeSnapshot's rows
+// (:114-136). The run's output rows are kept until the case's files are written.
+struct BoundaryRun {
+  const CaseConfig *cfg = nullptr;
+  const CaseMaterial *mat = nullptr;
+  int64_t n = 0;
+  std::vector<double> x, vol, weight;
+  std::vector<uint8_t> codes, latSets;
+  int axis = 0, lateral = 1;
+  double loadingSign = 1.0, gaugeLength = 0.0, lateralGauge = 0.0, interiorVolume = 0.0;
+  double crossSectionArea = 0.0;
   int64_t countNeg = 0, countPos = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    countNeg += latNeg[i] ? 1 : 0;
-    countPos += latPos[i] ? 1 : 0;
-  }
-
-  // observationBounds / observationWeights (include/ObservationWindow.hpp:13-58).
-  double lo = x[axis], hi = lo;
-  for (int64_t i = 0; i < n; ++i) {
-    lo = std::min(lo, x[3 * i + axis]);
-    hi = std::max(hi, x[3 * i + axis]);
-  }
-  const double band = (cfg.regionLayers - 1) * cfg.spacing;
-  const double obsA = std::isnan(cfg.axialMin) ? lo + band : cfg.axialMin;
-  const double obsB = std::isnan(cfg.axialMax) ? hi - band : cfg.axialMax;
-  if (!std::isfinite(obsA) || !std::isfinite(obsB) || obsA < lo || obsB > hi || obsA >= obsB)
-    throw std::runtime_error(
-        "Observation window must lie within the body and have positive length.");
-  const double tol = 1e-8 * cfg.spacing;
-  std::vector<double> weight(n, 0.0);
-  bool foundA = false, foundB = false;
-  for (int64_t i = 0; i < n; ++i) {
-    const double xi = x[3 * i + axis];
-    const bool atA = std::abs(xi - obsA) < tol, atB = std::abs(xi - obsB) < tol;
-    foundA = foundA || atA;
-    foundB = foundB || atB;
-    if (xi < obsA - tol || xi > obsB + tol)
-      continue;
-    const bool interiorEnd = (atA && obsA > lo + tol) || (atB && obsB < hi - tol);
-    weight[i] = vol[i] * (interiorEnd ? 0.5 : 1.0);
-  }
-  if (!foundA || !foundB)
-    throw std::runtime_error("Observation bounds must coincide with reference nodal planes.");
-  double interiorVolume = 0.0;
-  for (int64_t i = 0; i < n; ++i)
-    if (weight[i] > 0.0)
-      interiorVolume += weight[i];
-  const int crossCount = axis == 0 ? cfg.ny : cfg.nx;
-  const double crossSectionArea =
-      static_cast<double>((crossCount - 1) * (cfg.nz - 1)) * cfg.spacing * cfg.spacing;
-
-  // The material must be the device's Johnson-Cook VUMAT.
-  const std::string plugin = std::filesystem::path(mat.material.pluginPath).filename().string();
-  std::string cm = mat.material.cmname;
-  std::transform(cm.begin(), cm.end(), cm.begin(), [](unsigned char ch) { return std::toupper(ch); });
-  if (cm.rfind("JOHNSONCOOK", 0) != 0 && plugin.rfind("jc_vumat_plugin", 0) != 0)
-    throw std::runtime_error("Only the Johnson-Cook VUMAT runs on the device; material " +
-                             mat.label + " uses " + plugin);
-  vpd_config vc{};
-  vc.horizon = cfg.horizon;
-  vc.density = cfg.density;
-  vc.nprops = static_cast<int>(mat.material.props.size());
-  vc.props = mat.material.props.data();
-  vc.nstatev = mat.material.nstatev;
-  vc.initial_state = mat.initialState.empty() ? nullptr : mat.initialState.data();
-  vc.reference_temperature = mat.referenceTemperature;
-  vc.char_length = cfg.spacing;
-  vc.material_density = mat.material.density;
-  if (cfg.damageMode == "none") {
-    vc.damage_mode = VPD_DAMAGE_NONE;
-  } else if (cfg.damageMode == "critical_stretch") {
-    if (cfg.damageInitiation <= 0.0)
-      throw std::runtime_error("critical_stretch requires initiation > 0.");
-    vc.damage_mode = VPD_DAMAGE_CRITICAL_STRETCH;
-  } else {
-    throw std::runtime_error("Unsupported damage model on the device: " + cfg.damageMode);
-  }
-  vc.damage_initiation = cfg.damageInitiation;
-  vc.damage_failure = cfg.damageFailure;
-  vc.return_map = VPD_RETURN_MAP_BISECTION;
-  vpd_context *raw = nullptr;
-  if (vpd_create(&vc, &raw) != VPD_OK)
-    throw std::runtime_error(vpd_last_error(nullptr));
-  std::unique_ptr<vpd_context, ContextDeleter> owner(raw);
-  vpd_context *ctx = raw;
-  check(ctx, vpd_set_particles(ctx, x.data(), vol.data(), n));
-  int64_t nnz = 0;
-  check(ctx, vpd_build_neighbors(ctx, &nnz));
-  check(ctx, vpd_set_constraints(ctx, codes.data()));
-  check(ctx, vpd_initialize_shape(ctx));
-  check(ctx, vpd_initialize_material(ctx));
-  check(ctx, vpd_initialize_dynamics(ctx));
-  double ramp0[3];
-  vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, 0.0, ramp0);
-  check(ctx, vpd_apply_constraints(ctx, ramp0));
-
-  std::vector<uint8_t> latSets(n);
-  for (int64_t i = 0; i < n; ++i)
-    latSets[i] = (latNeg[i] ? 1 : 0) | (latPos[i] ? 2 : 0);
-  vpd_diag_config dc{};
-  dc.load_axis = axis;
-  dc.lateral_axis = lateral;
-  dc.observation_weight = weight.data();
-  dc.lateral_sets = latSets.data();
-  dc.eqps_index = mat.eqpsIndex;
-  dc.indicator_count = static_cast<int>(mat.indicatorIndices.size());
-  if (dc.indicator_count > 8)
-    throw std::runtime_error("At most 8 internal-indicator state indices run on the device.");
-  for (int k = 0; k < dc.indicator_count; ++k)
-    dc.indicator_indices[k] = mat.indicatorIndices[k];
-  dc.indicator_min = mat.indicatorMin ? 1 : 0;
-  dc.indicator_one_minus = mat.indicatorOneMinus ? 1 : 0;
-  dc.capacity = std::min(cfg.steps, 2048);
-  check(ctx, vpd_enable_diagnostics(ctx, &dc));
-
   RunSummary summary;
-  summary.material = mat.label;
-  summary.observationMin = obsA;
-  summary.observationMax = obsB;
-  summary.analyticalPlaneStrainSlope = mat.elasticE / (1.0 - mat.elasticNu * mat.elasticNu);
   double internalEnergy = 0.0, externalWork = 0.0, previousPower = 0.0;
   double slopeX2 = 0.0, slopeXY = 0.0;
-  const double dt = cfg.dt;
-  std::vector<double> records(static_cast<size_t>(dc.capacity) * VPD_DIAG_STRIDE);
-  int processed = 0; // steps whose record has been consumed
+  std::string ts, snap;
 
-  // Host half of the per-step diagnostics (BoundaryValueCaseRunner.cpp:240-305).
-  auto consume = [&](const double *r, int step) {
+  BoundaryRun(const CaseConfig &c, const CaseMaterial &m) : cfg(&c), mat(&m) {}
+
+  void setup() {
+    const CaseConfig &cf = *cfg;
+    n = static_cast<int64_t>(cf.nx) * cf.ny * cf.nz;
+    x.assign(3 * n, 0.0);
+    vol.assign(n, 0.0);
+    if (vpd_make_patch(cf.nx, cf.ny, cf.nz, cf.spacing, x.data(), vol.data()) != VPD_OK)
+      throw std::runtime_error("Particle patch requires positive dimensions and spacing.");
+    axis = cf.component == "y" ? 1 : 0;
+    lateral = axis == 0 ? 1 : 0;
+    const double width =
+        static_cast<double>(cf.regionLayers - 1) * cf.spacing + 1.0e-9 * cf.spacing;
+    const auto support = region(x, n, cf.supportRegion.c_str(), width);
+    const auto loaded = region(x, n, cf.loadRegion.c_str(), width);
+    codes.assign(n, 0);
+    int64_t anchor = 0, pin = 0;
+    if (vpd_regional_constraint_codes(x.data(), n, axis, cf.regionLayers, cf.spacing,
+                                      cf.planeStrain ? 1 : 0, codes.data(), &anchor,
+                                      &pin) != VPD_OK)
+      throw std::runtime_error("Boundary-region selection returned no particles.");
+    loadingSign = cf.amplitude > 0.0 ? 1.0 : -1.0;
+    gaugeLength = std::abs(mean_of(x, loaded, axis) - mean_of(x, support, axis));
+    const auto latNeg = region(x, n, lateral == 0 ? "left" : "bottom", 1.0e-9 * cf.spacing);
+    const auto latPos = region(x, n, lateral == 0 ? "right" : "top", 1.0e-9 * cf.spacing);
+    lateralGauge = std::abs(mean_of(x, latPos, lateral) - mean_of(x, latNeg, lateral));
+    latSets.assign(n, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      countNeg += latNeg[i] ? 1 : 0;
+      countPos += latPos[i] ? 1 : 0;
+      latSets[i] = (latNeg[i] ? 1 : 0) | (latPos[i] ? 2 : 0);
+    }
+    // observationBounds / observationWeights (include/ObservationWindow.hpp:13-58).
+    double lo = x[axis], hi = lo;
+    for (int64_t i = 0; i < n; ++i) {
+      lo = std::min(lo, x[3 * i + axis]);
+      hi = std::max(hi, x[3 * i + axis]);
+    }
+    const double band = (cf.regionLayers - 1) * cf.spacing;
+    const double obsA = std::isnan(cf.axialMin) ? lo + band : cf.axialMin;
+    const double obsB = std::isnan(cf.axialMax) ? hi - band : cf.axialMax;
+    if (!std::isfinite(obsA) || !std::isfinite(obsB) || obsA < lo || obsB > hi || obsA >= obsB)
+      throw std::runtime_error(
+          "Observation window must lie within the body and have positive length.");
+    const double tol = 1e-8 * cf.spacing;
+    weight.assign(n, 0.0);
+    bool foundA = false, foundB = false;
+    for (int64_t i = 0; i < n; ++i) {
+      const double xi = x[3 * i + axis];
+      const bool atA = std::abs(xi - obsA) < tol, atB = std::abs(xi - obsB) < tol;
+      foundA = foundA || atA;
+      foundB = foundB || atB;
+      if (xi < obsA - tol || xi > obsB + tol)
+        continue;
+      const bool interiorEnd = (atA && obsA > lo + tol) || (atB && obsB < hi - tol);
+      weight[i] = vol[i] * (interiorEnd ? 0.5 : 1.0);
+    }
+    if (!foundA || !foundB)
+      throw std::runtime_error("Observation bounds must coincide with reference nodal planes.");
+    for (int64_t i = 0; i < n; ++i)
+      if (weight[i] > 0.0)
+        interiorVolume += weight[i];
+    const int crossCount = axis == 0 ? cf.ny : cf.nx;
+    crossSectionArea =
+        static_cast<double>((crossCount - 1) * (cf.nz - 1)) * cf.spacing * cf.spacing;
+    // The material must be the device's Johnson-Cook VUMAT.
+    const std::string plugin =
+        std::filesystem::path(mat->material.pluginPath).filename().string();
+    std::string cm = mat->material.cmname;
+    std::transform(cm.begin(), cm.end(), cm.begin(),
+                   [](unsigned char ch) { return std::toupper(ch); });
+    if (cm.rfind("JOHNSONCOOK", 0) != 0 && plugin.rfind("jc_vumat_plugin", 0) != 0)
+      throw std::runtime_error("Only the Johnson-Cook VUMAT runs on the device; material " +
+                               mat->label + " uses " + plugin);
+    summary.material = mat->label;
+    summary.observationMin = obsA;
+    summary.observationMax = obsB;
+    summary.analyticalPlaneStrainSlope = mat->elasticE / (1.0 - mat->elasticNu * mat->elasticNu);
+  }
+
+  vpd_config context_config() const {
+    const CaseConfig &cf = *cfg;
+    vpd_config vc{};
+    vc.horizon = cf.horizon;
+    vc.density = cf.density;
+    vc.nprops = static_cast<int>(mat->material.props.size());
+    vc.props = mat->material.props.data();
+    vc.nstatev = mat->material.nstatev;
+    vc.initial_state = mat->initialState.empty() ? nullptr : mat->initialState.data();
+    vc.reference_temperature = mat->referenceTemperature;
+    vc.char_length = cf.spacing;
+    vc.material_density = mat->material.density;
+    if (cf.damageMode == "none") {
+      vc.damage_mode = VPD_DAMAGE_NONE;
+    } else if (cf.damageMode == "critical_stretch") {
+      if (cf.damageInitiation <= 0.0)
+        throw std::runtime_error("critical_stretch requires initiation > 0.");
+      vc.damage_mode = VPD_DAMAGE_CRITICAL_STRETCH;
+    } else {
+      throw std::runtime_error("Unsupported damage model on the device: " + cf.damageMode);
+    }
+    vc.damage_initiation = cf.damageInitiation;
+    vc.damage_failure = cf.damageFailure;
+    vc.return_map = VPD_RETURN_MAP_BISECTION;
+    return vc;
+  }
+
+  // The diagnostics configuration (weights and face sets of this run's particles).
+  vpd_diag_config diag_config(int capacity) const {
+    vpd_diag_config dc{};
+    dc.load_axis = axis;
+    dc.lateral_axis = lateral;
+    dc.observation_weight = weight.data();
+    dc.lateral_sets = latSets.data();
+    dc.eqps_index = mat->eqpsIndex;
+    dc.indicator_count = static_cast<int>(mat->indicatorIndices.size());
+    if (dc.indicator_count > 8)
+      throw std::runtime_error("At most 8 internal-indicator state indices run on the device.");
+    for (int k = 0; k < dc.indicator_count; ++k)
+      dc.indicator_indices[k] = mat->indicatorIndices[k];
+    dc.indicator_min = mat->indicatorMin ? 1 : 0;
+    dc.indicator_one_minus = mat->indicatorOneMinus ? 1 : 0;
+    dc.capacity = capacity;
+    return dc;
+  }
+
+  void ramp(int step, double out[3]) const {
+    vpd_smooth_ramp(cfg->amplitude, cfg->riseTime, static_cast<double>(step) * cfg->dt, out);
+  }
+
+  bool snapshot_step(int step) const {
+    return step % cfg->snapshotEvery == 0 || step == cfg->steps;
+  }
+
+  void consume(const double *r, int step) {
+    const CaseConfig &cf = *cfg;
+    const double dt = cf.dt;
     const double time = static_cast<double>(step) * dt;
-    double ramp[3];
-    vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, time, ramp);
+    double rp[3];
+    ramp(step, rp);
     internalEnergy += r[VPD_DIAG_INTERNAL_ENERGY];
     const double net = std::sqrt(r[VPD_DIAG_NET_FORCE_X] * r[VPD_DIAG_NET_FORCE_X] +
                                  r[VPD_DIAG_NET_FORCE_Y] * r[VPD_DIAG_NET_FORCE_Y] +
@@ -578,7 +593,7 @@ RunSummary run_material(const CaseConfig &cfg, const CaseMaterial &mat, std::ost
     const double maxIndicator = r[VPD_DIAG_MAX_INDICATOR];
     summary.finite = summary.finite && r[VPD_DIAG_NONFINITE_MATERIAL] == 0.0 &&
                      r[VPD_DIAG_NONFINITE_FORCE] == 0.0;
-    const double power = reaction * ramp[1];
+    const double power = reaction * rp[1];
     externalWork += 0.5 * dt * (previousPower + power);
     previousPower = power;
     const double kinetic = r[VPD_DIAG_KINETIC];
@@ -586,7 +601,7 @@ RunSummary run_material(const CaseConfig &cfg, const CaseMaterial &mat, std::ost
         std::max({std::abs(externalWork), std::abs(internalEnergy) + kinetic, 1.0e-12});
     const double energyError = std::abs(kinetic + internalEnergy - externalWork) / scale;
     const double kineticRatio = kinetic / std::max(std::abs(internalEnergy), 1.0e-12);
-    const double engineeringStrain = loadingSign * ramp[0] / gaugeLength;
+    const double engineeringStrain = loadingSign * rp[0] / gaugeLength;
     const double lateralStrain = (r[VPD_DIAG_LATERAL_POS] / static_cast<double>(countPos) -
                                   r[VPD_DIAG_LATERAL_NEG] / static_cast<double>(countNeg)) /
                                  lateralGauge;
@@ -607,41 +622,123 @@ RunSummary run_material(const CaseConfig &cfg, const CaseMaterial &mat, std::ost
       slopeX2 += engineeringStrain * engineeringStrain;
       slopeXY += engineeringStrain * averageStress;
     }
-    if (step % cfg.outputEvery == 0 || step == 1 || step == cfg.steps) {
-      const double vals[] = {time, ramp[0], ramp[1], reaction, engineeringStrain, lateralStrain,
+    if (step % cf.outputEvery == 0 || step == 1 || step == cf.steps) {
+      const double vals[] = {time, rp[0], rp[1], reaction, engineeringStrain, lateralStrain,
                              nominalStress, averageAxialStrain, averageStress, maxEqps,
                              maxIndicator, kinetic, internalEnergy, externalWork, energyError,
                              net};
-      ts << mat.label << ',' << step;
+      ts += mat->label + ',' + std::to_string(step);
       for (double v : vals)
-        ts << ',' << num(v, 16);
-      ts << ',' << (summary.finite ? 1 : 0) << '\n';
+        ts += ',' + num(v, 16);
+      ts += summary.finite ? ",1\n" : ",0\n";
     }
-  };
+  }
+
+  // The run's particles start at `offset` in the downloaded block fields.
+  void snapshot(int step, int64_t offset, int nstatev, const double *u, const double *v,
+                const double *stress, const double *strain, const double *statev,
+                const double *dmg) {
+    const double time = static_cast<double>(step) * cfg->dt;
+    std::string line;
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t j = offset + i;
+      const double *x0 = &x[3 * i], *ui = &u[3 * j], *vi = &v[3 * j];
+      const double *st = &statev[static_cast<size_t>(j) * nstatev];
+      const double eqps = mat->eqpsIndex >= 0 ? st[mat->eqpsIndex] : 0.0;
+      const double vals[] = {x0[0], x0[1], x0[2], ui[0], ui[1], ui[2],
+                             x0[0] + ui[0], x0[1] + ui[1], x0[2] + ui[2], vi[0], vi[1], vi[2],
+                             strain[9 * j + 4 * axis], stress[9 * j + 4 * axis], eqps,
+                             internal_indicator(*mat, st), dmg[j]};
+      line = mat->label + ',' + std::to_string(step) + ',' + num(time, 16) + ',' +
+             std::to_string(i);
+      for (double val : vals)
+        line += ',' + num(val, 16);
+      line += '\n';
+      snap += line;
+    }
+  }
+
+  void finish(double runtimeSeconds) {
+    const CaseConfig &cf = *cfg;
+    summary.earlyElasticSlope = slopeX2 > 0.0 ? slopeXY / slopeX2 : 0.0;
+    summary.earlyElasticRelativeError =
+        summary.analyticalPlaneStrainSlope > 0.0
+            ? std::abs(summary.earlyElasticSlope - summary.analyticalPlaneStrainSlope) /
+                  summary.analyticalPlaneStrainSlope
+            : 0.0;
+    summary.runtimeSeconds = runtimeSeconds;
+    summary.pass = summary.finite && summary.maxNetInternalForce <= cf.netForceTol &&
+                   summary.finalEnergyError <= cf.energyTol &&
+                   summary.finalKineticToInternal <= cf.kineticLimit &&
+                   summary.maxEqps >= cf.minPlastic;
+  }
+};
+
+// Fields of the whole block at a snapshot step.
+struct BlockFields {
+  std::vector<double> u, v, stress, strain, statev, dmg;
+  void download(vpd_context *ctx, int64_t n, int nstatev) {
+    u.resize(3 * n);
+    v.resize(3 * n);
+    stress.resize(9 * n);
+    strain.resize(9 * n);
+    dmg.resize(n);
+    statev.resize(static_cast<size_t>(nstatev) * n);
+    check(ctx, vpd_download(ctx, VPD_FIELD_DISPLACEMENT, u.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_VELOCITY, v.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_STRESS, stress.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_STRAIN, strain.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_STATEV, statev.data()));
+    check(ctx, vpd_download(ctx, VPD_FIELD_DAMAGE_INDEX, dmg.data()));
+  }
+};
+
+// runMaterial for one material alone: steps through vpd_step with the fused diagnostics.
+void run_material(BoundaryRun &run) {
+  const auto started = std::chrono::steady_clock::now();
+  const CaseConfig &cfg = *run.cfg;
+  run.setup();
+  const int64_t n = run.n;
+  const vpd_config vc = run.context_config();
+  vpd_context *raw = nullptr;
+  if (vpd_create(&vc, &raw) != VPD_OK)
+    throw std::runtime_error(vpd_last_error(nullptr));
+  std::unique_ptr<vpd_context, ContextDeleter> owner(raw);
+  vpd_context *ctx = raw;
+  check(ctx, vpd_set_particles(ctx, run.x.data(), run.vol.data(), n));
+  int64_t nnz = 0;
+  check(ctx, vpd_build_neighbors(ctx, &nnz));
+  check(ctx, vpd_set_constraints(ctx, run.codes.data()));
+  check(ctx, vpd_initialize_shape(ctx));
+  check(ctx, vpd_initialize_material(ctx));
+  check(ctx, vpd_initialize_dynamics(ctx));
+  double ramp0[3];
+  run.ramp(0, ramp0);
+  check(ctx, vpd_apply_constraints(ctx, ramp0));
+  const vpd_diag_config dc = run.diag_config(std::min(cfg.steps, 2048));
+  check(ctx, vpd_enable_diagnostics(ctx, &dc));
+
+  std::vector<double> records(static_cast<size_t>(dc.capacity) * VPD_DIAG_STRIDE);
+  int processed = 0; // steps whose record has been consumed
   auto drain = [&] {
     int count = 0;
     check(ctx, vpd_fetch_diagnostics(ctx, records.data(), dc.capacity, &count));
     for (int k = 0; k < count; ++k)
-      consume(&records[static_cast<size_t>(k) * VPD_DIAG_STRIDE], ++processed);
+      run.consume(&records[static_cast<size_t>(k) * VPD_DIAG_STRIDE], ++processed);
   };
-
-  const int nstatev = mat.material.nstatev;
-  std::vector<double> u(3 * n), v(3 * n), stress(9 * n), strain(9 * n), dmg(n),
-      statev(static_cast<size_t>(nstatev) * n);
+  BlockFields fields;
   int pending = 0;
   for (int step = 1; step <= cfg.steps; ++step) {
-    const double time = static_cast<double>(step) * dt;
-    const bool snapshot = step % cfg.snapshotEvery == 0 || step == cfg.steps;
+    const bool snapshot = run.snapshot_step(step);
     vpd_step_args a{};
-    a.dt = dt;
-    a.time = time;
-    a.step_time = (step - 1) * dt;
-    vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, time, a.ramp);
+    a.dt = cfg.dt;
+    a.time = static_cast<double>(step) * cfg.dt;
+    a.step_time = (step - 1) * cfg.dt;
+    run.ramp(step, a.ramp);
     // The next predictor is fused into this step unless the state after this step is read.
     a.fuse_next = snapshot ? 0 : 1;
     if (a.fuse_next)
-      vpd_smooth_ramp(cfg.amplitude, cfg.riseTime, static_cast<double>(step + 1) * dt,
-                      a.ramp_next); // the time the next step recomputes, bit for bit
+      run.ramp(step + 1, a.ramp_next); // the time the next step recomputes, bit for bit
     a.want_fields = snapshot ? 1 : 0;
     if (pending == dc.capacity) {
       drain();
@@ -653,50 +750,162 @@ RunSummary run_material(const CaseConfig &cfg, const CaseMaterial &mat, std::ost
       continue;
     drain();
     pending = 0;
-    check(ctx, vpd_download(ctx, VPD_FIELD_DISPLACEMENT, u.data()));
-    check(ctx, vpd_download(ctx, VPD_FIELD_VELOCITY, v.data()));
-    check(ctx, vpd_download(ctx, VPD_FIELD_STRESS, stress.data()));
-    check(ctx, vpd_download(ctx, VPD_FIELD_STRAIN, strain.data()));
-    check(ctx, vpd_download(ctx, VPD_FIELD_STATEV, statev.data()));
-    check(ctx, vpd_download(ctx, VPD_FIELD_DAMAGE_INDEX, dmg.data()));
-    // writeSnapshot (BoundaryValueCaseRunner.cpp:114-136).
-    std::string line;
-    for (int64_t i = 0; i < n; ++i) {
-      const double *x0 = &x[3 * i], *ui = &u[3 * i], *vi = &v[3 * i];
-      const double *st = &statev[static_cast<size_t>(i) * nstatev];
-      const double eqps = mat.eqpsIndex >= 0 ? st[mat.eqpsIndex] : 0.0;
-      const double vals[] = {x0[0], x0[1], x0[2], ui[0], ui[1], ui[2],
-                             x0[0] + ui[0], x0[1] + ui[1], x0[2] + ui[2], vi[0], vi[1], vi[2],
-                             strain[9 * i + 4 * axis], stress[9 * i + 4 * axis], eqps,
-                             internal_indicator(mat, st), dmg[i]};
-      line = mat.label + ',' + std::to_string(step) + ',' + num(time, 16) + ',' +
-             std::to_string(i);
-      for (double val : vals)
-        line += ',' + num(val, 16);
-      line += '\n';
-      snap << line;
-    }
+    fields.download(ctx, n, run.mat->material.nstatev);
+    run.snapshot(step, 0, run.mat->material.nstatev, fields.u.data(), fields.v.data(),
+                 fields.stress.data(), fields.strain.data(), fields.statev.data(),
+                 fields.dmg.data());
   }
   if (pending)
     drain();
-
-  summary.earlyElasticSlope = slopeX2 > 0.0 ? slopeXY / slopeX2 : 0.0;
-  summary.earlyElasticRelativeError =
-      summary.analyticalPlaneStrainSlope > 0.0
-          ? std::abs(summary.earlyElasticSlope - summary.analyticalPlaneStrainSlope) /
-                summary.analyticalPlaneStrainSlope
-          : 0.0;
-  summary.runtimeSeconds =
-      std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
-  summary.pass = summary.finite && summary.maxNetInternalForce <= cfg.netForceTol &&
-                 summary.finalEnergyError <= cfg.energyTol &&
-                 summary.finalKineticToInternal <= cfg.kineticLimit &&
-                 summary.maxEqps >= cfg.minPlastic;
-  return summary;
+  run.finish(std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count());
 }
 
-// runBoundaryValueCase (BoundaryValueCaseRunner.cpp:309-440).
-int run_boundary_case(const CaseConfig &cfg) {
+// The materials of several boundary cases as specimens of one particle block
+// (vpd_set_specimens with half-cosine band tables, vpd_set_specimen_bands), stepped together
+// with per-specimen diagnostics. Each run's rows equal those of run_material on the same case,
+// bit for bit: every specimen keeps its own neighbour rows, time step, band values, step count
+// and diagnostics chunks. The runs must share the device context's settings (material,
+// solver density, damage) and the diagnostics configuration (load axis, state indices).
+void run_batch(std::vector<BoundaryRun> &runs) {
+  const auto started = std::chrono::steady_clock::now();
+  for (auto &r : runs)
+    r.setup();
+  const BoundaryRun &head = runs.front();
+  const vpd_config vc = head.context_config();
+  for (const auto &r : runs) {
+    const vpd_config o = r.context_config();
+    const CaseMaterial &a = *head.mat, &b = *r.mat;
+    if (o.density != vc.density || o.damage_mode != vc.damage_mode ||
+        o.damage_initiation != vc.damage_initiation || o.damage_failure != vc.damage_failure ||
+        a.material.props != b.material.props || a.material.nstatev != b.material.nstatev ||
+        a.material.density != b.material.density || a.initialState != b.initialState ||
+        a.referenceTemperature != b.referenceTemperature || a.eqpsIndex != b.eqpsIndex ||
+        a.indicatorIndices != b.indicatorIndices || a.indicatorMin != b.indicatorMin ||
+        a.indicatorOneMinus != b.indicatorOneMinus || r.axis != head.axis)
+      throw std::runtime_error("Cases " + head.cfg->name + " and " + r.cfg->name +
+                               " cannot share a block: material, solver density, damage, "
+                               "load axis and state indices must agree.");
+  }
+  const int count = static_cast<int>(runs.size());
+  std::vector<int64_t> first(count + 1, 0);
+  for (int s = 0; s < count; ++s)
+    first[s + 1] = first[s] + runs[s].n;
+  const int64_t n = first[count];
+  std::vector<double> x, vol, weight;
+  std::vector<uint8_t> codes, latSets;
+  std::vector<int32_t> rowptr(1, 0), col;
+  for (auto &r : runs) { // each specimen's own neighbour rows, at its own horizon
+    x.insert(x.end(), r.x.begin(), r.x.end());
+    vol.insert(vol.end(), r.vol.begin(), r.vol.end());
+    weight.insert(weight.end(), r.weight.begin(), r.weight.end());
+    codes.insert(codes.end(), r.codes.begin(), r.codes.end());
+    latSets.insert(latSets.end(), r.latSets.begin(), r.latSets.end());
+    vpd_config rc = r.context_config();
+    vpd_context *tmp = nullptr;
+    if (vpd_create(&rc, &tmp) != VPD_OK)
+      throw std::runtime_error(vpd_last_error(nullptr));
+    std::unique_ptr<vpd_context, ContextDeleter> hold(tmp);
+    check(tmp, vpd_set_particles(tmp, r.x.data(), r.vol.data(), r.n));
+    int64_t nnz = 0;
+    check(tmp, vpd_build_neighbors(tmp, &nnz));
+    std::vector<int32_t> rp(r.n + 1), cl(std::max<int64_t>(1, nnz));
+    check(tmp, vpd_get_neighbors(tmp, rp.data(), cl.data()));
+    const int32_t base = static_cast<int32_t>(x.size() / 3 - r.n), offset = rowptr.back();
+    for (int64_t i = 1; i <= r.n; ++i)
+      rowptr.push_back(offset + rp[i]);
+    for (int64_t k = 0; k < nnz; ++k)
+      col.push_back(base + cl[k]);
+  }
+  vpd_context *raw = nullptr;
+  if (vpd_create(&vc, &raw) != VPD_OK)
+    throw std::runtime_error(vpd_last_error(nullptr));
+  std::unique_ptr<vpd_context, ContextDeleter> owner(raw);
+  vpd_context *ctx = raw;
+  check(ctx, vpd_set_particles(ctx, x.data(), vol.data(), n));
+  if (col.empty())
+    col.push_back(0);
+  check(ctx, vpd_set_neighbors(ctx, rowptr.data(), col.data(), rowptr.back()));
+  check(ctx, vpd_set_constraints(ctx, codes.data()));
+  check(ctx, vpd_initialize_shape(ctx));
+  check(ctx, vpd_initialize_material(ctx));
+  check(ctx, vpd_initialize_dynamics(ctx));
+  std::vector<double> dts(count), vel(count, 0.0), table;
+  std::vector<int32_t> last(count);
+  std::vector<int64_t> offsets(count + 1, 0);
+  int maxSteps = 0;
+  for (int s = 0; s < count; ++s) {
+    const BoundaryRun &r = runs[s];
+    dts[s] = r.cfg->dt;
+    last[s] = r.cfg->steps;
+    maxSteps = std::max(maxSteps, r.cfg->steps);
+    for (int k = 0; k <= r.cfg->steps; ++k) { // the ramp at step·dt, as run_material has it
+      double rp[3];
+      r.ramp(k, rp);
+      table.insert(table.end(), rp, rp + 3);
+    }
+    offsets[s + 1] = offsets[s] + r.cfg->steps + 1;
+  }
+  check(ctx, vpd_set_specimens(ctx, count, first.data(), dts.data(), vel.data(), last.data()));
+  check(ctx, vpd_set_specimen_bands(ctx, offsets.data(), table.data()));
+  vpd_diag_config dc = head.diag_config(std::min(maxSteps, 1024));
+  dc.observation_weight = weight.data();
+  dc.lateral_sets = latSets.data();
+  check(ctx, vpd_enable_diagnostics(ctx, &dc));
+
+  std::vector<double> records(static_cast<size_t>(dc.capacity) * count * VPD_DIAG_STRIDE);
+  int processed = 0;
+  auto drain = [&] {
+    int steps = 0;
+    check(ctx, vpd_fetch_diagnostics(ctx, records.data(), dc.capacity, &steps));
+    for (int k = 0; k < steps; ++k) {
+      ++processed;
+      for (int s = 0; s < count; ++s)
+        if (processed <= runs[s].cfg->steps)
+          runs[s].consume(&records[(static_cast<size_t>(k) * count + s) * VPD_DIAG_STRIDE],
+                          processed);
+    }
+  };
+  const int nstatev = head.mat->material.nstatev;
+  BlockFields fields;
+  int pending = 0;
+  for (int step = 1; step <= maxSteps; ++step) {
+    bool snapshot = false, midRun = false;
+    for (const auto &r : runs)
+      if (step <= r.cfg->steps && r.snapshot_step(step)) {
+        snapshot = true;
+        midRun = midRun || step < r.cfg->steps;
+      }
+    if (pending == dc.capacity) {
+      drain();
+      pending = 0;
+    }
+    // A specimen's last step is never followed by a prediction; a mid-run snapshot needs
+    // the end-of-step state, so that step is left unfused.
+    check(ctx, vpd_step_specimens(ctx, step, midRun ? 0 : 1, snapshot ? 1 : 0));
+    ++pending;
+    if (!snapshot)
+      continue;
+    drain();
+    pending = 0;
+    fields.download(ctx, n, nstatev);
+    for (int s = 0; s < count; ++s)
+      if (step <= runs[s].cfg->steps && runs[s].snapshot_step(step))
+        runs[s].snapshot(step, first[s], nstatev, fields.u.data(), fields.v.data(),
+                         fields.stress.data(), fields.strain.data(), fields.statev.data(),
+                         fields.dmg.data());
+  }
+  if (pending)
+    drain();
+  const double runtime =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+  for (auto &r : runs)
+    r.finish(runtime);
+}
+
+// runBoundaryValueCase's outputs (BoundaryValueCaseRunner.cpp:341-440) for a case whose
+// material runs are done: timeseries.csv, field_snapshots.csv, metadata.json and the summary
+// line per material. Returns 0 when every material passes, 2 otherwise.
+int write_boundary_case(const CaseConfig &cfg, const std::vector<const BoundaryRun *> &runs) {
   std::filesystem::create_directories(cfg.outputDir);
   std::ofstream ts(cfg.outputDir + "/timeseries.csv");
   std::ofstream snap(cfg.outputDir + "/field_snapshots.csv");
@@ -710,17 +919,17 @@ int run_boundary_case(const CaseConfig &cfg) {
   snap << "material,step,time_ms,particle,x0_mm,y0_mm,z0_mm,ux_mm,uy_mm,uz_mm,"
           "x_mm,y_mm,z_mm,vx_mm_per_ms,vy_mm_per_ms,vz_mm_per_ms,axial_strain,"
           "axial_stress_MPa,eqps,internal_indicator,simple_damage\n";
-  std::vector<RunSummary> summaries;
   const long particles = static_cast<long>(cfg.nx) * cfg.ny * cfg.nz;
-  for (const auto &m : cfg.materials) {
-    RunSummary s = run_material(cfg, m, ts, snap);
+  for (const BoundaryRun *r : runs) {
+    ts << r->ts;
+    snap << r->snap;
+    const RunSummary &s = r->summary;
     std::cout << cfg.name << '/' << s.material << ": particles=" << particles
               << " runtime_s=" << s.runtimeSeconds << " energy_error=" << s.finalEnergyError
               << " max_eqps=" << s.maxEqps << " pass=" << (s.pass ? "true" : "false") << '\n';
-    summaries.push_back(s);
   }
-  const bool pass = std::all_of(summaries.begin(), summaries.end(),
-                                [](const RunSummary &s) { return s.pass; });
+  const bool pass = std::all_of(runs.begin(), runs.end(),
+                                [](const BoundaryRun *r) { return r->summary.pass; });
   std::ofstream md(cfg.outputDir + "/metadata.json");
   md << "{\n  \"name\": \"" << cfg.name << "\",\n"
      << "  \"solver\": \"correspondence peridynamics\",\n"
@@ -747,9 +956,9 @@ int run_boundary_case(const CaseConfig &cfg) {
         "support upper midplane point\"},\n"
      << "  \"dt_ms\": " << num(cfg.dt, 6) << ",\n  \"steps\": " << cfg.steps
      << ",\n  \"materials\": [\n";
-  for (size_t i = 0; i < summaries.size(); ++i) {
-    const auto &s = summaries[i];
-    const auto &m = cfg.materials[i];
+  for (size_t i = 0; i < runs.size(); ++i) {
+    const auto &s = runs[i]->summary;
+    const auto &m = *runs[i]->mat;
     md << "    {\"name\": \"" << s.material << "\", \"status\": \"" << (s.pass ? "pass" : "fail")
        << "\", \"plugin_file\": \""
        << std::filesystem::path(m.material.pluginPath).filename().string()
@@ -773,7 +982,7 @@ int run_boundary_case(const CaseConfig &cfg) {
        << ", \"finalNominalStress_MPa\": " << json_number(s.finalNominalStress)
        << ", \"finalEngineeringStrain\": " << json_number(s.finalEngineeringStrain)
        << ", \"finite\": " << (s.finite ? "true" : "false") << "}"
-       << (i + 1 == summaries.size() ? "\n" : ",\n");
+       << (i + 1 == runs.size() ? "\n" : ",\n");
   }
   md << "  ],\n  \"acceptance\": {\"energyToleranceRelative\": " << num(cfg.energyTol, 6)
      << ", \"netForceTolerance_N\": " << num(cfg.netForceTol, 6)
@@ -786,6 +995,43 @@ int run_boundary_case(const CaseConfig &cfg) {
      << "  \"legacyBlockPolicy\": \"Material-independent fixed-width blocks of 256 points; a "
         "non-full final block duplicates its last valid point and discards padded outputs.\"\n}\n";
   return pass ? 0 : 2;
+}
+
+// runBoundaryValueCase (BoundaryValueCaseRunner.cpp:309-440): the materials one after another.
+int run_boundary_case(const CaseConfig &cfg) {
+  std::vector<BoundaryRun> runs;
+  runs.reserve(cfg.materials.size());
+  std::vector<const BoundaryRun *> done;
+  for (const auto &m : cfg.materials) {
+    runs.emplace_back(cfg, m);
+    run_material(runs.back());
+    done.push_back(&runs.back());
+  }
+  return write_boundary_case(cfg, done);
+}
+
+// Several boundary cases (every material of every case) as one batch.
+int run_boundary_batch(const std::vector<CaseConfig> &cases) {
+  std::vector<BoundaryRun> runs;
+  size_t total = 0;
+  for (const auto &c : cases)
+    total += c.materials.size();
+  runs.reserve(total);
+  for (const auto &c : cases)
+    for (const auto &m : c.materials)
+      runs.emplace_back(c, m);
+  if (runs.empty())
+    throw std::runtime_error("No materials to run.");
+  run_batch(runs);
+  int status = 0;
+  size_t k = 0;
+  for (const auto &c : cases) {
+    std::vector<const BoundaryRun *> mine;
+    for (size_t m = 0; m < c.materials.size(); ++m)
+      mine.push_back(&runs[k++]);
+    status = std::max(status, write_boundary_case(c, mine));
+  }
+  return status;
 }
 
 
@@ -1094,6 +1340,27 @@ int run_demo_case(const CaseConfig &cfg) {
 }
 
 } // namespace
+
+extern "C" int vpd_run_cases_json(const char *const *case_paths, int count,
+                                  const char *plugin_directory) {
+  try {
+    if (!case_paths || count < 1)
+      throw std::runtime_error("No case files given.");
+    std::vector<CaseConfig> cases;
+    for (int k = 0; k < count; ++k) {
+      if (!case_paths[k])
+        throw std::runtime_error("Failed to open JSON file: (null)");
+      cases.push_back(load_case(case_paths[k], plugin_directory ? plugin_directory : ""));
+      if (cases.back().type != "vumat_pd_boundary_case")
+        throw std::runtime_error("Batched runs take vumat_pd_boundary_case files; got: " +
+                                 cases.back().type);
+    }
+    return run_boundary_batch(cases);
+  } catch (const std::exception &error) {
+    std::cerr << "VUMAT-PD demo case failed: " << error.what() << "\n";
+    return 1;
+  }
+}
 
 extern "C" int vpd_run_case_json(const char *case_path, const char *plugin_directory) {
   try {

@@ -126,6 +126,8 @@ def lib():
         L.vpd_select_boundary_region.argtypes = [_D, C.c_int64, C.c_char_p, C.c_double, _U8]
         L.vpd_select_boundary_region.restype = C.c_int64
         L.vpd_run_case_json.argtypes = [C.c_char_p, C.c_char_p]
+        L.vpd_run_cases_json.argtypes = [C.POINTER(C.c_char_p), C.c_int, C.c_char_p]
+        L.vpd_set_specimen_bands.argtypes = [P, C.c_void_p, C.c_void_p]
         L.vpd_enable_diagnostics.argtypes = [P, C.POINTER(DiagConfig)]
         L.vpd_fetch_diagnostics.argtypes = [P, _D, C.c_int, C.POINTER(C.c_int)]
         L.vpd_set_return_map.argtypes = [P, C.c_int, C.c_double]
@@ -228,6 +230,13 @@ def select_boundary_region(x, side, width):
     if count < 0:
         raise VpdError("boundary-region selection failed for side %r, width %r" % (side, width))
     return mask.astype(bool)
+
+
+def run_cases_json(case_paths, plugin_directory=""):
+    """vpd_run_cases_json: several boundary cases as one batch on the GPU; returns the exit
+    status (0 all pass, 2 a material failed, 1 error)."""
+    arr = (C.c_char_p * len(case_paths))(*[str(p).encode() for p in case_paths])
+    return lib().vpd_run_cases_json(arr, len(case_paths), str(plugin_directory).encode())
 
 
 def run_case_json(case_path, plugin_directory=""):
