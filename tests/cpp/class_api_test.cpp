@@ -272,10 +272,77 @@ void material() {
   require(throws([&] { VumatPdCorrespondenceMaterial x(bad); }, "spatial"),
           "spatial stress frame rejected");
   bad = jc(1.0);
+  bad.material.cmname = "PROBE";
+  bad.material.pluginPath = "contract_probe_plugin.so";
+  require(throws([&] { VumatPdCorrespondenceMaterial x(bad); }, "Johnson-Cook and elastic"),
+          "a routine the device does not hold is rejected");
+  bad = jc(1.0);
   bad.material.cmname = "ELASTIC";
   bad.material.pluginPath = "elastic_vumat_plugin.so";
-  require(throws([&] { VumatPdCorrespondenceMaterial x(bad); }, "Johnson-Cook"),
-          "non-JC routine rejected");
+  bad.material.props = {210000.0};
+  require(throws([&] { VumatPdCorrespondenceMaterial x(bad); }, "2 props"),
+          "elastic routine without nu rejected");
+}
+
+// The elastic routine (materials/examples/Elastic.f) through the same class, as the
+// reference's adapter runs any plugin its config names (VumatMaterialAdapter.cpp:27-75).
+// Hooke's law on the Flanagan-Taylor increment: the corotational stress grows by
+// λ tr(dε) I + 2μ dε with dε = D̄·dt, and the spatial stress is R σ̄ Rᵀ; the state,
+// the temperature and the energies are carried unchanged. A general (non-symmetric) F makes
+// every shear component distinct, so a misordered Voigt slot fails. The same increments,
+// small enough to stay elastic, through the Johnson-Cook routine give the same stress.
+void elasticMaterial() {
+  const double E = 210000.0, nu = 0.30, dt = 2.5e-5;
+  const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+  VumatPdCorrespondenceConfig c;
+  c.material.cmname = "ELASTIC";
+  c.material.pluginPath = "elastic_vumat_plugin.so";
+  c.material.nstatev = 1;
+  c.material.props = {E, nu};
+  c.material.density = 7800;
+  c.initialState = {4.5};
+  VumatPdCorrespondenceConfig j = c;
+  j.material.cmname = "JOHNSONCOOK";
+  j.material.pluginPath = "jc_vumat_plugin.so";
+  j.material.nstatev = 6;
+  j.material.props = {E, nu, 350.0, 500.0, 0.025, 0.36, 1.0, 1.0, 293.0, 1793.0, 7800.0,
+                      0.9, 500.0, 1.0};
+  j.initialState = {};
+  VumatPdCorrespondenceMaterial el(c), jcm(j);
+  const std::vector<Tensor3> path = {
+      eye(),
+      {1 + 1.0e-4, 2.0e-5, -1.5e-5, 3.5e-5, 1 - 3.0e-5, 0.5e-5, -2.5e-5, 4.0e-5, 1 + 2.0e-5},
+      {1 + 1.8e-4, 3.0e-5, -1.0e-5, 6.5e-5, 1 - 5.0e-5, 1.5e-5, -4.0e-5, 7.0e-5, 1 + 3.0e-5}};
+  MaterialState se, sj;
+  el.initialize({path[0]}, se);
+  jcm.initialize({path[0]}, sj);
+  jcm.update({path[0]}, sj, dt, 0, 0); // the JC routine's first call is its init branch
+  Tensor3 sbar{}, Vn = eye(), Rn = eye();
+  for (int step = 1; step <= 2; ++step) {
+    const auto kin = updateCorotationalKinematics(path[step - 1], path[step], Vn, Rn, dt);
+    double tr = 0;
+    for (int a = 0; a < 3; ++a)
+      tr += kin.unrotatedRateOfDeformation[4 * a] * dt;
+    for (int q = 0; q < 9; ++q)
+      sbar[q] += 2 * mu * kin.unrotatedRateOfDeformation[q] * dt + (q % 4 == 0 ? lam * tr : 0);
+    const Tensor3 expect = rotateCorotationalStressToSpatial(sbar, kin.rotationNP1);
+    Vn = kin.leftStretchNP1;
+    Rn = kin.rotationNP1;
+    const auto out = el.update({path[step]}, se, dt, step * dt, step * dt);
+    double scale = 0;
+    for (double v : expect)
+      scale = std::max(scale, std::abs(v));
+    require(maxDiff(out.rawStress[0], expect) <= 1e-10 * scale, "elastic stress, Hooke's law");
+    require(maxDiff(se.rawStressN[0], sbar) <= 1e-10 * scale, "elastic corotational history");
+    require(se.vumatStateN[0].size() == 1 && se.vumatStateN[0][0] == 4.5, "state carried");
+    require(se.temperatureN[0] == 293.0 && out.internalEnergy[0] == 0.0 &&
+                out.inelasticEnergy[0] == 0.0,
+            "temperature and energies carried");
+    const auto oj = jcm.update({path[step]}, sj, dt, step * dt, step * dt);
+    require(sj.vumatStateN[0][1] == 0.0, "the JC increments stay elastic");
+    require(maxDiff(oj.rawStress[0], out.rawStress[0]) <= 1e-12 * scale,
+            "elastic routine equals the JC routine below yield");
+  }
 }
 } // namespace
 
@@ -283,7 +350,8 @@ int main() {
   const std::pair<const char *, std::function<void()>> tests[] = {
       {"neighbor lattice", neighborLattice}, {"correspondence sampling", correspondenceSampling},
       {"corotational", corotational},         {"outer damage", outerDamage},
-      {"integrator", integrator},             {"material", material}};
+      {"integrator", integrator},             {"material", material},
+      {"elastic material", elasticMaterial}};
   for (const auto &t : tests) {
     std::printf("[class-api] %s\n", t.first);
     std::fflush(stdout);

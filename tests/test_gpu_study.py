@@ -1,7 +1,8 @@
-"""The time-step study on the GPU runner (vumat-pd_amd/study.py) against the same three runs
-of the reference's own runner (oracle/_ref, on this host's CPU): the study completes with
-the reference study's keys, and its coarse/fine comparisons agree with those computed from
-the reference runs' outputs."""
+"""The time-step study (vumat-pd_amd/study.py), its three time steps batched on the GPU,
+against the same three runs of the reference's own runner (oracle/_ref, on this host's CPU):
+the study completes with the reference study's keys, each variant's outputs equal those of the
+same case file run alone on the GPU byte for byte, and the coarse/fine comparisons agree with
+those computed from the reference runs' outputs."""
 import json
 import os
 import sys
@@ -43,20 +44,25 @@ def test_time_step_study_vs_reference_runs(vpd, ref, tmp_path):
     assert [r["label"] for r in report["runs"]] == ["dt_x2", "dt_x1", "dt_x0p5"]
     assert [r["steps"] for r in report["runs"]] == [120, 240, 480]
     assert all(r["returncode"] == 0 and r["result_sha256"] for r in report["runs"])
-    # The same three runs through the reference's runner, compared with the same metrics.
+    assert (tmp_path / "study" / "batch.log").is_file()
+    # Batched == alone, and the same three runs through the reference's runner.
     results = []
     for r in report["runs"]:
-        sel = json.loads((tmp_path / "study" / (r["label"] + ".json")).read_text())
+        case_file = tmp_path / "study" / (r["label"] + ".json")
+        sel = json.loads(case_file.read_text())
+        sel["output"]["directory"] = str(tmp_path / "alone" / r["label"])
+        alone = tmp_path / ("alone_" + r["label"] + ".json")
+        alone.write_text(json.dumps(sel))
+        assert vpd.run_case_json(str(alone)) == 0
+        for f in ("timeseries.csv", "field_snapshots.csv"):
+            assert (tmp_path / "study" / r["label"] / f).read_bytes() == \
+                (tmp_path / "alone" / r["label"] / f).read_bytes(), (r["label"], f)
         sel["output"]["directory"] = str(tmp_path / "ref" / r["label"])
         path = tmp_path / ("ref_" + r["label"] + ".json")
         path.write_text(json.dumps(sel))
         assert ref.ref().ref_run_case_json(os.fsencode(str(path)),
                                            os.fsencode(ref.REF_DIR)) == 0
-        out = sel["output"]["directory"]
-        results.append({"dt_ms": r["dt_ms"],
-                        "history": study.csv_rows(os.path.join(out, "timeseries.csv")),
-                        "fields": {x["particle"]: x for x in
-                                   study.csv_rows(os.path.join(out, "field_snapshots.csv"))}})
+        results.append(study.VariantRun.load(sel["output"]["directory"], r["dt_ms"]))
     expect = [study.compare(results[i], results[i + 1], 0.6) for i in range(2)]
     for got, want in zip(report["comparisons"], expect):
         assert got["common_history_times"] == want["common_history_times"]

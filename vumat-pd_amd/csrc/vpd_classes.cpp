@@ -268,14 +268,22 @@ void ExplicitTimeIntegrator::correctVelocity(ExplicitDynamicState &state,
 // ------------------------------------------------------------------ material
 
 namespace {
-bool namesJohnsonCook(const VumatMaterialConfig &m) {
+// The device routine a material config names, by its plugin file or cmname; -1 for any
+// other. The reference's adapter loads whichever plugin the config names
+// (VumatMaterialAdapter.cpp:27-75); the device holds the two example routines
+// (materials/examples/JohnsonCook.f90, Elastic.f).
+int deviceMaterial(const VumatMaterialConfig &m) {
   std::string plugin = m.pluginPath;
   const size_t slash = plugin.find_last_of("/\\");
   if (slash != std::string::npos)
     plugin = plugin.substr(slash + 1);
   std::string cm = m.cmname;
   std::transform(cm.begin(), cm.end(), cm.begin(), ::toupper);
-  return plugin.rfind("jc_vumat_plugin", 0) == 0 || cm == "JOHNSONCOOK";
+  if (plugin.rfind("jc_vumat_plugin", 0) == 0 || cm.rfind("JOHNSONCOOK", 0) == 0)
+    return VPD_MATERIAL_JOHNSON_COOK;
+  if (plugin.rfind("elastic_vumat_plugin", 0) == 0 || cm.rfind("ELASTIC", 0) == 0)
+    return VPD_MATERIAL_ELASTIC;
+  return -1;
 }
 } // namespace
 
@@ -294,8 +302,10 @@ VumatPdCorrespondenceMaterial::VumatPdCorrespondenceMaterial(VumatPdCorresponden
     throw std::runtime_error("Unsupported tensor convention.");
   if (config_.material.stressFrame != "corotational" && config_.material.stressFrame != "spatial")
     throw std::runtime_error("Unsupported stress frame.");
-  if (!namesJohnsonCook(config_.material))
-    throw std::runtime_error("The GPU material path implements the Johnson-Cook VUMAT only.");
+  const int model = deviceMaterial(config_.material);
+  if (model < 0)
+    throw std::runtime_error(
+        "The GPU material path implements the Johnson-Cook and elastic VUMATs only.");
   if (config_.kinematicsMode != KinematicsMode::Corotational)
     throw std::runtime_error("Only corotational kinematics is implemented by this host.");
   if (config_.material.stressFrame != "corotational")
@@ -305,8 +315,11 @@ VumatPdCorrespondenceMaterial::VumatPdCorrespondenceMaterial(VumatPdCorresponden
   if (!config_.initialState.empty() &&
       static_cast<int>(config_.initialState.size()) != config_.material.nstatev)
     throw std::runtime_error("Initial VUMAT state size does not match nstatev.");
-  if (config_.material.props.size() < 14 || config_.material.nstatev < 6)
+  if (model == VPD_MATERIAL_JOHNSON_COOK &&
+      (config_.material.props.size() < 14 || config_.material.nstatev < 6))
     throw std::runtime_error("JohnsonCook VUMAT requires 14 props and 6 states");
+  if (model == VPD_MATERIAL_ELASTIC && config_.material.props.size() < 2)
+    throw std::runtime_error("The elastic VUMAT requires 2 props (E, nu).");
 }
 
 void VumatPdCorrespondenceMaterial::initialize(const std::vector<Tensor3> &f0,
@@ -350,6 +363,7 @@ MaterialUpdateResult VumatPdCorrespondenceMaterial::update(const std::vector<Ten
     if ((int)row.size() != nst)
       throw std::runtime_error("stateOld entry size does not match nstatev.");
   vpd_config cfg = geometryConfig(1.0);
+  cfg.material_model = deviceMaterial(config_.material);
   cfg.nprops = (int)config_.material.props.size();
   cfg.props = config_.material.props.data();
   cfg.nstatev = nst;
@@ -372,7 +386,8 @@ MaterialUpdateResult VumatPdCorrespondenceMaterial::update(const std::vector<Ten
   ctx.ok(vpd_upload(ctx.get(), VPD_FIELD_LEFT_STRETCH_N, flat(state.leftStretchN)));
   ctx.ok(vpd_upload(ctx.get(), VPD_FIELD_ROTATION_N, flat(state.rotationN)));
   ctx.ok(vpd_upload(ctx.get(), VPD_FIELD_STRESS_N, flat(state.rawStressN)));
-  ctx.ok(vpd_upload(ctx.get(), VPD_FIELD_STATEV, sv.data()));
+  if (nst > 0)
+    ctx.ok(vpd_upload(ctx.get(), VPD_FIELD_STATEV, sv.data()));
   ctx.ok(vpd_upload(ctx.get(), VPD_FIELD_TEMPERATURE, state.temperatureN.data()));
   ctx.ok(vpd_upload(ctx.get(), VPD_FIELD_INTERNAL_ENERGY, state.internalEnergyN.data()));
   ctx.ok(vpd_upload(ctx.get(), VPD_FIELD_INELASTIC_ENERGY, state.inelasticEnergyN.data()));
@@ -394,7 +409,8 @@ MaterialUpdateResult VumatPdCorrespondenceMaterial::update(const std::vector<Ten
   ctx.ok(vpd_download(ctx.get(), VPD_FIELD_INTERNAL_ENERGY, out.internalEnergy.data()));
   ctx.ok(vpd_download(ctx.get(), VPD_FIELD_INELASTIC_ENERGY, out.inelasticEnergy.data()));
   ctx.ok(vpd_download(ctx.get(), VPD_FIELD_STRESS_N, flat(state.rawStressN)));
-  ctx.ok(vpd_download(ctx.get(), VPD_FIELD_STATEV, sv.data()));
+  if (nst > 0)
+    ctx.ok(vpd_download(ctx.get(), VPD_FIELD_STATEV, sv.data()));
   ctx.ok(vpd_download(ctx.get(), VPD_FIELD_TEMPERATURE, state.temperatureN.data()));
   state.deformationGradientN = f;
   state.leftStretchN = out.leftStretch;
