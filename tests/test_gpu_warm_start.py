@@ -209,6 +209,22 @@ def any_network(vpd):
     return net
 
 
+@pytest.mark.parametrize("mode", ["BISECTION", "BISECTION_TIGHT", "NEWTON"])
+def test_material_compacted_equals_fused(vpd, mode):
+    # Without run diagnostics the material stage is compacted (k_material_screen finishes
+    # the points whose return map ends at its first iteration, reusing that iteration's
+    # yield, and lists the rest for k_material_return_list); with them every point goes
+    # through the one fused kernel. Both give the same bits.
+    banner("%s return map: compacted == fused" % mode.lower())
+    m = getattr(vpd, "RETURN_MAP_" + mode)
+    a = plastic_bar(vpd, m, diag=True)
+    b = plastic_bar(vpd, m)
+    assert (b.get("statev")[:, 4] > 0).sum() > 100  # plastic flow on this step
+    for name in ("u", "v", "a", "stressN", "statev", "temp", "eint", "einel", "damage"):
+        assert np.array_equal(a.get(name), b.get(name)), name
+    assert np.array_equal(a.get("iterations"), b.get("iterations"))
+
+
 def test_learned_step_compacted_equals_split(vpd):
     # The learned update runs compacted (screen → network and Newton on the plastic list)
     # unless run diagnostics are on, which take the uncompacted split path. Both give the

@@ -922,6 +922,32 @@ int ensure_list(vpd_context *c) {
   return VPD_OK;
 }
 
+// The compacted material update (k_material_screen, then the return map on the listed
+// plastic points only; see vpd_kernels.hpp): the network seeds the list first in the learned
+// mode.
+template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g, int grid) {
+  if (int s = ensure_list(c))
+    return s;
+  const CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n)};
+  if (int s = timed(c, K_MATERIAL, [&] {
+        k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
+        k_material_screen<MODE><<<grid, 256, 0, c->stream>>>(g, cl);
+      }))
+    return s;
+  if (MODE == RM_LEARNED) {
+    if (int s = timed(c, K_MLP_SEED, [&] {
+          k_mlp_seed<SEED_SLOTS><<<mlp_grid(c->n), 256, 0, c->stream>>>(
+              0, c->listCount, c->list, c->incList, c->n, c->statev, c->n, c->mlp->theta,
+              c->mlp->norm, c->seed);
+        }))
+      return s;
+    g.seed = c->seed;
+  }
+  return timed(c, K_RETURN_MAP, [&] {
+    k_material_return_list<MODE><<<grid_of(c->n), 256, 0, c->stream>>>(g, cl);
+  });
+}
+
 template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
   const int grid = (int)(ceil_div(g.nblocks, 8) * 8); // grid8_of(n), or the diagnostics' chunks
   if (c->materialModel == VPD_MATERIAL_ELASTIC) {
@@ -930,42 +956,36 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
     return timed(c, K_MATERIAL,
                  [&] { k_material<RM_ELASTIC, false, false><<<grid, 256, 0, c->stream>>>(g); });
   }
+  if (c->returnMap == RM_LEARNED && !c->mlp)
+    return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
+  if (!DIAG) {
+    switch (c->returnMap) {
+    case RM_BISECTION:
+      return launch_material_compact<RM_BISECTION>(c, g, grid);
+    case RM_BISECTION_TIGHT:
+      return launch_material_compact<RM_BISECTION_TIGHT>(c, g, grid);
+    case RM_NEWTON:
+      return launch_material_compact<RM_NEWTON>(c, g, grid);
+    default:
+      return launch_material_compact<RM_LEARNED>(c, g, grid);
+    }
+  }
+  // With run diagnostics each workgroup also reduces its chunk's material terms: one pass
+  // over every point.
   switch (c->returnMap) {
   case RM_BISECTION:
     return timed(c, K_MATERIAL,
-                 [&] { k_material<RM_BISECTION, false, DIAG><<<grid, 256, 0, c->stream>>>(g); });
+                 [&] { k_material<RM_BISECTION, false, true><<<grid, 256, 0, c->stream>>>(g); });
   case RM_BISECTION_TIGHT:
     return timed(c, K_MATERIAL, [&] {
-      k_material<RM_BISECTION_TIGHT, false, DIAG><<<grid, 256, 0, c->stream>>>(g);
+      k_material<RM_BISECTION_TIGHT, false, true><<<grid, 256, 0, c->stream>>>(g);
     });
   case RM_NEWTON:
     return timed(c, K_MATERIAL,
-                 [&] { k_material<RM_NEWTON, false, DIAG><<<grid, 256, 0, c->stream>>>(g); });
+                 [&] { k_material<RM_NEWTON, false, true><<<grid, 256, 0, c->stream>>>(g); });
   default: {
-    if (!c->mlp)
-      return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
-    if (!DIAG) { // screen, then the network and the Newton solve on the plastic points only
-      if (int s = ensure_list(c))
-        return s;
-      const CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n)};
-      if (int s = timed(c, K_MATERIAL, [&] {
-            k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
-            k_material_screen<RM_LEARNED><<<grid, 256, 0, c->stream>>>(g, cl);
-          }))
-        return s;
-      if (int s = timed(c, K_MLP_SEED, [&] {
-            k_mlp_seed<SEED_SLOTS><<<mlp_grid(c->n), 256, 0, c->stream>>>(
-                0, c->listCount, c->list, c->incList, c->n, c->statev, c->n, c->mlp->theta,
-                c->mlp->norm, c->seed);
-          }))
-        return s;
-      g.seed = c->seed;
-      return timed(c, K_RETURN_MAP, [&] {
-        k_material_return_list<RM_LEARNED><<<grid_of(c->n), 256, 0, c->stream>>>(g, cl);
-      });
-    }
     if (int s = timed(c, K_MATERIAL,
-                      [&] { k_material<RM_LEARNED, true, DIAG><<<grid, 256, 0, c->stream>>>(g); }))
+                      [&] { k_material<RM_LEARNED, true, true><<<grid, 256, 0, c->stream>>>(g); }))
       return s;
     if (int s = timed(c, K_MLP_SEED, [&] {
           k_mlp_seed<SEED_ALL><<<mlp_grid(c->n), 256, 0, c->stream>>>(
@@ -975,7 +995,7 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
       return s;
     g.seed = c->seed;
     return timed(c, K_RETURN_MAP, [&] {
-      k_material_return<RM_LEARNED, DIAG><<<grid, 256, 0, c->stream>>>(g);
+      k_material_return<RM_LEARNED, true><<<grid, 256, 0, c->stream>>>(g);
     });
   }
   }
@@ -2008,12 +2028,11 @@ int vpd_run_specimens(vpd_context *c, int32_t firstStep, int32_t count) {
   if (c->predicted && firstStep != c->batch.step + 1)
     return fail(c, VPD_ERR_STATE, "The fused prediction was made for step " +
                                       std::to_string(c->batch.step + 1) + ".");
-  if (c->returnMap == RM_LEARNED) {
-    if (!c->mlp)
-      return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
+  if (c->returnMap == RM_LEARNED && !c->mlp)
+    return fail(c, VPD_ERR_STATE, "The learned return map needs a warm-start network.");
+  if (c->materialModel != VPD_MATERIAL_ELASTIC)
     if (int s = ensure_list(c)) // no allocation may happen while the step is captured
       return s;
-  }
   if (int s = prepare_plans(c)) // nor a plan build
     return s;
   c->batch.step = firstStep;

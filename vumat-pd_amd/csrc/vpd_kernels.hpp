@@ -1434,13 +1434,15 @@ __global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
   diag_block_store<D_NET_X, 0>(dv, g.diag.partials + blk * DIAG_STRIDE);
 }
 
-// ---- the compacted material update of the learned return map ----------------------------
-// Most points of a step are elastic, and the warm-start network only helps the plastic ones.
-// k_material_screen runs the fused update for every point whose return map would end at its
-// first iteration and appends the others to a list (one atomic per wavefront); the network
-// and the Newton solve then run on that dense list only (k_mlp_seed_list,
-// k_material_return_list). Each point's arithmetic is that of the fused kernel, so the
-// result does not depend on the list order.
+// ---- the compacted material update -----------------------------------------------------
+// Most points of a step are elastic, and the plastic ones sit scattered through the block
+// (about one lane in five of a wavefront in the necking bar): in k_material every wavefront
+// with one plastic lane runs the whole return-map loop. k_material_screen runs the fused
+// update for every point whose return map ends at its first iteration (reusing that
+// iteration's yield) and appends the others to a list (one atomic per workgroup); the return
+// map (and, in the learned mode, the network first: k_mlp_seed) then runs on that dense
+// list only (k_material_return_list). Each point's arithmetic is that of the fused kernel,
+// so the result is bit for bit the same and does not depend on the list order.
 struct CompactArgs {
   int *count;   // number of listed points (zeroed before the screen)
   int *list;    // particle index per slot
@@ -1470,6 +1472,7 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
   T3 f1, r1;
   double inc[6], sOld[6], st[6];
   bool plastic = false;
+  JcFirst first{};
   if (live) {
     T3 f0, v0, r0;
 #pragma unroll
@@ -1505,8 +1508,18 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
         sOld[c] = g.sig(c, i);
         st[c] = g.statev(c, i);
       }
-      plastic = !jc_elastic(g.jc, inc, sOld, st, g.rtol);
+      plastic = !jc_elastic<MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT,
+                            MODE == RM_BISECTION_TIGHT>(g.jc, inc, sOld, st, g.rtol, &first);
     }
+  }
+  // The elastic points finish before the workgroup meets at the append, so that no
+  // wavefront waits there with its update still to do.
+  if (live && !plastic) {
+    double sNew[6];
+    double eint = g.eint[i], einel = g.einel[i];
+    ReturnMapStats stats;
+    jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats, &first);
+    material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t mask = __ballot(plastic);
@@ -1529,15 +1542,7 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
 #pragma unroll
     for (int c = 0; c < 6; ++c)
       cl.inc(c, slot) = inc[c];
-    return;
   }
-  if (!live)
-    return;
-  double sNew[6];
-  double eint = g.eint[i], einel = g.einel[i];
-  ReturnMapStats stats;
-  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
-  material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
 }
 
 template <int MODE>

@@ -416,12 +416,21 @@ __device__ __forceinline__ void jc_yield_slope(const JcParams &p, double epOld, 
   dY = soft * (dHard * rfac + (dep > 0.0 ? hard * p.rateC / dep : 0.0));
 }
 
-// Whether jc_point<MODE> (MODE ≠ RM_BISECTION) ends at its first iteration, Δε̄ᵖ = 0: the
-// init branch, or a trial stress that does not exceed the current yield stress by more than
-// the solve tolerance. Evaluates exactly the expressions jc_point's first iteration does.
+// The first iteration's thermal factor 1 − θᵐ and yield stress (the max with the stored
+// one), as jc_elastic evaluates them; jc_point takes them instead of evaluating them again.
+struct JcFirst {
+  double soft, yield;
+};
+
+// Whether jc_point<MODE> ends at its first iteration, Δε̄ᵖ = 0: the init branch, or a trial
+// stress that leaves the yield stress at Δ = 0 as the solve's first iteration accepts it.
+// The bisections (BISECT; TIGHT for RM_BISECTION_TIGHT) end there when |r| < tol or r < 0
+// (JohnsonCook.f90:119-127), the Newton forms when r does not exceed rtol·q. Evaluates
+// exactly the expressions jc_point's first iteration does and returns them in `first`.
+template <bool BISECT = false, bool TIGHT = false>
 __device__ __forceinline__ bool jc_elastic(const JcParams &p, const double inc[6],
                                            const double stressOld[6], const double state[6],
-                                           double rtol) {
+                                           double rtol, JcFirst *first = nullptr) {
   if (state[0] == 0.0)
     return true;
   double trial[6];
@@ -443,20 +452,30 @@ __device__ __forceinline__ bool jc_elastic(const JcParams &p, const double inc[6
   const double qTrial =
       sqrt(1.5 * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
                   2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5])));
-  const double yield = fmax(jc_yield(p, state[1], 0.0, state[2]), state[3]);
+  const double soft = jc_softening(p, state[2]);
+  const double yield = fmax(jc_yield_soft(p, state[1], 0.0, soft), state[3]);
+  if (first) {
+    first->soft = soft;
+    first->yield = yield;
+  }
   const double r0 = fmax(0.0, qTrial) - yield;
+  if (BISECT) // at Δ = 0 the bisection's residual is q − Y (qNew = max(0, q))
+    return fabs(r0) < (TIGHT ? rtol * qTrial : p.tolerance) || r0 < 0.0;
   return !(r0 > 0.0 && fabs(r0) >= rtol * qTrial);
 }
 
 // One material point of JohnsonCook.f90:74-156. in: stressOld[6], strainInc[6], state[6]
 // (in/out), energies (in/out); writes stressNew[6]; returns the iteration count (0 for the
 // init branch). The bookkeeping after the solve is the reference's for every solver.
+// `first`, when given, holds jc_elastic's evaluation of the first iteration's 1 − θᵐ and
+// yield for the same state (the same bits jc_point would compute).
 template <int MODE>
 __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const double inc[6],
                                         const double stressOld[6], double state[6],
                                         double &eint, double &einel, double stressNew[6],
                                         double seed = 0.0, double rtol = 1.0e-12,
-                                        ReturnMapStats *stats = nullptr) {
+                                        ReturnMapStats *stats = nullptr,
+                                        const JcFirst *first = nullptr) {
   double trial[6];
   const double traceInc = inc[0] + inc[1] + inc[2];
 #pragma unroll
@@ -497,7 +516,7 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
     constexpr bool SCREEN = MODE != RM_BISECTION_UNSCREENED;
     const double tol = MODE == RM_BISECTION_TIGHT ? rtol * qTrial : p.tolerance;
     const int maxIt = MODE == RM_BISECTION_TIGHT ? 200 : 50;
-    const double soft = jc_softening(p, temperature); // the same at every iteration
+    const double soft = first ? first->soft : jc_softening(p, temperature); // every iteration
     const double rateScale = SCREEN ? 1.0 / (dt * p.rate0) : 0.0;
     double epLower = 0.0, epUpper = qTrial / (2.0 * p.mu);
     // JohnsonCook.f90:119-136: iterations 1..maxIt; iteration k evaluates the yield at the
@@ -535,8 +554,12 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
         }
       }
       const double epRate = epInc / dt;
-      yield = jc_yield_soft(p, epOld + epInc, epRate, soft);
-      yield = fmax(yield, oldYield);
+      if (first && iteration == 1) { // Δ = 0: ε̄ + 0 = ε̄ and the rate is ε̇₀, as in jc_elastic
+        yield = first->yield;
+      } else {
+        yield = jc_yield_soft(p, epOld + epInc, epRate, soft);
+        yield = fmax(yield, oldYield);
+      }
       const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
       const double residual = qNew - yield;
       if (fabs(residual) < tol || (epInc == 0.0 && residual < 0.0)) {
@@ -560,8 +583,8 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
   } else {
     // Iteration 1 tests Δ = 0 exactly as the reference does (elastic exit).
     iteration = 1;
-    const double soft = jc_softening(p, temperature);
-    yield = fmax(jc_yield_soft(p, epOld, 0.0, soft), oldYield);
+    const double soft = first ? first->soft : jc_softening(p, temperature);
+    yield = first ? first->yield : fmax(jc_yield_soft(p, epOld, 0.0, soft), oldYield);
     const double r0 = fmax(0.0, qTrial) - yield;
     if (r0 > 0.0 && fabs(r0) >= rtol * qTrial) {
       double lo = 0.0, hi = qTrial / (3.0 * p.mu);
