@@ -9,8 +9,12 @@ safeguarded Newton finish to |r| ≤ 1e-12·q, bisection fallback), 10000 steps.
 
 Every `log_every` steps one unfused step keeps the fields, and the log records the plastic
 points of that step, their mean return-map iterations, how many needed a bisection fallback
-inside the Newton solve, peak ε̄ᵖ and temperature rise, broken bonds and the particles whose
-bonds are all broken (those `deletionEnabled` would flag, PdSimpleDamage.cpp:81-84).
+inside the Newton solve, peak ε̄ᵖ and temperature rise, the mean damage index (the broken
+fraction of each particle's bonds), the particles whose bonds are all broken (those
+`deletionEnabled` would flag, PdSimpleDamage.cpp:81-84), and the chunk's own throughput.
+The summary splits the run at the first logged step whose mean damage index reaches 1%:
+the intact phase before it (every bond pass sums nearly every bond) and the fragmenting
+phase after it (the passes skip more and more broken bonds), each with its throughput.
 
     python impact.py [--steps 10000] [--log-every 500] [--return-map learned|newton|bisection]
 """
@@ -72,20 +76,42 @@ def log_record(bar, step, tref=293.0):
 
 def run(bar, steps, log_every):
     """`steps` steps in chunks of log_every (fused inside a chunk, its last step unfused and
-    logged). Returns (seconds, log)."""
+    logged). Returns (seconds, log); logging downloads are not timed."""
     log = []
-    bar.ctx.synchronize()
-    t0 = time.perf_counter()
+    seconds = 0.0
     done = 0
     while done < steps:
         k = min(log_every, steps - done)
-        bar.step(k, want_fields=True)
-        done += k
         bar.ctx.synchronize()
-        t_pause = time.perf_counter()
-        log.append(log_record(bar, done))
-        t0 += time.perf_counter() - t_pause  # logging downloads are not timed
-    return time.perf_counter() - t0, log
+        t0 = time.perf_counter()
+        bar.step(k, want_fields=True)
+        bar.ctx.synchronize()
+        dt = time.perf_counter() - t0
+        seconds += dt
+        done += k
+        rec = log_record(bar, done)
+        rec.update(chunk_steps=k, chunk_seconds=dt, chunk_particle_steps_per_s=bar.n * k / dt)
+        log.append(rec)
+    return seconds, log
+
+
+INTACT_DAMAGE = 0.01  # mean damage index that ends the intact phase
+
+
+def phases(log, n):
+    """Throughput of the intact phase (chunks ending below INTACT_DAMAGE mean damage index)
+    and of the rest of the run."""
+    out = {}
+    split = next((k for k, r in enumerate(log) if r["mean_damage_index"] >= INTACT_DAMAGE),
+                 len(log))
+    for name, part in (("intact", log[:split]), ("fragmenting", log[split:])):
+        steps = sum(r["chunk_steps"] for r in part)
+        secs = sum(r["chunk_seconds"] for r in part)
+        out[name] = dict(steps=steps, last_step=part[-1]["step"] if part else None,
+                         seconds=secs, particle_steps_per_s=n * steps / secs if secs else None,
+                         plastic_points_at_end=part[-1]["plastic_points"] if part else None)
+    out["split_mean_damage_index"] = INTACT_DAMAGE
+    return out
 
 
 def main(argv=None):
@@ -110,7 +136,7 @@ def main(argv=None):
                velocity_rise_ms=a.rise,
                bonds=bar.ctx.nnz, steps=a.steps, return_map=a.return_map, setup_s=setup,
                seconds=seconds, particle_steps_per_s=bar.n * a.steps / seconds,
-               launch_tuning=tuned,
+               launch_tuning=tuned, phases=phases(log, bar.n),
                logged_fallback_rate=(sum(r["fallback_points"] for r in log) / plastic
                                      if plastic else 0.0),
                log=log)
