@@ -434,9 +434,16 @@ int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *p
 /* ---- the return-map warm start (no reference counterpart) -------------------------- */
 /* An 8 → 64 → 64 → 64 → 1 ReLU network mapping (6 Voigt strain increments, ε̄ᵖ, T) to a
  * Δε̄ᵖ estimate that seeds the Newton return map (vpd_return_map_device mode 3). Trained on
- * the GPU in FP32 (forward, backward, Adam); inference runs as bf16 MFMA with FP32
- * accumulation. theta: 8961 FP32 values, W1 (64×8), b1, W2 (64×64), b2, W3 (64×64), b3,
- * W4 (1×64), b4; norm: input mean[8], std[8], output mean, std. */
+ * the GPU (forward, backward, Adam; FP32 master weights and optimiser state) with bf16 MFMA
+ * products and FP32 accumulation by default, or with plain FP32 kernels
+ * (vpd_mlp_set_training_form); inference runs as bf16 MFMA with FP32 accumulation.
+ * theta: 8961 FP32 values, W1 (64×8), b1, W2 (64×64), b2, W3 (64×64), b3, W4 (1×64), b4;
+ * norm: input mean[8], std[8], output mean, std. Gradient sums run in a fixed order, so a
+ * training run is deterministic in either form. */
+enum vpd_mlp_training_form {
+  VPD_MLP_TRAIN_FP32 = 0, /* one thread per output, FP32 arithmetic throughout */
+  VPD_MLP_TRAIN_MFMA = 1  /* bf16 operands on the matrix cores, FP32 accumulation (default) */
+};
 int vpd_mlp_create(int max_batch, vpd_mlp **out);
 void vpd_mlp_destroy(vpd_mlp *mlp);
 const char *vpd_mlp_last_error(const vpd_mlp *mlp);
@@ -448,6 +455,8 @@ int vpd_mlp_loss_and_gradient(vpd_mlp *mlp, const float *x, const float *t, int 
 int vpd_mlp_set_dataset(vpd_mlp *mlp, const float *x, const float *t, int64_t n, uint64_t seed);
 /* `steps` Adam steps on minibatches of the dataset; *last_loss = loss of the last batch. */
 int vpd_mlp_train(vpd_mlp *mlp, int steps, int batch, float lr, float *last_loss);
+/* The arithmetic of vpd_mlp_loss_and_gradient and vpd_mlp_train (enum vpd_mlp_training_form). */
+int vpd_mlp_set_training_form(vpd_mlp *mlp, int form);
 /* Raw network outputs (normalised units) for normalised inputs, through the MFMA kernel. */
 int vpd_mlp_predict(vpd_mlp *mlp, const float *x, int64_t n, float *y);
 /* Return-map seeds Δε̄ᵖ for n points from device planes: inc[c*inc_stride + i] (c < 6) and

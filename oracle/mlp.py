@@ -93,3 +93,45 @@ def features(strain_inc, eqps, temp, norm):
     x = np.concatenate([np.asarray(strain_inc), np.asarray(eqps)[:, None],
                         np.asarray(temp)[:, None]], 1)
     return (x - norm[0]) / norm[1]
+
+
+def bf16(a):
+    """Round to bfloat16 (round to nearest, ties to even) and back to float64: the top 16
+    bits of the IEEE binary32 pattern after rounding the low 16."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def loss_and_grad_bf16(theta, x, t, rnd=bf16):
+    """loss_and_grad with every matrix-product operand rounded by `rnd` where the GPU's bf16
+    MFMA training step rounds it (sums in FP64 here, FP32 on the GPU): the inputs, hidden
+    activations (after ReLU), the weight matrices W1-W4 of the forward pass and W2, W3 of
+    the backward pass, the pre-activation gradients dz3, dz2, dz1, and dy where it multiplies
+    h3ᵀ and sums into b4. Biases, the output, dy itself and dz3's factor W4 stay unrounded.
+    With rnd = identity it is loss_and_grad."""
+    W1, b1, W2, b2, W3, b3, W4, b4 = unflatten(np.asarray(theta, np.float64))
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    xr = rnd(x)
+    h1 = rnd(np.maximum(f32(xr @ rnd(W1).T + b1), 0.0))
+    h2 = rnd(np.maximum(f32(h1 @ rnd(W2).T + b2), 0.0))
+    h3 = rnd(np.maximum(f32(h2 @ rnd(W3).T + b3), 0.0))
+    y = f32((h3 @ rnd(W4).T)[:, 0] + b4[0])
+    B = x.shape[0]
+    r = y - t
+    loss = np.mean(r * r)
+    dy = f32((2.0 / B) * r)[:, None]
+    dyr = rnd(dy)
+    gW4 = dyr.T @ h3
+    gb4 = dyr.sum(0)
+    dz3 = rnd(f32((dy @ W4) * (h3 > 0)))
+    gW3 = dz3.T @ h2
+    gb3 = dz3.sum(0)
+    dz2 = rnd(f32((dz3 @ rnd(W3)) * (h2 > 0)))
+    gW2 = dz2.T @ h1
+    gb2 = dz2.sum(0)
+    dz1 = rnd(f32((dz2 @ rnd(W2)) * (h1 > 0)))
+    gW1 = dz1.T @ xr
+    gb1 = dz1.sum(0)
+    g = np.concatenate([a.ravel() for a in (gW1, gb1, gW2, gb2, gW3, gb3, gW4, gb4)])
+    return loss, g

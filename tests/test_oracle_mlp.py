@@ -1,6 +1,7 @@
 """Pins the FP64 warm-start network oracle: analytic gradients against central finite
 differences, loss decrease under Adam on a smooth target, the Adam update against its
-closed form on one coordinate."""
+closed form on one coordinate; the bf16 rounding on known values and the bf16-pipeline
+statement against the FP64 gradient with its rounding switched off."""
 import numpy as np
 
 
@@ -58,3 +59,39 @@ def test_adam_closed_form_and_training_progress():
         loss, g = mlp.loss_and_grad(theta, x, t)
         theta, m, v = mlp.adam(theta, g, m, v, step, lr=3e-3)
     assert loss < 0.2 * l0
+
+
+def test_bf16_rounding_known_values():
+    import mlp
+    # bf16 spacing at 1 is 2^-7: 1 + 2^-8 is a tie (to even: 1), just above it rounds up.
+    v = np.array([1.0, 1 + 2.0 ** -8, 1 + 2.0 ** -8 + 2.0 ** -20, 1 + 3 * 2.0 ** -8, -2.5,
+                  3.0e38, 1e-30])
+    r = mlp.bf16(v)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1 + 2.0 ** -7
+    assert r[3] == 1 + 2.0 ** -6  # tie between 1 + 2^-7 and 1 + 2^-6: to the even one
+    assert r[4] == -2.5
+    assert abs(r[5] / 3.0e38 - 1) <= 2.0 ** -8 and abs(r[6] / 1e-30 - 1) <= 2.0 ** -8
+
+
+def test_bf16_pipeline_without_rounding_is_the_network_gradient():
+    # The bf16-pipeline statement with its rounding replaced by the identity keeps only the
+    # FP32 conversions of the GPU step: within FP32 rounding of the FP64 gradient, block by
+    # block (a dropped term or a transposed factor would be off by order 1).
+    import mlp
+    rng = np.random.default_rng(4)
+    theta = mlp.init(2) + 0.02 * rng.standard_normal(mlp.n_params())
+    x = rng.standard_normal((300, 8))
+    t = rng.standard_normal(300)
+    l0, g0 = mlp.loss_and_grad(theta, x, t)
+    l1, g1 = mlp.loss_and_grad_bf16(theta, x.astype(np.float32), t, rnd=lambda a: a)
+    assert abs(l1 - l0) <= 1e-6 * l0
+    o = 0
+    for shape in mlp.shapes():
+        k = int(np.prod(shape))
+        a, b = g1[o:o + k], g0[o:o + k]
+        assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b), shape
+        o += k
+    # With the rounding, the gradient moves by bf16-sized amounts, not more.
+    l2, g2 = mlp.loss_and_grad_bf16(theta, x, t)
+    cos = g2 @ g0 / (np.linalg.norm(g2) * np.linalg.norm(g0))
+    assert 0.99 < cos < 1 - 1e-9

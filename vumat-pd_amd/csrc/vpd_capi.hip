@@ -59,6 +59,13 @@ struct vpd_mlp {
   int *order = nullptr;
   int64_t nData = 0, cursor = 0;
   int adamStep = 0;
+  // bf16 MFMA training (VPD_MLP_TRAIN_MFMA): feature-major layer inputs and gradients over
+  // the padded batch, dy, and the per-chunk gradient partials.
+  int form = VPD_MLP_TRAIN_MFMA;
+  int64_t ld = 0;
+  short *xT = nullptr, *h1T = nullptr, *h2T = nullptr, *h3T = nullptr, *dz1T = nullptr,
+        *dz2T = nullptr, *dz3T = nullptr;
+  float *dyT = nullptr, *gpart = nullptr;
   std::string err;
 };
 
@@ -3247,6 +3254,20 @@ void mlp_forward_backward(vpd_mlp *m, int B) {
   wgrad(MLP_IN, H, m->dz1, m->xb, MLP_OFF_W1, MLP_OFF_B1);
 }
 
+// The bf16 MFMA step over the batch in xb/yb (see vpd_mlp.hpp): the gradient to m->grad and,
+// with `adam`, the Adam update; per-point squared errors to m->sq.
+void mlp_step_mfma(vpd_mlp *m, int B, bool adam, float lr, float c1, float c2) {
+  hipStream_t s = m->stream;
+  const int64_t ld = ceil_div(B, MLP_GRAD_CHUNK) * MLP_GRAD_CHUNK;
+  MlpTrainBufs b{B, ld, m->xb, m->yb, m->xT, m->h1T, m->h2T, m->h3T, m->dz1T, m->dz2T, m->dz3T,
+                 m->dyT, m->sq};
+  const int64_t tiles = ld / 16, chunks = ld / MLP_GRAD_CHUNK;
+  k_mlp_fwd_bwd<<<(int)std::min<int64_t>(ceil_div(tiles, 4), 256), 256, 0, s>>>(b, m->theta);
+  k_mlp_wgrad<<<dim3((unsigned)ceil_div(chunks, 4), 3), 256, 0, s>>>(b, m->gpart);
+  k_mlp_reduce_adam<<<(int)ceil_div(MLP_PARAMS, 256), 256, 0, s>>>(
+      (int)chunks, m->gpart, m->grad, m->theta, m->m, m->v, adam ? 1 : 0, lr, c1, c2);
+}
+
 int mlp_batch_loss(vpd_mlp *m, int B, float *loss) {
   std::vector<float> sq(B);
   MLPTRY(m, hipMemcpyAsync(sq.data(), m->sq, sizeof(float) * B, hipMemcpyDeviceToHost, m->stream));
@@ -3257,6 +3278,16 @@ int mlp_batch_loss(vpd_mlp *m, int B, float *loss) {
   if (loss)
     *loss = (float)(acc / B);
   return VPD_OK;
+}
+
+int mlp_alloc_mfma(vpd_mlp *m) {
+  m->ld = ceil_div(m->maxB, MLP_GRAD_CHUNK) * MLP_GRAD_CHUNK;
+  const int64_t ld = m->ld, H = MLP_H;
+  return malloc_n(m, m->xT, MLP_IN * ld) || malloc_n(m, m->h1T, H * ld) ||
+         malloc_n(m, m->h2T, H * ld) || malloc_n(m, m->h3T, H * ld) ||
+         malloc_n(m, m->dz1T, H * ld) || malloc_n(m, m->dz2T, H * ld) ||
+         malloc_n(m, m->dz3T, H * ld) || malloc_n(m, m->dyT, ld) ||
+         malloc_n(m, m->gpart, ld / MLP_GRAD_CHUNK * MLP_PARAMS);
 }
 
 } // namespace
@@ -3281,7 +3312,8 @@ int vpd_mlp_create(int maxBatch, vpd_mlp **out) {
       malloc_n(m, m->yb, B) || malloc_n(m, m->h1, B * H) || malloc_n(m, m->h2, B * H) ||
       malloc_n(m, m->h3, B * H) || malloc_n(m, m->y, B) || malloc_n(m, m->dy, B) ||
       malloc_n(m, m->dz3, B * H) || malloc_n(m, m->dz2, B * H) || malloc_n(m, m->dz1, B * H) ||
-      malloc_n(m, m->part, chunks * H * (H + 1)) || malloc_n(m, m->sq, B)) {
+      malloc_n(m, m->part, chunks * H * (H + 1)) || malloc_n(m, m->sq, B) ||
+      mlp_alloc_mfma(m)) {
     vpd_mlp_destroy(m);
     return mfail(nullptr, VPD_ERR_DEVICE, "Allocating the warm-start network failed.");
   }
@@ -3312,6 +3344,11 @@ void vpd_mlp_destroy(vpd_mlp *m) {
                     &m->dz1,   &m->part, &m->sq,  &m->xs,  &m->ys};
   for (float **b : bufs)
     dfree(*b);
+  short **sbufs[] = {&m->xT, &m->h1T, &m->h2T, &m->h3T, &m->dz1T, &m->dz2T, &m->dz3T};
+  for (short **b : sbufs)
+    dfree(*b);
+  dfree(m->dyT);
+  dfree(m->gpart);
   dfree(m->order);
   if (m->stream)
     (void)hipStreamDestroy(m->stream);
@@ -3360,7 +3397,10 @@ int vpd_mlp_loss_and_gradient(vpd_mlp *m, const float *x, const float *t, int B,
   MLPTRY(m, hipMemcpyAsync(m->xb, x, sizeof(float) * B * MLP_IN, hipMemcpyHostToDevice,
                            m->stream));
   MLPTRY(m, hipMemcpyAsync(m->yb, t, sizeof(float) * B, hipMemcpyHostToDevice, m->stream));
-  mlp_forward_backward(m, B);
+  if (m->form == VPD_MLP_TRAIN_MFMA)
+    mlp_step_mfma(m, B, false, 0.f, 1.f, 1.f);
+  else
+    mlp_forward_backward(m, B);
   MLPTRY(m, hipGetLastError());
   if (grad)
     MLPTRY(m, hipMemcpyAsync(grad, m->grad, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost,
@@ -3403,16 +3443,29 @@ int vpd_mlp_train(vpd_mlp *m, int steps, int batch, float lr, float *lastLoss) {
                                                                     m->order, m->xs, m->ys, m->xb,
                                                                     m->yb);
     m->cursor = (m->cursor + batch) % m->nData;
-    mlp_forward_backward(m, batch);
     ++m->adamStep;
     const float c1 = 1.f - powf(0.9f, (float)m->adamStep);
     const float c2 = 1.f - powf(0.999f, (float)m->adamStep);
+    if (m->form == VPD_MLP_TRAIN_MFMA) {
+      mlp_step_mfma(m, batch, true, lr, c1, c2);
+      continue;
+    }
+    mlp_forward_backward(m, batch);
     k_mlp_adam<<<(int)ceil_div(MLP_PARAMS, 256), 256, 0, m->stream>>>(MLP_PARAMS, m->theta,
                                                                        m->grad, m->m, m->v, lr, c1,
                                                                        c2);
   }
   MLPTRY(m, hipGetLastError());
   return mlp_batch_loss(m, batch, lastLoss);
+}
+
+int vpd_mlp_set_training_form(vpd_mlp *m, int form) {
+  if (!m)
+    return mfail(m, VPD_ERR_ARGUMENT, "Null network.");
+  if (form != VPD_MLP_TRAIN_FP32 && form != VPD_MLP_TRAIN_MFMA)
+    return mfail(m, VPD_ERR_ARGUMENT, "Unknown training form.");
+  m->form = form;
+  return VPD_OK;
 }
 
 int vpd_mlp_predict(vpd_mlp *m, const float *x, int64_t n, float *y) {

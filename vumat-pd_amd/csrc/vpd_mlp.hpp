@@ -385,6 +385,336 @@ __global__ void __launch_bounds__(256) k_mlp_infer_raw(int64_t n, const float *_
   }
 }
 
+// ------------------------------------------------------------------ training (bf16 MFMA)
+// The training step as three kernels, every product on the matrix cores
+// (v_mfma_f32_16x16x16_bf16: bf16 operands, FP32 accumulation), in the operand layouts of
+// the inference above:
+//  1. k_mlp_fwd_bwd: per 16-point tile and wavefront, the forward pass (as mlp_tile), the
+//     loss gradient dy = 2(y − t)/B and the backward pass down to the first layer's
+//     pre-activation gradient: dz3 = W4ᵀ dy ⊙ [h3 > 0] elementwise, then
+//     dz2 = (W3ᵀ dz3) ⊙ [h2 > 0] and dz1 = (W2ᵀ dz2) ⊙ [h1 > 0] as MFMAs with Wᵀ tiles as
+//     the A operand and the previous gradient's accumulator tile as the B operand (the same
+//     chaining as the forward pass). The layer inputs (x, h1, h2, h3) and gradients (dz1-3)
+//     go to memory as bf16, feature-major ([feature][point]), dy as FP32.
+//  2. k_mlp_wgrad: the weight and bias gradients as GEMMs over the batch, gW = dz·hᵀ with the
+//     points as the MFMA K dimension (feature-major storage makes each operand a 4-point,
+//     8-byte load), the bias gradient as the product with a ones column; one wavefront per
+//     64-point chunk writes its partial gradient.
+//  3. k_mlp_reduce_adam: the partials summed per parameter in chunk order, then (optionally)
+//     the Adam update of k_mlp_adam. The sums run in a fixed order: a training run is
+//     deterministic.
+// Weights reach the wavefronts through LDS: each workgroup converts θ once into the bf16
+// operand tiles (72 tiles of 64 lanes × 4) and the FP32 bias/W4 vectors in the D layout.
+constexpr int MLP_WG_TILES = 72; // w1 4, w2 16, w3 16, w4 4, w2ᵀ 16, w3ᵀ 16
+constexpr int MLP_WG_T_W1 = 0, MLP_WG_T_W2 = 4, MLP_WG_T_W3 = 20, MLP_WG_T_W4 = 36,
+              MLP_WG_T_W2T = 40, MLP_WG_T_W3T = 56;
+constexpr int MLP_WG_VECS = 16; // b1 4, b2 4, b3 4, W4 4 (f32x4 per lane, D layout)
+constexpr int MLP_GRAD_CHUNK = 64; // points per wavefront of k_mlp_wgrad
+
+// The bf16 operand tile `tile` for lane `lane` (see the A-layout above), from θ.
+__device__ __forceinline__ bf16x4 mlp_operand(const float *__restrict__ theta, int tile,
+                                              int lane) {
+  const int r = lane & 15, q = lane >> 4;
+  f32x4 v = {0.f, 0.f, 0.f, 0.f};
+  if (tile < MLP_WG_T_W2) { // w1[t]: W1[16t + r][4q + i], inputs 8..15 zero
+    if (q < 2)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        v[i] = theta[MLP_OFF_W1 + (16 * tile + r) * MLP_IN + 4 * q + i];
+  } else if (tile < MLP_WG_T_W4) { // w2/w3[t][s]: W[16t + r][16s + 4q + i]
+    const int off = tile < MLP_WG_T_W3 ? MLP_OFF_W2 : MLP_OFF_W3;
+    const int ts = tile - (tile < MLP_WG_T_W3 ? MLP_WG_T_W2 : MLP_WG_T_W3);
+    const int t = ts >> 2, sl = ts & 3;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      v[i] = theta[off + (16 * t + r) * MLP_H + 16 * sl + 4 * q + i];
+  } else if (tile < MLP_WG_T_W2T) { // w4[s]: row 0 only, W4[16s + 4q + i]
+    if (r == 0)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        v[i] = theta[MLP_OFF_W4 + 16 * (tile - MLP_WG_T_W4) + 4 * q + i];
+  } else { // w2ᵀ/w3ᵀ[t][s]: A[16t + r][16s + 4q + i] = W[16s + 4q + i][16t + r]
+    const int off = tile < MLP_WG_T_W3T ? MLP_OFF_W2 : MLP_OFF_W3;
+    const int ts = tile - (tile < MLP_WG_T_W3T ? MLP_WG_T_W2T : MLP_WG_T_W3T);
+    const int t = ts >> 2, sl = ts & 3;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      v[i] = theta[off + (16 * sl + 4 * q + i) * MLP_H + 16 * t + r];
+  }
+  return bf16_of(v);
+}
+
+__device__ __forceinline__ bool bf16_positive(short h) { return h > 0; }
+
+// Pre-activation gradient tile: g ⊙ [h > 0] (h the layer's bf16 output), as bf16.
+__device__ __forceinline__ bf16x4 masked_bf16(f32x4 g, bf16x4 h) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    g[i] = bf16_positive(h[i]) ? g[i] : 0.f;
+  return bf16_of(g);
+}
+
+// Stores a D-layout bf16 tile group (features 16t + 4q + i of point n = r) feature-major.
+__device__ __forceinline__ void store_feature_major(short *__restrict__ out, int64_t ld,
+                                                    int64_t n, int q, const bf16x4 (&v)[4]) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      out[(int64_t)(16 * t + 4 * q + i) * ld + n] = v[t][i];
+}
+
+struct MlpTrainBufs {
+  int B;      // batch points (the loss is the mean over these)
+  int64_t ld; // padded batch: a multiple of MLP_GRAD_CHUNK (points beyond B are zero-gradient)
+  const float *xb, *yb; // normalised inputs [B][8], targets [B]
+  short *xT, *h1T, *h2T, *h3T, *dz1T, *dz2T, *dz3T; // bf16 feature-major [features][ld]
+  float *dy;  // [ld]
+  float *sq;  // per-point squared error [B]
+};
+
+__global__ void __launch_bounds__(256) k_mlp_fwd_bwd(MlpTrainBufs b,
+                                                     const float *__restrict__ theta) {
+  __shared__ bf16x4 wl[MLP_WG_TILES][64];
+  __shared__ f32x4 vl[MLP_WG_VECS][64];
+  for (int e = threadIdx.x; e < MLP_WG_TILES * 64; e += 256)
+    wl[e >> 6][e & 63] = mlp_operand(theta, e >> 6, e & 63);
+  for (int e = threadIdx.x; e < MLP_WG_VECS * 64; e += 256) {
+    const int v = e >> 6, l = e & 63, t = v & 3;
+    const int off = v < 4 ? MLP_OFF_B1 : v < 8 ? MLP_OFF_B2 : v < 12 ? MLP_OFF_B3 : MLP_OFF_W4;
+    vl[v][l] = *(const f32x4 *)(theta + off + 16 * t + 4 * (l >> 4));
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int64_t tiles = b.ld / 16;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wave >= tiles)
+    return;
+  const float b4 = theta[MLP_OFF_B4];
+  const float invB = 2.0f / (float)b.B;
+  for (int64_t tile = wave; tile < tiles; tile += (int64_t)gridDim.x * 4) {
+    const int64_t n = tile * 16 + r;
+    const bool real = n < b.B;
+    // Forward (the arithmetic of mlp_tile, weights from LDS).
+    bf16x4 x = {0, 0, 0, 0};
+    if (q < 2 && real)
+      x = bf16_of(*(const f32x4 *)(b.xb + n * MLP_IN + 4 * q));
+    bf16x4 h1[4], h2[4], h3[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      h1[t] = relu_bf16(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[MLP_WG_T_W1 + t][lane], x,
+                                                                   vl[t][lane], 0, 0, 0));
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      acc[t] = vl[4 + t][lane];
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[MLP_WG_T_W2 + 4 * t + sl][lane],
+                                                           h1[sl], acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      h2[t] = relu_bf16(acc[t]);
+      acc[t] = vl[8 + t][lane];
+    }
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[MLP_WG_T_W3 + 4 * t + sl][lane],
+                                                           h2[sl], acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      h3[t] = relu_bf16(acc[t]);
+    f32x4 out = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl)
+      out = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[MLP_WG_T_W4 + sl][lane], h3[sl], out,
+                                                      0, 0, 0);
+    // Loss gradient of point r (lanes 0-15), then to every lane of that point.
+    float dyr = 0.f;
+    if (q == 0 && real) {
+      const float e = (out[0] + b4) - b.yb[n];
+      dyr = invB * e;
+      b.sq[n] = e * e;
+    }
+    if (q == 0)
+      b.dy[n] = dyr;
+    dyr = __shfl(dyr, r, 64);
+    // Backward.
+    bf16x4 d3[4], d2[4], d1[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const f32x4 w4 = vl[12 + t][lane];
+      f32x4 g;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        g[i] = w4[i] * dyr;
+      d3[t] = masked_bf16(g, h3[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[MLP_WG_T_W3T + 4 * t + sl][lane],
+                                                           d3[sl], acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      d2[t] = masked_bf16(acc[t], h2[t]);
+      acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int sl = 0; sl < 4; ++sl)
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wl[MLP_WG_T_W2T + 4 * t + sl][lane],
+                                                           d2[sl], acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      d1[t] = masked_bf16(acc[t], h1[t]);
+    // Layer inputs and gradients, feature-major, for the weight-gradient GEMMs.
+    if (q < 2)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        b.xT[(int64_t)(4 * q + i) * b.ld + n] = x[i];
+    store_feature_major(b.h1T, b.ld, n, q, h1);
+    store_feature_major(b.h2T, b.ld, n, q, h2);
+    store_feature_major(b.h3T, b.ld, n, q, h3);
+    store_feature_major(b.dz1T, b.ld, n, q, d1);
+    store_feature_major(b.dz2T, b.ld, n, q, d2);
+    store_feature_major(b.dz3T, b.ld, n, q, d3);
+  }
+}
+
+// A K-slice operand of the gradient GEMMs: features 16·tile + r, points p0 + 4q .. + 3 of a
+// feature-major bf16 array (8 bytes, aligned).
+__device__ __forceinline__ bf16x4 fm_operand(const short *__restrict__ a, int64_t ld, int tile,
+                                             int r, int q, int64_t p0) {
+  return *(const bf16x4 *)(a + (int64_t)(16 * tile + r) * ld + p0 + 4 * q);
+}
+
+// Partial gradients of one 64-point chunk per wavefront; blockIdx.y picks the parameters:
+// 0 = W3, b3; 1 = W2, b2; 2 = W1, b1, W4, b4. part[chunk][MLP_PARAMS].
+__global__ void __launch_bounds__(256) k_mlp_wgrad(MlpTrainBufs b, float *__restrict__ part) {
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int64_t chunk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (chunk * MLP_GRAD_CHUNK >= b.ld)
+    return;
+  const int group = blockIdx.y;
+  const bf16x4 ones = r == 0 ? bf16x4{0x3F80, 0x3F80, 0x3F80, 0x3F80} : bf16x4{0, 0, 0, 0};
+  float *out = part + chunk * MLP_PARAMS;
+  f32x4 gw[4][4], gb[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    gb[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      gw[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  if (group < 2) { // gW[m][k] = Σ_p dz[m][p] h[k][p]; gb[m] = Σ_p dz[m][p]
+    const short *dz = group == 0 ? b.dz3T : b.dz2T;
+    const short *h = group == 0 ? b.h2T : b.h1T;
+    for (int k = 0; k < MLP_GRAD_CHUNK / 16; ++k) {
+      const int64_t p0 = chunk * MLP_GRAD_CHUNK + 16 * k;
+      bf16x4 a[4], bb[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        a[t] = fm_operand(dz, b.ld, t, r, q, p0);
+        bb[t] = fm_operand(h, b.ld, t, r, q, p0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          gw[t][u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], bb[u], gw[t][u], 0, 0, 0);
+        gb[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], ones, gb[t], 0, 0, 0);
+      }
+    }
+    const int offW = group == 0 ? MLP_OFF_W3 : MLP_OFF_W2;
+    const int offB = group == 0 ? MLP_OFF_B3 : MLP_OFF_B2;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = 16 * t + 4 * q + i;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          out[offW + m * MLP_H + 16 * u + r] = gw[t][u][i];
+        if (r == 0)
+          out[offB + m] = gb[t][i];
+      }
+    return;
+  }
+  // gW1[m][k] = Σ_p dz1[m][p] x[k][p] (k < 8), gb1; gW4[k] = Σ_p dy[p] h3[k][p], gb4.
+  f32x4 g4[4], gb4 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    g4[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < MLP_GRAD_CHUNK / 16; ++k) {
+    const int64_t p0 = chunk * MLP_GRAD_CHUNK + 16 * k;
+    const bf16x4 xs = r < MLP_IN ? fm_operand(b.xT, b.ld, 0, r, q, p0) : bf16x4{0, 0, 0, 0};
+    f32x4 dv = {0.f, 0.f, 0.f, 0.f};
+    if (r == 0)
+      dv = *(const f32x4 *)(b.dy + p0 + 4 * q);
+    const bf16x4 dyA = bf16_of(dv); // row 0 of the A operand: dy of the K-slice's points
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bf16x4 a = fm_operand(b.dz1T, b.ld, t, r, q, p0);
+      gw[t][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xs, gw[t][0], 0, 0, 0);
+      gb[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ones, gb[t], 0, 0, 0);
+      g4[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(dyA, fm_operand(b.h3T, b.ld, t, r, q, p0),
+                                                        g4[t], 0, 0, 0);
+    }
+    gb4 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(dyA, ones, gb4, 0, 0, 0);
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = 16 * t + 4 * q + i;
+      if (r < MLP_IN)
+        out[MLP_OFF_W1 + m * MLP_IN + r] = gw[t][0][i];
+      if (r == 0)
+        out[MLP_OFF_B1 + m] = gb[t][i];
+    }
+  if (q == 0) { // row m = 0 of the W4 and b4 products: lanes 0-15, component 0
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      out[MLP_OFF_W4 + 16 * t + r] = g4[t][0];
+    if (r == 0)
+      out[MLP_OFF_B4] = gb4[0];
+  }
+}
+
+// grad[p] = Σ_chunks part[c][p] in chunk order; with `adam`, the Adam update of k_mlp_adam.
+__global__ void __launch_bounds__(256) k_mlp_reduce_adam(int chunks,
+                                                         const float *__restrict__ part,
+                                                         float *__restrict__ grad,
+                                                         float *__restrict__ theta,
+                                                         float *__restrict__ m,
+                                                         float *__restrict__ v, int adam,
+                                                         float lr, float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= MLP_PARAMS)
+    return;
+  float g = 0.f;
+  for (int c = 0; c < chunks; ++c)
+    g += part[(int64_t)c * MLP_PARAMS + p];
+  grad[p] = g;
+  if (!adam)
+    return;
+  const float mi = 0.9f * m[p] + 0.1f * g;
+  const float vi = 0.999f * v[p] + 0.001f * g * g;
+  m[p] = mi;
+  v[p] = vi;
+  theta[p] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
 // Grid of the inference kernels: 2 four-wavefront workgroups per CU (the kernels' register
 // use allows 2 wavefronts per SIMD), each wavefront walking many 16-point tiles so that the
 // weights are loaded once per wavefront; at most one wavefront per tile.

@@ -143,6 +143,7 @@ def lib():
         L.vpd_mlp_loss_and_gradient.argtypes = [P, _F, _F, C.c_int, C.POINTER(C.c_float), _F]
         L.vpd_mlp_set_dataset.argtypes = [P, _F, _F, C.c_int64, C.c_uint64]
         L.vpd_mlp_train.argtypes = [P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]
+        L.vpd_mlp_set_training_form.argtypes = [P, C.c_int]
         L.vpd_mlp_predict.argtypes = [P, _F, C.c_int64, _F]
         L.vpd_mlp_seed_device.argtypes = [P, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                           C.c_int64, C.c_void_p, C.POINTER(C.c_double)]
@@ -578,6 +579,13 @@ class WarmStart:
         loss = C.c_float()
         self._check(lib().vpd_mlp_train(self.h, steps, batch, lr, C.byref(loss)))
         return loss.value
+
+    TRAIN_FP32, TRAIN_MFMA = 0, 1
+
+    def set_training_form(self, form):
+        """vpd_mlp_set_training_form: TRAIN_MFMA (bf16 matrix-core products, FP32
+        accumulation; the default) or TRAIN_FP32."""
+        self._check(lib().vpd_mlp_set_training_form(self.h, form))
 
     def predict(self, x):
         x = np.ascontiguousarray(x, np.float32)

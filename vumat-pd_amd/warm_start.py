@@ -6,7 +6,8 @@
    increment the return map received (6 Voigt components), the equivalent plastic strain and
    the temperature before the update, and the target is the converged Δε̄ᵖ.
 2. Fit the input/output normalisation, then train the 8 → 64 → 64 → 64 → 1 network on the GPU
-   with the project's FP32 forward/backward kernels and Adam.
+   with Adam: the forward and backward products as bf16 MFMA with FP32 accumulation (the
+   default) or the FP32 kernels (--form fp32).
 3. Save θ and the normalisation (an .npz of FP32 arrays).
 
     python vumat-pd_amd/warm_start.py --pairs 2000000 --out vumat-pd_amd/warm_start_jc.npz
@@ -135,11 +136,12 @@ def normalisation(x, y):
     return np.concatenate([mx, sx, [y.mean(), max(y.std(), 1e-30)]]).astype(np.float32)
 
 
-def train(x, y, steps=20000, batch=4096, lr=1e-3, seed=7, verbose=True):
+def train(x, y, steps=20000, batch=4096, lr=1e-3, seed=7, verbose=True, form="mfma"):
     norm = normalisation(x, y)
     xn = ((x - norm[:8]) / norm[8:16]).astype(np.float32)
     yn = ((y - norm[16]) / norm[17]).astype(np.float32)
     net = vpd.WarmStart(batch)
+    net.set_training_form(net.TRAIN_MFMA if form == "mfma" else net.TRAIN_FP32)
     net.set_params(initial_params(seed), norm)
     net.set_dataset(xn, yn, seed=seed)
     t0 = time.perf_counter()
@@ -151,8 +153,9 @@ def train(x, y, steps=20000, batch=4096, lr=1e-3, seed=7, verbose=True):
         loss = net.train(min(chunk, steps - s), batch, rate)
         if verbose:
             print("[warm-start] step %d loss %.4e" % (s + chunk, loss), flush=True)
+    net.train_seconds = time.perf_counter() - t0
     if verbose:
-        print("[warm-start] trained %d steps in %.1f s" % (steps, time.perf_counter() - t0))
+        print("[warm-start] trained %d steps (%s) in %.2f s" % (steps, form, net.train_seconds))
     return net, norm, loss
 
 
@@ -175,9 +178,10 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "warm_start_jc.npz"))
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--eval-seed", type=int, default=98765)
+    ap.add_argument("--form", choices=("mfma", "fp32"), default="mfma")
     a = ap.parse_args()
     rec = log_pairs(a.pairs, seed=a.seed)
-    net, norm, loss = train(rec["x"], rec["y"], steps=a.steps)
+    net, norm, loss = train(rec["x"], rec["y"], steps=a.steps, form=a.form)
     save(a.out, net)
     print("[warm-start] saved %s (final batch loss %.4e, %d pairs)" % (a.out, loss, len(rec["y"])))
     held = log_pairs(200000, seed=a.eval_seed, verbose=False)
