@@ -66,6 +66,9 @@ struct vpd_mlp {
   short *xT = nullptr, *h1T = nullptr, *h2T = nullptr, *h3T = nullptr, *dz1T = nullptr,
         *dz2T = nullptr, *dz3T = nullptr;
   float *dyT = nullptr, *gpart = nullptr;
+  // vpd_mlp_seed_device: the count and list of the points the network seeds (grown on demand)
+  int *seedCount = nullptr, *seedList = nullptr;
+  int64_t seedCap = 0;
   std::string err;
 };
 
@@ -152,7 +155,7 @@ struct vpd_context {
   double *inc = nullptr, *seed = nullptr;
   int *fallbacks = nullptr;
   // The compacted learned update: list length, particle per slot, strain increments per slot.
-  int *listCount = nullptr, *list = nullptr;
+  int *listCount = nullptr, *list = nullptr, *firstList = nullptr;
   double *incList = nullptr;
   std::vector<TimingRec> recs;
   std::vector<hipEvent_t> eventPool;
@@ -923,19 +926,19 @@ int launch_defgrad(vpd_context *c) {
 
 // Buffers of the compacted learned update (plastic-point list), allocated on first use.
 int ensure_list(vpd_context *c) {
-  if (!c->list && (dalloc(c, c->listCount, 1) || dalloc(c, c->list, c->n) ||
-                   dalloc(c, c->incList, 6 * c->n)))
+  if (!c->list && (dalloc(c, c->listCount, 2) || dalloc(c, c->list, c->n) ||
+                   dalloc(c, c->firstList, c->n) || dalloc(c, c->incList, 6 * c->n)))
     return VPD_ERR_DEVICE;
   return VPD_OK;
 }
 
 // The compacted material update (k_material_screen, then the return map on the listed
-// plastic points only; see vpd_kernels.hpp): the network seeds the list first in the learned
-// mode.
+// plastic points only; see vpd_kernels.hpp): in the learned mode the network first seeds the
+// listed points that have no positive state 5.
 template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g, int grid) {
   if (int s = ensure_list(c))
     return s;
-  const CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n)};
+  const CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n), c->firstList};
   if (int s = timed(c, K_MATERIAL, [&] {
         k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
         k_material_screen<MODE><<<grid, 256, 0, c->stream>>>(g, cl);
@@ -943,9 +946,9 @@ template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g,
     return s;
   if (MODE == RM_LEARNED) {
     if (int s = timed(c, K_MLP_SEED, [&] {
-          k_mlp_seed<SEED_SLOTS><<<mlp_grid(c->n), 256, 0, c->stream>>>(
-              0, c->listCount, c->list, c->incList, c->n, c->statev, c->n, c->mlp->theta,
-              c->mlp->norm, c->seed);
+          k_mlp_seed<SEED_LIST><<<mlp_grid(c->n), 256, 0, c->stream>>>(
+              0, c->listCount + 1, c->firstList, c->list, c->incList, c->n, c->statev, c->n,
+              c->mlp->theta, c->mlp->norm, c->seed);
         }))
       return s;
     g.seed = c->seed;
@@ -996,7 +999,7 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
       return s;
     if (int s = timed(c, K_MLP_SEED, [&] {
           k_mlp_seed<SEED_ALL><<<mlp_grid(c->n), 256, 0, c->stream>>>(
-              c->n, nullptr, nullptr, c->inc, c->n, c->statev, c->n, c->mlp->theta,
+              c->n, nullptr, nullptr, nullptr, c->inc, c->n, c->statev, c->n, c->mlp->theta,
               c->mlp->norm, c->seed);
         }))
       return s;
@@ -1292,6 +1295,7 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->seed);
   dfree(c->listCount);
   dfree(c->list);
+  dfree(c->firstList);
   dfree(c->incList);
   dfree(c->broken);
   dfree(c->anyBroken);
@@ -1359,6 +1363,7 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   dfree(c->fallbacks);
   dfree(c->listCount); // sized for the previous block
   dfree(c->list);
+  dfree(c->firstList);
   dfree(c->incList);
   c->haveNeighbors = c->haveShape = c->haveMaterial = c->haveDynamics = false;
   c->haveFnew = c->haveFields = c->haveForce = c->predicted = false;
@@ -3350,6 +3355,8 @@ void vpd_mlp_destroy(vpd_mlp *m) {
   dfree(m->dyT);
   dfree(m->gpart);
   dfree(m->order);
+  dfree(m->seedCount);
+  dfree(m->seedList);
   if (m->stream)
     (void)hipStreamDestroy(m->stream);
   delete m;
@@ -3495,9 +3502,23 @@ int vpd_mlp_seed_device(vpd_mlp *m, const double *inc, int64_t incStride, const 
   hipEvent_t a, b;
   MLPTRY(m, hipEventCreate(&a));
   MLPTRY(m, hipEventCreate(&b));
+  if (n > m->seedCap) {
+    dfree(m->seedList);
+    m->seedCap = 0;
+    if ((!m->seedCount && malloc_n(m, m->seedCount, 2)) || malloc_n(m, m->seedList, n))
+      return VPD_ERR_DEVICE;
+    m->seedCap = n;
+  }
   MLPTRY(m, hipEventRecord(a, m->stream));
-  k_mlp_seed<SEED_ALL><<<mlp_grid(n), 256, 0, m->stream>>>(
-      n, nullptr, nullptr, inc, incStride, state, stateStride, m->theta, m->norm, seed);
+  // State 5 where positive, the network on the list of the other points.
+  k_zero_count<<<1, 1, 0, m->stream>>>(m->seedCount);
+  const int64_t chunks = ceil_div(n, 256), groups = std::min<int64_t>(chunks, 512);
+  k_seed_state5<<<(int)groups, 256, 0, m->stream>>>(n, ceil_div(chunks, groups), state,
+                                                     stateStride, m->seedCount, m->seedList,
+                                                     seed);
+  k_mlp_seed<SEED_LIST><<<mlp_grid(n), 256, 0, m->stream>>>(
+      0, m->seedCount, m->seedList, nullptr, inc, incStride, state, stateStride, m->theta,
+      m->norm, seed);
   MLPTRY(m, hipGetLastError());
   MLPTRY(m, hipEventRecord(b, m->stream));
   MLPTRY(m, hipEventSynchronize(b));

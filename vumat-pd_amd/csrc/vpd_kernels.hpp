@@ -1440,21 +1440,125 @@ __global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
 // with one plastic lane runs the whole return-map loop. k_material_screen runs the fused
 // update for every point whose return map ends at its first iteration (reusing that
 // iteration's yield) and appends the others to a list (one atomic per workgroup); the return
-// map (and, in the learned mode, the network first: k_mlp_seed) then runs on that dense
-// list only (k_material_return_list). Each point's arithmetic is that of the fused kernel,
-// so the result is bit for bit the same and does not depend on the list order.
+// map then runs on that dense list only (k_material_return_list). In the learned mode a
+// listed point whose state 5 (its previous plastic increment) is positive is seeded with it;
+// the others (first yield, or yield again after an elastic increment) are also appended to a
+// second list of their slots, and the network runs on that list only (k_mlp_seed) before the
+// return map. Each point's arithmetic is that of the fused kernel, so the result is bit for
+// bit the same and does not depend on the list order.
 struct CompactArgs {
-  int *count;   // number of listed points (zeroed before the screen)
+  int *count;   // count[0]: listed points; count[1]: slots in `first` (8-byte aligned, both
+                // zeroed before the screen)
   int *list;    // particle index per slot
   Planes inc;   // strain increments per slot (6 planes of the particle-count capacity)
+  int *first;   // learned mode: the slots whose state 5 is not positive
 };
 
-__global__ void k_zero_count(int *count) { *count = 0; }
+__global__ void k_zero_count(int *count) {
+  count[0] = 0;
+  count[1] = 0;
+}
+
+// Appends to two lists behind one 64-bit counter (count[0] in its low half, count[1] in its
+// high half), with one device atomic per workgroup for both: returns this thread's positions
+// for `takeA` (list 0) and `takeB` (list 1). Every thread of the workgroup calls; `base` holds
+// 10 ints of LDS. Appends from many workgroups to one address serialise in the memory
+// system, so a kernel appends once per workgroup at most.
+struct Appended {
+  int a, b;
+};
+__device__ __forceinline__ Appended group_append2(bool takeA, bool takeB, int *count,
+                                                  int *base) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t ma = __ballot(takeA), mb = __ballot(takeB);
+  if (lane == 0) {
+    base[wv] = __popcll(ma);
+    base[4 + wv] = __popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ta = 0, tb = 0;
+    for (int w = 0; w < 4; ++w) {
+      const int ca = base[w], cb = base[4 + w];
+      base[w] = ta;
+      base[4 + w] = tb;
+      ta += ca;
+      tb += cb;
+    }
+    unsigned long long old = 0;
+    if (ta + tb > 0)
+      old = atomicAdd((unsigned long long *)count,
+                      (unsigned long long)ta + ((unsigned long long)tb << 32));
+    base[8] = (int)(old & 0xFFFFFFFFull);
+    base[9] = (int)(old >> 32);
+  }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1ull;
+  const Appended pos{base[8] + base[wv] + (int)__popcll(ma & below),
+                     base[9] + base[4 + wv] + (int)__popcll(mb & below)};
+  __syncthreads(); // `base` may be reused
+  return pos;
+}
+
+// Inclusive prefix sum over the 256 threads of a workgroup (`red` holds 4 ints of LDS).
+__device__ __forceinline__ int group_inclusive_sum(int v, int *red) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off);
+    if (lane >= off)
+      v += t;
+  }
+  if (lane == 63)
+    red[wv] = v;
+  __syncthreads();
+  for (int w = 0; w < wv; ++w)
+    v += red[w];
+  __syncthreads(); // `red` may be reused
+  return v;
+}
+
+// The first pass of a listed network seed over n points (k_mlp_seed<SEED_LIST> is the
+// second): seed[s] = state 5 where it is positive; the other points are appended to
+// `entries` (count[0] zeroed before). Workgroup g handles the 256-point chunks
+// [g·per, (g+1)·per): it counts, appends its total with one atomic, then writes the entries
+// (each thread its own points, in order).
+__global__ void __launch_bounds__(256) k_seed_state5(int64_t n, int64_t per,
+                                                     const double *__restrict__ state,
+                                                     int64_t stateStride, int *count,
+                                                     int *entries, double *seed) {
+  __shared__ int red[4];
+  __shared__ int groupBase;
+  const int64_t c0 = (int64_t)blockIdx.x * per;
+  const int64_t c1 = min(c0 + per, (n + 255) / 256);
+  int mine = 0;
+  for (int64_t c = c0; c < c1; ++c) {
+    const int64_t s = c * 256 + threadIdx.x;
+    if (s < n) {
+      const double prev = state[4 * stateStride + s];
+      if (prev > 0.0)
+        seed[s] = prev;
+      else
+        ++mine;
+    }
+  }
+  const int incl = group_inclusive_sum(mine, red);
+  if (threadIdx.x == 255)
+    groupBase = incl > 0 ? atomicAdd(count, incl) : 0;
+  __syncthreads();
+  if (mine == 0)
+    return;
+  int pos = groupBase + incl - mine;
+  for (int64_t c = c0; c < c1; ++c) {
+    const int64_t s = c * 256 + threadIdx.x;
+    if (s < n && !(state[4 * stateStride + s] > 0.0))
+      entries[pos++] = (int)s;
+  }
+}
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, CompactArgs cl) {
-  __shared__ int waveBase[4];
-  __shared__ int blockBase;
+  __shared__ int base[10];
   const int64_t blk = xcd_block(g.nblocks);
   if (blk < 0)
     return;
@@ -1521,28 +1625,18 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
     jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats, &first);
     material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
   }
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const uint64_t mask = __ballot(plastic);
-  if (lane == 0)
-    waveBase[wv] = __popcll(mask);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int total = 0;
-    for (int w = 0; w < 4; ++w) {
-      const int c = waveBase[w];
-      waveBase[w] = total;
-      total += c;
-    }
-    blockBase = total > 0 ? atomicAdd(cl.count, total) : 0;
-  }
-  __syncthreads();
+  // In the learned mode the network's points (no positive state-5 seed) also go to the
+  // second list, with the same append.
+  const bool net = MODE == RM_LEARNED && plastic && !(st[4] > 0.0);
+  const Appended pos = group_append2(plastic, net, cl.count, base);
   if (plastic) {
-    const int slot = blockBase + waveBase[wv] + __popcll(mask & ((1ull << lane) - 1ull));
-    cl.list[slot] = (int)i;
+    cl.list[pos.a] = (int)i;
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-      cl.inc(c, slot) = inc[c];
+      cl.inc(c, pos.a) = inc[c];
   }
+  if (net)
+    cl.first[pos.b] = pos.a;
 }
 
 template <int MODE>
@@ -1562,8 +1656,9 @@ __global__ void __launch_bounds__(256) k_material_return_list(MaterialArgs g, Co
   ReturnMapStats stats;
   const Ramp none{0.0, 0.0, 0.0};
   const double dt = point_step(g.batch, i, g.dt, none, none).dt;
-  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[slot] : 0.0,
-                 g.rtol, &stats);
+  // Learned mode: state 5 where positive, else the network's estimate for this slot.
+  const double seed = MODE != RM_LEARNED ? 0.0 : st[4] > 0.0 ? st[4] : g.seed[slot];
+  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, seed, g.rtol, &stats);
   T3 f1, r1;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
