@@ -154,9 +154,12 @@ struct vpd_context {
   vpd_mlp *mlp = nullptr;
   double *inc = nullptr, *seed = nullptr;
   int *fallbacks = nullptr;
-  // The compacted learned update: list length, particle per slot, strain increments per slot.
+  // The compacted material update: list lengths, particle per slot, the network's slots,
+  // strain increments per slot. listZeroed: the lengths are known to be zero (a force pass
+  // cleared them after the last material update), so the next update need not clear them.
   int *listCount = nullptr, *list = nullptr, *firstList = nullptr;
   double *incList = nullptr;
+  bool listZeroed = false, capturing = false;
   std::vector<TimingRec> recs;
   std::vector<hipEvent_t> eventPool;
   // Run diagnostics (vpd_enable_diagnostics): per-chunk partial rows, the step records kept
@@ -926,9 +929,12 @@ int launch_defgrad(vpd_context *c) {
 
 // Buffers of the compacted learned update (plastic-point list), allocated on first use.
 int ensure_list(vpd_context *c) {
-  if (!c->list && (dalloc(c, c->listCount, 2) || dalloc(c, c->list, c->n) ||
-                   dalloc(c, c->firstList, c->n) || dalloc(c, c->incList, 6 * c->n)))
-    return VPD_ERR_DEVICE;
+  if (!c->list) {
+    c->listZeroed = false;
+    if (dalloc(c, c->listCount, 2) || dalloc(c, c->list, c->n) ||
+        dalloc(c, c->firstList, c->n) || dalloc(c, c->incList, 6 * c->n))
+      return VPD_ERR_DEVICE;
+  }
   return VPD_OK;
 }
 
@@ -939,8 +945,12 @@ template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g,
   if (int s = ensure_list(c))
     return s;
   const CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n), c->firstList};
+  // A captured step clears the lengths itself: its replays may follow other work.
+  const bool clear = !c->listZeroed || c->capturing;
+  c->listZeroed = false;
   if (int s = timed(c, K_MATERIAL, [&] {
-        k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
+        if (clear)
+          k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
         k_material_screen<MODE><<<grid, 256, 0, c->stream>>>(g, cl);
       }))
     return s;
@@ -1140,9 +1150,13 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   g.batch = c->batch;
   if (int s = prepare_plans(c))
     return s;
-  return timed(c, K_FORCE, [&] {
-    launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag, gather_of(c, 1));
-  });
+  g.zeroCount = c->listCount; // the pass clears the material lists for the next update
+  if (int s = timed(c, K_FORCE, [&] {
+        launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag, gather_of(c, 1));
+      }))
+    return s;
+  c->listZeroed = c->listCount && !c->capturing;
+  return VPD_OK;
 }
 
 int require_block(vpd_context *c) {
@@ -2061,6 +2075,7 @@ int vpd_run_specimens(vpd_context *c, int32_t firstStep, int32_t count) {
   c->batch.stepDev = c->stepDev;
   hipGraph_t graph = nullptr;
   HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
   int s = launch_defgrad(c);
   if (!s)
     s = launch_material(c, 1.0, false);
@@ -2069,6 +2084,8 @@ int vpd_run_specimens(vpd_context *c, int32_t firstStep, int32_t count) {
   if (!s)
     k_advance_step<<<1, 64, 0, c->stream>>>(c->stepDev);
   const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+  c->capturing = false;
+  c->listZeroed = false;
   c->batch = eager;
   c->timing = timing;
   if (s) {

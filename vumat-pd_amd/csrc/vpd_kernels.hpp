@@ -1692,7 +1692,17 @@ struct ForceArgs {
   DiagArgs diag;
   Batch batch;
   int take; // windowed gathers: bonds a lane takes from one window at most (see window_stop)
+  // When set, the pass clears these two list lengths of the compacted material update (the
+  // step's material stage has run; the next one appends from zero).
+  int *zeroCount = nullptr;
 };
+
+__device__ __forceinline__ void clear_list_lengths(const ForceArgs &g) {
+  if (g.zeroCount && blockIdx.x == 0 && threadIdx.x == 0) {
+    g.zeroCount[0] = 0;
+    g.zeroCount[1] = 0;
+  }
+}
 
 // A of particle j from its 5 component pairs.
 __device__ __forceinline__ T3 load_amap(PairPlanes amap, int64_t j) {
@@ -1872,6 +1882,7 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
 // reaches the chunk's reduction.
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true, bool DIAG = false>
 __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
+  clear_list_lengths(g);
   static_assert(!DIAG || INTEGRATE, "the diagnostics read the corrected velocity");
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
   if (blk < 0)
@@ -1904,6 +1915,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
 // lanes in the window steps; the rounds loop exits on the wavefront's first particle.
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W>
 __global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
+  clear_list_lengths(g);
   extern __shared__ d2v wlds[];
   d2v *win = wlds + wave_slot() * (7 * W);
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
@@ -1936,6 +1948,7 @@ __global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W, bool PREFETCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFETCH ? 1 : 5)))
     k_force_p(ForceArgs g, WindowPlan plan) {
+  clear_list_lengths(g);
   extern __shared__ d2v wlds[];
   d2v *win = wlds + wave_slot() * (7 * W); // (unused with PREFETCH)
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
@@ -1997,6 +2010,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(group_waves<7, W>())))
     k_force_g(ForceArgs g, GroupPlan plan) {
+  clear_list_lengths(g);
   extern __shared__ d2v wlds[];
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
   if (blk < 0)
