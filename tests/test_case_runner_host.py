@@ -168,3 +168,26 @@ def test_material_point_cases_are_refused(tmp_path, capfd):
     err = capfd.readouterr().err.strip()
     assert rc == 1 and "vumat_material_point_case" in err
 
+
+
+def test_command_line_runner(tmp_path):
+    # vpd_pd_run: the reference's usage (src/apps/main_pd_case_runner.cpp:15-27), its default
+    # case when called without arguments, one case per call, several cases as one batch; every
+    # failure here happens while reading the files, before any device work.
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "vumat-pd_amd", "build", "vpd_pd_run")
+    if not os.path.exists(exe):
+        pytest.skip("vumat-pd_amd/build/vpd_pd_run is not built")
+    r = subprocess.run([exe, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.startswith("Usage: vpd_pd_run [case.json")
+    assert "default: examples/affine_bar_elastic.json" in r.stdout
+    r = subprocess.run([exe], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 1
+    assert r.stderr.strip() == ("VUMAT-PD demo case failed: Failed to open JSON file: "
+                                "examples/affine_bar_elastic.json")
+    missing = str(tmp_path / "missing.json")
+    for args in ([missing], [missing, str(tmp_path / "other.json")]):
+        r = subprocess.run([exe] + args, capture_output=True, text=True)
+        assert r.returncode == 1
+        assert r.stderr.strip() == "VUMAT-PD demo case failed: Failed to open JSON file: " + missing

@@ -23,3 +23,11 @@ def test_phase_split_never_damaged():
     p = impact.phases([rec(500, 500, 1.0, 0.0)], 10)
     assert p["intact"]["steps"] == 500
     assert p["fragmenting"]["steps"] == 0 and p["fragmenting"]["particle_steps_per_s"] is None
+
+
+def test_chunk_ends():
+    # Every 100 steps over the first 500 (where the damage front forms), then every 500.
+    assert impact.chunk_ends(1700, 500, 100, 500) == [100, 200, 300, 400, 500, 1000, 1500, 1700]
+    assert impact.chunk_ends(250, 500, 100, 500) == [100, 200, 250]
+    assert impact.chunk_ends(1000, 500) == [500, 1000]
+    assert impact.chunk_ends(1000, 400, 150, 500) == [150, 300, 450, 500, 900, 1000]

@@ -7,7 +7,8 @@ MPa → J/m³ factor 1e6 of the reference's configs/johnson_cook_heat_units.json
 workToHeat), critical-stretch damage s_c = 0.02, the learned return map (network seed,
 safeguarded Newton finish to |r| ≤ 1e-12·q, bisection fallback), 10000 steps.
 
-Every `log_every` steps one unfused step keeps the fields, and the log records the plastic
+Every `log_every` steps (every `early_log_every` over the first `early_steps`, where the
+damage front forms) one unfused step keeps the fields, and the log records the plastic
 points of that step, their mean return-map iterations, how many needed a bisection fallback
 inside the Newton solve, peak ε̄ᵖ and temperature rise, the mean damage index (the broken
 fraction of each particle's bonds), the particles whose bonds are all broken (those
@@ -16,7 +17,8 @@ The summary splits the run at the first logged step whose mean damage index reac
 the intact phase before it (every bond pass sums nearly every bond) and the fragmenting
 phase after it (the passes skip more and more broken bonds), each with its throughput.
 
-    python impact.py [--steps 10000] [--log-every 500] [--return-map learned|newton|bisection]
+    python impact.py [--steps 10000] [--log-every 500] [--early-log-every 100] [--early-steps 500]
+                     [--return-map learned|newton|bisection]
 """
 from __future__ import annotations
 
@@ -74,14 +76,28 @@ def log_record(bar, step, tref=293.0):
                 mean_damage_index=float(dmg.mean()))
 
 
-def run(bar, steps, log_every):
-    """`steps` steps in chunks of log_every (fused inside a chunk, its last step unfused and
-    logged). Returns (seconds, log); logging downloads are not timed."""
+def chunk_ends(steps, log_every, early_every=0, early_steps=0):
+    """The logged steps: every early_every steps up to early_steps (the bar's intact start,
+    where the damage front forms within a few hundred steps), then every log_every steps,
+    and the last step."""
+    ends, done = [], 0
+    while done < steps:
+        if early_every and done < early_steps:
+            done = min(done + early_every, early_steps, steps)
+        else:
+            done = min(done + log_every, steps)
+        ends.append(done)
+    return ends
+
+
+def run(bar, steps, log_every, early_every=0, early_steps=0):
+    """`steps` steps in chunks ending at chunk_ends() (fused inside a chunk, its last step
+    unfused and logged). Returns (seconds, log); logging downloads are not timed."""
     log = []
     seconds = 0.0
     done = 0
-    while done < steps:
-        k = min(log_every, steps - done)
+    for end in chunk_ends(steps, log_every, early_every, early_steps):
+        k = end - done
         bar.ctx.synchronize()
         t0 = time.perf_counter()
         bar.step(k, want_fields=True)
@@ -118,6 +134,9 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--log-every", type=int, default=500)
+    ap.add_argument("--early-log-every", type=int, default=100,
+                    help="log cadence over the first --early-steps steps (0: none)")
+    ap.add_argument("--early-steps", type=int, default=500)
     ap.add_argument("--return-map", choices=sorted(MODES), default="learned")
     ap.add_argument("--nx", type=int, default=400)
     ap.add_argument("--ny", type=int, default=70)
@@ -130,7 +149,7 @@ def main(argv=None):
     setup = time.perf_counter() - t0
     tuned = bar.ctx.autotune(3)
     bar.reset()
-    seconds, log = run(bar, a.steps, a.log_every)
+    seconds, log = run(bar, a.steps, a.log_every, a.early_log_every, a.early_steps)
     plastic = sum(r["plastic_points"] for r in log)
     out = dict(workload="voided_jc_bar_%dx%dx%d" % (a.nx, a.ny, a.ny), particles=bar.n,
                velocity_rise_ms=a.rise,
