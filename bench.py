@@ -4,15 +4,17 @@ Johnson-Cook VUMAT on one MI355X, on the 500k-particle JC tensile bar (200×50×
 1 mm/ms, dt = 0.5·Δx/c_p), with the dominant kernel's HBM roofline and the reference's CPU
 path timed on this host beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--from-step S] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--from-step S] [--chunk C]
+                    [--no-cpu-baseline]
 
 Prints one JSON line. A step is one fused increment (F + damage bond pass, material point
 update, force bond pass + velocity-Verlet) over all 500k particles, inputs resident in HBM.
 The timed window is steps S .. S+K−1 (default 4001-5000, the plastic tail of the bar's
 5000-step run: yielding starts near step 3500, and by step 4000 some 75k points take a
 plastic increment per step); the bar is advanced to step S−1−W untimed and runs W warmup
-steps first. The same window is timed with the learned return map from the same state, and
-a second window from step 1 (elastic) is reported beside it.
+steps first. The same window is timed with the learned return map on a second context of
+the bar from the same state, the two alternating chunks of --chunk steps so that drift over
+the run reaches both alike; a second window from step 1 (elastic) is reported beside it.
 """
 from __future__ import annotations
 
@@ -162,6 +164,25 @@ def timed_window(bar, steps):
     return elapsed, kt
 
 
+def timed_interleaved(bars, steps, chunk):
+    """`steps` timed steps of each bar in `bars`, in alternating chunks of `chunk` steps
+    (bar 0's chunk, bar 1's chunk, ...), so that clock and temperature drift over the window
+    reaches every bar alike. Returns per bar (seconds, summed per-kernel HIP-event times)."""
+    out = [[0.0, {}] for _ in bars]
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        for b, acc in zip(bars, out):
+            elapsed, kt = timed_window(b, k)
+            acc[0] += elapsed
+            for name, (ms, launches) in kt.items():
+                tot = acc[1].setdefault(name, [0.0, 0])
+                tot[0] += ms
+                tot[1] += launches
+        done += k
+    return [(e, {k: tuple(v) for k, v in kt.items()}) for e, kt in out]
+
+
 def timed_run(bar, steps, warmup):
     """Warmup steps, reset to the state before step 1, then `steps` timed steps from step 1.
     Returns (seconds, per-kernel HIP-event times of every TIMING_STRIDE-th step)."""
@@ -185,7 +206,8 @@ def return_map_study(points, seed=4321):
     """The return-map solvers on `points` plastic material points logged from seed-fixed plain
     runs of the 30×8×8 JC bar (the warm start's training workload, a held-out seed): kernel
     time, mean iterations and the largest stress deviation from the FP64 tight bisection
-    (|r| ≤ 1e-14·q), per solver. The learned time includes the network's inference."""
+    (|r| ≤ 1e-14·q), per solver. The learned rows run seeding and solve in one kernel
+    (vpd_return_map_learned_device), so their time includes the network's inference."""
     import numpy as np
     import vpd
     import warm_start as ws
@@ -217,7 +239,7 @@ def return_map_study(points, seed=4321):
             if mode in (vpd.RETURN_MAP_LEARNED, -1):
                 r = vpd.return_map_learned(net, props, dt, inc, stress,
                                            state if mode >= 0 else no5, z, z)
-                total = r["ms"] + r["seed_ms"]
+                total = r["ms"]
             else:
                 r = vpd.return_map(mode, props, dt, inc, stress, state, z, z)
                 total = r["ms"]
@@ -231,7 +253,8 @@ def return_map_study(points, seed=4321):
                "max_eqps_inc_err": float(np.abs(r["state"][4] - exact["state"][4]).max()),
                "newton_fallbacks_per_point": float(r["fallbacks"].mean())}
         if mode in (vpd.RETURN_MAP_LEARNED, -1):
-            row["seed_ms"] = r["seed_ms"]
+            row["seeds_needing_network"] = int((~(state[4] > 0.0) if mode >= 0
+                                                else np.ones(m, bool)).sum())
             row["seed_median_rel_err"] = float(np.median(
                 np.abs(r["seed"] - exact["state"][4]) / np.maximum(exact["state"][4], 1e-30)))
         out[name] = row
@@ -340,6 +363,9 @@ def main():
                     help="first timed step: the bar is advanced untimed to step from-step − 1 − "
                          "warmup, then runs `warmup` steps and times `steps` steps (4001 times "
                          "the plastic phase of the 5000-step run; 1 the start)")
+    ap.add_argument("--chunk", type=int, default=100,
+                    help="steps per chunk of the timed window; the plain and the learned "
+                         "return map alternate chunks")
     ap.add_argument("--no-step1-window", action="store_true",
                     help="skip the second timed window from step 1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -385,7 +411,9 @@ def main():
     print("[bench] launch tuning %s" % tuned, file=sys.stderr, flush=True)
 
     # The timed window: steps from_step .. from_step + steps − 1, after an untimed advance and
-    # `warmup` untimed steps. Both return-map modes start from the same snapshot.
+    # `warmup` untimed steps. The learned return map runs on a second context of the same bar
+    # from the same snapshot with the same launch tuning; the two time alternating chunks of
+    # the window (timed_interleaved).
     first = args.from_step
     warm = min(args.warmup, first - 1)
     bar.reset()
@@ -393,13 +421,25 @@ def main():
     if advance:
         print("[bench] advancing to step %d untimed" % advance, file=sys.stderr, flush=True)
         bar.step(advance)
-    snap = bar.snapshot() if not args.no_warm_start else None
+    bars = [bar]
+    if not args.no_warm_start:
+        snap = bar.snapshot()
+        learned = vpd.Bar(w["nx"], w["ny"], w["nz"], w["spacing"], w["horizon"], JC_PROPS, dt,
+                          RHO, velocity=w["velocity"], damage=w["damage"],
+                          return_map=vpd.RETURN_MAP_LEARNED, warm_start=vpd.load_warm_start())
+        learned.ctx.set_tuning(**tuned)
+        learned.restore(snap)
+        del snap
+        bars.append(learned)
     if warm:
-        bar.step(warm)
+        for b in bars:
+            b.step(warm)
     cpu_state = bar.snapshot() if not args.no_cpu_baseline else None
-    print("[bench] timing %d plain steps from step %d" % (args.steps, first), file=sys.stderr,
-          flush=True)
-    elapsed, kt = timed_window(bar, args.steps)
+    print("[bench] timing %d steps from step %d, %s in alternating chunks of %d"
+          % (args.steps, first, "plain and learned" if len(bars) > 1 else "plain",
+             args.chunk), file=sys.stderr, flush=True)
+    timed = timed_interleaved(bars, args.steps, args.chunk)
+    elapsed, kt = timed[0]
     u = bar.get("u")
     if not np.isfinite(u).all():
         raise SystemExit("non-finite displacement after the timed run")
@@ -446,22 +486,16 @@ def main():
         "result": {"max_eqps": win["max_eqps"], "broken_bonds": win["broken_bonds"]},
     }
     if not args.no_warm_start:
-        # The same bar, snapshot and launch tuning with the learned return map (compacted:
-        # network and Newton on the plastic points only).
-        print("[bench] timing %d warm-start steps from step %d" % (args.steps, first),
-              file=sys.stderr, flush=True)
-        net = vpd.load_warm_start()
-        bar.restore(snap)
-        bar.ctx.set_return_map(vpd.RETURN_MAP_LEARNED, 1e-12, net)
-        if warm:
-            bar.step(warm)
-        ws_elapsed, ws_kt = timed_window(bar, args.steps)
-        ws = window_result(bar, first, args.steps, ws_elapsed, ws_kt, n)
+        # The same window with the learned return map (compacted: Newton on the plastic
+        # points only, the network on those without a positive state 5).
+        ws_elapsed, ws_kt = timed[1]
+        learned = bars.pop()
+        ws = window_result(learned, first, args.steps, ws_elapsed, ws_kt, n)
         ws["unit"] = "particle-steps/s"
         ws["over_plain"] = ws["value"] / value
         out["warm_start"] = ws
-        bar.ctx.set_return_map(vpd.RETURN_MAP_BISECTION, 1e-12)
-        del snap
+        learned.ctx.close()
+        del learned
     if not args.no_step1_window and first > 1:
         print("[bench] timing %d plain steps from step 1" % args.steps, file=sys.stderr,
               flush=True)

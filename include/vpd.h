@@ -431,6 +431,29 @@ int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *p
                           double *ener_intern_new, double *ener_inelas_new, int32_t *iterations,
                           int32_t *fallbacks, double *elapsed_ms);
 
+/* The largest errors of the FP32 hardware log2 and exp2 over every input the bisection's
+ * FP32 yield screen can give them (every float in [0.5, 1] for log2, absolute error; every
+ * float in [0, 1] for exp2, relative error), each against the FP64 function. The screen's
+ * error bound is built on these figures; this call measures them on the device at hand. No
+ * reference counterpart. */
+int vpd_screen_accuracy(double *log2_abs_error, double *exp2_rel_error);
+
+/* The learned return map (mode 3) on device arrays with the network attached instead of a
+ * seed array: a point with a positive state 5 is seeded with it and the others with the
+ * network's estimate, which runs in dense tiles on a list of those points (the seeds of
+ * vpd_mlp_seed_device), then the Newton solve. seed_out (device, nblock doubles, may be NULL)
+ * receives the seeds; *elapsed_ms covers seeding and solve.
+ * Uses the network's work buffers: calls with one network must not overlap. Other arguments
+ * as for vpd_return_map_device. No reference counterpart. */
+int vpd_return_map_learned_device(vpd_mlp *mlp, int64_t nblock, int nstatev,
+                                  const double *props, int nprops, double dt, double rtol,
+                                  const double *strain_inc, const double *stress_old,
+                                  const double *state_old, const double *ener_intern_old,
+                                  const double *ener_inelas_old, double *seed_out,
+                                  double *stress_new, double *state_new, double *ener_intern_new,
+                                  double *ener_inelas_new, int32_t *iterations,
+                                  int32_t *fallbacks, double *elapsed_ms);
+
 /* ---- the return-map warm start (no reference counterpart) -------------------------- */
 /* An 8 → 64 → 64 → 64 → 1 ReLU network mapping (6 Voigt strain increments, ε̄ᵖ, T) to a
  * Δε̄ᵖ estimate that seeds the Newton return map (vpd_return_map_device mode 3). Trained on

@@ -78,8 +78,27 @@ def test_return_map_modes(vpd, orc):
     assert it[3] <= 3.0   # an exact seed needs only the acceptance test
 
 
+def test_screen_hardware_error_bounds(vpd):
+    """The FP32 yield screen's error budget (vpd_math.hpp) assumes the hardware log2 errs by
+    at most kScreenLog2Hw (absolute, every float in [0.5, 1]) and the hardware exp2 by at most
+    kScreenExp2Hw (relative, every float in [0, 1]); measured exhaustively on this device."""
+    import os
+    import re
+    banner("screen: hardware log2/exp2 error bounds")
+    src = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                            "vumat-pd_amd", "csrc", "vpd_math.hpp")).read()
+    bound = {k: float(re.search(r"%s = ([0-9.e+-]+);" % k, src).group(1))
+             for k in ("kScreenLog2Hw", "kScreenExp2Hw")}
+    log2_err, exp2_err = vpd.screen_accuracy()
+    print("log2 %.4e (bound %.1e), exp2 %.4e (bound %.1e)"
+          % (log2_err, bound["kScreenLog2Hw"], exp2_err, bound["kScreenExp2Hw"]))
+    # Not exact: the FP32 results are rounded, so each function errs by a fraction of an ulp.
+    assert 0.0 < log2_err <= bound["kScreenLog2Hw"]
+    assert 0.0 < exp2_err <= bound["kScreenExp2Hw"]
+
+
 @pytest.mark.parametrize("case", ["plastic", "first_yield", "heated", "high_rate", "tiny",
-                                  "reference_props"])
+                                  "reference_props", "wide"])
 def test_screened_bisection_equals_unscreened(vpd, case):
     """The parity bisection decides most iterations from an FP32 estimate with an error bound
     and evaluates the FP64 yield only where that cannot decide: every output (stress, state
@@ -106,6 +125,14 @@ def test_screened_bisection_equals_unscreened(vpd, case):
     elif case == "reference_props":
         props = JC_PROPS_REFERENCE
         state[1] = rng.random(n) * 2.0
+    elif case == "wide":  # increments over five decades, any history and temperature
+        n = 1000000
+        inc, stress, state = plastic_block(n, 12)
+        inc *= 10.0 ** rng.uniform(-4.0, 1.0, n)
+        state[1] = 10.0 ** rng.uniform(-6.0, 0.5, n)
+        state[2] = 293.0 + rng.random(n) * 1400.0
+        state[4] = np.where(rng.random(n) < 0.3, 0.0, state[4])
+        dt = 1e-5
     eint = rng.random(n)
     einel = rng.random(n)
     a = vpd.return_map(0, props, dt, inc, stress, state, eint, einel)
@@ -332,3 +359,25 @@ def test_trained_warm_start_gain(vpd):
     assert it["learned"] <= it["state5_seed"]
     assert it["network_everywhere"] < 0.8 * it["no_seed"]
     assert it["learned"] < 0.25 * it["bisection_tight"]
+
+
+def test_fused_learned_return_map_equals_two_pass(vpd):
+    # vpd_return_map_learned_device seeds inside the return-map kernel (each wavefront runs the
+    # network on its own points without a positive state 5). It must give the seeds and every
+    # output of the two-call form (vpd_mlp_seed_device, then mode 3 with those seeds), bit for
+    # bit, on a block mixing continuing and first-yield points, ragged at the end.
+    banner("learned return map: fused seeding == two-pass")
+    net = vpd.load_warm_start()
+    n = 100003
+    inc, stress, state = plastic_block(n, 21)
+    rng = np.random.default_rng(5)
+    state[4] = np.where(rng.random(n) < 0.4, 0.0, state[4])  # 40% without a state-5 seed
+    z = np.zeros(n)
+    dt = 8.5377e-5
+    a = vpd.return_map_learned(net, JC_PROPS, dt, inc, stress, state, z, z)
+    b = vpd.return_map_learned(net, JC_PROPS, dt, inc, stress, state, z, z, fused=False)
+    for k in ("seed", "stress", "state", "eint", "einel", "iterations", "fallbacks"):
+        assert np.array_equal(a[k], b[k]), k
+    first = ~(state[4] > 0.0)
+    assert np.array_equal(a["seed"][~first], state[4][~first])
+    assert (a["seed"][first] > 0).mean() > 0.5

@@ -68,6 +68,7 @@ struct vpd_mlp {
   float *dyT = nullptr, *gpart = nullptr;
   // vpd_mlp_seed_device: the count and list of the points the network seeds (grown on demand)
   int *seedCount = nullptr, *seedList = nullptr;
+  double *seedBuf = nullptr; // vpd_return_map_learned_device: the network's seed per point
   int64_t seedCap = 0;
   std::string err;
 };
@@ -157,7 +158,7 @@ struct vpd_context {
   // The compacted material update: list lengths, particle per slot, the network's slots,
   // strain increments per slot. listZeroed: the lengths are known to be zero (a force pass
   // cleared them after the last material update), so the next update need not clear them.
-  int *listCount = nullptr, *list = nullptr, *firstList = nullptr;
+  int *listCount = nullptr, *list = nullptr;
   double *incList = nullptr;
   bool listZeroed = false, capturing = false;
   std::vector<TimingRec> recs;
@@ -932,19 +933,23 @@ int ensure_list(vpd_context *c) {
   if (!c->list) {
     c->listZeroed = false;
     if (dalloc(c, c->listCount, 2) || dalloc(c, c->list, c->n) ||
-        dalloc(c, c->firstList, c->n) || dalloc(c, c->incList, 6 * c->n))
+        dalloc(c, c->incList, 6 * c->n))
       return VPD_ERR_DEVICE;
   }
   return VPD_OK;
 }
 
 // The compacted material update (k_material_screen, then the return map on the listed
-// plastic points only; see vpd_kernels.hpp): in the learned mode the network first seeds the
-// listed points that have no positive state 5.
+// plastic points only; see vpd_kernels.hpp): in the learned mode the list kernel's last
+// wavefronts run the network on the listed points without a positive state 5.
 template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g, int grid) {
   if (int s = ensure_list(c))
     return s;
-  const CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n), c->firstList};
+  CompactArgs cl{c->listCount, c->list, planes(c->incList, c->n), c->n, nullptr, nullptr};
+  if (MODE == RM_LEARNED) {
+    cl.theta = c->mlp->theta;
+    cl.norm = c->mlp->norm;
+  }
   // A captured step clears the lengths itself: its replays may follow other work.
   const bool clear = !c->listZeroed || c->capturing;
   c->listZeroed = false;
@@ -954,15 +959,6 @@ template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g,
         k_material_screen<MODE><<<grid, 256, 0, c->stream>>>(g, cl);
       }))
     return s;
-  if (MODE == RM_LEARNED) {
-    if (int s = timed(c, K_MLP_SEED, [&] {
-          k_mlp_seed<SEED_LIST><<<mlp_grid(c->n), 256, 0, c->stream>>>(
-              0, c->listCount + 1, c->firstList, c->list, c->incList, c->n, c->statev, c->n,
-              c->mlp->theta, c->mlp->norm, c->seed);
-        }))
-      return s;
-    g.seed = c->seed;
-  }
   return timed(c, K_RETURN_MAP, [&] {
     k_material_return_list<MODE><<<grid_of(c->n), 256, 0, c->stream>>>(g, cl);
   });
@@ -1309,7 +1305,6 @@ void vpd_destroy(vpd_context *c) {
   dfree(c->seed);
   dfree(c->listCount);
   dfree(c->list);
-  dfree(c->firstList);
   dfree(c->incList);
   dfree(c->broken);
   dfree(c->anyBroken);
@@ -1377,7 +1372,6 @@ int vpd_set_particles(vpd_context *c, const double *x, const double *vol, int64_
   dfree(c->fallbacks);
   dfree(c->listCount); // sized for the previous block
   dfree(c->list);
-  dfree(c->firstList);
   dfree(c->incList);
   c->haveNeighbors = c->haveShape = c->haveMaterial = c->haveDynamics = false;
   c->haveFnew = c->haveFields = c->haveForce = c->predicted = false;
@@ -3064,18 +3058,76 @@ int vpd_build_neighbors_noalloc(const double *x, int64_t n, double horizon, int3
   return VPD_OK;
 }
 
-int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *props, int nprops,
+// The largest errors of the two FP32 hardware steps of the bisection's yield screen
+// (jc_yield_screen) over every input the screen can give them: v_log_f32 on every float in
+// [0.5, 1] (absolute, against the FP64 log2; 1 is where a mantissa just below 1 rounds) and
+// v_exp_f32 on every float in [0, 1] (relative, against the FP64 exp2). Each workgroup reduces its share and makes one atomic max (the bits
+// of non-negative doubles order like their values).
+__device__ __forceinline__ void worst_of_group(double e, unsigned long long *worst) {
+  __shared__ double red[4];
+  for (int off = 32; off > 0; off >>= 1)
+    e = fmax(e, __shfl_xor(e, off));
+  if ((threadIdx.x & 63) == 0)
+    red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicMax(worst, (unsigned long long)__double_as_longlong(
+                         fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))));
+}
+__global__ void __launch_bounds__(256) k_screen_log2_error(unsigned long long *worst) {
+  double e = 0.0;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i <= (1u << 23); i += gridDim.x * 256) {
+    const float m = __uint_as_float(0x3F000000u + i); // 0.5·(1 + i·2⁻²³)
+    e = fmax(e, fabs((double)__builtin_amdgcn_logf(m) - log2((double)m)));
+  }
+  worst_of_group(e, worst);
+}
+__global__ void __launch_bounds__(256) k_screen_exp2_error(unsigned long long *worst) {
+  double e = 0.0;
+  for (uint32_t b = blockIdx.x * 256 + threadIdx.x; b <= 0x3F800000u; b += gridDim.x * 256) {
+    const float f = __uint_as_float(b); // every float in [0, 1]
+    const double ref = exp2((double)f);
+    e = fmax(e, fabs((double)__builtin_amdgcn_exp2f(f) - ref) / ref);
+  }
+  worst_of_group(e, worst);
+}
+
+int vpd_screen_accuracy(double *log2_abs_error, double *exp2_rel_error) {
+  if (!log2_abs_error || !exp2_rel_error)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "Invalid accuracy arguments.");
+  unsigned long long *worst = nullptr;
+  if (hipMalloc((void **)&worst, 2 * sizeof(unsigned long long)) != hipSuccess)
+    return fail(nullptr, VPD_ERR_DEVICE, "HIP allocation failed.");
+  unsigned long long host[2] = {0ull, 0ull};
+  hipError_t e = hipMemcpy(worst, host, sizeof(host), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    k_screen_log2_error<<<1024, 256>>>(worst);
+    k_screen_exp2_error<<<4096, 256>>>(worst + 1);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(host, worst, sizeof(host), hipMemcpyDeviceToHost);
+  (void)hipFree(worst);
+  if (e != hipSuccess)
+    return fail(nullptr, VPD_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e));
+  std::memcpy(log2_abs_error, &host[0], sizeof(double));
+  std::memcpy(exp2_rel_error, &host[1], sizeof(double));
+  return VPD_OK;
+}
+
+static int return_map_run(int mode, int64_t nblock, int nstatev, const double *props, int nprops,
                           double dt, double rtol, const double *strainInc,
                           const double *stressOld, const double *stateOld,
                           const double *enerInternOld, const double *enerInelasOld,
                           const double *seed, double *stressNew, double *stateNew,
                           double *enerInternNew, double *enerInelasNew, int32_t *iterations,
-                          int32_t *fallbacks, double *elapsedMs) {
+                          int32_t *fallbacks, double *elapsedMs, vpd_mlp *net = nullptr,
+                          double *seedOut = nullptr) {
   if (mode < RM_BISECTION || mode > VPD_RETURN_MAP_BISECTION_UNSCREENED)
     return fail(nullptr, VPD_ERR_ARGUMENT, "Unknown return-map mode.");
   if (!props || nprops < 14 || nstatev < 6)
     return fail(nullptr, VPD_ERR_ARGUMENT, "JohnsonCook VUMAT requires 14 props and 6 states");
-  if (mode == RM_LEARNED && !seed)
+  if (mode == RM_LEARNED && !seed && !net)
     return fail(nullptr, VPD_ERR_ARGUMENT, "The learned return map needs a seed array.");
   if (!(dt > 0.0))
     return fail(nullptr, VPD_ERR_ARGUMENT, "VUMAT update requires dt > 0.");
@@ -3094,6 +3146,22 @@ int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *p
   g.enerInternOld = enerInternOld;
   g.enerInelasOld = enerInelasOld;
   g.seed = seed;
+  double *seeds = nullptr; // the learned mode with the network: its seeds, then the solve
+  if (net) {
+    if (nblock > net->seedCap) {
+      dfree(net->seedList);
+      dfree(net->seedBuf);
+      net->seedCap = 0;
+      if ((!net->seedCount &&
+           hipMalloc((void **)&net->seedCount, 2 * sizeof(int)) != hipSuccess) ||
+          hipMalloc((void **)&net->seedList, sizeof(int) * (size_t)nblock) != hipSuccess ||
+          hipMalloc((void **)&net->seedBuf, sizeof(double) * (size_t)nblock) != hipSuccess)
+        return fail(nullptr, VPD_ERR_DEVICE, "HIP allocation failed.");
+      net->seedCap = nblock;
+    }
+    seeds = seedOut ? seedOut : net->seedBuf;
+    g.seed = seeds;
+  }
   g.stressNew = stressNew;
   g.stateNew = stateNew;
   g.enerInternNew = enerInternNew;
@@ -3118,6 +3186,14 @@ int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *p
     k_vumat_jc<RM_BISECTION_UNSCREENED><<<grid_of(nblock), 256>>>(g);
     break;
   default:
+    if (net) {
+      k_zero_count<<<1, 1>>>(net->seedCount);
+      k_seed_state5<<<(int)ceil_div(nblock, 2048), 256>>>(nblock, stateOld, nblock,
+                                                           net->seedCount, net->seedList, seeds);
+      k_mlp_seed<SEED_LIST><<<mlp_grid(nblock), 256>>>(0, net->seedCount, net->seedList,
+                                                       nullptr, strainInc, nblock, stateOld,
+                                                       nblock, net->theta, net->norm, seeds);
+    }
     k_vumat_jc<RM_LEARNED><<<grid_of(nblock), 256>>>(g);
   }
   hipError_t e = hipGetLastError();
@@ -3134,6 +3210,34 @@ int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *p
   if (elapsedMs)
     *elapsedMs = ms;
   return VPD_OK;
+}
+
+int vpd_return_map_device(int mode, int64_t nblock, int nstatev, const double *props, int nprops,
+                          double dt, double rtol, const double *strainInc,
+                          const double *stressOld, const double *stateOld,
+                          const double *enerInternOld, const double *enerInelasOld,
+                          const double *seed, double *stressNew, double *stateNew,
+                          double *enerInternNew, double *enerInelasNew, int32_t *iterations,
+                          int32_t *fallbacks, double *elapsedMs) {
+  return return_map_run(mode, nblock, nstatev, props, nprops, dt, rtol, strainInc, stressOld,
+                        stateOld, enerInternOld, enerInelasOld, seed, stressNew, stateNew,
+                        enerInternNew, enerInelasNew, iterations, fallbacks, elapsedMs);
+}
+
+int vpd_return_map_learned_device(vpd_mlp *mlp, int64_t nblock, int nstatev,
+                                  const double *props, int nprops, double dt, double rtol,
+                                  const double *strainInc, const double *stressOld,
+                                  const double *stateOld, const double *enerInternOld,
+                                  const double *enerInelasOld, double *seedOut,
+                                  double *stressNew, double *stateNew, double *enerInternNew,
+                                  double *enerInelasNew, int32_t *iterations, int32_t *fallbacks,
+                                  double *elapsedMs) {
+  if (!mlp)
+    return fail(nullptr, VPD_ERR_ARGUMENT, "The learned return map needs a warm-start network.");
+  return return_map_run(RM_LEARNED, nblock, nstatev, props, nprops, dt, rtol, strainInc,
+                        stressOld, stateOld, enerInternOld, enerInelasOld, nullptr, stressNew,
+                        stateNew, enerInternNew, enerInelasNew, iterations, fallbacks, elapsedMs,
+                        mlp, seedOut);
 }
 
 int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstatev, int *nfieldv,
@@ -3374,6 +3478,7 @@ void vpd_mlp_destroy(vpd_mlp *m) {
   dfree(m->order);
   dfree(m->seedCount);
   dfree(m->seedList);
+  dfree(m->seedBuf);
   if (m->stream)
     (void)hipStreamDestroy(m->stream);
   delete m;
@@ -3529,10 +3634,8 @@ int vpd_mlp_seed_device(vpd_mlp *m, const double *inc, int64_t incStride, const 
   MLPTRY(m, hipEventRecord(a, m->stream));
   // State 5 where positive, the network on the list of the other points.
   k_zero_count<<<1, 1, 0, m->stream>>>(m->seedCount);
-  const int64_t chunks = ceil_div(n, 256), groups = std::min<int64_t>(chunks, 512);
-  k_seed_state5<<<(int)groups, 256, 0, m->stream>>>(n, ceil_div(chunks, groups), state,
-                                                     stateStride, m->seedCount, m->seedList,
-                                                     seed);
+  k_seed_state5<<<(int)ceil_div(n, 2048), 256, 0, m->stream>>>(n, state, stateStride,
+                                                                m->seedCount, m->seedList, seed);
   k_mlp_seed<SEED_LIST><<<mlp_grid(n), 256, 0, m->stream>>>(
       0, m->seedCount, m->seedList, nullptr, inc, incStride, state, stateStride, m->theta,
       m->norm, seed);

@@ -8,6 +8,7 @@
 
 #include "vpd_diag.hpp"
 #include "vpd_math.hpp"
+#include "vpd_mlp.hpp"
 
 #include <hip/hip_runtime.h>
 #include <climits>
@@ -1441,17 +1442,19 @@ __global__ void __launch_bounds__(256) k_material_return(MaterialArgs g) {
 // update for every point whose return map ends at its first iteration (reusing that
 // iteration's yield) and appends the others to a list (one atomic per workgroup); the return
 // map then runs on that dense list only (k_material_return_list). In the learned mode a
-// listed point whose state 5 (its previous plastic increment) is positive is seeded with it;
-// the others (first yield, or yield again after an elastic increment) are also appended to a
-// second list of their slots, and the network runs on that list only (k_mlp_seed) before the
-// return map. Each point's arithmetic is that of the fused kernel, so the result is bit for
-// bit the same and does not depend on the list order.
+// listed point whose state 5 (its previous plastic increment) is positive is seeded with it
+// and goes to the front of the list; the others (first yield, or yield again after an
+// elastic increment) go to its back; the list kernel visits the back first, so its first
+// wavefronts hold only them and run the network on them (wave_network_seed) before their
+// Newton solves; no other wavefront loads the network. Each point's arithmetic is that of the fused kernel, so the result is
+// bit for bit the same and does not depend on the list order.
 struct CompactArgs {
-  int *count;   // count[0]: listed points; count[1]: slots in `first` (8-byte aligned, both
-                // zeroed before the screen)
-  int *list;    // particle index per slot
-  Planes inc;   // strain increments per slot (6 planes of the particle-count capacity)
-  int *first;   // learned mode: the slots whose state 5 is not positive
+  int *count;   // count[0]: slots from the front, count[1]: slots from the back (one 8-byte
+                // word, zeroed before the screen)
+  int *list;    // particle index per slot; back slot b is list[capacity − 1 − b]
+  Planes inc;   // strain increments per slot (6 planes of the capacity)
+  int64_t capacity;
+  const float *theta, *norm; // learned mode: the network (vpd_mlp)
 };
 
 __global__ void k_zero_count(int *count) {
@@ -1459,16 +1462,21 @@ __global__ void k_zero_count(int *count) {
   count[1] = 0;
 }
 
-// Appends to two lists behind one 64-bit counter (count[0] in its low half, count[1] in its
-// high half), with one device atomic per workgroup for both: returns this thread's positions
-// for `takeA` (list 0) and `takeB` (list 1). Every thread of the workgroup calls; `base` holds
-// 10 ints of LDS. Appends from many workgroups to one address serialise in the memory
-// system, so a kernel appends once per workgroup at most.
-struct Appended {
-  int a, b;
+// Slot of list position t, or −1 past the end: the back's slots first (in the learned mode
+// the network's points, whose wavefronts then start first and overlap the network's latency
+// with the rest of the list), then the front's.
+__device__ __forceinline__ int64_t list_slot(const CompactArgs &cl, int64_t t, int front,
+                                             int back) {
+  return t < back ? cl.capacity - 1 - t : t < (int64_t)front + back ? t - back : -1;
+}
+
+// Positions within this workgroup's share of two lists: every thread calls; returns the
+// thread's rank among the workgroup's threads taking A (and B), and the workgroup's totals.
+// `base` holds 8 ints of LDS.
+struct GroupRanks {
+  int a, b, totalA, totalB;
 };
-__device__ __forceinline__ Appended group_append2(bool takeA, bool takeB, int *count,
-                                                  int *base) {
+__device__ __forceinline__ GroupRanks group_ranks(bool takeA, bool takeB, int *base) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint64_t ma = __ballot(takeA), mb = __ballot(takeB);
   if (lane == 0) {
@@ -1476,26 +1484,42 @@ __device__ __forceinline__ Appended group_append2(bool takeA, bool takeB, int *c
     base[4 + wv] = __popcll(mb);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int ta = 0, tb = 0;
-    for (int w = 0; w < 4; ++w) {
-      const int ca = base[w], cb = base[4 + w];
-      base[w] = ta;
-      base[4 + w] = tb;
-      ta += ca;
-      tb += cb;
+  GroupRanks r{0, 0, 0, 0};
+  for (int w = 0; w < 4; ++w) {
+    if (w < wv) {
+      r.a += base[w];
+      r.b += base[4 + w];
     }
+    r.totalA += base[w];
+    r.totalB += base[4 + w];
+  }
+  const uint64_t below = (1ull << lane) - 1ull;
+  r.a += (int)__popcll(ma & below);
+  r.b += (int)__popcll(mb & below);
+  __syncthreads(); // `base` may be reused
+  return r;
+}
+
+// One append per workgroup to two lists behind one 64-bit counter (count[0] in its low half,
+// count[1] in its high half): returns this thread's positions for `takeA` and `takeB`. Every
+// thread of the workgroup calls; `base` holds 10 ints of LDS. Appends from many workgroups to
+// one address serialise in the memory system, so a kernel appends once per workgroup at most.
+struct Appended {
+  int a, b;
+};
+__device__ __forceinline__ Appended group_append2(bool takeA, bool takeB, int *count,
+                                                  int *base) {
+  const GroupRanks r = group_ranks(takeA, takeB, base);
+  if (threadIdx.x == 0) {
     unsigned long long old = 0;
-    if (ta + tb > 0)
+    if (r.totalA + r.totalB > 0)
       old = atomicAdd((unsigned long long *)count,
-                      (unsigned long long)ta + ((unsigned long long)tb << 32));
+                      (unsigned long long)r.totalA + ((unsigned long long)r.totalB << 32));
     base[8] = (int)(old & 0xFFFFFFFFull);
     base[9] = (int)(old >> 32);
   }
   __syncthreads();
-  const uint64_t below = (1ull << lane) - 1ull;
-  const Appended pos{base[8] + base[wv] + (int)__popcll(ma & below),
-                     base[9] + base[4 + wv] + (int)__popcll(mb & below)};
+  const Appended pos{base[8] + r.a, base[9] + r.b};
   __syncthreads(); // `base` may be reused
   return pos;
 }
@@ -1519,41 +1543,39 @@ __device__ __forceinline__ int group_inclusive_sum(int v, int *red) {
 }
 
 // The first pass of a listed network seed over n points (k_mlp_seed<SEED_LIST> is the
-// second): seed[s] = state 5 where it is positive; the other points are appended to
-// `entries` (count[0] zeroed before). Workgroup g handles the 256-point chunks
-// [g·per, (g+1)·per): it counts, appends its total with one atomic, then writes the entries
-// (each thread its own points, in order).
-__global__ void __launch_bounds__(256) k_seed_state5(int64_t n, int64_t per,
-                                                     const double *__restrict__ state,
+// second): seed[s] = state 5 (component 4 of the state planes, stride stateStride) where it
+// is positive; the other points are appended to `entries` (count[0] zeroed before). Each
+// thread takes 8 consecutive points, their loads in flight together, and each workgroup
+// appends its share with one atomic.
+__global__ void __launch_bounds__(256) k_seed_state5(int64_t n, const double *__restrict__ state,
                                                      int64_t stateStride, int *count,
                                                      int *entries, double *seed) {
   __shared__ int red[4];
   __shared__ int groupBase;
-  const int64_t c0 = (int64_t)blockIdx.x * per;
-  const int64_t c1 = min(c0 + per, (n + 255) / 256);
-  int mine = 0;
-  for (int64_t c = c0; c < c1; ++c) {
-    const int64_t s = c * 256 + threadIdx.x;
-    if (s < n) {
-      const double prev = state[4 * stateStride + s];
-      if (prev > 0.0)
-        seed[s] = prev;
-      else
-        ++mine;
+  const int64_t p0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 8;
+  double prev[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    prev[j] = p0 + j < n ? state[4 * stateStride + p0 + j] : 1.0;
+  uint32_t mine = 0; // bit j: point p0 + j takes the network's seed
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (prev[j] > 0.0) {
+      if (p0 + j < n)
+        seed[p0 + j] = prev[j];
+    } else {
+      mine |= 1u << j;
     }
   }
-  const int incl = group_inclusive_sum(mine, red);
+  const int cnt = __popc(mine);
+  const int incl = group_inclusive_sum(cnt, red);
   if (threadIdx.x == 255)
     groupBase = incl > 0 ? atomicAdd(count, incl) : 0;
   __syncthreads();
-  if (mine == 0)
-    return;
-  int pos = groupBase + incl - mine;
-  for (int64_t c = c0; c < c1; ++c) {
-    const int64_t s = c * 256 + threadIdx.x;
-    if (s < n && !(state[4 * stateStride + s] > 0.0))
-      entries[pos++] = (int)s;
-  }
+  int pos = groupBase + incl - cnt;
+  for (int j = 0; j < 8; ++j)
+    if ((mine >> j) & 1u)
+      entries[pos++] = (int)(p0 + j);
 }
 
 template <int MODE>
@@ -1622,42 +1644,61 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
     double sNew[6];
     double eint = g.eint[i], einel = g.einel[i];
     ReturnMapStats stats;
-    jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats, &first);
+    jc_point<MODE, true>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats,
+                         &first);
     material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
   }
-  // In the learned mode the network's points (no positive state-5 seed) also go to the
-  // second list, with the same append.
-  const bool net = MODE == RM_LEARNED && plastic && !(st[4] > 0.0);
-  const Appended pos = group_append2(plastic, net, cl.count, base);
+  // Learned mode: the points the network seeds (no positive state 5) go to the back.
+  const bool back = MODE == RM_LEARNED && plastic && !(st[4] > 0.0);
+  const Appended pos = group_append2(plastic && !back, back, cl.count, base);
   if (plastic) {
-    cl.list[pos.a] = (int)i;
+    const int64_t slot = back ? cl.capacity - 1 - pos.b : pos.a;
+    cl.list[slot] = (int)i;
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-      cl.inc(c, pos.a) = inc[c];
+      cl.inc(c, slot) = inc[c];
   }
-  if (net)
-    cl.first[pos.b] = pos.a;
 }
 
 template <int MODE>
 __global__ void __launch_bounds__(256) k_material_return_list(MaterialArgs g, CompactArgs cl) {
-  const int64_t slot = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= *cl.count)
-    return;
-  const int64_t i = cl.list[slot];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int front = cl.count[0], back = cl.count[1];
+  if (t - (threadIdx.x & 63) >= (int64_t)front + back)
+    return; // a wavefront past the list (the others keep every lane: the network needs them)
+  const int64_t slot = list_slot(cl, t, front, back);
+  const bool valid = slot >= 0;
+  int64_t i = 0;
   double inc[6], sOld[6], sNew[6], st[6];
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    inc[c] = cl.inc(c, slot);
-    sOld[c] = g.sig(c, i);
-    st[c] = g.statev(c, i);
+  for (int c = 0; c < 6; ++c)
+    inc[c] = sOld[c] = st[c] = 0.0;
+  if (valid) {
+    i = cl.list[slot];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      inc[c] = cl.inc(c, slot);
+      sOld[c] = g.sig(c, i);
+      st[c] = g.statev(c, i);
+    }
   }
+  double seed = 0.0;
+  if constexpr (MODE == RM_LEARNED) {
+    // State 5 where positive (the front); the network on the back's points, which fill the
+    // list's first wavefronts.
+    __shared__ float xs[4][64 * MLP_IN];
+    __shared__ double ys[4][64];
+    const bool net = valid && t < back;
+    const int wv = threadIdx.x >> 6;
+    const double est = wave_network_seed(net, inc, st[1], st[2], cl.theta, cl.norm, xs[wv], ys[wv]);
+    seed = net ? est : st[4];
+  }
+  if (!valid)
+    return;
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
   const Ramp none{0.0, 0.0, 0.0};
   const double dt = point_step(g.batch, i, g.dt, none, none).dt;
-  // Learned mode: state 5 where positive, else the network's estimate for this slot.
-  const double seed = MODE != RM_LEARNED ? 0.0 : st[4] > 0.0 ? st[4] : g.seed[slot];
   jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, seed, g.rtol, &stats);
   T3 f1, r1;
 #pragma unroll
@@ -1692,12 +1733,12 @@ struct ForceArgs {
   DiagArgs diag;
   Batch batch;
   int take; // windowed gathers: bonds a lane takes from one window at most (see window_stop)
-  // When set, the pass clears these two list lengths of the compacted material update (the
+  // When set, the pass clears the two list lengths of the compacted material update (the
   // step's material stage has run; the next one appends from zero).
   int *zeroCount = nullptr;
 };
 
-__device__ __forceinline__ void clear_list_lengths(const ForceArgs &g) {
+__device__ __forceinline__ void clear_list_length(const ForceArgs &g) {
   if (g.zeroCount && blockIdx.x == 0 && threadIdx.x == 0) {
     g.zeroCount[0] = 0;
     g.zeroCount[1] = 0;
@@ -1882,7 +1923,7 @@ __device__ __forceinline__ void force_one(const ForceArgs &g, int64_t i, double 
 // reaches the chunk's reduction.
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE = true, bool DIAG = false>
 __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
-  clear_list_lengths(g);
+  clear_list_length(g);
   static_assert(!DIAG || INTEGRATE, "the diagnostics read the corrected velocity");
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
   if (blk < 0)
@@ -1915,7 +1956,7 @@ __global__ void __launch_bounds__(256) k_force(ForceArgs g) {
 // lanes in the window steps; the rounds loop exits on the wavefront's first particle.
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W>
 __global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
-  clear_list_lengths(g);
+  clear_list_length(g);
   extern __shared__ d2v wlds[];
   d2v *win = wlds + wave_slot() * (7 * W);
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
@@ -1948,7 +1989,7 @@ __global__ void __launch_bounds__(256) k_force_w(ForceArgs g) {
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W, bool PREFETCH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFETCH ? 1 : 5)))
     k_force_p(ForceArgs g, WindowPlan plan) {
-  clear_list_lengths(g);
+  clear_list_length(g);
   extern __shared__ d2v wlds[];
   d2v *win = wlds + wave_slot() * (7 * W); // (unused with PREFETCH)
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
@@ -2010,7 +2051,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREFET
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int W>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(group_waves<7, W>())))
     k_force_g(ForceArgs g, GroupPlan plan) {
-  clear_list_lengths(g);
+  clear_list_length(g);
   extern __shared__ d2v wlds[];
   const int64_t blk = xcd_block(g.rd.blocksPerRound);
   if (blk < 0)
@@ -2250,11 +2291,9 @@ struct VumatArgs {
   int *iterations, *fallbacks;
 };
 
+// One point of the device VUMAT: the Johnson-Cook update of point k with the given seed.
 template <int MODE>
-__global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.nblock)
-    return;
+__device__ __forceinline__ void vumat_jc_point(const VumatArgs &g, int64_t k, double seed) {
   const int64_t nb = g.nblock;
   double inc[6], sOld[6], sNew[6], st[6];
 #pragma unroll
@@ -2276,8 +2315,7 @@ __global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
       g.fieldNew[k + nb * c] = g.fieldOld[k + nb * c];
   double eint = g.enerInternOld[k], einel = g.enerInelasOld[k];
   ReturnMapStats stats;
-  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, g.seed ? g.seed[k] : 0.0, g.rtol,
-                 &stats);
+  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, seed, g.rtol, &stats);
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     g.stressNew[k + nb * c] = sNew[c];
@@ -2291,6 +2329,14 @@ __global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
     g.iterations[k] = stats.iterations;
   if (g.fallbacks)
     g.fallbacks[k] = stats.fallbacks;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.nblock)
+    return;
+  vumat_jc_point<MODE>(g, k, g.seed ? g.seed[k] : 0.0);
 }
 
 } // namespace vpd

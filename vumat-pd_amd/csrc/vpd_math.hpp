@@ -306,20 +306,30 @@ __device__ __forceinline__ double jc_yield(const JcParams &p, double ep, double 
 // evaluated yield; so an iteration whose residual is far from 0 and from ±tol can take its
 // decision from an estimate. The estimate forms (A + B ε̄ⁿ)·soft·(1 + C ln ρ), ρ = Δ/(dt ε̇₀),
 // with log2 x = e + log2(m) for x = m·2^e (m in [0.5, 1), exponent exact, the FP32 hardware
-// log2 of m) and 2^t = 2^⌊t⌋·2^(t − ⌊t⌋) (the FP32 hardware exp2 of the fraction); the
-// rest is FP64. With the hardware log2/exp2 within 2 ulp (about 2.4e-7), |log2 x| errs by at
-// most 3e-7, ε̄ⁿ by a relative (0.21·|n| + 0.28)·1e-6 and ln ρ by 2.1e-7; `margin` bounds the
-// estimate's distance from the exact FP64 value with at least 5x of slack on those terms:
-//   1e-5·|soft|·[(|A| + (1 + |n|)·|B ε̄ⁿ|)·|1 + C ln ρ| + (|A| + |B ε̄ⁿ|)·|C|·(|ln ρ| + 1)].
+// log2 of m rounded to FP32) and 2^t = 2^⌊t⌋·2^(t − ⌊t⌋) (the FP32 hardware exp2 of the
+// fraction rounded to FP32); the rest is FP64. Error budget (FP32 steps; FP64 rounding is
+// covered by the caller's 1e-14 term):
+//  * log2 x: the hardware log2 errs by at most kScreenLog2Hw on every float in [0.5, 1]
+//    (measured exhaustively, vpd_screen_accuracy), and rounding m to FP32 moves log2 m by at
+//    most 2⁻²⁴/ln 2;
+//  * 2^f: the hardware exp2 errs by a relative kScreenExp2Hw at most on every float in
+//    [0, 1], and rounding t − ⌊t⌋ to FP32 moves it by 2⁻²⁵ at most;
+//  so ε̄ⁿ errs by a relative e_pw = ln 2·(|n|·e_log + 2⁻²⁵) + kScreenExp2Hw and ln ρ by
+// e_lr = ln 2·e_log absolutely (e_log = kScreenLog2Hw + 2⁻²⁴/ln 2), and the estimate by at most
+//   |soft|·(|B ε̄ⁿ|·|1 + C ln ρ|·e_pw + |A + B ε̄ⁿ|·|C|·e_lr)
+// to first order; `margin` is four times that.
 // Returns false (no estimate) for arguments outside the range the FP32 steps cover.
+constexpr double kScreenLog2Hw = 6.0e-8; // ≥ 5.945e-8 measured on MI355X (0.997·2⁻²⁴)
+constexpr double kScreenExp2Hw = 8.4e-8; // ≥ 8.317e-8 measured on MI355X (0.698·2⁻²³)
 __device__ __forceinline__ double screen_log2(double x) { // x > 0, normal
   const float m = (float)__builtin_amdgcn_frexp_mant(x);
   return (double)__builtin_amdgcn_frexp_exp(x) + (double)__builtin_amdgcn_logf(m);
 }
-__device__ __forceinline__ bool jc_yield_screen(const JcParams &p, double ep, double dep,
-                                                double rateScale, double soft, double &y,
-                                                double &margin) {
-  double pw = 0.0; // ε̄ⁿ
+// The estimate's two transcendental terms: ε̄ⁿ (pw) and ln ρ (lr, 0 for Δ = 0: the rate is
+// then ε̇₀). False for arguments outside the range the FP32 steps cover.
+__device__ __forceinline__ bool screen_terms(const JcParams &p, double ep, double dep,
+                                             double rateScale, double &pw, double &lr) {
+  pw = 0.0;
   if (ep != 0.0) {
     if (!(ep > 1e-300 && ep < 1e300))
       return false;
@@ -331,21 +341,54 @@ __device__ __forceinline__ bool jc_yield_screen(const JcParams &p, double ep, do
   } else if (!(p.hardN > 0.0)) {
     return false; // pow(0, n) for n <= 0 is left to the exact evaluation
   }
-  double lr = 0.0; // ln ρ (0 for Δ = 0: the rate is then ε̇₀)
+  lr = 0.0;
   if (dep != 0.0) {
     const double rho = dep * rateScale;
     if (!(rho > 1e-300 && rho < 1e300))
       return false;
     lr = 0.6931471805599453 * screen_log2(rho);
   }
+  return true;
+}
+// Four times the first-order error bound of the estimate (above).
+__device__ __forceinline__ double screen_margin(const JcParams &p, double soft, double bpw,
+                                                double hard, double rfac) {
+  constexpr double ln2 = 0.6931471805599453;
+  constexpr double eLog = kScreenLog2Hw + 5.9604644775390625e-8 / ln2; // + 2⁻²⁴/ln 2
+  const double ePw = ln2 * (fabs(p.hardN) * eLog + 2.98023223876953125e-8) + kScreenExp2Hw;
+  return 4.0 * fabs(soft) *
+         (fabs(bpw) * fabs(rfac) * ePw + fabs(hard) * fabs(p.rateC) * (ln2 * eLog));
+}
+__device__ __forceinline__ bool jc_yield_screen(const JcParams &p, double ep, double dep,
+                                                double rateScale, double soft, double &y,
+                                                double &margin) {
+  double pw, lr;
+  if (!screen_terms(p, ep, dep, rateScale, pw, lr))
+    return false;
   const double bpw = p.b0 * pw;
   const double hard = p.a0 + bpw;
   const double rfac = 1.0 + p.rateC * lr;
   y = hard * soft * rfac;
-  const double absA = fabs(p.a0), absB = fabs(bpw);
-  margin = 1e-5 * fabs(soft) *
-           ((absA + (1.0 + fabs(p.hardN)) * absB) * fabs(rfac) +
-            (absA + absB) * fabs(p.rateC) * (fabs(lr) + 1.0));
+  margin = screen_margin(p, soft, bpw, hard, rfac);
+  return true;
+}
+// The estimate with its slope dY/dΔ at Δ = dep > 0 (jc_yield_slope's expressions on the
+// estimated terms), for the FP32 phase of the Newton solvers.
+__device__ __forceinline__ bool jc_yield_slope_screen(const JcParams &p, double epOld,
+                                                      double dep, double rateScale,
+                                                      double soft, double &y, double &dy,
+                                                      double &margin) {
+  const double ep = epOld + dep;
+  double pw, lr;
+  if (!screen_terms(p, ep, dep, rateScale, pw, lr))
+    return false;
+  const double bpw = p.b0 * pw;
+  const double hard = p.a0 + bpw;
+  const double rfac = 1.0 + p.rateC * lr;
+  y = hard * soft * rfac;
+  const double dHard = ep > 0.0 ? p.b0 * p.hardN * pw / ep : 0.0;
+  dy = soft * (dHard * rfac + hard * p.rateC / dep);
+  margin = screen_margin(p, soft, bpw, hard, rfac);
   return true;
 }
 
@@ -356,8 +399,10 @@ __device__ __forceinline__ bool jc_yield_screen(const JcParams &p, double ep, do
 //                 plain return map at the fast modes' accuracy.
 //   RM_NEWTON     safeguarded Newton on r(Δ) = (q − 3μΔ) − max(Y(ε̄+Δ, Δ/dt, T), Y_old)
 //                 inside the bracket [0, q/3μ], seeded with state 5 (the reference's seed),
-//                 falling back to bisection whenever a step leaves the bracket; stops at
-//                 |r| <= rtol·q_trial.
+//                 falling back to bisection whenever a step leaves the bracket; its first
+//                 steps use the screen's FP32 estimate of Y and dY/dΔ (while the estimate
+//                 decides the residual's sign), the rest the FP64 yield; stops at
+//                 |r| <= rtol·q_trial on an FP64 residual. Iterations count both kinds.
 //   RM_LEARNED    RM_NEWTON seeded with an external Δε̄ᵖ estimate (the learned warm start).
 // Both bisections screen their iterations (jc_yield_screen): an iteration evaluates the FP64
 // yield only when the estimate cannot decide, or when the loop ends there; iterates, the
@@ -468,8 +513,11 @@ __device__ __forceinline__ bool jc_elastic(const JcParams &p, const double inc[6
 // (in/out), energies (in/out); writes stressNew[6]; returns the iteration count (0 for the
 // init branch). The bookkeeping after the solve is the reference's for every solver.
 // `first`, when given, holds jc_elastic's evaluation of the first iteration's 1 − θᵐ and
-// yield for the same state (the same bits jc_point would compute).
-template <int MODE>
+// yield for the same state (the same bits jc_point would compute). FIRST_EXIT: the caller
+// knows (jc_elastic) that the solve ends at its first iteration with Δε̄ᵖ = 0 and passes
+// `first`; the solver loops are then left out and that iteration's outcome is taken
+// (iteration 1, Δε̄ᵖ = 0, the stored yield first->yield), the bits the full routine gives.
+template <int MODE, bool FIRST_EXIT = false>
 __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const double inc[6],
                                         const double stressOld[6], double state[6],
                                         double &eint, double &einel, double stressNew[6],
@@ -512,7 +560,11 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
 
   double epInc = 0.0, yield = oldYield;
   int iteration = 0;
-  if (MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT || MODE == RM_BISECTION_UNSCREENED) {
+  if constexpr (FIRST_EXIT) {
+    iteration = 1;
+    yield = first->yield;
+  } else if (MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT ||
+             MODE == RM_BISECTION_UNSCREENED) {
     constexpr bool SCREEN = MODE != RM_BISECTION_UNSCREENED;
     const double tol = MODE == RM_BISECTION_TIGHT ? rtol * qTrial : p.tolerance;
     const int maxIt = MODE == RM_BISECTION_TIGHT ? 200 : 50;
@@ -592,6 +644,34 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
       if (!(d > lo && d < hi))
         d = 0.5 * (lo + hi);
       int fallbacks = 0;
+      // FP32 phase: Newton steps on the screen's estimate of the yield and its slope while
+      // the estimated residual lies farther from 0 than twice the estimate's error bound (its
+      // sign is then the exact residual's, so the bracket stays valid); the FP64 Newton
+      // below finishes from there, usually in two evaluations.
+      const double rateScale = 1.0 / (dt * p.rate0);
+      for (int it = 0; it < 16; ++it) {
+        double Y, dY, margin;
+        if (!jc_yield_slope_screen(p, epOld, d, rateScale, soft, Y, dY, margin))
+          break;
+        const bool capped = Y < oldYield;
+        const double r = (qTrial - 3.0 * p.mu * d) - (capped ? oldYield : Y);
+        if (fabs(r) <= 2.0 * margin + 1e-14 * (qTrial + fabs(Y)))
+          break;
+        ++iteration;
+        if (r > 0.0)
+          lo = d;
+        else
+          hi = d;
+        const double slope = -3.0 * p.mu - (capped ? 0.0 : dY);
+        double next = d - r / slope;
+        if (!(next > lo && next < hi)) {
+          next = 0.5 * (lo + hi);
+          ++fallbacks;
+        }
+        if (next == d)
+          break;
+        d = next;
+      }
       for (int it = 0; it < 60; ++it) {
         ++iteration;
         double Y, dY;

@@ -278,6 +278,56 @@ __device__ __forceinline__ bf16x4 normalised_bf16(const double x[4], const doubl
   return bf16_of(v);
 }
 
+// The network's estimate for the lanes of one wavefront that ask for it (`need`), inside a
+// kernel that holds their inputs in registers: 6 strain increments, ε̄ᵖ and T, normalised in
+// FP64 and rounded to FP32 exactly as k_mlp_seed does. Those lanes' inputs go to the
+// wavefront's LDS (`xs`: 64 × 8 floats, `ys`: 64 doubles), then 16-point tiles of them run
+// through mlp_tile (up to 4 tiles) with every lane active, and each asking lane gets back
+// max(0, y·std + mean), the value k_mlp_seed writes for the same point (one MFMA column per
+// point: the tile a point shares does not change its bits). Every lane of the wavefront must
+// call (the branch on the ballot is uniform); a wavefront with no asking lane returns at once.
+__device__ __forceinline__ double wave_network_seed(bool need, const double inc[6], double ep,
+                                                    double temp,
+                                                    const float *__restrict__ theta,
+                                                    const float *__restrict__ norm, float *xs,
+                                                    double *ys) {
+  const uint64_t mask = __ballot(need);
+  if (mask == 0)
+    return 0.0;
+  const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  if (need) {
+    const double x[MLP_IN] = {inc[0], inc[1], inc[2], inc[3], inc[4], inc[5], ep, temp};
+#pragma unroll
+    for (int c = 0; c < MLP_IN; ++c)
+      xs[MLP_IN * rank + c] =
+          (float)((x[c] - (double)norm[c]) * (1.0 / (double)norm[MLP_IN + c]));
+  }
+  MlpWeights w;
+  mlp_load_weights(theta, lane, w);
+  const double mean = norm[2 * MLP_IN], sdev = norm[2 * MLP_IN + 1];
+  const int count = __popcll(mask);
+  __builtin_amdgcn_s_waitcnt(0xC07F); // lgkmcnt(0): this wavefront's LDS stores are done
+  __builtin_amdgcn_wave_barrier();
+  for (int t = 0; 16 * t < count; ++t) {
+    const int pt = 16 * t + r;
+    f32x4 v = {0.f, 0.f, 0.f, 0.f}; // a missing point normalises to 0, as in k_mlp_seed
+    if (q < 2 && pt < count)
+      v = *(const f32x4 *)(xs + MLP_IN * pt + 4 * q);
+    bf16x4 x = {0, 0, 0, 0};
+    if (q < 2)
+      x = bf16_of(v);
+    const float y = mlp_tile(w, x);
+    if (lane < 16 && pt < count) {
+      const double est = (double)y * sdev + mean;
+      ys[pt] = est > 0.0 ? est : 0.0;
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_wave_barrier();
+  return need ? ys[rank] : 0.0;
+}
+
 // Return-map seeds. A point whose previous plastic increment (state 5, component 4 of the
 // JC state planes, stride stateStride) is positive keeps it as its seed: on a continuing
 // plastic path it is closer to the solution than the network's estimate (about 2% median

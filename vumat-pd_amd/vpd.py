@@ -150,6 +150,9 @@ def lib():
         L.vpd_return_map_device.argtypes = ([C.c_int, C.c_int64, C.c_int, _D, C.c_int, C.c_double,
                                              C.c_double] + [C.c_void_p] * 12 +
                                             [C.POINTER(C.c_double)])
+        L.vpd_return_map_learned_device.argtypes = ([P, C.c_int64, C.c_int, _D, C.c_int,
+                                                     C.c_double, C.c_double] + [C.c_void_p] * 12
+                                                    + [C.POINTER(C.c_double)])
         L.vpd_build_neighbors_noalloc.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_void_p,
                                                   C.c_void_p, C.c_int64,
                                                   C.POINTER(C.c_int64), C.c_void_p]
@@ -611,6 +614,15 @@ def _return_map_on_device(mode, props, dt, nb, nst, ins, seed_ptr, rtol):
                 einel=outs[3].read(), iterations=iters, fallbacks=fbs, ms=ms.value)
 
 
+def screen_accuracy():
+    """vpd_screen_accuracy: (largest absolute error of the FP32 hardware log2 on [0.5, 1],
+    largest relative error of the FP32 hardware exp2 on [0, 1]), measured on the device."""
+    a, b = C.c_double(), C.c_double()
+    if lib().vpd_screen_accuracy(C.byref(a), C.byref(b)):
+        raise VpdError(lib().vpd_last_error(None).decode())
+    return a.value, b.value
+
+
 def return_map(mode, props, dt, strain_inc, stress_old, state_old, eint_old, einel_old,
                seed=None, rtol=1e-12):
     """vpd_return_map_device on device copies of column-major blocks (6, nb) / (nstatev, nb).
@@ -624,14 +636,29 @@ def return_map(mode, props, dt, strain_inc, stress_old, state_old, eint_old, ein
 
 
 def return_map_learned(net, props, dt, strain_inc, stress_old, state_old, eint_old, einel_old,
-                       rtol=1e-12):
-    """The learned return map on a device block: the network's bf16 MFMA seeds
-    (vpd_mlp_seed_device, timed) feed vpd_return_map_device's Newton solve (timed). Returns
-    return_map's dict plus seed_ms and the seeds."""
+                       rtol=1e-12, fused=True):
+    """The learned return map on a device block. fused: vpd_return_map_learned_device, one
+    timed kernel that seeds (state 5 where positive, the network elsewhere) and solves; else
+    vpd_mlp_seed_device (timed as seed_ms) then vpd_return_map_device mode 3 (timed). Returns
+    return_map's dict plus the seeds (and seed_ms for the two-call form)."""
     nb = strain_inc.shape[1]
     D = DeviceBuffer
     ins = [D(strain_inc), D(stress_old), D(state_old), D(eint_old), D(einel_old)]
     seed = D(np.zeros(nb))
+    if fused:
+        nst = state_old.shape[0]
+        outs = [D(np.zeros((6, nb))), D(np.zeros((nst, nb))), D(np.zeros(nb)), D(np.zeros(nb))]
+        it = D(np.zeros(nb))
+        fb = D(np.zeros(nb))
+        ms = C.c_double()
+        if lib().vpd_return_map_learned_device(net.h, nb, nst, _f64(props), len(props), dt, rtol,
+                                               *[b.ptr for b in ins], seed.ptr,
+                                               *[b.ptr for b in outs], it.ptr, fb.ptr,
+                                               C.byref(ms)):
+            raise VpdError(lib().vpd_last_error(None).decode())
+        return dict(stress=outs[0].read(), state=outs[1].read(), eint=outs[2].read(),
+                    einel=outs[3].read(), iterations=it.read().view(np.int32)[:nb],
+                    fallbacks=fb.read().view(np.int32)[:nb], ms=ms.value, seed=seed.read())
     seed_ms = C.c_double()
     if lib().vpd_mlp_seed_device(net.h, ins[0].ptr, nb, ins[2].ptr, nb, nb, seed.ptr,
                                  C.byref(seed_ms)):
