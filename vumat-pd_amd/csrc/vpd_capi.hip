@@ -66,9 +66,7 @@ struct vpd_mlp {
   short *xT = nullptr, *h1T = nullptr, *h2T = nullptr, *h3T = nullptr, *dz1T = nullptr,
         *dz2T = nullptr, *dz3T = nullptr;
   float *dyT = nullptr, *gpart = nullptr;
-  // vpd_mlp_seed_device: the count and list of the points the network seeds (grown on demand)
-  int *seedCount = nullptr, *seedList = nullptr;
-  double *seedBuf = nullptr; // vpd_return_map_learned_device: the network's seed per point
+  double *seedBuf = nullptr; // vpd_return_map_learned_device: the seed per point (on demand)
   int64_t seedCap = 0;
   std::string err;
 };
@@ -1004,9 +1002,8 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
                       [&] { k_material<RM_LEARNED, true, true><<<grid, 256, 0, c->stream>>>(g); }))
       return s;
     if (int s = timed(c, K_MLP_SEED, [&] {
-          k_mlp_seed<SEED_ALL><<<mlp_grid(c->n), 256, 0, c->stream>>>(
-              c->n, nullptr, nullptr, nullptr, c->inc, c->n, c->statev, c->n, c->mlp->theta,
-              c->mlp->norm, c->seed);
+          k_mlp_seed<<<mlp_grid(c->n), 256, 0, c->stream>>>(
+              c->n, c->inc, c->n, c->statev, c->n, c->mlp->theta, c->mlp->norm, c->seed);
         }))
       return s;
     g.seed = c->seed;
@@ -3148,14 +3145,10 @@ static int return_map_run(int mode, int64_t nblock, int nstatev, const double *p
   g.seed = seed;
   double *seeds = nullptr; // the learned mode with the network: its seeds, then the solve
   if (net) {
-    if (nblock > net->seedCap) {
-      dfree(net->seedList);
+    if (!seedOut && nblock > net->seedCap) {
       dfree(net->seedBuf);
       net->seedCap = 0;
-      if ((!net->seedCount &&
-           hipMalloc((void **)&net->seedCount, 2 * sizeof(int)) != hipSuccess) ||
-          hipMalloc((void **)&net->seedList, sizeof(int) * (size_t)nblock) != hipSuccess ||
-          hipMalloc((void **)&net->seedBuf, sizeof(double) * (size_t)nblock) != hipSuccess)
+      if (hipMalloc((void **)&net->seedBuf, sizeof(double) * (size_t)nblock) != hipSuccess)
         return fail(nullptr, VPD_ERR_DEVICE, "HIP allocation failed.");
       net->seedCap = nblock;
     }
@@ -3186,14 +3179,9 @@ static int return_map_run(int mode, int64_t nblock, int nstatev, const double *p
     k_vumat_jc<RM_BISECTION_UNSCREENED><<<grid_of(nblock), 256>>>(g);
     break;
   default:
-    if (net) {
-      k_zero_count<<<1, 1>>>(net->seedCount);
-      k_seed_state5<<<(int)ceil_div(nblock, 2048), 256>>>(nblock, stateOld, nblock,
-                                                           net->seedCount, net->seedList, seeds);
-      k_mlp_seed<SEED_LIST><<<mlp_grid(nblock), 256>>>(0, net->seedCount, net->seedList,
-                                                       nullptr, strainInc, nblock, stateOld,
-                                                       nblock, net->theta, net->norm, seeds);
-    }
+    if (net)
+      k_seed_block<<<(int)ceil_div(nblock, 2048), 256>>>(nblock, strainInc, nblock, stateOld,
+                                                          nblock, net->theta, net->norm, seeds);
     k_vumat_jc<RM_LEARNED><<<grid_of(nblock), 256>>>(g);
   }
   hipError_t e = hipGetLastError();
@@ -3476,8 +3464,6 @@ void vpd_mlp_destroy(vpd_mlp *m) {
   dfree(m->dyT);
   dfree(m->gpart);
   dfree(m->order);
-  dfree(m->seedCount);
-  dfree(m->seedList);
   dfree(m->seedBuf);
   if (m->stream)
     (void)hipStreamDestroy(m->stream);
@@ -3624,21 +3610,11 @@ int vpd_mlp_seed_device(vpd_mlp *m, const double *inc, int64_t incStride, const 
   hipEvent_t a, b;
   MLPTRY(m, hipEventCreate(&a));
   MLPTRY(m, hipEventCreate(&b));
-  if (n > m->seedCap) {
-    dfree(m->seedList);
-    m->seedCap = 0;
-    if ((!m->seedCount && malloc_n(m, m->seedCount, 2)) || malloc_n(m, m->seedList, n))
-      return VPD_ERR_DEVICE;
-    m->seedCap = n;
-  }
   MLPTRY(m, hipEventRecord(a, m->stream));
-  // State 5 where positive, the network on the list of the other points.
-  k_zero_count<<<1, 1, 0, m->stream>>>(m->seedCount);
-  k_seed_state5<<<(int)ceil_div(n, 2048), 256, 0, m->stream>>>(n, state, stateStride,
-                                                                m->seedCount, m->seedList, seed);
-  k_mlp_seed<SEED_LIST><<<mlp_grid(n), 256, 0, m->stream>>>(
-      0, m->seedCount, m->seedList, nullptr, inc, incStride, state, stateStride, m->theta,
-      m->norm, seed);
+  // State 5 where positive, the network on the other points of each 2048-point group.
+  k_seed_block<<<(int)ceil_div(n, 2048), 256, 0, m->stream>>>(n, inc, incStride, state,
+                                                               stateStride, m->theta, m->norm,
+                                                               seed);
   MLPTRY(m, hipGetLastError());
   MLPTRY(m, hipEventRecord(b, m->stream));
   MLPTRY(m, hipEventSynchronize(b));

@@ -1542,17 +1542,24 @@ __device__ __forceinline__ int group_inclusive_sum(int v, int *red) {
   return v;
 }
 
-// The first pass of a listed network seed over n points (k_mlp_seed<SEED_LIST> is the
-// second): seed[s] = state 5 (component 4 of the state planes, stride stateStride) where it
-// is positive; the other points are appended to `entries` (count[0] zeroed before). Each
-// thread takes 8 consecutive points, their loads in flight together, and each workgroup
-// appends its share with one atomic.
-__global__ void __launch_bounds__(256) k_seed_state5(int64_t n, const double *__restrict__ state,
-                                                     int64_t stateStride, int *count,
-                                                     int *entries, double *seed) {
+// Return-map seeds for a block of n points in one launch: workgroup g takes the 2048 points
+// from 2048·g (8 per thread, their loads in flight together), writes state 5 (component 4 of
+// the state planes, stride stateStride) where it is positive, lists the others in its LDS and
+// runs the network on them in 16-point tiles, tile t on wavefront t mod 4: the seed
+// k_mlp_seed gives each of those points (one MFMA column per point, inputs normalised
+// alike), with no device-wide list and no atomic.
+__global__ void __launch_bounds__(256) k_seed_block(int64_t n, const double *__restrict__ inc,
+                                                    int64_t incStride,
+                                                    const double *__restrict__ state,
+                                                    int64_t stateStride,
+                                                    const float *__restrict__ theta,
+                                                    const float *__restrict__ norm,
+                                                    double *__restrict__ seed) {
+  __shared__ int ids[2048];
   __shared__ int red[4];
-  __shared__ int groupBase;
-  const int64_t p0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 8;
+  __shared__ int total;
+  const int64_t g0 = (int64_t)blockIdx.x * 2048;
+  const int64_t p0 = g0 + 8 * (int64_t)threadIdx.x;
   double prev[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j)
@@ -1569,13 +1576,57 @@ __global__ void __launch_bounds__(256) k_seed_state5(int64_t n, const double *__
   }
   const int cnt = __popc(mine);
   const int incl = group_inclusive_sum(cnt, red);
-  if (threadIdx.x == 255)
-    groupBase = incl > 0 ? atomicAdd(count, incl) : 0;
-  __syncthreads();
-  int pos = groupBase + incl - cnt;
+  int pos = incl - cnt;
   for (int j = 0; j < 8; ++j)
     if ((mine >> j) & 1u)
-      entries[pos++] = (int)(p0 + j);
+      ids[pos++] = 8 * (int)threadIdx.x + j;
+  if (threadIdx.x == 255)
+    total = incl;
+  __syncthreads();
+  const int count = total, tiles = (count + 15) / 16;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
+  if (wv >= tiles)
+    return; // (uniform per wavefront; no barrier follows)
+  MlpWeights w;
+  mlp_load_weights(theta, lane, w);
+  const double mean = norm[2 * MLP_IN], sdev = norm[2 * MLP_IN + 1];
+  double mu[4], inv[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int f = q < 2 ? 4 * q + c : c;
+    mu[c] = norm[f];
+    inv[c] = 1.0 / (double)norm[MLP_IN + f];
+  }
+  for (int tile = wv; tile < tiles; tile += 4) {
+    const int e = 16 * tile + r;
+    const int64_t s = e < count ? g0 + ids[e] : 0;
+    // This lane's 4 raw inputs (lanes 0-15: increments 0-3; lanes 16-31: increments 4-5,
+    // ε̄ᵖ, T); missing points normalise to 0.
+    double v[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      v[c] = mu[c];
+    if (q < 2 && e < count) {
+      if (q == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          v[c] = inc[c * incStride + s];
+      } else {
+        v[0] = inc[4 * incStride + s];
+        v[1] = inc[5 * incStride + s];
+        v[2] = state[1 * stateStride + s];
+        v[3] = state[2 * stateStride + s];
+      }
+    }
+    bf16x4 x = {0, 0, 0, 0};
+    if (q < 2)
+      x = normalised_bf16(v, mu, inv);
+    const float y = mlp_tile(w, x);
+    if (q == 0 && e < count) {
+      const double est = (double)y * sdev + mean;
+      seed[s] = est > 0.0 ? est : 0.0;
+    }
+  }
 }
 
 template <int MODE>

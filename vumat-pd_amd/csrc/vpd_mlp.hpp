@@ -328,39 +328,26 @@ __device__ __forceinline__ double wave_network_seed(bool need, const double inc[
   return need ? ys[rank] : 0.0;
 }
 
-// Return-map seeds. A point whose previous plastic increment (state 5, component 4 of the
-// JC state planes, stride stateStride) is positive keeps it as its seed: on a continuing
-// plastic path it is closer to the solution than the network's estimate (about 2% median
-// error). A point with state 5 = 0 (first yield, or reloading after an elastic increment),
-// where that seed says nothing, takes the network's estimate from the 6 strain increments,
-// ε̄ᵖ and T (components 1 and 2): seed = max(0, y·std + mean).
-//   SEED_ALL   every point s < nPoints (particle s, increments at inc[c·incStride + s]).
-//              Each wavefront walks 16-point tiles with a grid stride, reading state 5 first;
-//              a tile whose points all carry a state-5 seed skips the network and its inputs.
-//   SEED_LIST  only the `*count` points listed in `entries`: entry e names slot
-//              entries[e], whose increments are at inc[c·incStride + slot] and whose particle
-//              is particles[slot] (the slot itself when `particles` is null); the seed goes
-//              to seed[slot]. The list holds the points without a state-5 seed, so every
-//              tile runs the network.
-// A wavefront loads the weights at its first tile that needs them; wavefronts with no tile
-// exit at once, and the tile loop (and the skip) is uniform per wavefront, so every MFMA runs
-// with all lanes active. A point's estimate depends on its own inputs only (one MFMA column),
-// not on the tile it shares.
-enum SeedForm { SEED_ALL = 0, SEED_LIST = 1 };
-template <int FORM>
-__global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__restrict__ count,
-                                                  const int *__restrict__ entries,
-                                                  const int *__restrict__ particles,
-                                                  const double *__restrict__ inc,
+// Return-map seeds for n points (particle s, increments at inc[c·incStride + s]). A point
+// whose previous plastic increment (state 5, component 4 of the JC state planes, stride
+// stateStride) is positive keeps it as its seed: on a continuing plastic path it is closer to
+// the solution than the network's estimate (about 2% median error). A point with state 5 = 0
+// (first yield, or reloading after an elastic increment), where that seed says nothing, takes
+// the network's estimate from the 6 strain increments, ε̄ᵖ and T (components 1 and 2):
+// seed = max(0, y·std + mean). Each wavefront walks 16-point tiles with a grid stride,
+// reading state 5 first; a tile whose points all carry a state-5 seed skips the network and
+// its inputs, and a wavefront loads the weights at its first tile that needs them.
+// Wavefronts with no tile exit at once, and the tile loop (and the skip) is uniform per
+// wavefront, so every MFMA runs with all lanes active. A point's estimate depends on its own
+// inputs only (one MFMA column), not on the tile it shares.
+__global__ void __launch_bounds__(256) k_mlp_seed(int64_t n, const double *__restrict__ inc,
                                                   int64_t incStride,
                                                   const double *__restrict__ state,
                                                   int64_t stateStride,
                                                   const float *__restrict__ theta,
                                                   const float *__restrict__ norm,
                                                   double *__restrict__ seed) {
-  constexpr bool LISTED = FORM == SEED_LIST;
   const int lane = threadIdx.x & 63, q = lane >> 4, r = lane & 15;
-  const int64_t n = LISTED ? (int64_t)*count : nPoints;
   const int64_t tiles = (n + 15) / 16;
   const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int64_t waves = (int64_t)gridDim.x * 4;
@@ -370,14 +357,9 @@ __global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__
   bool loaded = false;
   double mean = 0.0, sdev = 0.0, mu[4], inv[4];
   for (int64_t tile = wave; tile < tiles; tile += waves) {
-    const int64_t e = tile * 16 + r;
-    int64_t s = 0, i = 0; // slot and particle of this lane's point
-    if (e < n) {
-      s = LISTED ? (int64_t)entries[e] : e;
-      i = LISTED && particles ? (int64_t)particles[s] : s;
-    }
-    const double prev = !LISTED && q == 0 && e < n ? state[4 * stateStride + i] : 0.0;
-    const bool needNet = q == 0 && e < n && !(prev > 0.0);
+    const int64_t s = tile * 16 + r;
+    const double prev = q == 0 && s < n ? state[4 * stateStride + s] : 1.0; // state 5
+    const bool needNet = q == 0 && s < n && !(prev > 0.0);
     float y = 0.f;
     if (__ballot(needNet) != 0) { // uniform: the tile has a point without a state-5 seed
       if (!loaded) {
@@ -398,7 +380,7 @@ __global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__
 #pragma unroll
       for (int c = 0; c < 4; ++c)
         v[c] = mu[c];
-      if (q < 2 && e < n) {
+      if (q < 2 && s < n) {
         if (q == 0) {
 #pragma unroll
           for (int c = 0; c < 4; ++c)
@@ -406,8 +388,8 @@ __global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__
         } else {
           v[0] = inc[4 * incStride + s];
           v[1] = inc[5 * incStride + s];
-          v[2] = state[1 * stateStride + i];
-          v[3] = state[2 * stateStride + i];
+          v[2] = state[1 * stateStride + s];
+          v[3] = state[2 * stateStride + s];
         }
       }
       bf16x4 x = {0, 0, 0, 0};
@@ -415,9 +397,9 @@ __global__ void __launch_bounds__(256) k_mlp_seed(int64_t nPoints, const int *__
         x = normalised_bf16(v, mu, inv);
       y = mlp_tile(w, x);
     }
-    if (q == 0 && e < n) {
-      const double est = (double)y * sdev + mean;
-      seed[s] = needNet ? (est > 0.0 ? est : 0.0) : prev;
+    if (q == 0 && s < n) {
+      const double e = (double)y * sdev + mean;
+      seed[s] = needNet ? (e > 0.0 ? e : 0.0) : prev;
     }
   }
 }
