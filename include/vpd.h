@@ -280,7 +280,9 @@ int vpd_attach_warm_start(vpd_context *ctx, vpd_mlp *mlp);
  *    160} stages runs of W consecutive particles' data in each wavefront's LDS and serves
  *    the bonds that fall in them from there (the LDS reservation is then at least the
  *    windows'); W in {256, 384, 480, 512, 640} does so once per 256-thread workgroup
- *    (form 3);
+ *    (form 3); with form 4, shared rows, W in {16, 32, 64} is the lanes per particle, each
+ *    computing one bond's term per W-bond chunk, summed in bond order (for small blocks,
+ *    where one lane per particle leaves the GPU idle);
  *  - bonds per window (windowed gathers): the most bonds a particle takes from one window
  *    before the wavefront moves on, 0 for no limit, at most 1024;
  *  - window form (windowed gathers): 0 walks the bond lists during the pass; 1 replays a
@@ -290,10 +292,12 @@ int vpd_attach_warm_start(vpd_context *ctx, vpd_mlp *mlp);
  *    shared by the workgroup's 256 particles (per window and particle the neighbours' LDS
  *    offsets, ≈ 2 B per bond). Forms 1 and 2 need a window of 64, 80, 96 or 128 and 1 to 8
  *    bonds per window, form 3 a window of 256, 384, 480, 512 or 640 and no limit (0),
- *    form 0 a window of at most 160; a step with another combination fails with
+ *    form 4 a window of 16, 32 or 64 and no limit, form 0 a window of 64 to 160; a step
+ *    with another combination fails with
  *    VPD_ERR_ARGUMENT.
- * The sum order, and so every result bit, is the same for every setting. vpd_autotune times
- * both passes over the grid of these on the context's own block and keeps the fastest;
+ * The sum order, and so every result bit, is the same for every setting (sweeps do not apply
+ * to form 4). vpd_autotune times both passes over the grid of these on the context's own
+ * block (form 4 on blocks of at most 262,144 particles) and keeps the fastest;
  * tuned[] (VPD_TUNE_COUNT ints) receives the values indexed by enum vpd_tuning. */
 enum vpd_tuning {
   VPD_TUNE_DEFGRAD_LDS = 0,

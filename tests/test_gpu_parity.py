@@ -286,6 +286,12 @@ def test_fused_equals_unfused(vpd, orc):
     dict(defgrad_window=256, force_window=256, defgrad_window_plan=3, force_window_plan=3,
          defgrad_lds=64000, force_lds=50000),
     dict(defgrad_window=640, force_window=512, defgrad_window_plan=3, force_window_plan=3),
+    dict(defgrad_window=16, force_window=16, defgrad_window_plan=4, force_window_plan=4),
+    dict(defgrad_window=32, force_window=512, defgrad_window_plan=4, force_window_plan=3,
+         defgrad_lds=64000),
+    dict(defgrad_window=80, force_window=64, defgrad_window_bonds=7, defgrad_window_plan=2,
+         force_window_plan=4, force_lds=36408),
+    dict(defgrad_window=64, force_window=32, defgrad_window_plan=4, force_window_plan=4),
 ])
 def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
     # Launch knobs change how the bond passes are scheduled, never what they sum or in which
@@ -306,7 +312,7 @@ def test_launch_tuning_is_bitwise_neutral(vpd, knobs):
 
 @pytest.mark.parametrize("voids,take,plan", [(0.0, 0, 0), (0.08, 0, 0), (0.08, 5, 0), (0.0, 2, 0),
                                             (0.08, 6, 1), (0.08, 7, 2), (0.0, 8, 2), (0.08, 0, 3),
-                                            (0.0, 0, 3)])
+                                            (0.0, 0, 3), (0.08, 0, 4)])
 def test_windowed_gathers_on_irregular_rows(vpd, voids, take, plan):
     # Voids make every row irregular: the windows then cover fewer bonds per step, and the
     # results must still equal the direct gathers bit for bit, with or without a limit on the
@@ -318,7 +324,7 @@ def test_windowed_gathers_on_irregular_rows(vpd, voids, take, plan):
               density=7.85e-3, velocity=300.0, damage=(0.002, 0.0), points=points)
     a = vpd.Bar(**kw)
     b = vpd.Bar(**kw)
-    window = 256 if plan == 3 else 64
+    window = {3: 256, 4: 16}.get(plan, 64)
     b.ctx.set_tuning(defgrad_window=window, force_window=window, defgrad_window_bonds=take,
                      force_window_bonds=take, defgrad_window_plan=plan, force_window_plan=plan)
     a.step(50, want_fields=True)
@@ -336,6 +342,9 @@ FORMS = {
                              force_window_bonds=8, defgrad_window_plan=2, force_window_plan=2),
     "group": dict(defgrad_window=384, force_window=256, defgrad_window_plan=3,
                   force_window_plan=3),
+    "rows": dict(defgrad_window=16, force_window=16, defgrad_window_plan=4, force_window_plan=4),
+    "rows-64": dict(defgrad_window=64, force_window=32, defgrad_window_plan=4,
+                    force_window_plan=4),
 }
 
 
@@ -447,7 +456,7 @@ def test_sampled_kernel_timing(vpd):
 def test_window_knob_limits(vpd):
     b = vpd.Bar(nx=6, ny=4, nz=4, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
                 density=7.85e-3, velocity=1.0)
-    for bad in (32, 100, 400):
+    for bad in (48, 100, 400):
         with pytest.raises(vpd.VpdError, match="window"):
             b.ctx.set_tuning(force_window=bad)
     b.ctx.set_tuning(force_window=0, defgrad_window=160)
@@ -458,7 +467,7 @@ def test_window_knob_limits(vpd):
     b.ctx.set_tuning(force_window_bonds=1024, defgrad_window_bonds=6)
     t = b.ctx.get_tuning()
     assert (t["force_window_bonds"], t["defgrad_window_bonds"]) == (1024, 6)
-    for bad in (-1, 4):
+    for bad in (-1, 5):
         with pytest.raises(vpd.VpdError, match="Window form"):
             b.ctx.set_tuning(force_window_plan=bad)
     # A planned form needs a planned window size and 1..8 bonds per window: the step refuses
@@ -483,8 +492,20 @@ def test_window_knob_limits(vpd):
     b.ctx.set_tuning(force_window=384, force_window_bonds=0, force_window_plan=3,
                      defgrad_window=640, defgrad_window_bonds=0, defgrad_window_plan=3)
     b.step(1)
+    # Shared rows go only with 16, 32 or 64 lanes per particle and no bond limit, and windows
+    # 16 and 32 only with them.
+    for knobs in (dict(force_window=16, force_window_plan=3), dict(force_window=32,
+                                                                    force_window_plan=0),
+                  dict(force_window=80, force_window_plan=4),
+                  dict(force_window=16, force_window_plan=4, force_window_bonds=3)):
+        b.ctx.set_tuning(**dict(dict(force_window_bonds=0), **knobs))
+        with pytest.raises(vpd.VpdError, match="Shared rows"):
+            b.step(1)
+    for lanes in (16, 32, 64):
+        b.ctx.set_tuning(force_window=lanes, force_window_bonds=0, force_window_plan=4)
+        b.step(1)
     b.ctx.set_tuning(defgrad_window=160, defgrad_window_plan=0)
-    b.ctx.set_tuning(force_window_plan=0)
+    b.ctx.set_tuning(force_window=384, force_window_plan=0)
     with pytest.raises(vpd.VpdError, match="Walked windows need"):
         b.step(1)
     b.ctx.set_tuning(force_window_plan=1, force_window_bonds=4)

@@ -66,17 +66,17 @@ def assert_same(batch, s, bar, what):
 
 
 @pytest.mark.parametrize("return_map,windows", [(0, 0), (2, 0), (0, 80), (2, 128), (0, 96),
-                                                (0, 384)],
+                                                (0, 384), (2, 16)],
                          ids=["bisection", "newton", "bisection-windowed",
                               "newton-windowed-limited", "bisection-planned-prefetch",
-                              "bisection-group-windows"])
+                              "bisection-group-windows", "newton-shared-rows"])
 def test_batch_equals_specimens_alone(vpd, return_map, windows):
     banner("batched specimens vs each alone, return map %d, windows %d" % (return_map, windows))
     damage = (0.003, 0.0)
     sps = specimens(vpd)
     batch = vpd.SpecimenBatch(sps, JC_PROPS, RHO, damage=damage, return_map=return_map)
     take = {128: 7, 96: 6}.get(windows, 0)
-    form = {96: 2, 384: 3}.get(windows, 0)
+    form = {96: 2, 384: 3, 16: 4}.get(windows, 0)
     batch.ctx.set_tuning(defgrad_window=windows, force_window=windows,
                          defgrad_window_bonds=take, force_window_bonds=take,
                          defgrad_window_plan=form, force_window_plan=form)
@@ -172,9 +172,9 @@ def test_batch_errors(vpd):
 
 
 @pytest.mark.parametrize("return_map,planned", [(0, False), (2, False), (3, False), (0, True),
-                                                (0, "group")],
+                                                (0, "group"), (0, "rows")],
                          ids=["bisection", "newton", "learned", "bisection-planned",
-                              "bisection-group-windows"])
+                              "bisection-group-windows", "bisection-shared-rows"])
 def test_graph_replay_equals_eager(vpd, return_map, planned):
     """Every replay of the captured step against eager steps, with eager steps and host
     synchronisations between the replays (the graph replays with planned windows in the
@@ -189,6 +189,9 @@ def test_graph_replay_equals_eager(vpd, return_map, planned):
     if planned == "group":
         graph.ctx.set_tuning(defgrad_window=512, force_window=256, defgrad_window_plan=3,
                              force_window_plan=3)
+    elif planned == "rows":
+        graph.ctx.set_tuning(defgrad_window=64, force_window=32, defgrad_window_plan=4,
+                             force_window_plan=4)
     elif planned:
         graph.ctx.set_tuning(defgrad_window=80, force_window=64, defgrad_window_bonds=7,
                              force_window_bonds=8, defgrad_window_plan=2, force_window_plan=1)

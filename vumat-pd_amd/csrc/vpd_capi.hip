@@ -653,6 +653,19 @@ template <class F> void with_planned_window(int window, F &&f) {
 // window. The workgroup holds one window: `pairs` planes of W pairs (the force pass's 7
 // planes of 640 pairs take 70 KB, two workgroups per CU), within the CU's 160 KB.
 constexpr int kGroupWindows[] = {256, 384, 480, 512, 640};
+// Blocks up to this many particles also try shared rows (gather form 4) in the autotuner,
+// with each of these lanes per particle.
+constexpr int64_t kRowBlockMax = 262144;
+constexpr int kRowLanes[] = {16, 32, 64};
+bool row_lanes_supported(int lanes) { return lanes == 16 || lanes == 32 || lanes == 64; }
+template <class F> void with_row_lanes(int lanes, F &&f) {
+  if (lanes == 16)
+    f(std::integral_constant<int, 16>{});
+  else if (lanes == 32)
+    f(std::integral_constant<int, 32>{});
+  else
+    f(std::integral_constant<int, 64>{});
+}
 constexpr int kMaxGroupLds = kMaxLds;
 bool group_window_supported(int window) {
   for (int w : kGroupWindows)
@@ -682,8 +695,8 @@ template <class F> void with_group_window(int window, F &&f) {
 }
 
 // How a bond pass gathers its neighbour data: window 0 (direct) or W, the bond limit per
-// window, the form (0 cursor walk, 1 planned, 2 planned with prefetch, 3 group windows) and
-// the plan of the form.
+// window, the form (0 cursor walk, 1 planned, 2 planned with prefetch, 3 group windows, 4
+// shared rows, whose "window" is the lanes per particle) and the plan of the form.
 struct Gather {
   int window = 0, take = 0, mode = 0;
   WindowPlan plan{};
@@ -817,6 +830,13 @@ GroupPlan group_plan_of(const vpd_context *c, int pass) {
 // Builds the plan a gather needs (none for direct and walked gathers; a walked gather is
 // checked for a supported window).
 int ensure_plan(vpd_context *c, int pass, const Gather &gx) {
+  if (gx.mode == 4 || (gx.window > 0 && gx.window < 64)) {
+    if (!row_lanes_supported(gx.window) || gx.mode != 4 || gx.take != 0)
+      return fail(c, VPD_ERR_ARGUMENT, "Shared rows (form 4) go with a window of 16, 32 or 64 "
+                                       "(lanes per particle) and no bond limit per window (0); "
+                                       "windows 16 and 32 only with them.");
+    return VPD_OK;
+  }
   if (gx.window > 0 && gx.mode == 0 && !window_supported(gx.window))
     return fail(c, VPD_ERR_ARGUMENT, "Walked windows need a window of 64, 80, 96, 128 or 160 "
                                      "particles.");
@@ -862,6 +882,16 @@ template <bool DAMAGE>
 void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompression,
                       const Batch &batch, const Gather &gx) {
   const int window = gx.window;
+  if (window > 0 && gx.mode == 4) {
+    with_row_lanes(window, [&](auto w) {
+      constexpr int R = decltype(w)::value;
+      k_defgrad_m<DAMAGE, R><<<(int)ceil_div(rd.n, 256 / R), 256, lds, c->stream>>>(
+          rd.n, planes(c->x, c->n), pairs(c->xv, c->n), upairs(c), sell_of(c),
+          planes(c->kinv, c->n), planes(c->fnew, c->n), c->broken, c->anyBroken,
+          c->initiation, c->failure, useCompression, batch);
+    });
+    return;
+  }
   if (window > 0 && gx.mode == 3) {
     with_group_window(window, [&](auto w) {
       k_defgrad_g<DAMAGE, decltype(w)::value>
@@ -1056,7 +1086,13 @@ void launch_force_t(vpd_context *c, const ForceArgs &g, int lds, const Gather &g
   const int window = gx.window;
   ForceArgs h = g;
   h.take = take_arg(gx.take);
-  if (!DIAG && window > 0 && gx.mode == 3)
+  if (!DIAG && window > 0 && gx.mode == 4)
+    with_row_lanes(window, [&](auto w) {
+      constexpr int R = decltype(w)::value;
+      k_force_m<DAMAGE, FUSE, INTEGRATE, R>
+          <<<(int)ceil_div(h.n, 256 / R), 256, lds, c->stream>>>(h);
+    });
+  else if (!DIAG && window > 0 && gx.mode == 3)
     with_group_window(window, [&](auto w) {
       k_force_g<DAMAGE, FUSE, INTEGRATE, decltype(w)::value>
           <<<grid_rounds(h.rd), 256, std::max(lds, group_lds(7, window)), c->stream>>>(
@@ -2409,10 +2445,12 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
       return fail(c, VPD_ERR_ARGUMENT, "Sweeps per bond pass must lie in [1, 1024].");
     (knob == VPD_TUNE_DEFGRAD_ROUNDS ? c->roundsDefgrad : c->roundsForce) = value;
   } else if (knob == VPD_TUNE_DEFGRAD_WINDOW || knob == VPD_TUNE_FORCE_WINDOW) {
-    if (value != 0 && !window_supported(value) && !group_window_supported(value))
+    if (value != 0 && !row_lanes_supported(value) && !window_supported(value) &&
+        !group_window_supported(value))
       return fail(c, VPD_ERR_ARGUMENT,
                   "Gather window must be 0 (direct) or one of 64, 80, 96, 128, 160 particles "
-                  "(256, 384, 480, 512, 640 with group windows).");
+                  "(256, 384, 480, 512, 640 with group windows; 16, 32, 64 lanes per particle "
+                  "with shared rows).");
     (knob == VPD_TUNE_DEFGRAD_WINDOW ? c->winDefgrad : c->winForce) = value;
   } else if (knob == VPD_TUNE_DEFGRAD_WINDOW_BONDS || knob == VPD_TUNE_FORCE_WINDOW_BONDS) {
     if (value < 0 || value > kMaxTake)
@@ -2420,9 +2458,9 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
                   "Bonds per lane and window must lie in [0, 1024] (0: no limit).");
     (knob == VPD_TUNE_DEFGRAD_WINDOW_BONDS ? c->takeDefgrad : c->takeForce) = value;
   } else if (knob == VPD_TUNE_DEFGRAD_WINDOW_PLAN || knob == VPD_TUNE_FORCE_WINDOW_PLAN) {
-    if (value < 0 || value > 3)
+    if (value < 0 || value > 4)
       return fail(c, VPD_ERR_ARGUMENT, "Window form must be 0 (walk), 1 (planned), 2 (planned "
-                                       "with prefetch) or 3 (group windows).");
+                                       "with prefetch), 3 (group windows) or 4 (shared rows).");
     (knob == VPD_TUNE_DEFGRAD_WINDOW_PLAN ? c->modeDefgrad : c->modeForce) = value;
   } else {
     return fail(c, VPD_ERR_ARGUMENT, "Unknown tuning knob.");
@@ -2701,6 +2739,13 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
           return s;
       }
     }
+    if (c->n <= kRowBlockMax) // shared rows: for blocks that leave most lanes idle
+      for (int lanes : kRowLanes) {
+        Gather gx = walk(lanes, 0);
+        gx.mode = 4;
+        if (int s = trial(pass, 0, 1, gx))
+          return s;
+      }
     Gather chosen = pick[pass];
     if (int s = ensure_plan(c, pass, chosen))
       return s;
@@ -2711,7 +2756,8 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
                                              (chosen.mode == 2 ? 2 : 1);
     for (int cap : ldsChoices)
       for (int r : roundChoices) {
-        if ((cap != 0 && cap <= own) || (r > 1 && r > ceil_div(c->n, 256)))
+        if ((cap != 0 && cap <= own) || (r > 1 && r > ceil_div(c->n, 256)) ||
+            (chosen.mode == 4 && r > 1)) // shared rows launch one sweep
           continue;
         if (int s = trial(pass, cap, r, chosen))
           return s;
