@@ -86,6 +86,8 @@ def bond_pass_device_kernel(kernel, tuning):
         return "%s_w<%s, %d>" % (head, args, window)
     if form == 3:
         return "%s_g<%s, %d>" % (head, args, window)
+    if form == 4:
+        return "%s_m<%s, %d>" % (head, args, window)
     return "%s_p<%s, %d, %s>" % (head, args, window, "true" if form == 2 else "false")
 
 
@@ -329,7 +331,7 @@ def small_bar_graph(steps=2000):
 
     dt = cfl_dt(1.0)
     bar = vpd.Bar(30, 8, 8, 1.0, 3.015, JC_PROPS, dt, RHO, velocity=1.0)
-    bar.ctx.autotune(3)
+    bar.autotune_in_steps(3)
     bar.step(20)
     bar.reset()
     bar.ctx.synchronize()
@@ -407,7 +409,7 @@ def main():
         bar.ctx.set_tuning(**json.loads(args.tuning))
         tuned = bar.ctx.get_tuning()
     else:
-        tuned = bar.ctx.get_tuning() if args.no_autotune else bar.ctx.autotune(3)
+        tuned = bar.ctx.get_tuning() if args.no_autotune else bar.autotune_in_steps(3)
     print("[bench] launch tuning %s" % tuned, file=sys.stderr, flush=True)
 
     # The timed window: steps from_step .. from_step + steps − 1, after an untimed advance and

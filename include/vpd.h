@@ -315,6 +315,15 @@ enum vpd_tuning {
 int vpd_set_tuning(vpd_context *ctx, int knob, int value);
 int vpd_get_tuning(vpd_context *ctx, int knob, int *value);
 int vpd_autotune(vpd_context *ctx, int reps, int *tuned);
+/* The settings of bond pass `pass` (0 F, 1 force) that the context's last vpd_autotune timed
+ * again in its final stage (up to 4), fastest first: `*count` (at most `max`) full tunings of
+ * VPD_TUNE_COUNT ints in knobs[], each the context's current tuning with that pass's five
+ * knobs replaced, and their mean trial times in ms[] (may be NULL). *count is 0 before any
+ * autotuning. A caller can time these in its own steps (the trials time each pass alone, and
+ * settings within a few percent of each other there can rank otherwise in a step). No
+ * reference counterpart. */
+int vpd_autotune_candidates(vpd_context *ctx, int pass, int max, int *knobs, float *ms,
+                            int *count);
 
 /* ---- run diagnostics ------------------------------------------------------------- */
 /* The per-step observables of runMaterial (src/host/BoundaryValueCaseRunner.cpp:231-305),

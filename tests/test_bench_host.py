@@ -27,6 +27,8 @@ def test_bond_pass_device_kernels():
     assert g(80, 0) == "k_force_w<true, true, true, 80>"
     assert g(64, 1) == "k_force_p<true, true, true, 64, false>"
     assert g(512, 3) == "k_force_g<true, true, true, 512>"
+    assert f(64, 4) == "k_defgrad_m<true, 64>"
+    assert g(16, 4) == "k_force_m<true, true, true, 16>"
 
 
 def test_traffic_only_for_the_profiled_kernel():

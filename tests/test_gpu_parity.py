@@ -515,6 +515,44 @@ def test_window_knob_limits(vpd):
     b.step(1)
 
 
+def test_autotune_and_in_step_confirmation(vpd):
+    # The autotuner leaves the state as it was and sets its pick; its re-timed candidates per
+    # pass come fastest first and include the pick. The in-step confirmation restores the
+    # state bit for bit and sets one of those candidates; the run then equals an untuned one.
+    banner("autotune, candidates and in-step confirmation on a damaged bar")
+    kw = dict(nx=24, ny=8, nz=8, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,
+              density=7.85e-3, velocity=300.0, damage=(0.002, 0.0))
+    ref, b = vpd.Bar(**kw), vpd.Bar(**kw)
+    assert b.ctx.autotune_candidates(0) == [] and b.ctx.autotune_candidates(1) == []
+    ref.step(15)
+    b.step(15)
+    assert b.get("damage").sum() > 0
+    before = b.snapshot()
+    pick = b.ctx.autotune(2)
+    assert b.ctx.get_tuning() == pick
+    for bond_pass, prefix in ((0, "defgrad_"), (1, "force_")):
+        cands = b.ctx.autotune_candidates(bond_pass)
+        assert 1 <= len(cands) <= 4
+        assert [ms for _, ms in cands] == sorted(ms for _, ms in cands)
+        keys = [k for k in pick if k.startswith(prefix)]
+        assert any(all(c[k] == pick[k] for k in keys) for c, _ in cands)
+        other = [k for k in pick if not k.startswith(prefix)]
+        assert all(all(c[k] == pick[k] for k in other) for c, _ in cands)
+    assert len(b.ctx.autotune_candidates(1, 1)) == 1
+    with pytest.raises(vpd.VpdError, match="Bond pass"):
+        b.ctx.autotune_candidates(2)
+    kept = b.autotune_in_steps(reps=2, steps=5, warmup=2, rounds=1)
+    assert b.ctx.get_tuning() == kept
+    assert b.step_index == 15
+    after = b.snapshot()
+    for k in before:
+        assert np.array_equal(before[k], after[k]), k
+    ref.step(20)
+    b.step(20)
+    for name in ["u", "v", "stressN", "statev", "damage"]:
+        assert np.array_equal(b.get(name), ref.get(name)), name
+
+
 @pytest.mark.parametrize("take,plan,gwin", [(0, 0, 0), (8, 0, 0), (6, 1, 0), (7, 2, 0),
                                             (0, 3, (512, 480)), (0, 3, (480, 480)),
                                             (0, 3, (640, 640))])

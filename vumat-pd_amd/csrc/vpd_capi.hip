@@ -128,8 +128,16 @@ struct vpd_context {
   int takeDefgrad = 0, takeForce = 0;
   // Form of the windowed gathers per bond pass: 0 walks the bond lists, 1 replays a window
   // plan, 2 replays it with the next window prefetched (WindowPlan in vpd_kernels.hpp), 3
-  // replays a plan of windows shared by each workgroup (GroupPlan). Chosen by vpd_autotune.
+  // replays a plan of windows shared by each workgroup (GroupPlan), 4 shares each particle's
+  // row among a group of lanes (k_defgrad_m, k_force_m). Chosen by vpd_autotune.
   int modeDefgrad = 0, modeForce = 0;
+  // The settings of each bond pass that the last vpd_autotune timed again in its final stage,
+  // fastest first (vpd_autotune_candidates).
+  struct TunedSetting {
+    int lds, rounds, window, take, mode;
+    float ms;
+  };
+  std::vector<TunedSetting> tunedTop[2];
   // The window plan of each bond pass, built for the pass's window and bond limit when first
   // needed after the bond lists or those knobs change.
   struct PlanBuffers {
@@ -2468,6 +2476,32 @@ int vpd_set_tuning(vpd_context *c, int knob, int value) {
   return VPD_OK;
 }
 
+int vpd_autotune_candidates(vpd_context *c, int pass, int max, int *knobs, float *ms,
+                            int *count) {
+  if (!c || !count || (max > 0 && !knobs))
+    return fail(c, VPD_ERR_ARGUMENT, "Null argument.");
+  if (pass != 0 && pass != 1)
+    return fail(c, VPD_ERR_ARGUMENT, "Bond pass must be 0 (F) or 1 (force).");
+  const auto &top = c->tunedTop[pass];
+  const int k = std::max(0, std::min(max, (int)top.size()));
+  for (int r = 0; r < k; ++r) {
+    int *t = knobs + (int64_t)r * VPD_TUNE_COUNT;
+    for (int knob = 0; knob < VPD_TUNE_COUNT; ++knob)
+      if (int s = vpd_get_tuning(c, knob, t + knob))
+        return s;
+    const vpd_context::TunedSetting &x = top[r];
+    t[pass == 0 ? VPD_TUNE_DEFGRAD_LDS : VPD_TUNE_FORCE_LDS] = x.lds;
+    t[pass == 0 ? VPD_TUNE_DEFGRAD_ROUNDS : VPD_TUNE_FORCE_ROUNDS] = x.rounds;
+    t[pass == 0 ? VPD_TUNE_DEFGRAD_WINDOW : VPD_TUNE_FORCE_WINDOW] = x.window;
+    t[pass == 0 ? VPD_TUNE_DEFGRAD_WINDOW_BONDS : VPD_TUNE_FORCE_WINDOW_BONDS] = x.take;
+    t[pass == 0 ? VPD_TUNE_DEFGRAD_WINDOW_PLAN : VPD_TUNE_FORCE_WINDOW_PLAN] = x.mode;
+    if (ms)
+      ms[r] = x.ms;
+  }
+  *count = k;
+  return VPD_OK;
+}
+
 int vpd_get_tuning(vpd_context *c, int knob, int *value) {
   if (!c || !value)
     return fail(c, VPD_ERR_ARGUMENT, "Null argument.");
@@ -2627,9 +2661,9 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
     return VPD_ERR_DEVICE;
   // Mean time of `runs` launches after a warming one.
   auto time_setting = [&](int pass, int lds, int rounds, const Gather &gx, int runs,
-                          float *mean) -> int {
+                          float *mean, float *fastest = nullptr) -> int {
     g.rd = rounds_of(c->n, rounds);
-    float total = 0.f;
+    float total = 0.f, low = 1e30f;
     for (int r = 0; r <= runs; ++r) {
       HIPTRY(c, hipMemsetAsync(flush, r & 0xFF, flushBytes, c->stream));
       const int64_t planeBytes = (int64_t)sizeof(double) * 4 * c->n; // u, or X and V
@@ -2654,10 +2688,14 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
       HIPTRY(c, hipEventSynchronize(b));
       float ms = 0.f;
       HIPTRY(c, hipEventElapsedTime(&ms, a, b));
-      if (r > 0) // the first run warms caches and code
+      if (r > 0) { // the first run warms caches and code
         total += ms;
+        low = std::min(low, ms);
+      }
     }
     *mean = total / runs;
+    if (fastest)
+      *fastest = low;
     return VPD_OK;
   };
   auto keep_if_best = [&](int pass, int lds, int rounds, const Gather &gx, float ms) {
@@ -2668,10 +2706,11 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
       pick[pass] = gx;
     }
   };
-  // One setting with `reps` runs; keeps it if it is the fastest yet.
+  // One setting with `reps` runs, ranked by its fastest run (one slow run, e.g. a clock dip,
+  // then cannot push a setting out of the final stage); keeps it if it is the fastest yet.
   auto trial = [&](int pass, int lds, int rounds, const Gather &gx) -> int {
-    float ms = 0.f;
-    if (int s = time_setting(pass, lds, rounds, gx, reps, &ms))
+    float mean = 0.f, ms = 0.f;
+    if (int s = time_setting(pass, lds, rounds, gx, reps, &mean, &ms))
       return s;
     seen[pass].push_back(Candidate{lds, rounds, gx, ms});
     keep_if_best(pass, lds, rounds, gx, ms);
@@ -2686,9 +2725,10 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
   // Stage 1: sweeps × LDS cap × window (walked, no bond limit per window); stage 2, at the
   // sweeps stage 1 chose: every window with each bond limit (walked, at stage 1's LDS cap),
   // every planned form, window and bond limit with each LDS cap, and the group windows with
-  // each LDS cap; stage 3: sweeps × LDS
-  // cap again for the form, window and limit chosen; stage 4: the fastest few settings timed
-  // again with four times the runs, against timing noise between close ones.
+  // each LDS cap (these two also at one sweep), and shared rows on small blocks; stage 3: sweeps × LDS
+  // cap again for the form, window and limit chosen; stage 4: the fastest four settings (at
+  // most two per form) timed again with four times the runs and ranked by their mean,
+  // against timing noise between close ones.
   for (int pass = 0; pass < 2; ++pass) {
     for (int win : windowChoices)
       for (int lds : ldsChoices) {
@@ -2702,6 +2742,10 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
         }
       }
     const int lds = pickLds[pass], rounds = pickRounds[pass];
+    // The planned and group forms also run at one sweep: their staging changes which sweep
+    // count suits them, and stage 3 varies the sweeps of the chosen form only.
+    const std::vector<int> formRounds =
+        rounds == 1 ? std::vector<int>{1} : std::vector<int>{rounds, 1};
     for (int win : windowChoices)
       for (int take : takeChoices)
         if (win > 0)
@@ -2720,8 +2764,9 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
             Gather gx = walk(win, take);
             gx.mode = mode;
             gx.plan = plan_of(c, pass);
-            if (int s = trial(pass, cap, rounds, gx))
-              return s;
+            for (int r : formRounds)
+              if (int s = trial(pass, cap, r, gx))
+                return s;
           }
       }
     for (int win : kGroupWindows) {
@@ -2735,8 +2780,9 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
       for (int cap : ldsChoices) {
         if (cap != 0 && cap <= group_lds(pairsOf[pass], win))
           continue;
-        if (int s = trial(pass, cap, rounds, gx))
-          return s;
+        for (int r : formRounds)
+          if (int s = trial(pass, cap, r, gx))
+            return s;
       }
     }
     if (c->n <= kRowBlockMax) // shared rows: for blocks that leave most lanes idle
@@ -2769,11 +2815,17 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
       return x.lds == y.lds && x.rounds == y.rounds && x.gx.window == y.gx.window &&
              x.gx.take == y.gx.take && x.gx.mode == y.gx.mode;
     };
+    // The fastest four distinct settings, at most two of one gather form, so that the final
+    // stage (and a caller timing the candidates in steps) sees more than one form.
     std::vector<Candidate> top;
     for (const Candidate &x : cand) {
       if (top.size() == 4)
         break;
-      if (std::none_of(top.begin(), top.end(), [&](const Candidate &y) { return same(x, y); }))
+      const auto ofForm = std::count_if(top.begin(), top.end(), [&](const Candidate &y) {
+        return y.gx.mode == x.gx.mode && (y.gx.window > 0) == (x.gx.window > 0);
+      });
+      if (ofForm < 2 &&
+          std::none_of(top.begin(), top.end(), [&](const Candidate &y) { return same(x, y); }))
         top.push_back(x);
     }
     // Within 3% of the fastest the timings do not separate the settings reliably; of those,
@@ -2788,6 +2840,12 @@ int vpd_autotune(vpd_context *c, int reps, int *tuned) {
         return s;
       fastest = std::min(fastest, x.ms);
     }
+    std::stable_sort(top.begin(), top.end(),
+                     [](const Candidate &x, const Candidate &y) { return x.ms < y.ms; });
+    c->tunedTop[pass].clear();
+    for (const Candidate &x : top)
+      c->tunedTop[pass].push_back(vpd_context::TunedSetting{
+          x.lds, x.rounds, x.gx.window, x.gx.take, x.gx.mode, x.ms});
     best[pass] = 1e30f;
     int fewest = INT_MAX;
     for (const Candidate &x : top)

@@ -115,6 +115,8 @@ def lib():
         L.vpd_set_tuning.argtypes = [P, C.c_int, C.c_int]
         L.vpd_get_tuning.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
         L.vpd_autotune.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
+        L.vpd_autotune_candidates.argtypes = [P, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                              C.POINTER(C.c_float), C.POINTER(C.c_int)]
         L.vpd_collect_timing.argtypes = [P, C.POINTER(KernelTime), C.c_int,
                                          C.POINTER(C.c_int)]
         L.vpd_make_patch.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, _D, _D]
@@ -417,6 +419,18 @@ class Context:
         out = (C.c_int * len(self.TUNE_KEYS))()
         self._check(lib().vpd_autotune(self.h, reps, out))
         return dict(zip(self.TUNE_KEYS, list(out)))
+
+    def autotune_candidates(self, bond_pass, max_count=4):
+        """The settings of bond pass `bond_pass` (0 F, 1 force) the last autotune timed again in
+        its final stage, fastest first: [(tuning dict, mean trial ms)] (vpd_autotune_candidates)."""
+        n = len(self.TUNE_KEYS)
+        knobs = (C.c_int * (n * max_count))()
+        ms = (C.c_float * max_count)()
+        cnt = C.c_int()
+        self._check(lib().vpd_autotune_candidates(self.h, bond_pass, max_count, knobs, ms,
+                                                  C.byref(cnt)))
+        return [(dict(zip(self.TUNE_KEYS, list(knobs[r * n:(r + 1) * n]))), ms[r])
+                for r in range(cnt.value)]
 
     def set_tuning(self, **knobs):
         for k, v in knobs.items():
@@ -749,6 +763,44 @@ class Bar:
 
     STATE_FIELDS = ("u", "v", "a", "fN", "vN", "rN", "stressN", "statev", "temp", "eint",
                     "einel")
+
+    def autotune_in_steps(self, reps=3, steps=40, warmup=10, rounds=2, top=3):
+        """vpd_autotune, then its `top` fastest settings per bond pass timed in this bar's own
+        steps: the force pass's with the F pass at the autotuner's pick, then the F pass's with
+        the force pass so chosen. Each setting runs `warmup` + `steps` steps from the current
+        state, `rounds` times in alternation with the others, and the fastest total is kept.
+        The autotuner times each pass alone, and settings it ranks within a few percent can
+        rank otherwise inside a step. The state is restored afterwards (bit for bit, every
+        setting giving the same results) and the kept tuning is set and returned."""
+        import time
+
+        best = self.ctx.autotune(reps)
+        snap = self.snapshot()
+        for bond_pass in (1, 0):
+            keys = [k for k in self.ctx.TUNE_KEYS
+                    if k.startswith("force_" if bond_pass else "defgrad_")]
+            settings = []
+            for cand, _ in self.ctx.autotune_candidates(bond_pass, top):
+                t = dict(best, **{k: cand[k] for k in keys})
+                if t not in settings:
+                    settings.append(t)
+            if len(settings) < 2:
+                continue
+            total = [0.0] * len(settings)
+            for _ in range(rounds):
+                for i, t in enumerate(settings):
+                    self.ctx.set_tuning(**t)
+                    self.restore(snap)
+                    self.step(warmup)
+                    self.ctx.synchronize()
+                    t0 = time.perf_counter()
+                    self.step(steps)
+                    self.ctx.synchronize()
+                    total[i] += time.perf_counter() - t0
+            best = settings[int(np.argmin(total))]
+        self.ctx.set_tuning(**best)
+        self.restore(snap)
+        return best
 
     def snapshot(self):
         """The end-of-step state a following step reads (host copies), with the step index and
