@@ -323,15 +323,19 @@ def sweep_measure(steps, warmup, end_steps=2000):
             "speedup": t_seq / t_batch, "setup_s": setup}
 
 
-def small_bar_graph(steps=2000):
+def small_bar_graph(steps=2000, tuning=None):
     """The CPU-sized JC bar (30×8×8, Δx = 1 mm, δ = 3.015Δx, dt = 0.5·CFL, 1 mm/ms, damage
     off) for `steps` steps: host-issued fused steps (vpd_step) against the same bar as a
-    one-specimen batch replaying one captured HIP graph per step. Launch-bound at this size."""
+    one-specimen batch replaying one captured HIP graph per step. `tuning`: launch knobs to
+    set instead of autotuning."""
     import vpd
 
     dt = cfl_dt(1.0)
     bar = vpd.Bar(30, 8, 8, 1.0, 3.015, JC_PROPS, dt, RHO, velocity=1.0)
-    bar.autotune_in_steps(3)
+    if tuning:
+        bar.ctx.set_tuning(**tuning)
+    else:
+        bar.autotune_in_steps(3)
     bar.step(20)
     bar.reset()
     bar.ctx.synchronize()
@@ -353,7 +357,8 @@ def small_bar_graph(steps=2000):
     return {"workload": "jc_tensile_bar_30x8x8", "particles": bar.n, "steps": steps,
             "eager_ms_per_step": elapsed / steps * 1e3,
             "graph_ms_per_step": t_graph / steps * 1e3,
-            "eager_value": bar.n * steps / elapsed, "graph_value": bar.n * steps / t_graph}
+            "eager_value": bar.n * steps / elapsed, "graph_value": bar.n * steps / t_graph,
+            "launch_tuning": bar.ctx.get_tuning()}
 
 
 def main():
@@ -383,11 +388,18 @@ def main():
     ap.add_argument("--return-map-points", type=int, default=1000000)
     ap.add_argument("--sweep", action="store_true",
                     help="measure the 48-bar batched sweep instead (JSON on stdout)")
+    ap.add_argument("--small-bar", action="store_true",
+                    help="measure only the 30x8x8 bar's 2000-step run, eager and graph-replayed "
+                         "(JSON on stdout; --tuning fixes its launch knobs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this path is not sharded across GPUs. "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.small_bar:
+        print(json.dumps(small_bar_graph(tuning=json.loads(args.tuning) if args.tuning
+                                         else None)))
+        return
     if args.sweep:
         out = sweep_measure(min(args.steps, 100), min(args.warmup, 5))
         out["small_bar"] = small_bar_graph()

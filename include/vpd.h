@@ -46,7 +46,11 @@ enum vpd_return_map {
   /* vpd_return_map_device only: mode 0 with the FP64 yield evaluated at every iteration (modes
    * 0 and 1 evaluate it only where an FP32 estimate with an error bound cannot decide the
    * iteration's branch, with the same iterates and results bit for bit); for checking. */
-  VPD_RETURN_MAP_BISECTION_UNSCREENED = 4
+  VPD_RETURN_MAP_BISECTION_UNSCREENED = 4,
+  /* vpd_return_map_device only: mode 0 on lane groups (32 lanes per point, each round
+   * evaluating every iterate of the next five iterations at once), the form vpd_step uses
+   * for short plastic lists of blocks of up to 32,768 particles; the same bits as mode 0. */
+  VPD_RETURN_MAP_BISECTION_GROUPED = 5
 };
 
 /* The VUMAT the material stage runs (the plugin of VumatMaterialConfig::pluginPath). */

@@ -515,6 +515,31 @@ def test_window_knob_limits(vpd):
     b.step(1)
 
 
+@pytest.mark.parametrize("mode,shape,plastic", [(0, (30, 8, 8), 500), (1, (30, 8, 8), 500),
+                                                (0, (40, 16, 16), 4200)],
+                         ids=["parity-bisection", "tight-bisection", "long-list"])
+def test_grouped_plastic_list_equals_serial(vpd, mode, shape, plastic):
+    # Small blocks run the bisections' plastic list on lane groups (several iterations per
+    # round) while it is short, and on one lane per point past 4096 points (the long-list
+    # case crosses over); with run diagnostics the material stage runs every point on one
+    # lane. The two must agree bit for bit, iteration counts (state 6) included.
+    banner("plastic list on lane groups vs one lane per point, return map %d, %s"
+           % (mode, shape))
+    kw = dict(spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5, density=7.85e-3,
+              velocity=60.0, damage=(0.003, 0.0), return_map=mode)
+    grouped, serial = vpd.Bar(*shape, **kw), vpd.Bar(*shape, **kw)
+    serial.ctx.enable_diagnostics(0, 1, capacity=200)
+    for k in (40, 80):
+        grouped.step(k)
+        serial.step(k)
+        for name in ("u", "v", "a", "stressN", "statev", "damage"):
+            assert np.array_equal(grouped.get(name), serial.get(name)), (k, name)
+        print("step %d: %d plastic points" % (grouped.step_index,
+                                              (grouped.get("statev")[:, 4] > 0).sum()))
+    st = grouped.get("statev")
+    assert (st[:, 4] > 0).sum() > plastic and st[:, 5].max() > 100
+
+
 def test_autotune_and_in_step_confirmation(vpd):
     # The autotuner leaves the state as it was and sets its pick; its re-timed candidates per
     # pass come fastest first and include the pick. The in-step confirmation restores the

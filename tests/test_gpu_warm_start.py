@@ -140,8 +140,13 @@ def test_screened_bisection_equals_unscreened(vpd, case):
     plastic = (b["state"][4] > 0).mean()
     print("%s: plastic %.3f, mean iterations %.2f, screened %.3f ms, every-iteration %.3f ms"
           % (case, plastic, b["iterations"].mean(), a["ms"], b["ms"]))
+    # The same bisection on lane groups (several iterations per round).
+    c = vpd.return_map(vpd.RETURN_MAP_BISECTION_GROUPED, props, dt, inc, stress, state, eint,
+                       einel)
+    print("%s: lane groups %.3f ms, max iterations %d" % (case, c["ms"], b["iterations"].max()))
     for k in ("stress", "state", "eint", "einel", "iterations"):
         assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(c[k], b[k]), ("lane groups", k)
     assert case == "tiny" or plastic > 0.1
 
 

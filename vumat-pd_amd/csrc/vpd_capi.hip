@@ -978,6 +978,11 @@ int ensure_list(vpd_context *c) {
 // The compacted material update (k_material_screen, then the return map on the listed
 // plastic points only; see vpd_kernels.hpp): in the learned mode the list kernel's last
 // wavefronts run the network on the listed points without a positive state 5.
+// Blocks of up to kGroupedListMax particles run the bisections' plastic list with
+// k_material_return_list_g: on lane groups of kListLanes per point while the list is short.
+constexpr int64_t kGroupedListMax = 32768;
+constexpr int kListLanes = 32;
+
 template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g, int grid) {
   if (int s = ensure_list(c))
     return s;
@@ -996,6 +1001,13 @@ template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g,
       }))
     return s;
   return timed(c, K_RETURN_MAP, [&] {
+    if constexpr (MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT) {
+      if (c->n <= kGroupedListMax) { // short lists: lane groups per point
+        k_material_return_list_g<MODE, kListLanes>
+            <<<(int)ceil_div(c->n * kListLanes, 256), 256, 0, c->stream>>>(g, cl);
+        return;
+      }
+    }
     k_material_return_list<MODE><<<grid_of(c->n), 256, 0, c->stream>>>(g, cl);
   });
 }
@@ -3224,7 +3236,7 @@ static int return_map_run(int mode, int64_t nblock, int nstatev, const double *p
                           double *enerInternNew, double *enerInelasNew, int32_t *iterations,
                           int32_t *fallbacks, double *elapsedMs, vpd_mlp *net = nullptr,
                           double *seedOut = nullptr) {
-  if (mode < RM_BISECTION || mode > VPD_RETURN_MAP_BISECTION_UNSCREENED)
+  if (mode < RM_BISECTION || mode > VPD_RETURN_MAP_BISECTION_GROUPED)
     return fail(nullptr, VPD_ERR_ARGUMENT, "Unknown return-map mode.");
   if (!props || nprops < 14 || nstatev < 6)
     return fail(nullptr, VPD_ERR_ARGUMENT, "JohnsonCook VUMAT requires 14 props and 6 states");
@@ -3281,6 +3293,9 @@ static int return_map_run(int mode, int64_t nblock, int nstatev, const double *p
     break;
   case VPD_RETURN_MAP_BISECTION_UNSCREENED:
     k_vumat_jc<RM_BISECTION_UNSCREENED><<<grid_of(nblock), 256>>>(g);
+    break;
+  case VPD_RETURN_MAP_BISECTION_GROUPED:
+    k_vumat_jc_g<kListLanes><<<(int)ceil_div(nblock * kListLanes, 256), 256>>>(g);
     break;
   default:
     if (net)

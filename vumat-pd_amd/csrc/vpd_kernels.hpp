@@ -1760,6 +1760,59 @@ __global__ void __launch_bounds__(256) k_material_return_list(MaterialArgs g, Co
   material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
 }
 
+// The bisections' list on lane groups, for short lists (small blocks), where one lane per
+// point leaves the GPU idle and the list waits on the longest chain of iterations: G lanes
+// (8, 16, 32 or 64) share each listed point and run its bisection log2(G) iterations per round
+// (jc_point's lane groups: the single lane's bits); the group's first lane writes it. A round
+// costs a few single-lane iterations, so a list longer than kGroupedLanes / G points (more
+// than about two wavefronts per SIMD) runs one lane per point instead; the grid covers the
+// block's every point with G lanes.
+constexpr int64_t kGroupedLanes = 131072;
+template <int MODE, int G>
+__device__ __forceinline__ void list_point_bisect(const MaterialArgs &g, const CompactArgs &cl,
+                                                  int64_t t, int front, int back, bool lead) {
+  const int64_t slot = list_slot(cl, t, front, back);
+  const int64_t i = cl.list[slot];
+  double inc[6], sOld[6], sNew[6], st[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    inc[c] = cl.inc(c, slot);
+    sOld[c] = g.sig(c, i);
+    st[c] = g.statev(c, i);
+  }
+  double eint = g.eint[i], einel = g.einel[i];
+  ReturnMapStats stats;
+  const Ramp none{0.0, 0.0, 0.0};
+  const double dt = point_step(g.batch, i, g.dt, none, none).dt;
+  jc_point<MODE, false, G>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
+  if (!lead)
+    return;
+  T3 f1, r1;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) {
+    f1.m[c] = g.fN(c, i);
+    r1.m[c] = g.rN(c, i);
+  }
+  material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
+}
+template <int MODE, int G>
+__global__ void __launch_bounds__(256) k_material_return_list_g(MaterialArgs g, CompactArgs cl) {
+  static_assert(MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT, "bisections only");
+  const int front = cl.count[0], back = cl.count[1];
+  const int64_t count = (int64_t)front + back;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (count * G <= kGroupedLanes) {
+    const int64_t t = gid / G;
+    if (t >= count)
+      return; // the whole group
+    list_point_bisect<MODE, G>(g, cl, t, front, back, (threadIdx.x & (G - 1)) == 0);
+  } else {
+    if (gid >= count)
+      return;
+    list_point_bisect<MODE, 1>(g, cl, gid, front, back, true);
+  }
+}
+
 // Bond pass 2. PdCorrespondenceSkeleton.cpp:213-227: f_i = Σ_k (1 − d_ik)(A_i ξ − A_j(−ξ)) V_j
 // in CSR order (a broken bond contributes exactly zero, so it is skipped), then the
 // velocity-Verlet corrector (ExplicitTimeIntegrator.cpp:36-52) and the constraints at t;
@@ -2475,10 +2528,12 @@ struct VumatArgs {
   int *iterations, *fallbacks;
 };
 
-// One point of the device VUMAT: the Johnson-Cook update of point k with the given seed.
-template <int MODE>
+// One point of the device VUMAT: the Johnson-Cook update of point k with the given seed. G > 1:
+// the G lanes of a lane group share the point (jc_point's lane groups); the first writes it.
+template <int MODE, int G = 1>
 __device__ __forceinline__ void vumat_jc_point(const VumatArgs &g, int64_t k, double seed) {
   const int64_t nb = g.nblock;
+  const bool lead = G == 1 || (threadIdx.x & (G - 1)) == 0;
   double inc[6], sOld[6], sNew[6], st[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -2486,20 +2541,24 @@ __device__ __forceinline__ void vumat_jc_point(const VumatArgs &g, int64_t k, do
     sOld[c] = g.stressOld[k + nb * c];
     st[c] = g.stateOld[k + nb * c];
   }
-  if (g.stretchNew)
-    for (int c = 0; c < 6; ++c)
-      g.stretchNew[k + nb * c] = g.stretchOld[k + nb * c];
-  for (int c = 6; c < g.nstatev; ++c)
-    g.stateNew[k + nb * c] = g.stateOld[k + nb * c];
-  if (g.defgradNew)
-    for (int c = 0; c < 9; ++c)
-      g.defgradNew[k + nb * c] = g.defgradOld[k + nb * c];
-  if (g.fieldNew)
-    for (int c = 0; c < g.nfieldv; ++c)
-      g.fieldNew[k + nb * c] = g.fieldOld[k + nb * c];
+  if (lead) {
+    if (g.stretchNew)
+      for (int c = 0; c < 6; ++c)
+        g.stretchNew[k + nb * c] = g.stretchOld[k + nb * c];
+    for (int c = 6; c < g.nstatev; ++c)
+      g.stateNew[k + nb * c] = g.stateOld[k + nb * c];
+    if (g.defgradNew)
+      for (int c = 0; c < 9; ++c)
+        g.defgradNew[k + nb * c] = g.defgradOld[k + nb * c];
+    if (g.fieldNew)
+      for (int c = 0; c < g.nfieldv; ++c)
+        g.fieldNew[k + nb * c] = g.fieldOld[k + nb * c];
+  }
   double eint = g.enerInternOld[k], einel = g.enerInelasOld[k];
   ReturnMapStats stats;
-  jc_point<MODE>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, seed, g.rtol, &stats);
+  jc_point<MODE, false, G>(g.jc, g.dt, inc, sOld, st, eint, einel, sNew, seed, g.rtol, &stats);
+  if (!lead)
+    return;
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     g.stressNew[k + nb * c] = sNew[c];
@@ -2521,6 +2580,14 @@ __global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
   if (k >= g.nblock)
     return;
   vumat_jc_point<MODE>(g, k, g.seed ? g.seed[k] : 0.0);
+}
+
+// The parity bisection of the device VUMAT on lane groups of G (8, 16, 32 or 64) per point.
+template <int G> __global__ void __launch_bounds__(256) k_vumat_jc_g(VumatArgs g) {
+  const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (k >= g.nblock)
+    return; // the whole group
+  vumat_jc_point<RM_BISECTION, G>(g, k, 0.0);
 }
 
 } // namespace vpd

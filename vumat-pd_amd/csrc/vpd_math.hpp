@@ -517,7 +517,10 @@ __device__ __forceinline__ bool jc_elastic(const JcParams &p, const double inc[6
 // knows (jc_elastic) that the solve ends at its first iteration with Δε̄ᵖ = 0 and passes
 // `first`; the solver loops are then left out and that iteration's outcome is taken
 // (iteration 1, Δε̄ᵖ = 0, the stored yield first->yield), the bits the full routine gives.
-template <int MODE, bool FIRST_EXIT = false>
+// G > 1 (8, 16, 32 or 64; the screened bisections only): the G lanes of an aligned group call
+// this for the same point with the same inputs, and its bisection runs log2(G) iterations per
+// round (below); every lane of the group returns the same results, the single lane's bits.
+template <int MODE, bool FIRST_EXIT = false, int G = 1>
 __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const double inc[6],
                                         const double stressOld[6], double state[6],
                                         double &eint, double &einel, double stressNew[6],
@@ -580,7 +583,108 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
     // the literal loop's.
     bool done = false, exact = true; // iteration 1 is evaluated exactly
     iteration = 1;
-    while (!done) {
+    static_assert(G == 1 || ((G == 8 || G == 16 || G == 32 || G == 64) && SCREEN),
+                  "lane groups of 8, 16, 32 or 64");
+    if constexpr (G > 1) {
+      // Lane groups: iteration 1 as below, then rounds of L = log2(G) iterations. The G − 1
+      // iterates the next L iterations can visit form a binary tree (heap order: node 1 is the
+      // round's first iterate, node 2m + 1 follows node m when its residual is ≥ 0, 2m when
+      // < 0); group lane m evaluates node m, whose iterate it reaches by applying the loop's
+      // own bracket updates along the path its index encodes, with the single loop's screen
+      // or exact evaluation. The group then walks the decided path with those updates; each
+      // node's outcome (end, or the residual's sign) is what the single loop finds there, so
+      // the iterates, the iteration count and the stored yield are the same bits.
+      constexpr int L = G == 8 ? 3 : G == 16 ? 4 : G == 32 ? 5 : 6;
+      const int lane = __lane_id(), gl = lane & (G - 1), base = lane & ~(G - 1);
+      {
+        const double epRate = epInc / dt;
+        if (first) {
+          yield = first->yield;
+        } else {
+          yield = jc_yield_soft(p, epOld + epInc, epRate, soft);
+          yield = fmax(yield, oldYield);
+        }
+        const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
+        const double residual = qNew - yield;
+        if (fabs(residual) < tol || (epInc == 0.0 && residual < 0.0)) {
+          done = true;
+        } else {
+          if (residual >= 0.0)
+            epLower = fmax(epLower, epInc);
+          else
+            epUpper = fmin(epUpper, epInc);
+          epInc = fmin(fmax(state[4], epLower), epUpper); // iteration 1 moves to the seed
+          iteration = iteration + 1;
+        }
+      }
+      while (!done) {
+        int depth = -1;
+        double lo = epLower, hi = epUpper, e = epInc;
+        if (gl > 0) {
+          depth = 31 - __builtin_clz((unsigned)gl);
+          for (int d = depth - 1; d >= 0; --d) {
+            if ((gl >> d) & 1)
+              lo = fmax(lo, e);
+            else
+              hi = fmin(hi, e);
+            e = 0.5 * (lo + hi);
+          }
+        }
+        const int itNode = iteration + depth;
+        bool nodeEnds = false, nodeUp = false;
+        double nodeYield = 0.0;
+        if (gl > 0 && itNode <= maxIt) {
+          const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * e);
+          bool decided = false;
+          double ra = 0.0;
+          if (itNode != maxIt) {
+            double ya, margin;
+            if (jc_yield_screen(p, epOld + e, e, rateScale, soft, ya, margin)) {
+              ra = qNew - fmax(ya, oldYield);
+              const double slack = margin + 1e-14 * (qNew + fabs(ya));
+              decided = fabs(ra) >= tol + slack && !(e == 0.0 && ra < 0.0);
+            }
+          }
+          if (decided) {
+            nodeUp = ra >= 0.0;
+          } else {
+            double yv = jc_yield_soft(p, epOld + e, e / dt, soft);
+            yv = fmax(yv, oldYield);
+            const double residual = qNew - yv;
+            nodeYield = yv;
+            nodeEnds = fabs(residual) < tol || (e == 0.0 && residual < 0.0);
+            nodeUp = residual >= 0.0;
+          }
+        }
+        const uint64_t ends = __ballot(nodeEnds) >> base;
+        const uint64_t ups = __ballot(nodeUp) >> base;
+        int node = 1, last = 0;
+        for (int d = 0; d < L; ++d) {
+          if ((ends >> node) & 1ull) {
+            last = node;
+            done = true;
+            break;
+          }
+          const bool up = (ups >> node) & 1ull;
+          if (up)
+            epLower = fmax(epLower, epInc);
+          else
+            epUpper = fmin(epUpper, epInc);
+          epInc = 0.5 * (epLower + epUpper);
+          if (iteration == maxIt) { // evaluated exactly (it is the last allowed)
+            last = node;
+            done = true;
+            break;
+          }
+          iteration = iteration + 1;
+          node = 2 * node + (up ? 1 : 0);
+        }
+        const double y = __shfl(nodeYield, base + last);
+        if (done)
+          yield = y;
+      }
+    }
+    while (G == 1 && !done) {
       if (SCREEN) {
         while (!exact) {
           const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);

@@ -28,6 +28,7 @@ DAMAGE_NONE, DAMAGE_CRITICAL_STRETCH = 0, 1
 MATERIAL_JOHNSON_COOK, MATERIAL_ELASTIC = 0, 1
 RETURN_MAP_BISECTION, RETURN_MAP_BISECTION_TIGHT, RETURN_MAP_NEWTON, RETURN_MAP_LEARNED = 0, 1, 2, 3
 RETURN_MAP_BISECTION_UNSCREENED = 4  # vpd_return_map_device only
+RETURN_MAP_BISECTION_GROUPED = 5  # vpd_return_map_device only: mode 0 on lane groups
 MLP_PARAMS, MLP_NORM = 8961, 18
 
 
