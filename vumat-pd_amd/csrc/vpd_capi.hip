@@ -991,8 +991,9 @@ template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g,
     cl.theta = c->mlp->theta;
     cl.norm = c->mlp->norm;
   }
-  // A captured step clears the lengths itself: its replays may follow other work.
-  const bool clear = !c->listZeroed || c->capturing;
+  // A step captured by vpd_run_specimens finds them zero at every replay: they are cleared
+  // before the first, and each replay's force pass clears them for the next.
+  const bool clear = !c->listZeroed;
   c->listZeroed = false;
   if (int s = timed(c, K_MATERIAL, [&] {
         if (clear)
@@ -2114,6 +2115,9 @@ int vpd_run_specimens(vpd_context *c, int32_t firstStep, int32_t count) {
     if (int s = launch_predict(c, 1.0, none))
       return s;
   k_set_step<<<1, 64, 0, c->stream>>>(c->stepDev, firstStep);
+  if (c->listCount && !c->listZeroed) // the first replay's list lengths (see below)
+    k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
+  c->listZeroed = c->listCount != nullptr;
   HIPTRY(c, hipGetLastError());
   // One fused step, reading its step number from the device counter, captured once.
   const bool timing = c->timing;
