@@ -2226,12 +2226,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(group_
 // Shared rows (gather form 4), for small blocks, where one lane per particle leaves the GPU
 // nearly idle and a step waits on each lane's chain of ~120 dependent bond gathers: a group
 // of R lanes (16, 32 or 64: a quarter, half or whole wavefront) owns one particle. In each
-// chunk of R bonds every lane computes one bond's term (the expressions of the direct
-// kernels) into the group's LDS; then lane c adds component c of the chunk's terms in CSR
-// order (9 components for F, 3 for the force). Each component's sum is the direct kernel's
-// sum term for term, so the result is the same bits. A group exits alone (a particle outside
-// the block or a finished specimen); groups share only their wavefront, so no workgroup
-// barrier is used.
+// chunk of kRowChunks·R bonds every lane computes the terms of kRowChunks bonds (the
+// expressions of the direct kernels; all their loads issued before any term, slots past the
+// row's end loading its last bond and left unread) into the group's LDS; then lane c adds
+// component c of the chunk's terms in CSR order (9 components for F, 3 for the force). Each
+// component's sum is the direct kernel's sum term for term, so the result is the same bits.
+// A group exits alone (a particle outside the block or a finished specimen); groups share
+// only their wavefront, so no workgroup barrier is used.
+constexpr int kRowChunks = 2;
 // The group's LDS terms are written (read) by all its lanes before any lane goes on; the
 // compiler keeps its memory accesses on their side of this point.
 __device__ __forceinline__ void row_lds_sync() {
@@ -2245,7 +2247,8 @@ __global__ void __launch_bounds__(256)
     k_defgrad_m(int64_t n, Planes x, PairPlanes xv, UPairs u, Sell bonds, Planes kinv,
                 Planes fout, uint32_t *broken, uint8_t *anyBroken, double initiation,
                 double failure, int useCompression, Batch batch) {
-  __shared__ double terms[256 / R][R][9];
+  constexpr int CH = kRowChunks;
+  __shared__ double terms[256 / R][CH * R][9];
   const int gp = threadIdx.x / R, l = threadIdx.x % R;
   const int64_t i = (int64_t)blockIdx.x * (256 / R) + gp;
   if (i >= n || !point_active(batch, i))
@@ -2254,33 +2257,46 @@ __global__ void __launch_bounds__(256)
   const double x0 = x(0, i), x1 = x(1, i), x2 = x(2, i);
   const double yi0 = x0 + u(0, i), yi1 = x1 + u(1, i), yi2 = x2 + u(2, i);
   const int len = bonds.rowptr[i + 1] - bonds.rowptr[i];
+  const int64_t sliceBase = bonds.base[i >> 6];
   double num = 0.0; // component l of Σ V_j (y ⊗ ξ), l < 9
   bool fresh = false;
-  for (int k0 = 0; k0 < len; k0 += R) {
-    const int k = k0 + l;
-    if (k < len) {
-      const int64_t pos = sell_pos(bonds, i, k);
-      const int j = bonds.col[pos];
-      const d2v a = xv(0, j), b = xv(1, j);
-      const d2v ua = u.pair(0, j), ub = u.pair(1, j);
-      const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
-      const double yij[3] = {(a.x + ua.x) - yi0, (a.y + ua.y) - yi1, (b.x + ub.x) - yi2};
+  for (int k0 = 0; k0 < len; k0 += CH * R) {
+    int64_t pos[CH];
+    int j[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      pos[c] = sell_slot(sliceBase, i, min(k0 + c * R + l, len - 1));
+      j[c] = bonds.col[pos[c]];
+    }
+    d2v a[CH], b[CH], ua[CH], ub[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      a[c] = xv(0, j[c]);
+      b[c] = xv(1, j[c]);
+      ua[c] = u.pair(0, j[c]);
+      ub[c] = u.pair(1, j[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const double xi[3] = {a[c].x - x0, a[c].y - x1, b[c].x - x2};
+      const double yij[3] = {(a[c].x + ua[c].x) - yi0, (a[c].y + ua[c].y) - yi1,
+                             (b[c].x + ub[c].x) - yi2};
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          terms[gp][l][3 * r + c] = b.y * yij[r] * xi[c];
-      if (DAMAGE) {
-        const uint32_t bit = 1u << (pos & 31);
-        if (!(broken[pos >> 5] & bit) && stretch_breaks(test, xi, yij)) {
-          atomicOr(&broken[pos >> 5], bit);
+        for (int q = 0; q < 3; ++q)
+          terms[gp][c * R + l][3 * r + q] = b[c].y * yij[r] * xi[q];
+      if (DAMAGE && k0 + c * R + l < len) {
+        const uint32_t bit = 1u << (pos[c] & 31);
+        if (!(broken[pos[c] >> 5] & bit) && stretch_breaks(test, xi, yij)) {
+          atomicOr(&broken[pos[c] >> 5], bit);
           fresh = true;
         }
       }
     }
     row_lds_sync(); // the group's terms are in LDS
     if (l < 9) {
-      const int m1 = min(R, len - k0);
+      const int m1 = min(CH * R, len - k0);
       for (int m = 0; m < m1; ++m)
         num += terms[gp][m][l];
     }
@@ -2308,7 +2324,8 @@ __global__ void __launch_bounds__(256)
 template <bool DAMAGE, bool FUSE_NEXT, bool INTEGRATE, int R>
 __global__ void __launch_bounds__(256) k_force_m(ForceArgs g) {
   clear_list_length(g);
-  __shared__ double terms[256 / R][R][3];
+  constexpr int CH = kRowChunks;
+  __shared__ double terms[256 / R][CH * R][3];
   const int gp = threadIdx.x / R, l = threadIdx.x % R;
   const int64_t i = (int64_t)blockIdx.x * (256 / R) + gp;
   if (i >= g.n || !point_active(g.batch, i))
@@ -2316,33 +2333,44 @@ __global__ void __launch_bounds__(256) k_force_m(ForceArgs g) {
   const double x0 = g.x(0, i), x1 = g.x(1, i), x2 = g.x(2, i);
   const T3 ai = load_amap(g.amap, i);
   const int len = g.bonds.rowptr[i + 1] - g.bonds.rowptr[i];
+  const int64_t sliceBase = g.bonds.base[i >> 6];
   const int shift = R * ((threadIdx.x & 63) / R); // the group's bits in a ballot
   const uint64_t mask = R == 64 ? ~0ull : (1ull << (R & 63)) - 1;
   double acc = 0.0; // component l of the force density, l < 3
-  for (int k0 = 0; k0 < len; k0 += R) {
-    const int k = k0 + l;
-    bool live = false; // a broken bond contributes exactly zero: it is skipped
-    if (k < len) {
-      const int64_t pos = sell_pos(g.bonds, i, k);
-      live = !(DAMAGE && (g.broken[pos >> 5] & (1u << (pos & 31))));
-      if (live) {
-        const int j = g.bonds.col[pos];
-        const d2v a = g.xv(0, j), b = g.xv(1, j);
-        const T3 aj = load_amap(g.amap, j);
-        const double xi[3] = {a.x - x0, a.y - x1, b.x - x2};
-        double term[3];
-        force_bond_term(ai, aj, xi, b.y, term);
+  for (int k0 = 0; k0 < len; k0 += CH * R) {
+    int64_t pos[CH];
+    int j[CH];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-          terms[gp][l][d] = term[d];
-      }
+    for (int c = 0; c < CH; ++c) {
+      pos[c] = sell_slot(sliceBase, i, min(k0 + c * R + l, len - 1));
+      j[c] = g.bonds.col[pos[c]];
     }
-    const uint64_t lives = (__ballot(live) >> shift) & mask;
+    d2v a[CH], b[CH];
+    T3 aj[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      a[c] = g.xv(0, j[c]);
+      b[c] = g.xv(1, j[c]);
+      aj[c] = load_amap(g.amap, j[c]);
+    }
+    uint64_t lives[CH]; // a broken bond contributes exactly zero: it is skipped
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const bool live = k0 + c * R + l < len &&
+                        !(DAMAGE && (g.broken[pos[c] >> 5] & (1u << (pos[c] & 31))));
+      lives[c] = (__ballot(live) >> shift) & mask;
+      const double xi[3] = {a[c].x - x0, a[c].y - x1, b[c].x - x2};
+      double term[3];
+      force_bond_term(ai, aj[c], xi, b[c].y, term);
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        terms[gp][c * R + l][d] = term[d];
+    }
     row_lds_sync();
     if (l < 3) {
-      const int m1 = min(R, len - k0);
+      const int m1 = min(CH * R, len - k0);
       for (int m = 0; m < m1; ++m)
-        if ((lives >> m) & 1ull)
+        if ((lives[m / R] >> (m % R)) & 1ull)
           acc += terms[gp][m][l];
     }
     row_lds_sync();
