@@ -162,7 +162,7 @@ struct vpd_context {
   double *inc = nullptr, *seed = nullptr;
   int *fallbacks = nullptr;
   // The compacted material update: list lengths, particle per slot, the network's slots,
-  // strain increments per slot. listZeroed: the lengths are known to be zero (a force pass
+  // per slot the strain increments and the first iteration's 1 − θᵐ and yield. listZeroed: the lengths are known to be zero (a force pass
   // cleared them after the last material update), so the next update need not clear them.
   int *listCount = nullptr, *list = nullptr;
   double *incList = nullptr;
@@ -969,7 +969,7 @@ int ensure_list(vpd_context *c) {
   if (!c->list) {
     c->listZeroed = false;
     if (dalloc(c, c->listCount, 2) || dalloc(c, c->list, c->n) ||
-        dalloc(c, c->incList, 6 * c->n))
+        dalloc(c, c->incList, 8 * c->n))
       return VPD_ERR_DEVICE;
   }
   return VPD_OK;

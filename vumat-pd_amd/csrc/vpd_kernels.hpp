@@ -1452,7 +1452,8 @@ struct CompactArgs {
   int *count;   // count[0]: slots from the front, count[1]: slots from the back (one 8-byte
                 // word, zeroed before the screen)
   int *list;    // particle index per slot; back slot b is list[capacity − 1 − b]
-  Planes inc;   // strain increments per slot (6 planes of the capacity)
+  Planes inc;   // per slot: the strain increments (planes 0-5) and the first iteration's
+                // 1 − θᵐ and yield as the screen evaluated them (planes 6, 7; JcFirst)
   int64_t capacity;
   const float *theta, *norm; // learned mode: the network (vpd_mlp)
 };
@@ -1708,6 +1709,8 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
 #pragma unroll
     for (int c = 0; c < 6; ++c)
       cl.inc(c, slot) = inc[c];
+    cl.inc(6, slot) = first.soft;
+    cl.inc(7, slot) = first.yield;
   }
 }
 
@@ -1746,11 +1749,12 @@ __global__ void __launch_bounds__(256) k_material_return_list(MaterialArgs g, Co
   }
   if (!valid)
     return;
+  const JcFirst first{cl.inc(6, slot), cl.inc(7, slot)};
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
   const Ramp none{0.0, 0.0, 0.0};
   const double dt = point_step(g.batch, i, g.dt, none, none).dt;
-  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, seed, g.rtol, &stats);
+  jc_point<MODE>(g.jc, dt, inc, sOld, st, eint, einel, sNew, seed, g.rtol, &stats, &first);
   T3 f1, r1;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
@@ -1780,11 +1784,13 @@ __device__ __forceinline__ void list_point_bisect(const MaterialArgs &g, const C
     sOld[c] = g.sig(c, i);
     st[c] = g.statev(c, i);
   }
+  const JcFirst first{cl.inc(6, slot), cl.inc(7, slot)};
   double eint = g.eint[i], einel = g.einel[i];
   ReturnMapStats stats;
   const Ramp none{0.0, 0.0, 0.0};
   const double dt = point_step(g.batch, i, g.dt, none, none).dt;
-  jc_point<MODE, false, G>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats);
+  jc_point<MODE, false, G>(g.jc, dt, inc, sOld, st, eint, einel, sNew, 0.0, g.rtol, &stats,
+                           &first);
   if (!lead)
     return;
   T3 f1, r1;
