@@ -2374,10 +2374,16 @@ __global__ void __launch_bounds__(256) k_force_m(ForceArgs g) {
     }
     row_lds_sync();
     if (l < 3) {
-      const int m1 = min(CH * R, len - k0);
-      for (int m = 0; m < m1; ++m)
-        if ((lives[m / R] >> (m % R)) & 1ull)
-          acc += terms[gp][m][l];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int mc = min(R, len - k0 - c * R);
+        // A select, not a branch, so that the LDS reads need not wait on the sums; a broken
+        // bond leaves the sum as it was (not + 0.0, which would turn −0.0 into +0.0).
+        for (int m = 0; m < mc; ++m) {
+          const double t = terms[gp][c * R + m][l];
+          acc = ((lives[c] >> m) & 1ull) ? acc + t : acc;
+        }
+      }
     }
     row_lds_sync();
   }
