@@ -326,29 +326,27 @@ __device__ __forceinline__ double screen_log2(double x) { // x > 0, normal
   return (double)__builtin_amdgcn_frexp_exp(x) + (double)__builtin_amdgcn_logf(m);
 }
 // The estimate's two transcendental terms: ε̄ⁿ (pw) and ln ρ (lr, 0 for Δ = 0: the rate is
-// then ε̇₀). False for arguments outside the range the FP32 steps cover.
+// then ε̇₀). False for arguments outside the range the FP32 steps cover (pow(0, n) for n ≤ 0
+// included: left to the exact evaluation). Branch-free: each term is evaluated on its
+// argument or, where that is out of range, on 1, and the results are selected, so that the
+// checks (which in practice always pass) cost selects rather than divergent branches.
 __device__ __forceinline__ bool screen_terms(const JcParams &p, double ep, double dep,
                                              double rateScale, double &pw, double &lr) {
-  pw = 0.0;
-  if (ep != 0.0) {
-    if (!(ep > 1e-300 && ep < 1e300))
-      return false;
-    const double t = p.hardN * screen_log2(ep);
-    if (!(fabs(t) < 1000.0))
-      return false;
-    const double k = floor(t);
-    pw = __builtin_amdgcn_ldexp((double)__builtin_amdgcn_exp2f((float)(t - k)), (int)k);
-  } else if (!(p.hardN > 0.0)) {
-    return false; // pow(0, n) for n <= 0 is left to the exact evaluation
-  }
-  lr = 0.0;
-  if (dep != 0.0) {
-    const double rho = dep * rateScale;
-    if (!(rho > 1e-300 && rho < 1e300))
-      return false;
-    lr = 0.6931471805599453 * screen_log2(rho);
-  }
-  return true;
+  const bool epZero = ep == 0.0;
+  const bool epOk = ep > 1e-300 && ep < 1e300;
+  const double t = p.hardN * screen_log2(epOk ? ep : 1.0);
+  const bool tOk = fabs(t) < 1000.0;
+  const double tc = tOk ? t : 0.0;
+  const double k = floor(tc);
+  const double pwEst =
+      __builtin_amdgcn_ldexp((double)__builtin_amdgcn_exp2f((float)(tc - k)), (int)k);
+  pw = epZero ? 0.0 : pwEst;
+  const bool depZero = dep == 0.0;
+  const double rho = dep * rateScale;
+  const bool rhoOk = rho > 1e-300 && rho < 1e300;
+  const double lrEst = 0.6931471805599453 * screen_log2(rhoOk ? rho : 1.0);
+  lr = depZero ? 0.0 : lrEst;
+  return (epZero ? p.hardN > 0.0 : epOk && tOk) && (depZero || rhoOk);
 }
 // Four times the first-order error bound of the estimate (above).
 __device__ __forceinline__ double screen_margin(const JcParams &p, double soft, double bpw,
