@@ -357,18 +357,18 @@ __device__ __forceinline__ double screen_margin(const JcParams &p, double soft, 
   return 4.0 * fabs(soft) *
          (fabs(bpw) * fabs(rfac) * ePw + fabs(hard) * fabs(p.rateC) * (ln2 * eLog));
 }
+// y and margin are set also where it returns false (finite then, and not an estimate).
 __device__ __forceinline__ bool jc_yield_screen(const JcParams &p, double ep, double dep,
                                                 double rateScale, double soft, double &y,
                                                 double &margin) {
   double pw, lr;
-  if (!screen_terms(p, ep, dep, rateScale, pw, lr))
-    return false;
+  const bool ok = screen_terms(p, ep, dep, rateScale, pw, lr);
   const double bpw = p.b0 * pw;
   const double hard = p.a0 + bpw;
   const double rfac = 1.0 + p.rateC * lr;
   y = hard * soft * rfac;
   margin = screen_margin(p, soft, bpw, hard, rfac);
-  return true;
+  return ok;
 }
 // The estimate with its slope dY/dΔ at Δ = dep > 0 (jc_yield_slope's expressions on the
 // estimated terms), for the FP32 phase of the Newton solvers.
@@ -633,16 +633,12 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
         double nodeYield = 0.0;
         if (gl > 0 && itNode <= maxIt) {
           const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * e);
-          bool decided = false;
-          double ra = 0.0;
-          if (itNode != maxIt) {
-            double ya, margin;
-            if (jc_yield_screen(p, epOld + e, e, rateScale, soft, ya, margin)) {
-              ra = qNew - fmax(ya, oldYield);
-              const double slack = margin + 1e-14 * (qNew + fabs(ya));
-              decided = fabs(ra) >= tol + slack && !(e == 0.0 && ra < 0.0);
-            }
-          }
+          double ya, margin;
+          const bool est = jc_yield_screen(p, epOld + e, e, rateScale, soft, ya, margin);
+          const double ra = qNew - fmax(ya, oldYield);
+          const double slack = margin + 1e-14 * (qNew + fabs(ya));
+          const bool decided = itNode != maxIt && est && fabs(ra) >= tol + slack &&
+                               !(e == 0.0 && ra < 0.0);
           if (decided) {
             nodeUp = ra >= 0.0;
           } else {
@@ -686,14 +682,12 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
       if (SCREEN) {
         while (!exact) {
           const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
-          double ya, margin, ra = 0.0;
-          bool decided = false;
-          if (jc_yield_screen(p, epOld + epInc, epInc, rateScale, soft, ya, margin)) {
-            ra = qNew - fmax(ya, oldYield);
-            const double slack = margin + 1e-14 * (qNew + fabs(ya));
-            // Same sign as the exact residual, which is not below tol and ends nothing.
-            decided = fabs(ra) >= tol + slack && !(epInc == 0.0 && ra < 0.0);
-          }
+          double ya, margin;
+          const bool est = jc_yield_screen(p, epOld + epInc, epInc, rateScale, soft, ya, margin);
+          const double ra = qNew - fmax(ya, oldYield);
+          const double slack = margin + 1e-14 * (qNew + fabs(ya));
+          // Same sign as the exact residual, which is not below tol and ends nothing.
+          const bool decided = est && fabs(ra) >= tol + slack && !(epInc == 0.0 && ra < 0.0);
           if (!decided) {
             exact = true;
             break;
