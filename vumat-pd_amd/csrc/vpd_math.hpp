@@ -507,6 +507,72 @@ __device__ __forceinline__ bool jc_elastic(const JcParams &p, const double inc[6
   return !(r0 > 0.0 && fabs(r0) >= rtol * qTrial);
 }
 
+// The elastic trial stress of JohnsonCook.f90:75-81 (Voigt, engineering shear increments).
+__device__ __forceinline__ void jc_trial_stress(const JcParams &p, const double inc[6],
+                                                const double stressOld[6], double trial[6]) {
+  const double traceInc = inc[0] + inc[1] + inc[2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    trial[i] = stressOld[i] + p.lambda * traceInc + 2.0 * p.mu * inc[i];
+#pragma unroll
+  for (int i = 3; i < 6; ++i)
+    trial[i] = stressOld[i] + 2.0 * p.mu * inc[i];
+}
+// Its pressure, deviator and von Mises stress q (JohnsonCook.f90:108-112).
+__device__ __forceinline__ double jc_deviator(const double trial[6], double &pressure,
+                                              double dev[6]) {
+  pressure = (trial[0] + trial[1] + trial[2]) / 3.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    dev[i] = trial[i] - pressure;
+#pragma unroll
+  for (int i = 3; i < 6; ++i)
+    dev[i] = trial[i];
+  return sqrt(1.5 * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
+                     2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5])));
+}
+// The update after the solve (JohnsonCook.f90:138-154): radial return by Δε̄ᵖ = epInc, the
+// work and plastic-work energies, heating and the state variables.
+__device__ __forceinline__ void jc_update(const JcParams &p, const double trial[6],
+                                          double pressure, const double dev[6], double qTrial,
+                                          const double inc[6], const double stressOld[6],
+                                          double state[6], double &eint, double &einel,
+                                          double stressNew[6], double epOld,
+                                          double temperature, double epInc, double yield,
+                                          int iteration) {
+  double upd[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+    upd[i] = trial[i];
+  if (qTrial > 0.0) { // JohnsonCook.f90:138-142
+    const double scale = fmax(0.0, 1.0 - 3.0 * p.mu * epInc / qTrial);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      upd[i] = pressure + scale * dev[i];
+#pragma unroll
+    for (int i = 3; i < 6; ++i)
+      upd[i] = scale * dev[i];
+  }
+  const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
+  double work = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+    work = work + (stressOld[i] + upd[i]) * inc[i];
+  const double workInc = 0.5 * work;
+  const double plasticWork = 0.5 * epInc * qNew;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+    stressNew[i] = upd[i];
+  eint = eint + workInc / p.rho;
+  einel = einel + plasticWork / p.rho;
+  state[0] = 1.0;
+  state[1] = epOld + epInc;
+  state[2] = temperature + p.workToHeat * p.beta * plasticWork / (p.rho * p.cp);
+  state[3] = yield;
+  state[4] = epInc;
+  state[5] = state[5] + static_cast<double>(iteration);
+}
+
 // One material point of JohnsonCook.f90:74-156. in: stressOld[6], strainInc[6], state[6]
 // (in/out), energies (in/out); writes stressNew[6]; returns the iteration count (0 for the
 // init branch). The bookkeeping after the solve is the reference's for every solver.
@@ -526,13 +592,7 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
                                         ReturnMapStats *stats = nullptr,
                                         const JcFirst *first = nullptr) {
   double trial[6];
-  const double traceInc = inc[0] + inc[1] + inc[2];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    trial[i] = stressOld[i] + p.lambda * traceInc + 2.0 * p.mu * inc[i];
-#pragma unroll
-  for (int i = 3; i < 6; ++i)
-    trial[i] = stressOld[i] + 2.0 * p.mu * inc[i];
+  jc_trial_stress(p, inc, stressOld, trial);
 
   if (state[0] == 0.0) { // JohnsonCook.f90:92-102
     state[0] = 1.0;
@@ -547,17 +607,8 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
     return 0;
   }
   const double epOld = state[1], temperature = state[2], oldYield = state[3];
-  const double pressure = (trial[0] + trial[1] + trial[2]) / 3.0;
-  double dev[6];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    dev[i] = trial[i] - pressure;
-#pragma unroll
-  for (int i = 3; i < 6; ++i)
-    dev[i] = trial[i];
-  const double qTrial =
-      sqrt(1.5 * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2] +
-                  2.0 * (dev[3] * dev[3] + dev[4] * dev[4] + dev[5] * dev[5])));
+  double pressure, dev[6];
+  const double qTrial = jc_deviator(trial, pressure, dev);
 
   double epInc = 0.0, yield = oldYield;
   int iteration = 0;
@@ -799,38 +850,8 @@ __device__ __forceinline__ int jc_point(const JcParams &p, double dt, const doub
   }
   if (stats)
     stats->iterations += iteration;
-
-  double upd[6];
-#pragma unroll
-  for (int i = 0; i < 6; ++i)
-    upd[i] = trial[i];
-  if (qTrial > 0.0) { // JohnsonCook.f90:138-142
-    const double scale = fmax(0.0, 1.0 - 3.0 * p.mu * epInc / qTrial);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-      upd[i] = pressure + scale * dev[i];
-#pragma unroll
-    for (int i = 3; i < 6; ++i)
-      upd[i] = scale * dev[i];
-  }
-  const double qNew = fmax(0.0, qTrial - 3.0 * p.mu * epInc);
-  double work = 0.0;
-#pragma unroll
-  for (int i = 0; i < 6; ++i)
-    work = work + (stressOld[i] + upd[i]) * inc[i];
-  const double workInc = 0.5 * work;
-  const double plasticWork = 0.5 * epInc * qNew;
-#pragma unroll
-  for (int i = 0; i < 6; ++i)
-    stressNew[i] = upd[i];
-  eint = eint + workInc / p.rho;
-  einel = einel + plasticWork / p.rho;
-  state[0] = 1.0;
-  state[1] = epOld + epInc;
-  state[2] = temperature + p.workToHeat * p.beta * plasticWork / (p.rho * p.cp);
-  state[3] = yield;
-  state[4] = epInc;
-  state[5] = state[5] + static_cast<double>(iteration);
+  jc_update(p, trial, pressure, dev, qTrial, inc, stressOld, state, eint, einel, stressNew,
+            epOld, temperature, epInc, yield, iteration);
   return iteration;
 }
 
