@@ -459,7 +459,9 @@ int vpd_screen_accuracy(double *log2_abs_error, double *exp2_rel_error);
  * seed array: a point with a positive state 5 is seeded with it and the others with the
  * network's estimate, which runs in dense tiles on a list of those points (the seeds of
  * vpd_mlp_seed_device), then the Newton solve. seed_out (device, nblock doubles, may be NULL)
- * receives the seeds; *elapsed_ms covers seeding and solve.
+ * receives the seeds; without it only the network's seeds are written (to the network's work
+ * buffer) and the solve reads state 5 itself, the same seeds and results with less traffic.
+ * *elapsed_ms covers seeding and solve.
  * Uses the network's work buffers: calls with one network must not overlap. Other arguments
  * as for vpd_return_map_device. No reference counterpart. */
 int vpd_return_map_learned_device(vpd_mlp *mlp, int64_t nblock, int nstatev,

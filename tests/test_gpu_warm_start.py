@@ -367,10 +367,12 @@ def test_trained_warm_start_gain(vpd):
 
 
 def test_fused_learned_return_map_equals_two_pass(vpd):
-    # vpd_return_map_learned_device seeds inside the return-map kernel (each wavefront runs the
-    # network on its own points without a positive state 5). It must give the seeds and every
-    # output of the two-call form (vpd_mlp_seed_device, then mode 3 with those seeds), bit for
-    # bit, on a block mixing continuing and first-yield points, ragged at the end.
+    # vpd_return_map_learned_device seeds the block in one launch (each workgroup runs the
+    # network on its own points without a positive state 5), then solves. It must give the
+    # seeds and every output of the two-call form (vpd_mlp_seed_device, then mode 3 with those
+    # seeds), bit for bit, on a block mixing continuing and first-yield points, ragged at the
+    # end; and the same outputs without seed_out, where only the network's seeds are written
+    # and the solve takes state 5 itself.
     banner("learned return map: fused seeding == two-pass")
     net = vpd.load_warm_start()
     n = 100003
@@ -381,8 +383,11 @@ def test_fused_learned_return_map_equals_two_pass(vpd):
     dt = 8.5377e-5
     a = vpd.return_map_learned(net, JC_PROPS, dt, inc, stress, state, z, z)
     b = vpd.return_map_learned(net, JC_PROPS, dt, inc, stress, state, z, z, fused=False)
+    c = vpd.return_map_learned(net, JC_PROPS, dt, inc, stress, state, z, z, want_seeds=False)
     for k in ("seed", "stress", "state", "eint", "einel", "iterations", "fallbacks"):
         assert np.array_equal(a[k], b[k]), k
+        assert k == "seed" or np.array_equal(a[k], c[k]), ("without seed_out", k)
+    assert "seed" not in c
     first = ~(state[4] > 0.0)
     assert np.array_equal(a["seed"][~first], state[4][~first])
     assert (a["seed"][first] > 0).mean() > 0.5

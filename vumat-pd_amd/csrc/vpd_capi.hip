@@ -3274,6 +3274,7 @@ static int return_map_run(int mode, int64_t nblock, int nstatev, const double *p
     }
     seeds = seedOut ? seedOut : net->seedBuf;
     g.seed = seeds;
+    g.seedFromState5 = seedOut == nullptr; // without seed_out only the network's are written
   }
   g.stressNew = stressNew;
   g.stateNew = stateNew;
@@ -3304,7 +3305,8 @@ static int return_map_run(int mode, int64_t nblock, int nstatev, const double *p
   default:
     if (net)
       k_seed_block<<<(int)ceil_div(nblock, 2048), 256>>>(nblock, strainInc, nblock, stateOld,
-                                                          nblock, net->theta, net->norm, seeds);
+                                                          nblock, net->theta, net->norm, seeds,
+                                                          seedOut != nullptr);
     k_vumat_jc<RM_LEARNED><<<grid_of(nblock), 256>>>(g);
   }
   hipError_t e = hipGetLastError();
@@ -3417,6 +3419,7 @@ int vpd_vumat_adapter_call_device(int *nblock, int *ndir, int *nshr, int *nstate
   g.enerInelasNew = enerInelasNew;
   g.rtol = 0.0;
   g.seed = nullptr;
+  g.seedFromState5 = false;
   g.iterations = nullptr;
   g.fallbacks = nullptr;
   k_vumat_jc<RM_BISECTION><<<grid_of(g.nblock), 256>>>(g);
@@ -3737,7 +3740,7 @@ int vpd_mlp_seed_device(vpd_mlp *m, const double *inc, int64_t incStride, const 
   // State 5 where positive, the network on the other points of each 2048-point group.
   k_seed_block<<<(int)ceil_div(n, 2048), 256, 0, m->stream>>>(n, inc, incStride, state,
                                                                stateStride, m->theta, m->norm,
-                                                               seed);
+                                                               seed, true);
   MLPTRY(m, hipGetLastError());
   MLPTRY(m, hipEventRecord(b, m->stream));
   MLPTRY(m, hipEventSynchronize(b));

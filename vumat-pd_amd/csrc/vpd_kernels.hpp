@@ -1548,14 +1548,15 @@ __device__ __forceinline__ int group_inclusive_sum(int v, int *red) {
 // the state planes, stride stateStride) where it is positive, lists the others in its LDS and
 // runs the network on them in 16-point tiles, tile t on wavefront t mod 4: the seed
 // k_mlp_seed gives each of those points (one MFMA column per point, inputs normalised
-// alike), with no device-wide list and no atomic.
+// alike), with no device-wide list and no atomic. allSeeds false: only the network's seeds
+// are written (a reader takes state 5 itself where it is positive).
 __global__ void __launch_bounds__(256) k_seed_block(int64_t n, const double *__restrict__ inc,
                                                     int64_t incStride,
                                                     const double *__restrict__ state,
                                                     int64_t stateStride,
                                                     const float *__restrict__ theta,
                                                     const float *__restrict__ norm,
-                                                    double *__restrict__ seed) {
+                                                    double *__restrict__ seed, bool allSeeds) {
   __shared__ int ids[2048];
   __shared__ int red[4];
   __shared__ int total;
@@ -1569,7 +1570,7 @@ __global__ void __launch_bounds__(256) k_seed_block(int64_t n, const double *__r
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     if (prev[j] > 0.0) {
-      if (p0 + j < n)
+      if (allSeeds && p0 + j < n)
         seed[p0 + j] = prev[j];
     } else {
       mine |= 1u << j;
@@ -2566,6 +2567,9 @@ struct VumatArgs {
   double *tempNew, *stretchNew, *defgradNew, *fieldNew, *stressNew, *stateNew, *enerInternNew,
       *enerInelasNew;
   int *iterations, *fallbacks;
+  // The learned mode's seeds hold only the network's (k_seed_block, allSeeds false): a point
+  // with a positive state 5 is seeded with it.
+  bool seedFromState5;
 };
 
 // One point of the device VUMAT: the Johnson-Cook update of point k with the given seed. G > 1:
@@ -2619,7 +2623,12 @@ __global__ void __launch_bounds__(256) k_vumat_jc(VumatArgs g) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= g.nblock)
     return;
-  vumat_jc_point<MODE>(g, k, g.seed ? g.seed[k] : 0.0);
+  double seed = 0.0;
+  if (g.seed) {
+    const double s5 = MODE == RM_LEARNED && g.seedFromState5 ? g.stateOld[k + g.nblock * 4] : 0.0;
+    seed = s5 > 0.0 ? s5 : g.seed[k];
+  }
+  vumat_jc_point<MODE>(g, k, seed);
 }
 
 // The parity bisection of the device VUMAT on lane groups of G (8, 16, 32 or 64) per point.

@@ -651,11 +651,13 @@ def return_map(mode, props, dt, strain_inc, stress_old, state_old, eint_old, ein
 
 
 def return_map_learned(net, props, dt, strain_inc, stress_old, state_old, eint_old, einel_old,
-                       rtol=1e-12, fused=True):
+                       rtol=1e-12, fused=True, want_seeds=True):
     """The learned return map on a device block. fused: vpd_return_map_learned_device, one
     timed kernel that seeds (state 5 where positive, the network elsewhere) and solves; else
     vpd_mlp_seed_device (timed as seed_ms) then vpd_return_map_device mode 3 (timed). Returns
-    return_map's dict plus the seeds (and seed_ms for the two-call form)."""
+    return_map's dict plus the seeds (and seed_ms for the two-call form). want_seeds False
+    (fused only): no seed_out, so that only the network's seeds are written and read; the
+    results are the same and no seeds are returned."""
     nb = strain_inc.shape[1]
     D = DeviceBuffer
     ins = [D(strain_inc), D(stress_old), D(state_old), D(eint_old), D(einel_old)]
@@ -667,13 +669,17 @@ def return_map_learned(net, props, dt, strain_inc, stress_old, state_old, eint_o
         fb = D(np.zeros(nb))
         ms = C.c_double()
         if lib().vpd_return_map_learned_device(net.h, nb, nst, _f64(props), len(props), dt, rtol,
-                                               *[b.ptr for b in ins], seed.ptr,
+                                               *[b.ptr for b in ins],
+                                               seed.ptr if want_seeds else None,
                                                *[b.ptr for b in outs], it.ptr, fb.ptr,
                                                C.byref(ms)):
             raise VpdError(lib().vpd_last_error(None).decode())
-        return dict(stress=outs[0].read(), state=outs[1].read(), eint=outs[2].read(),
-                    einel=outs[3].read(), iterations=it.read().view(np.int32)[:nb],
-                    fallbacks=fb.read().view(np.int32)[:nb], ms=ms.value, seed=seed.read())
+        out = dict(stress=outs[0].read(), state=outs[1].read(), eint=outs[2].read(),
+                   einel=outs[3].read(), iterations=it.read().view(np.int32)[:nb],
+                   fallbacks=fb.read().view(np.int32)[:nb], ms=ms.value)
+        if want_seeds:
+            out["seed"] = seed.read()
+        return out
     seed_ms = C.c_double()
     if lib().vpd_mlp_seed_device(net.h, ins[0].ptr, nb, ins[2].ptr, nb, nb, seed.ptr,
                                  C.byref(seed_ms)):
