@@ -255,6 +255,27 @@ def test_bar_vs_reference_build(vpd, ref):
         assert np.array_equal(g.get("damage"), r.get("damage"))
 
 
+def test_defgrad_fields_after_swapped_steps(vpd):
+    # A plain step leaves F_{n+1} where F_n was (the two buffers swap instead of the material
+    # stage copying): the current F and the state's F_n read alike. A later F pass, or an
+    # uploaded F, goes to the other buffer and leaves F_n as it was.
+    banner("deformation-gradient fields across swapped steps")
+    b = vpd.Bar(nx=10, ny=4, nz=4, spacing=1.0, horizon=2.015, props=JC_PROPS, dt=8.5377e-5,
+                density=7.85e-3, velocity=300.0)
+    b.step(5)
+    F, FN = b.get("F"), b.get("fN")
+    assert np.array_equal(F, FN)
+    assert np.abs(FN - np.eye(3).reshape(9)[None, :]).max() > 0  # deformed
+    b.ctx.compute_deformation_gradient()
+    assert np.array_equal(b.get("fN"), FN)
+    assert np.array_equal(b.get("F"), F)  # the same u as the step's F pass
+    up = FN * 1.001
+    b.ctx.upload("F", up)
+    assert np.array_equal(b.get("F"), up) and np.array_equal(b.get("fN"), FN)
+    b.step(3)
+    assert np.array_equal(b.get("F"), b.get("fN"))
+
+
 def test_fused_equals_unfused(vpd, orc):
     banner("fused predict == separate predict")
     kw = dict(nx=16, ny=6, nz=6, spacing=1.0, horizon=3.015, props=JC_PROPS, dt=8.5377e-5,

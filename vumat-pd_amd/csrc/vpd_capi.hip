@@ -109,6 +109,10 @@ struct vpd_context {
   int *derr = nullptr;
   bool haveNeighbors = false, haveShape = false, haveMaterial = false, haveDynamics = false;
   bool haveFnew = false, haveFields = false, haveForce = false, predicted = false;
+  // F_{n+1} of an eager single-specimen step is not copied into fN: the material stage leaves
+  // it in fnew and the two buffers swap (swapF asks for that on the next material stage;
+  // fInN: the latest F, which VPD_FIELD_DEFGRAD returns, is in fN since the swap).
+  bool swapF = false, fInN = false;
   bool timing = false;
   // With timing on, the kernels of every timingStride-th step are timed (timingTick counts
   // steps; timeThis says whether the step being taken is one of them).
@@ -395,7 +399,7 @@ double *field_ptr(vpd_context *c, int field) {
   case VPD_FIELD_SHAPE_INVERSE:
     return c->kinv;
   case VPD_FIELD_DEFGRAD:
-    return c->fnew;
+    return c->fInN ? c->fN : c->fnew;
   default:
     return nullptr;
   }
@@ -946,6 +950,7 @@ void launch_defgrad_t(vpd_context *c, int lds, const Rounds &rd, int useCompress
 
 void launch_defgrad_with(vpd_context *c, bool dmg, int lds, int rounds, const Batch &batch,
                          const Gather &gx) {
+  c->fInN = false; // fnew receives the latest F
   const int useCompression = c->failure > 0.0 && std::isfinite(c->failure);
   const Rounds rd = rounds_of(c->n, rounds);
   if (dmg)
@@ -1073,6 +1078,8 @@ int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
   g.nblocks = ceil_div(c->n, 256);
   g.fnew = planes(c->fnew, c->n);
   g.fN = planes(c->fN, c->n);
+  g.copyF = !c->swapF;
+  g.fCur = c->swapF ? g.fnew : g.fN;
   g.vN = planes(c->vN, c->n);
   g.rN = planes(c->rN, c->n);
   g.sig = planes(c->sig, c->n);
@@ -1719,6 +1726,8 @@ int vpd_upload(vpd_context *c, int field, const void *host) {
       field == VPD_FIELD_DAMAGE_INDEX || field == VPD_FIELD_RETURN_MAP_ITERATIONS ||
       field == VPD_FIELD_REFERENCE_POSITION || field == VPD_FIELD_VOLUME)
     return fail(c, VPD_ERR_ARGUMENT, "Field is not writable.");
+  if (field == VPD_FIELD_DEFGRAD)
+    c->fInN = false; // the caller's F goes to fnew, the state's F_n stays in fN
   double *dst = field_ptr(c, field);
   if (!dst)
     return fail(c, VPD_ERR_ARGUMENT, "Unknown field.");
@@ -1913,8 +1922,18 @@ int vpd_step(vpd_context *c, const vpd_step_args *g) {
                 "The diagnostics record buffer is full; fetch the records first.");
   if (int s = launch_defgrad(c))
     return s;
-  if (int s = launch_material(c, g->dt, g->want_fields != 0, c->diagOn))
-    return s;
+  // One specimen, eager, no run diagnostics, the JC material: F_{n+1} stays in fnew and the
+  // buffers swap instead of the material stage copying it into fN.
+  c->swapF = !c->batch.id && !c->capturing && !c->diagOn &&
+             c->materialModel != VPD_MATERIAL_ELASTIC;
+  const int ms = launch_material(c, g->dt, g->want_fields != 0, c->diagOn);
+  if (c->swapF && !ms) {
+    std::swap(c->fN, c->fnew);
+    c->fInN = true;
+  }
+  c->swapF = false;
+  if (ms)
+    return ms;
   if (int s = launch_force(c, g->dt, g->ramp, g->fuse_next != 0, g->ramp_next,
                            g->want_fields != 0, c->diagOn))
     return s;
@@ -2177,7 +2196,8 @@ int vpd_compute_force_density(vpd_context *c, const double *cauchy, double *forc
   if (int s = put_planes(c, c->sigOut, cauchy, 9))
     return s;
   k_piola<<<grid_of(c->n), 256, 0, c->stream>>>(c->n, planes(c->sigOut, c->n),
-                                                planes(c->fnew, c->n), planes(c->kinv, c->n),
+                                                planes(c->fInN ? c->fN : c->fnew, c->n),
+                                                planes(c->kinv, c->n),
                                                 pairs(c->amap, c->n), c->derr);
   HIPTRY(c, hipGetLastError());
   if (int s = check_device(c))

@@ -1206,6 +1206,10 @@ struct MaterialArgs {
   int64_t n;
   int64_t nblocks;
   Planes fnew, fN, vN, rN, sig, statev, kinv;
+  // Compacted update: copyF false leaves F_{n+1} in fnew for the host to swap the two planes
+  // (no copy into fN); fCur is the plane the list kernels read F_{n+1} from.
+  Planes fCur;
+  bool copyF;
   PairPlanes amap; // A = P K⁻¹ as 5 component pairs (the last one (A_8, 0))
   double *temp, *eint, *einel;
   Planes sigOut, strainOut, incOut;
@@ -1675,9 +1679,13 @@ __global__ void __launch_bounds__(256) k_material_screen(MaterialArgs g, Compact
         for (int c = 0; c < 6; ++c)
           g.incOut(c, i) = inc[c];
       }
+      if (g.copyF) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+          g.fN(c, i) = f1.m[c];
+      }
 #pragma unroll
       for (int c = 0; c < 9; ++c) {
-        g.fN(c, i) = f1.m[c];
         g.vN(c, i) = k.V1.m[c];
         g.rN(c, i) = k.R1.m[c];
       }
@@ -1759,7 +1767,7 @@ __global__ void __launch_bounds__(256) k_material_return_list(MaterialArgs g, Co
   T3 f1, r1;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
-    f1.m[c] = g.fN(c, i);
+    f1.m[c] = g.fCur(c, i);
     r1.m[c] = g.rN(c, i);
   }
   material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
@@ -1797,7 +1805,7 @@ __device__ __forceinline__ void list_point_bisect(const MaterialArgs &g, const C
   T3 f1, r1;
 #pragma unroll
   for (int c = 0; c < 9; ++c) {
-    f1.m[c] = g.fN(c, i);
+    f1.m[c] = g.fCur(c, i);
     r1.m[c] = g.rN(c, i);
   }
   material_finish<false>(g, i, f1, r1, sNew, st, eint, einel, stats, sOld, nullptr, nullptr);
