@@ -274,6 +274,17 @@ def test_defgrad_fields_after_swapped_steps(vpd):
     assert np.array_equal(b.get("F"), up) and np.array_equal(b.get("fN"), FN)
     b.step(3)
     assert np.array_equal(b.get("F"), b.get("fN"))
+    # A material update right after a step (no new F) updates from the latest F, as before
+    # the buffers swapped: a zero increment, so the stress stays where the step left it (up
+    # to the return map's tolerance; the stale F_n would undo a whole step's increment, about
+    # E·2.5e-3 here).
+    c = vpd.Bar(nx=10, ny=4, nz=4, spacing=1.0, horizon=2.015, props=JC_PROPS, dt=8.5377e-5,
+                density=7.85e-3, velocity=300.0)
+    c.step(4)
+    sig, fn = c.get("stressN"), c.get("fN")
+    c.ctx.update_material(8.5377e-5)
+    assert np.array_equal(c.get("fN"), fn)
+    assert np.abs(c.get("stressN") - sig).max() <= 1e-4 * np.abs(sig).max()
 
 
 def test_fused_equals_unfused(vpd, orc):

@@ -1073,6 +1073,13 @@ template <bool DIAG> int launch_material_t(vpd_context *c, MaterialArgs &g) {
 // The material stage in the context's return-map mode; with `diag`, the run diagnostics'
 // material terms are reduced as well (vpd_step with diagnostics enabled).
 int launch_material(vpd_context *c, double dt, bool fields, bool diag = false) {
+  if (c->fInN) {
+    // The last step swapped the F buffers and no F pass has run since: the material stage
+    // updates from the latest F, as it did when the step copied F_{n+1} into both.
+    HIPTRY(c, hipMemcpyAsync(c->fnew, c->fN, sizeof(double) * 9 * c->n,
+                             hipMemcpyDeviceToDevice, c->stream));
+    c->fInN = false;
+  }
   MaterialArgs g;
   g.n = c->n;
   g.nblocks = ceil_div(c->n, 256);
