@@ -986,6 +986,9 @@ int ensure_list(vpd_context *c) {
 // Blocks of up to kGroupedListMax particles run the bisections' plastic list with
 // k_material_return_list_g: on lane groups of kListLanes per point while the list is short.
 constexpr int64_t kGroupedListMax = 32768;
+// Blocks of up to this many particles run the material screen in 64-thread workgroups, one
+// wavefront per CU where the 256-thread ones would stack four on a few CUs.
+constexpr int64_t kSmallScreenMax = 16384;
 constexpr int kListLanes = 32;
 
 template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g, int grid) {
@@ -1003,7 +1006,14 @@ template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g,
   if (int s = timed(c, K_MATERIAL, [&] {
         if (clear)
           k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
-        k_material_screen<MODE><<<grid, 256, 0, c->stream>>>(g, cl);
+        if (c->n <= kSmallScreenMax) { // one wavefront per workgroup: spread over more CUs
+          MaterialArgs gs = g;
+          gs.nblocks = ceil_div(c->n, 64);
+          k_material_screen<MODE>
+              <<<(int)(ceil_div(gs.nblocks, 8) * 8), 64, 0, c->stream>>>(gs, cl);
+        } else {
+          k_material_screen<MODE><<<grid, 256, 0, c->stream>>>(g, cl);
+        }
       }))
     return s;
   return timed(c, K_RETURN_MAP, [&] {

@@ -1482,7 +1482,7 @@ struct GroupRanks {
   int a, b, totalA, totalB;
 };
 __device__ __forceinline__ GroupRanks group_ranks(bool takeA, bool takeB, int *base) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6; // ≤ 4
   const uint64_t ma = __ballot(takeA), mb = __ballot(takeB);
   if (lane == 0) {
     base[wv] = __popcll(ma);
@@ -1490,7 +1490,7 @@ __device__ __forceinline__ GroupRanks group_ranks(bool takeA, bool takeB, int *b
   }
   __syncthreads();
   GroupRanks r{0, 0, 0, 0};
-  for (int w = 0; w < 4; ++w) {
+  for (int w = 0; w < nw; ++w) {
     if (w < wv) {
       r.a += base[w];
       r.b += base[4 + w];
