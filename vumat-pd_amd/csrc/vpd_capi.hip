@@ -986,8 +986,9 @@ int ensure_list(vpd_context *c) {
 // Blocks of up to kGroupedListMax particles run the bisections' plastic list with
 // k_material_return_list_g: on lane groups of kListLanes per point while the list is short.
 constexpr int64_t kGroupedListMax = 32768;
-// Blocks of up to this many particles run the material screen in 64-thread workgroups, one
-// wavefront per CU where the 256-thread ones would stack four on a few CUs.
+// Blocks of up to this many particles run the material screen and the list kernels in
+// 64-thread workgroups, spread over the CUs where 256-thread ones would stack four
+// wavefronts on a few.
 constexpr int64_t kSmallScreenMax = 16384;
 constexpr int kListLanes = 32;
 
@@ -1017,14 +1018,15 @@ template <int MODE> int launch_material_compact(vpd_context *c, MaterialArgs &g,
       }))
     return s;
   return timed(c, K_RETURN_MAP, [&] {
+    const int wg = c->n <= kSmallScreenMax ? 64 : 256; // small blocks: spread over the CUs
     if constexpr (MODE == RM_BISECTION || MODE == RM_BISECTION_TIGHT) {
       if (c->n <= kGroupedListMax) { // short lists: lane groups per point
         k_material_return_list_g<MODE, kListLanes>
-            <<<(int)ceil_div(c->n * kListLanes, 256), 256, 0, c->stream>>>(g, cl);
+            <<<(int)ceil_div(c->n * kListLanes, wg), wg, 0, c->stream>>>(g, cl);
         return;
       }
     }
-    k_material_return_list<MODE><<<grid_of(c->n), 256, 0, c->stream>>>(g, cl);
+    k_material_return_list<MODE><<<(int)ceil_div(c->n, wg), wg, 0, c->stream>>>(g, cl);
   });
 }
 
