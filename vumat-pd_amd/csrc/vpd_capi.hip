@@ -194,7 +194,9 @@ struct vpd_context {
   uint16_t *specId = nullptr;
   double *specDt = nullptr, *specVel = nullptr;
   int *specLast = nullptr;
-  int *stepDev = nullptr; // the step counter read by graph-replayed steps
+  // The two step-counter slots of graph-replayed steps (a captured pair of steps reads one
+  // each), and the slot the force pass being captured writes the next step's number to.
+  int *stepDev = nullptr, *stepNextDev = nullptr;
   double *specBand = nullptr; // vpd_set_specimen_bands: prescribed band rows
   int64_t *specBandOff = nullptr;
   int specCount = 0;
@@ -1227,11 +1229,12 @@ int launch_force(vpd_context *c, double dt, const double ramp[3], bool fuse,
   if (int s = prepare_plans(c))
     return s;
   g.zeroCount = c->listCount; // the pass clears the material lists for the next update
+  g.stepNext = c->stepNextDev; // captured steps: the next step's counter slot
   if (int s = timed(c, K_FORCE, [&] {
         launch_force_with(c, g, dmg, fuse, true, c->ldsForce, diag, gather_of(c, 1));
       }))
     return s;
-  c->listZeroed = c->listCount && !c->capturing;
+  c->listZeroed = c->listCount != nullptr; // (also in a captured step: each replay clears)
   return VPD_OK;
 }
 
@@ -2014,7 +2017,7 @@ int vpd_set_specimens(vpd_context *c, int count, const int64_t *first, const dou
                         "horizon apart.");
   if (dalloc(c, c->specId, c->n) || dalloc(c, c->specDt, count) ||
       dalloc(c, c->specVel, count) || dalloc(c, c->specLast, count) ||
-      (!c->stepDev && dalloc(c, c->stepDev, 1)))
+      (!c->stepDev && dalloc(c, c->stepDev, 2)))
     return VPD_ERR_DEVICE;
   HIPTRY(c, hipMemcpy(c->specId, id.data(), sizeof(uint16_t) * c->n, hipMemcpyHostToDevice));
   HIPTRY(c, hipMemcpy(c->specDt, dt, sizeof(double) * count, hipMemcpyHostToDevice));
@@ -2157,43 +2160,66 @@ int vpd_run_specimens(vpd_context *c, int32_t firstStep, int32_t count) {
     k_zero_count<<<1, 1, 0, c->stream>>>(c->listCount);
   c->listZeroed = c->listCount != nullptr;
   HIPTRY(c, hipGetLastError());
-  // One fused step, reading its step number from the device counter, captured once.
+  // Two fused steps captured once: the first reads its number from counter slot 0 and its
+  // force pass sets slot 1, the second reads slot 1 and sets slot 0 for the next replay, so
+  // the graph needs no kernel of its own to move the counter. An odd last step runs eagerly.
   const bool timing = c->timing;
   const Batch eager = c->batch;
   c->timing = false;
-  c->batch.stepDev = c->stepDev;
-  hipGraph_t graph = nullptr;
-  HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  c->capturing = true;
-  int s = launch_defgrad(c);
-  if (!s)
-    s = launch_material(c, 1.0, false);
-  if (!s)
-    s = launch_force(c, 1.0, none, true, none, false);
-  if (!s)
-    k_advance_step<<<1, 64, 0, c->stream>>>(c->stepDev);
-  const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
-  c->capturing = false;
-  c->listZeroed = false;
-  c->batch = eager;
-  c->timing = timing;
-  if (s) {
-    if (ec == hipSuccess && graph)
-      (void)hipGraphDestroy(graph);
-    return s;
-  }
-  HIPTRY(c, ec);
   hipGraphExec_t exec = nullptr;
-  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  HIPTRY(c, ei);
+  const int pairs = count / 2;
+  if (pairs > 0) {
+    hipGraph_t graph = nullptr;
+    HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int s = VPD_OK;
+    for (int h = 0; h < 2 && !s; ++h) {
+      c->batch.stepDev = c->stepDev + h;
+      c->stepNextDev = c->stepDev + (1 - h);
+      s = launch_defgrad(c);
+      if (!s)
+        s = launch_material(c, 1.0, false);
+      if (!s)
+        s = launch_force(c, 1.0, none, true, none, false);
+    }
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    c->capturing = false;
+    c->stepNextDev = nullptr;
+    c->batch = eager;
+    if (s) {
+      if (ec == hipSuccess && graph)
+        (void)hipGraphDestroy(graph);
+      c->listZeroed = false;
+      c->timing = timing;
+      return s;
+    }
+    HIPTRY(c, ec);
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPTRY(c, ei);
+  }
   hipError_t el = hipSuccess;
-  for (int k = 0; k < count && el == hipSuccess; ++k)
+  for (int k = 0; k < pairs && el == hipSuccess; ++k)
     el = hipGraphLaunch(exec, c->stream);
+  int s = VPD_OK;
+  if (el == hipSuccess && count % 2) { // the odd last step, eagerly, as the captured ones
+    c->batch.step = firstStep + count - 1;
+    s = launch_defgrad(c);
+    if (!s)
+      s = launch_material(c, 1.0, false);
+    if (!s)
+      s = launch_force(c, 1.0, none, true, none, false);
+    c->batch = eager;
+  }
+  c->timing = timing;
   const hipError_t es = hipStreamSynchronize(c->stream);
-  (void)hipGraphExecDestroy(exec);
+  if (exec)
+    (void)hipGraphExecDestroy(exec);
+  c->listZeroed = false;
   HIPTRY(c, el);
   HIPTRY(c, es);
+  if (s)
+    return s;
   c->batch.step = firstStep + count - 1;
   c->predicted = true; // every specimen past its last step, or predicted for the next one
   c->haveFnew = true;

@@ -274,7 +274,7 @@ struct Batch {
   const int *lastStep;
   int step; // the step being taken (1-based; 0 for the initial constraints)
   // When set, the step is read from this device counter instead, so that one captured HIP
-  // graph of a step serves every step (k_advance_step moves it on).
+  // graph serves every step (the graph's force passes move the counters on).
   const int *stepDev;
   // Prescribed band kinematics (vpd_set_specimen_bands): specimen s with
   // bandOff[s] < bandOff[s+1] takes (u, v, a) of step k from band[3·(bandOff[s] + k)], the
@@ -324,10 +324,6 @@ __global__ void k_set_step(int *stepDev, int step) {
     *stepDev = step;
 }
 
-__global__ void k_advance_step(int *stepDev) {
-  if (threadIdx.x == 0)
-    *stepDev += 1;
-}
 
 // Critical-stretch test of PdSimpleDamage.cpp:72-77 with a cheap exact filter.
 // The reference decides stretch = (‖y‖ − ‖ξ‖)/‖ξ‖ > s_c (or < −f), with two square roots and a
@@ -1855,12 +1851,21 @@ struct ForceArgs {
   // When set, the pass clears the two list lengths of the compacted material update (the
   // step's material stage has run; the next one appends from zero).
   int *zeroCount = nullptr;
+  // Captured steps: the counter slot the next step reads its number from (this step's plus
+  // one; the step's own slot is another, read by every block of this pass).
+  int *stepNext = nullptr;
 };
 
+// The force pass's one-thread bookkeeping for the next step: clear the material lists'
+// lengths and, in a captured step, set the next step's number.
 __device__ __forceinline__ void clear_list_length(const ForceArgs &g) {
-  if (g.zeroCount && blockIdx.x == 0 && threadIdx.x == 0) {
-    g.zeroCount[0] = 0;
-    g.zeroCount[1] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (g.zeroCount) {
+      g.zeroCount[0] = 0;
+      g.zeroCount[1] = 0;
+    }
+    if (g.stepNext)
+      *g.stepNext = g.batch.current() + 1;
   }
 }
 
